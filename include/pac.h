@@ -1,0 +1,237 @@
+/*
+ * pac.h -- C ABI of partitionandconquer_amd: the MI355X (gfx950) implementation of
+ * the partitioned neighbour-set intersection path of Byy-0024/PartitionAndConquer
+ * (triangle counting, local clustering coefficient, Jaccard similarity).
+ *
+ * The reference exposes this path as the C++ classes Graph / Partitioned_Graph
+ * (src/Graph.hpp:38-155) plus the argv protocol of bin/main (src/test.cpp:301-389).
+ * Every entry point below names the reference interface it replaces.  Plain
+ * pointers and sizes only; no C++ types or exceptions cross this boundary.
+ *
+ * Conventions
+ *   - Every int-returning function returns PAC_OK (0) or a negative PAC_ERR_*;
+ *     pac_last_error() then describes the failure (thread-local string).
+ *   - The caller owns every host buffer; the library owns device memory behind the
+ *     opaque handles.  Host output buffers must have the documented length.
+ *   - A pac_ctx owns one HIP stream on one device.  Handles are not re-entrant:
+ *     use one host thread per context.
+ *   - Vertex IDs are int32 on the boundary (reference VID, Graph.hpp:23) and uint32
+ *     inside; CSR offsets are uint64 on the boundary (reference EID, Graph.hpp:24)
+ *     and uint32 on the device, which limits a graph to < 2^32 directed entries.
+ *   - Counts are exact integers.  Ratios are computed from the exact integers as
+ *     (double)num/(double)den, rounded once to float where a float is returned.
+ */
+#ifndef PAC_H
+#define PAC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PAC_OK 0
+#define PAC_ERR_ARG (-1)         /* bad argument (NULL, size, range)                 */
+#define PAC_ERR_HIP (-2)         /* HIP runtime error                                */
+#define PAC_ERR_OOM (-3)         /* device or host allocation failed / budget hit    */
+#define PAC_ERR_FORMAT (-4)      /* malformed CSR: offsets, ordering, IDs, loops     */
+#define PAC_ERR_IO (-5)          /* file missing, unreadable or truncated            */
+#define PAC_ERR_STATE (-6)       /* call order (e.g. solve before prepare)           */
+#define PAC_ERR_UNSUPPORTED (-7) /* graph exceeds a documented limit                 */
+#define PAC_ERR_TIMEOUT (-8)     /* persistent queue made no progress in time        */
+
+typedef struct pac_ctx pac_ctx;
+typedef struct pac_graph pac_graph;
+
+/* Which intersection strategy runs on each partition. */
+enum {
+  PAC_DISPATCH_FIXED = 0,   /* fixed degree-threshold heuristic                      */
+  PAC_DISPATCH_LEARNED = 1, /* MLP over per-partition features (falls back to FIXED  *
+                             * when no model is loaded)                              */
+  PAC_FORCE_MERGE = 2,      /* wave64 merge-path sorted merge everywhere             */
+  PAC_FORCE_GALLOP = 3,     /* binary search of the shorter list in the longer one   */
+  PAC_FORCE_BITMAP = 4      /* LDS hashed bitmap of N+(u) + exact LDS confirmation   */
+};
+#define PAC_NUM_STRATEGIES 3 /* strategy ids: 0 merge, 1 gallop, 2 bitmap */
+
+/* How the per-partition work is launched. */
+enum {
+  PAC_LAUNCH_PLAIN = 0,      /* one kernel launch per partition, one workgroup/row    */
+  PAC_LAUNCH_PERSISTENT = 1, /* one persistent kernel over an atomic work queue       */
+  PAC_LAUNCH_GRAPH = 2       /* PERSISTENT, captured once in a hipGraph and replayed  */
+};
+
+typedef struct {
+  uint64_t num_vertices;   /* V                                                     */
+  uint64_t num_entries;    /* directed CSR entries = 2m (Graph::get_edge_num)        */
+  uint64_t num_edges;      /* undirected edges m = oriented edges                    */
+  uint64_t max_degree;
+  uint64_t max_out_degree; /* under degree orientation (d(u),u) < (d(v),v)           */
+  uint64_t work_W;         /* sum over oriented u->v of d+(u)+d+(v)                  */
+  uint64_t num_partitions; /* 0 before pac_graph_prepare                             */
+  uint64_t num_items;      /* persistent work items after pac_graph_prepare          */
+  double prepare_ms;       /* GPU time of the last pac_graph_prepare                 */
+} pac_graph_info;
+
+typedef struct {
+  double total_ms;      /* device time of the last solve (zeroing+intersection+epilogue) */
+  double isect_ms;      /* device time of the intersection launches alone               */
+  double epilogue_ms;   /* device time of the epilogue kernel                           */
+  uint64_t launches;    /* kernel launches in the intersection phase                    */
+  uint64_t strategy_partitions[PAC_NUM_STRATEGIES]; /* partitions per strategy          */
+} pac_timing;
+
+/* ---- context ------------------------------------------------------------------- */
+/* No reference counterpart (the reference is single-process OpenMP): binds a HIP
+ * device and creates the context's stream. */
+int pac_ctx_create(int device, pac_ctx **out);
+void pac_ctx_destroy(pac_ctx *ctx);
+/* Thread-local description of the last failure on this thread. */
+const char *pac_last_error(void);
+/* Device timing of the last solve on this context (hipEvents on the ctx stream). */
+int pac_ctx_last_timing(const pac_ctx *ctx, pac_timing *out);
+/* Number of compute units of the context's device. */
+int pac_ctx_num_cus(const pac_ctx *ctx);
+
+/* ---- graph construction (L1/L2 of the reference) ---------------------------------- */
+/* Replaces Graph::Graph(const std::vector<...>&) / the CSR members (Graph.hpp:99-100):
+ * uploads a symmetric CSR with sorted, duplicate-free, self-loop-free rows.
+ * vlist has V+1 entries, elist has nnz = vlist[V] entries.  Offsets and IDs are
+ * validated (PAC_ERR_FORMAT). */
+int pac_graph_upload(pac_ctx *ctx, const uint64_t *vlist, uint64_t num_vertices,
+                     const int32_t *elist, uint64_t nnz, pac_graph **out);
+/* Replaces Graph::Graph(const std::string csrdir, bool), Graph.cpp:92-137: reads
+ * <dir>/csr_vlist.bin (uint64 [V+1]) and <dir>/csr_elist.bin (int32 [nnz]). */
+int pac_graph_load_csr_dir(pac_ctx *ctx, const char *csrdir, pac_graph **out);
+/* Replaces Graph::Graph(const std::string graphfile), Graph.cpp:27-90 (text edge
+ * list: '#'/'%' comments, one-character separator, 1-based IDs shifted when no 0
+ * appears, self-loops dropped, symmetrized, rows sorted and deduplicated). */
+int pac_graph_load_text(pac_ctx *ctx, const char *path, pac_graph **out);
+/* Replaces Graph::save_as_csr_binfile, Graph.cpp:595-605 (same two files). */
+int pac_graph_save_csr_dir(const pac_graph *g, const char *csrdir);
+/* Seeded R-MAT generator on the GPU: ef*2^scale samples with quadrant probabilities
+ * (a,b,c,1-a-b-c), no noise, no label permutation, self-loops and duplicates
+ * removed, symmetrized.  No reference counterpart (the reference reads files). */
+int pac_graph_generate_rmat(pac_ctx *ctx, uint32_t scale, uint32_t edge_factor,
+                            double a, double b, double c, uint64_t seed,
+                            pac_graph **out);
+/* Seeded Erdos-Renyi G(n,p), p = avg_degree/(n-1). No reference counterpart. */
+int pac_graph_generate_er(pac_ctx *ctx, uint64_t n, double avg_degree, uint64_t seed,
+                          pac_graph **out);
+/* Copies the symmetric CSR back in the reference's types (Graph::get_csr_vlist /
+ * get_csr_elist, Graph.cpp:151-157).  vlist: V+1, elist: nnz. */
+int pac_graph_download_csr(const pac_graph *g, uint64_t *vlist, int32_t *elist);
+int pac_graph_get_info(const pac_graph *g, pac_graph_info *out);
+/* Structural check of the uploaded graph on the GPU; level 1 adds the symmetry
+ * check (every (u,v) has (v,u)). */
+int pac_graph_validate(const pac_graph *g, int level);
+void pac_graph_free(pac_graph *g);
+
+/* ---- partitioning (L3) ---------------------------------------------------------- */
+/* Replaces Partitioned_Graph::Partitioned_Graph(g, vertex2partition, threads),
+ * Graph.cpp:1017-1069: builds the degree-ordered oriented CSR on the GPU
+ * (histogram / scan / scatter into uint32 CSR), groups rows by partition and builds
+ * the persistent work queue.  v2p (V int32) assigns each vertex to a partition, as
+ * the reference's P<n>.bin does (test.cpp:242-243); NULL selects contiguous-ID
+ * partitions part(v) = v / ceil(V/num_partitions).  P = max(num_partitions,
+ * max(v2p)+1) (the reference takes max(v2p)+1, Graph.cpp:1021). */
+int pac_graph_prepare(pac_graph *g, uint32_t num_partitions, const int32_t *v2p);
+/* Per-partition oriented edge counts (P entries) after prepare. */
+int pac_graph_partition_sizes(const pac_graph *g, uint64_t *edges_per_partition);
+/* Graph::random_partition, Graph.cpp:564-570, with a seeded stream (host). */
+int pac_random_partition(uint64_t V, uint32_t P, uint64_t seed, int32_t *v2p);
+/* Graph::greedy_partition, Graph.cpp:898-967 (degree-descending,
+ * deficit-proportional; seeded instead of random_device) (host). */
+int pac_greedy_partition(const pac_graph *g, uint32_t P, uint64_t seed, int32_t *v2p);
+/* Partitioned_Graph::evaluate_partition, Graph.cpp:1460-1473: returns the balance
+ * ratio P*max_part_deg/max_deg for a vertex2partition. */
+int pac_evaluate_partition(const pac_graph *g, const int32_t *v2p, uint32_t P,
+                           double *balance_ratio);
+
+/* ---- solvers (L4/L5) -------------------------------------------------------------- */
+/* Replaces Graph::solve_triangle_counting_merge (Graph.cpp:355-368) and
+ * Partitioned_Graph::solve_triangle_counting_merge (Graph.cpp:1346-1362): *total = T.
+ * per_vertex (NULL or V uint64) receives t(u), the triangles through u.  The
+ * reference's *_galloping variants return 6T (Graph.cpp:342-353, 1326-1344); the
+ * C++ facade multiplies. */
+int pac_tc(pac_graph *g, int dispatch, int launch, uint64_t *total, uint64_t *per_vertex);
+/* Replaces Graph::solve_lcc_merge / _galloping (Graph.cpp:814-840) and the
+ * Partitioned_Graph versions (Graph.cpp:2197-2261): lcc64[u] =
+ * 2t(u)/(d(d-1)) in double (0.0 for d=0, NaN for d=1, as the reference), lcc32 =
+ * (float)lcc64.  Either output may be NULL.  total (may be NULL) receives T. */
+int pac_lcc(pac_graph *g, int dispatch, int launch, float *lcc32, double *lcc64,
+            uint64_t *total);
+/* Per-edge Jaccard over every undirected edge, in oriented-edge order (m entries):
+ * src/dst (original IDs, src has the smaller (degree, id)), inter = |N(u)&N(v)|,
+ * uni = d(u)+d(v)-inter, ratio = (float)((double)inter/(double)uni) -- the pair
+ * formula of Graph::solve_jsim_merge (Graph.cpp:808-809) applied to each edge.  Any
+ * output pointer may be NULL. */
+int pac_jaccard_edges(pac_graph *g, int dispatch, int launch, int32_t *src, int32_t *dst,
+                      uint32_t *inter, uint32_t *uni, float *ratio);
+/* Replaces Graph::solve_jsim_merge / _galloping (Graph.cpp:782-812): the n x n matrix
+ * res[i*n+j] = c/((double)(d_i+d_j) - c) over the query vertices q[0..n). */
+int pac_jaccard_queries(pac_graph *g, const int32_t *q, uint32_t n, double *res);
+/* Replaces Graph::batch_cnt_common_neis_merge(pairs) (Graph.cpp:413-447):
+ * res[i] = |N(u[i]) & N(v[i])| over the full symmetric lists. */
+int pac_cnt_common_pairs(pac_graph *g, const int32_t *u, const int32_t *v, uint64_t n,
+                         uint64_t *res);
+
+/* ---- batching ------------------------------------------------------------------- */
+/* Triangle counts of G prepared graphs in one persistent launch (or one hipGraph
+ * replay when launch == PAC_LAUNCH_GRAPH): totals[i] = T of gs[i].  No reference
+ * counterpart (the reference handles one graph per process). */
+int pac_tc_batch(pac_ctx *ctx, pac_graph *const *gs, uint32_t G, int dispatch, int launch,
+                 uint64_t *totals);
+
+/* ---- autotuning ------------------------------------------------------------------ */
+/* Tile configurations (waves per row, LDS list capacity, LDS bitmap bits). */
+#define PAC_NUM_TILES 4
+/* Times every tile configuration of the persistent kernel on the given graphs and
+ * keeps the fastest for this context; ms_out (PAC_NUM_TILES, may be NULL) receives
+ * the median times.  cache_path (may be NULL) stores/loads the choice keyed by the
+ * graphs' shape. Returns the chosen tile id in *tile. */
+int pac_autotune(pac_ctx *ctx, pac_graph *const *gs, uint32_t G, const char *cache_path,
+                 int *tile, double *ms_out);
+int pac_set_tile(pac_ctx *ctx, int tile);
+int pac_get_tile(const pac_ctx *ctx, int *tile);
+
+/* ---- learned dispatch ------------------------------------------------------------ */
+#define PAC_MLP_IN 16
+#define PAC_MLP_HIDDEN 64
+#define PAC_MLP_OUT 3
+#define PAC_MLP_NPARAMS (PAC_MLP_HIDDEN * PAC_MLP_IN + PAC_MLP_HIDDEN + \
+                         PAC_MLP_HIDDEN * PAC_MLP_HIDDEN + PAC_MLP_HIDDEN + \
+                         PAC_MLP_OUT * PAC_MLP_HIDDEN + PAC_MLP_OUT)
+/* Per-partition features (P x 16 floats) of a prepared graph. */
+int pac_partition_features(pac_graph *g, float *features);
+/* Times each strategy on each partition in isolation (one launch per partition and
+ * strategy, median of `reps`); ms has P x 3 entries. */
+int pac_profile_partitions(pac_graph *g, int reps, double *ms);
+/* The strategy the fixed heuristic picks per partition (P bytes). */
+int pac_fixed_choice(pac_graph *g, uint8_t *choice);
+/* The strategy the loaded MLP picks per partition (P bytes). */
+int pac_mlp_choice(pac_graph *g, uint8_t *choice);
+/* Set / get the MLP parameters (PAC_MLP_NPARAMS floats; stored on the device as
+ * bf16, so get returns the bf16-rounded values). */
+int pac_mlp_set_params(pac_ctx *ctx, const float *params);
+int pac_mlp_get_params(const pac_ctx *ctx, float *params);
+int pac_mlp_init(pac_ctx *ctx, uint64_t seed);
+/* Forward pass on the GPU: logits (n x 3) for features x (n x 16). */
+int pac_mlp_forward(pac_ctx *ctx, const float *x, uint64_t n, float *logits);
+/* Mean (weighted) softmax cross-entropy and its gradient w.r.t. the parameters,
+ * computed by the GPU backward kernels (weights may be NULL). */
+int pac_mlp_loss_grad(pac_ctx *ctx, const float *x, const int32_t *labels,
+                      const float *weights, uint64_t n, float *loss, float *grads);
+/* Trains the MLP on the GPU with Adam for `epochs` full-batch steps; loss_out
+ * (may be NULL) receives the final loss. */
+int pac_mlp_train(pac_ctx *ctx, const float *x, const int32_t *labels,
+                  const float *weights, uint64_t n, uint32_t epochs, float lr,
+                  float *loss_out);
+int pac_mlp_save(const pac_ctx *ctx, const char *path);
+int pac_mlp_load(pac_ctx *ctx, const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PAC_H */

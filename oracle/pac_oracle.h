@@ -1,0 +1,146 @@
+/*
+ * pac_oracle.h -- CPU/OpenMP restatement of the PartitionAndConquer neighbour-set
+ * intersection path (triangle counting, LCC, Jaccard), used ONLY as test
+ * infrastructure: by tests/, by __graft_entry__.smoke() as the checker, and by
+ * bench.py's cpu_baseline leg.  The product library (libpac.so) never links,
+ * loads or calls anything here.
+ *
+ * Every function cites the reference function it restates (file:line in
+ * Byy-0024/PartitionAndConquer).  Types follow the reference: vertex IDs are
+ * int32 (Graph.hpp:23 VID), CSR offsets are uint64 (Graph.hpp:24 EID), counts are
+ * size_t/uint64 and ratios double.
+ *
+ * Pinning (what checks each function independently of its own formula):
+ *   orc_text_to_csr      hand-written text fixtures (tests/golden/text_*.txt)
+ *   orc_cnt_*            std::set_intersection / Python set on random sorted lists
+ *   orc_tc_*             closed forms (K_n, wheel, cycle, K_ab, friendship),
+ *                        trace(A^3)/6 dense brute force, networkx.triangles
+ *   orc_lcc_*            diag(A^3), networkx.clustering (differs only at d=1)
+ *   orc_jsim_*           (A^2)_uv dense, networkx.jaccard_coefficient, closed forms
+ *   orc_jaccard_edges    (A^2)_uv dense on the edge set
+ *   orc_pgraph_*         partition-sum invariance against orc_tc_merge and
+ *                        brute force, for random and contiguous vertex2partition
+ *   orc_gen_*            structural invariants (symmetric, sorted, unique, no
+ *                        self-loops), edge-count expectations; the GPU generator
+ *                        must reproduce orc_gen_rmat bit-exactly
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- loaders / builders ---------------------------------------------------- */
+
+/* Graph::Graph(const std::string graphfile), Graph.cpp:27-90.
+ * Returns 0 on success, -1 if the file cannot be opened.  *vlist (V+1 uint64) and
+ * *elist (nnz int32) are malloc'd; free with orc_free. */
+int orc_text_to_csr(const char *path, uint64_t *V, uint64_t **vlist, int32_t **elist,
+                    uint64_t *nnz);
+
+/* Build a symmetric, sorted, deduplicated, self-loop-free CSR from an undirected
+ * edge list (the row-building half of Graph.cpp:70-85). */
+int orc_edges_to_csr(uint64_t V, const uint32_t *src, const uint32_t *dst, uint64_t ne,
+                     uint64_t **vlist, int32_t **elist, uint64_t *nnz);
+
+void orc_free(void *p);
+
+/* ---- pairwise intersections (Graph.cpp:186-198, 227-239, 272-285) ---------- */
+uint64_t orc_cnt_merge(const uint64_t *vl, const int32_t *el, int32_t u, int32_t v);
+uint64_t orc_cnt_less_than_merge(const uint64_t *vl, const int32_t *el, int32_t u,
+                                 int32_t v, int32_t thr);
+uint64_t orc_cnt_galloping(const uint64_t *vl, const int32_t *el, int32_t u, int32_t v);
+
+/* ---- triangle counting ---------------------------------------------------- */
+/* Graph::solve_triangle_counting_merge, Graph.cpp:355-368 -> T */
+uint64_t orc_tc_merge(const uint64_t *vl, const int32_t *el, uint64_t V, int nt);
+/* Graph::solve_triangle_counting_galloping, Graph.cpp:342-353 -> 6T */
+uint64_t orc_tc_galloping(const uint64_t *vl, const int32_t *el, uint64_t V, int nt);
+/* The loop of Graph.cpp:355-368 restricted to the listed rows u (CPU baseline
+ * sampling); returns the triangles counted at those rows. */
+uint64_t orc_tc_merge_rows(const uint64_t *vl, const int32_t *el, const int32_t *rows,
+                           uint64_t nrows, int nt);
+
+/* ---- LCC ------------------------------------------------------------------ */
+/* Graph::solve_lcc_merge, Graph.cpp:828-840 (res has V doubles) */
+void orc_lcc_merge(const uint64_t *vl, const int32_t *el, uint64_t V, double *res, int nt);
+/* Graph::solve_lcc_galloping, Graph.cpp:814-826 */
+void orc_lcc_galloping(const uint64_t *vl, const int32_t *el, uint64_t V, double *res,
+                       int nt);
+/* The integer numerator of Graph.cpp:834-837: sum over v in N(u) of
+ * cnt_common_neis_merge(u,v), which equals 2*t(u). */
+void orc_lcc_numer(const uint64_t *vl, const int32_t *el, uint64_t V, uint64_t *numer,
+                   int nt);
+/* Graph.cpp:828-840 restricted to listed rows (CPU baseline sampling). Returns the
+ * sum of the numerators (to keep the work observable). */
+uint64_t orc_lcc_merge_rows(const uint64_t *vl, const int32_t *el, const int32_t *rows,
+                            uint64_t nrows, double *res_rows, int nt);
+
+/* ---- Jaccard ---------------------------------------------------------------- */
+/* Graph::solve_jsim_merge, Graph.cpp:798-812 (res has n*n doubles) */
+void orc_jsim_merge(const uint64_t *vl, const int32_t *el, const int32_t *q, uint64_t n,
+                    double *res, int nt);
+/* Graph::solve_jsim_galloping, Graph.cpp:782-796 */
+void orc_jsim_galloping(const uint64_t *vl, const int32_t *el, const int32_t *q,
+                        uint64_t n, double *res, int nt);
+/* Per-edge form of the pair formula of Graph.cpp:808-809: for each listed edge
+ * (src[i],dst[i]): inter = cnt_common_neis_merge, uni = d(u)+d(v)-inter,
+ * ratio = (float)((double)inter/(double)uni). */
+void orc_jaccard_edges(const uint64_t *vl, const int32_t *el, const int32_t *src,
+                       const int32_t *dst, uint64_t ne, uint32_t *inter, uint32_t *uni,
+                       float *ratio, int nt);
+
+/* ---- Partitioned_Graph (Graph.cpp:1017-1069 and its solvers) ---------------- */
+typedef struct orc_pgraph orc_pgraph;
+orc_pgraph *orc_pgraph_build(const uint64_t *vl, const int32_t *el, uint64_t V,
+                             const int32_t *v2p, int nt);
+uint64_t orc_pgraph_num_partitions(const orc_pgraph *g);
+/* Partitioned_Graph::solve_triangle_counting_merge, Graph.cpp:1346-1362 -> T */
+uint64_t orc_pgraph_tc_merge(const orc_pgraph *g);
+/* Partitioned_Graph::solve_triangle_counting_galloping, Graph.cpp:1326-1344 -> 6T */
+uint64_t orc_pgraph_tc_galloping(const orc_pgraph *g);
+/* Partitioned_Graph::solve_lcc_merge, Graph.cpp:2230-2261 */
+void orc_pgraph_lcc_merge(const orc_pgraph *g, double *res);
+/* Per-partition partial of Graph.cpp:1346-1362 for intersection partition i. */
+uint64_t orc_pgraph_tc_merge_partial(const orc_pgraph *g, uint64_t i);
+void orc_pgraph_free(orc_pgraph *g);
+
+/* ---- random_partition (Graph.cpp:564-570) with a seeded stream instead of rand() */
+void orc_random_partition(uint64_t V, uint64_t P, uint64_t seed, int32_t *v2p);
+/* Contiguous-ID partition: part(v) = v / ceil(V/P). */
+void orc_contiguous_partition(uint64_t V, uint64_t P, int32_t *v2p);
+
+/* ---- seeded generators (no reference counterpart; test infrastructure) ----- */
+/* R-MAT, ef * 2^scale samples, quadrant probabilities (a,b,c,1-a-b-c), integer
+ * arithmetic only, so the GPU generator reproduces it bit for bit. */
+int orc_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                 uint64_t seed, uint64_t *V, uint64_t **vlist, int32_t **elist,
+                 uint64_t *nnz);
+/* Erdos-Renyi G(n,p), p = avg_deg/(n-1), row-parallel Batagelj-Brandes skipping. */
+int orc_gen_er(uint64_t n, double avg_deg, uint64_t seed, uint64_t *V, uint64_t **vlist,
+               int32_t **elist, uint64_t *nnz);
+
+/* ---- degree-ordered orientation statistics (the bench's work model) -------- */
+typedef struct {
+  uint64_t V, nnz, m;          /* vertices, directed entries (2m), undirected edges */
+  uint64_t max_deg, max_outdeg;
+  uint64_t W;                  /* sum over oriented u->v of (d+(u) + d+(v)) */
+  uint64_t sum_outdeg_sq;      /* sum_u d+(u)^2 */
+  uint64_t sum_deg_sq;         /* sum_u d(u)^2 */
+} orc_stats;
+void orc_oriented_stats(const uint64_t *vl, const int32_t *el, uint64_t V, orc_stats *s);
+
+/* ---- MLP oracle (16 -> 64 -> 64 -> 3, ReLU, softmax cross-entropy) ---------- */
+/* Weights are given as float (already rounded to bf16 by the caller when the GPU
+ * path stores bf16); all arithmetic is float with double accumulation where noted.
+ * Layout: W1[64][16], b1[64], W2[64][64], b2[64], W3[3][64], b3[3]. */
+void orc_mlp_forward(const float *params, const float *x, uint64_t n, float *logits);
+/* Mean softmax cross-entropy over n samples with integer labels; writes dL/dparams
+ * (same layout as params) and returns the loss. Sample weights may be NULL. */
+float orc_mlp_backward(const float *params, const float *x, const int32_t *labels,
+                       const float *weights, uint64_t n, float *grads);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,180 @@
+// pac_gen.hip -- seeded R-MAT generation straight into the device CSR.
+//
+// No reference counterpart: the reference only reads graphs from files
+// (Graph.cpp:27-137).  The generator is the synthetic-input side of the benchmarks:
+// ef*2^scale samples, each drawing `scale` quadrants with probabilities (a,b,c,d)
+// from a SplitMix64 stream (Steele, Lea, Flood 2014) seeded by a counter-based hash
+// of (seed, sample).  Only integer arithmetic is used, so the host restatement in
+// the oracle produces the identical edge list.  Self-loops are dropped, both
+// directions are emitted, and a device radix sort + unique + compaction gives the
+// symmetric CSR with sorted, duplicate-free rows.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "pac_internal.hpp"
+
+namespace pac {
+namespace {
+
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_rmat(uint64_t ns, uint32_t scale, uint64_t t1, uint64_t t2, uint64_t t3,
+                       uint64_t key, uint64_t *__restrict__ keys) {
+  const uint64_t sentinel = 1ull << (2 * scale);
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ns;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t st = mix64(key + i * kGamma);
+    uint32_t u = 0, v = 0;
+    for (uint32_t l = 0; l < scale; ++l) {
+      st += kGamma;
+      const uint64_t r = mix64(st) >> 11;
+      const uint32_t q = r < t1 ? 0u : (r < t2 ? 1u : (r < t3 ? 2u : 3u));
+      u = (u << 1) | (q >> 1);
+      v = (v << 1) | (q & 1u);
+    }
+    if (u == v) {
+      keys[2 * i] = sentinel;
+      keys[2 * i + 1] = sentinel;
+    } else {
+      keys[2 * i] = ((uint64_t)u << scale) | v;
+      keys[2 * i + 1] = ((uint64_t)v << scale) | u;
+    }
+  }
+}
+
+__global__ void k_unique_flags(const uint64_t *__restrict__ keys, uint64_t n, uint64_t sentinel,
+                               uint32_t *__restrict__ flags) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i];
+    flags[i] = (k != sentinel && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
+  }
+}
+
+__global__ void k_compact(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ flags,
+                          const uint32_t *__restrict__ pos, uint64_t n, uint32_t mask,
+                          uint32_t *__restrict__ adj, uint64_t *__restrict__ ckeys) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (flags[i]) {
+      const uint64_t k = keys[i];
+      adj[pos[i]] = (uint32_t)(k & mask);
+      ckeys[pos[i]] = k;
+    }
+  }
+}
+
+// off[u] = first position whose key has source >= u (binary search over sorted keys)
+__global__ void k_offsets(const uint64_t *__restrict__ ckeys, uint64_t nnz, uint32_t scale,
+                          uint32_t V, uint32_t *__restrict__ off) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u <= V;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t target = u << scale;
+    uint64_t lo = 0, n = nnz;
+    while (n > 0) {
+      const uint64_t half = n >> 1;
+      if (ckeys[lo + half] < target) {
+        lo += half + 1;
+        n -= half + 1;
+      } else {
+        n = half;
+      }
+    }
+    off[u] = (uint32_t)lo;
+  }
+}
+
+uint32_t grid1d(uint64_t n) {
+  uint64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > (1u << 20)) b = 1u << 20;
+  return (uint32_t)b;
+}
+
+}  // namespace
+
+int generate_rmat(Graph *g, uint32_t scale, uint32_t ef, double a, double b, double c,
+                  uint64_t seed) {
+  Ctx *ctx = g->ctx;
+  if (scale < 1 || scale > 26 || ef < 1 || ef > 1024) {
+    set_error("rmat: scale must be in [1,26] and edge factor in [1,1024]");
+    return PAC_ERR_ARG;
+  }
+  if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0)) {
+    set_error("rmat: probabilities must be >= 0 and a+b+c <= 1");
+    return PAC_ERR_ARG;
+  }
+  const uint64_t ns = (uint64_t)ef << scale;
+  const uint64_t n2 = 2 * ns;
+  if (n2 >= (1ull << 32)) {
+    set_error("rmat: %llu samples exceed the uint32 CSR limit", (unsigned long long)ns);
+    return PAC_ERR_UNSUPPORTED;
+  }
+  const double two53 = 9007199254740992.0;
+  const uint64_t t1 = (uint64_t)(a * two53), t2 = (uint64_t)((a + b) * two53),
+                 t3 = (uint64_t)((a + b + c) * two53);
+  uint64_t z = seed + kGamma;  // host copy of mix64(seed + gamma)
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  const uint64_t key = z ^ (z >> 31);
+  const uint32_t V = 1u << scale;
+  const uint64_t sentinel = 1ull << (2 * scale);
+
+  DevBuf keys_a, keys_b, flags, pos, tmp;
+  PAC_TRY(keys_a.reserve(sizeof(uint64_t) * n2));
+  PAC_TRY(keys_b.reserve(sizeof(uint64_t) * n2));
+  hipLaunchKernelGGL(k_rmat, dim3(grid1d(ns)), dim3(256), 0, ctx->stream, ns, scale, t1, t2, t3,
+                     key, keys_a.as<uint64_t>());
+  PAC_CHECK_HIP(hipGetLastError());
+  size_t tb = 0;
+  PAC_CHECK_HIP(rocprim::radix_sort_keys(nullptr, tb, keys_a.as<uint64_t>(),
+                                         keys_b.as<uint64_t>(), n2, 0u, 2 * scale + 1,
+                                         ctx->stream));
+  PAC_TRY(tmp.reserve(tb));
+  PAC_CHECK_HIP(rocprim::radix_sort_keys(tmp.p, tb, keys_a.as<uint64_t>(),
+                                         keys_b.as<uint64_t>(), n2, 0u, 2 * scale + 1,
+                                         ctx->stream));
+  PAC_TRY(flags.reserve(sizeof(uint32_t) * n2));
+  PAC_TRY(pos.reserve(sizeof(uint32_t) * n2));
+  hipLaunchKernelGGL(k_unique_flags, dim3(grid1d(n2)), dim3(256), 0, ctx->stream,
+                     keys_b.as<const uint64_t>(), n2, sentinel, flags.as<uint32_t>());
+  PAC_CHECK_HIP(hipGetLastError());
+  size_t sb = 0;
+  PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, sb, flags.as<uint32_t>(), pos.as<uint32_t>(),
+                                        0u, (size_t)n2, rocprim::plus<uint32_t>(), ctx->stream));
+  if (sb > tmp.bytes) {
+    tmp.release();
+    PAC_TRY(tmp.reserve(sb));
+  }
+  PAC_CHECK_HIP(rocprim::exclusive_scan(tmp.p, sb, flags.as<uint32_t>(), pos.as<uint32_t>(), 0u,
+                                        (size_t)n2, rocprim::plus<uint32_t>(), ctx->stream));
+  uint32_t last[2] = {0, 0};
+  PAC_CHECK_HIP(hipMemcpyAsync(&last[0], pos.as<uint32_t>() + (n2 - 1), 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  PAC_CHECK_HIP(hipMemcpyAsync(&last[1], flags.as<uint32_t>() + (n2 - 1), 4,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  const uint64_t nnz = (uint64_t)last[0] + last[1];
+  g->V = V;
+  g->nnz = nnz;
+  PAC_TRY(g->adj.reserve(sizeof(uint32_t) * (nnz ? nnz : 1)));
+  PAC_TRY(g->off.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  // compacted keys go to keys_a (free after the sort)
+  hipLaunchKernelGGL(k_compact, dim3(grid1d(n2)), dim3(256), 0, ctx->stream,
+                     keys_b.as<const uint64_t>(), flags.as<const uint32_t>(),
+                     pos.as<const uint32_t>(), n2, (uint32_t)(V - 1), g->adj.as<uint32_t>(),
+                     keys_a.as<uint64_t>());
+  hipLaunchKernelGGL(k_offsets, dim3(grid1d((uint64_t)V + 1)), dim3(256), 0, ctx->stream,
+                     keys_a.as<const uint64_t>(), nnz, scale, V, g->off.as<uint32_t>());
+  PAC_CHECK_HIP(hipGetLastError());
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return PAC_OK;
+}
+
+}  // namespace pac

@@ -1,0 +1,198 @@
+// pac_internal.hpp -- shared declarations of the gfx950 library behind include/pac.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/pac.h"
+
+namespace pac {
+
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+#define PAC_CHECK_HIP(call)                                                       \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ::pac::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                 \
+                       hipGetErrorString(e_));                                    \
+      return e_ == hipErrorOutOfMemory ? PAC_ERR_OOM : PAC_ERR_HIP;               \
+    }                                                                             \
+  } while (0)
+
+#define PAC_TRY(expr)         \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != PAC_OK) return rc_; \
+  } while (0)
+
+// Strategy ids (the enum order of the dispatch table and of the MLP outputs).
+enum Strategy : uint8_t { kMerge = 0, kGallop = 1, kBitmap = 2 };
+// What the intersection kernels accumulate.
+enum Mode : int { kModeTC = 0, kModeLCC = 1, kModeSUP = 2 };
+
+// Owning device buffer (hipMalloc / hipFree).
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grow-only (re)allocation; contents are not preserved.
+  int reserve(size_t n) {
+    if (n <= bytes && p) return PAC_OK;
+    release();
+    if (n == 0) n = 8;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(e));
+      return PAC_ERR_OOM;
+    }
+    bytes = n;
+    return PAC_OK;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+// One graph as the persistent kernel sees it (device-side descriptor).
+struct GraphDev {
+  const uint32_t *ooff;  // V+1 oriented row offsets
+  const uint32_t *oadj;  // m oriented neighbours, rows sorted by ID
+  const uint32_t *rows;  // rows with d+>0, grouped by partition
+  unsigned long long *total;  // T accumulator
+  unsigned long long *tvert;  // per-vertex triangle counts (LCC mode)
+  uint32_t *sup;              // per-oriented-edge support (SUP mode)
+  uint32_t V;
+  uint32_t part_base;         // offset of this graph's partitions in the dispatch table
+};
+
+// A unit of the persistent work queue: rows rows[lo..hi) of graph g, partition part;
+// when hi == lo+1 only the middle vertices at positions [c_lo, c_hi) of the row.
+struct WorkItem {
+  uint32_t g, part, lo, hi, c_lo, c_hi, cost_hi, cost_lo;
+};
+
+// Tile configurations: waves per row, LDS list capacity (entries), LDS bitmap bits.
+struct TileCfg {
+  int waves, cap, nbits;
+};
+extern const TileCfg kTiles[PAC_NUM_TILES];
+
+struct Ctx;
+
+struct Graph {
+  Ctx *ctx = nullptr;
+  uint32_t V = 0;
+  uint64_t nnz = 0;  // 2m
+  uint64_t m = 0;    // oriented edges
+  uint64_t max_deg = 0, max_outdeg = 0, W = 0;
+  // symmetric CSR
+  DevBuf off, adj, deg;
+  // oriented CSR
+  DevBuf odeg, ooff, oadj;
+  // partitioning and work items
+  bool prepared = false;
+  uint32_t P = 0;
+  std::vector<uint32_t> h_pstart;      // P+1: rows of partition k are rows[pstart[k]..pstart[k+1])
+  std::vector<uint64_t> h_pedges;      // P: oriented edges per partition
+  std::vector<float> h_features;       // P x 16
+  std::vector<uint8_t> h_fixed;        // P: fixed-heuristic strategy
+  std::vector<WorkItem> h_items;
+  DevBuf rows, items;
+  DevBuf rowwork;  // V: sum over c in N+(u) of d+(c)
+  // outputs
+  DevBuf total, tvert, sup, lcc32, lcc64;
+  double prepare_ms = 0;
+};
+
+struct Ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {};
+  pac_timing last{};
+  int tile = 1;
+  DevBuf scratch;     // generic temporary storage (library-owned)
+  DevBuf scratch2;
+  DevBuf queue;       // persistent work-queue head
+  DevBuf dispatch;    // per-partition strategy table
+  DevBuf gdesc;       // GraphDev array
+  DevBuf batch_items;
+  std::vector<const void *> batch_key;  // graph set whose items are in batch_items
+  // MLP
+  DevBuf mlp_params;  // bf16 parameters on device
+  bool mlp_loaded = false;
+  std::vector<float> mlp_host;  // float copy (bf16-rounded)
+  // hipGraph cache for the batched persistent launch
+  hipGraph_t hgraph = nullptr;
+  hipGraphExec_t hexec = nullptr;
+  std::vector<const void *> hgraph_key;
+  int hgraph_mode = -1, hgraph_tile = -1;
+};
+
+// ---- prepare.hip -----------------------------------------------------------------
+int build_symmetric_from_host(Graph *g, const uint64_t *vlist, const int32_t *elist);
+int orient(Graph *g);
+int graph_validate(const Graph *g, int level);
+
+// ---- gen.hip ---------------------------------------------------------------------
+int generate_rmat(Graph *g, uint32_t scale, uint32_t ef, double a, double b, double c,
+                  uint64_t seed);
+
+// ---- isect.hip -------------------------------------------------------------------
+// Allocate / zero the outputs of `mode` for each graph.
+int alloc_outputs(Graph *const *gs, uint32_t G, int mode);
+int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode);
+// Plain path: one launch per partition with the partition's strategy.
+int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
+                 uint64_t *launches);
+// Persistent path over the concatenated work items of gs.  Without use_graph the
+// caller zeroes the outputs first; with use_graph the zeroing is part of the graph.
+int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
+                      const uint8_t *strategy_per_part_concat, bool use_graph,
+                      uint64_t *launches);
+// One partition alone with one strategy (profiling).
+int launch_partition(Ctx *ctx, Graph *g, int mode, uint32_t part, int strat);
+int compute_rowwork(Graph *g);
+int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64);
+int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *inter,
+                     uint32_t *uni, float *ratio);
+int pair_counts(Ctx *ctx, Graph *g, const int32_t *d_u, const int32_t *d_v, uint64_t n,
+                uint64_t *d_res);
+int query_matrix(Ctx *ctx, Graph *g, const int32_t *d_q, uint32_t nq, double *d_res);
+int fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t val);
+
+// ---- mlp.hip ---------------------------------------------------------------------
+int mlp_forward_dev(Ctx *ctx, const float *d_x, uint64_t n, float *d_logits);
+int mlp_loss_grad_dev(Ctx *ctx, const float *d_x, const int32_t *d_labels,
+                      const float *d_w, uint64_t n, float *d_loss, float *d_grads);
+int mlp_adam_step(Ctx *ctx, float *d_master, float *d_m, float *d_v, const float *d_g,
+                  float lr, int step);
+int mlp_store_bf16(Ctx *ctx, const float *d_master);
+
+// ---- host helpers ----------------------------------------------------------------
+int fixed_heuristic(const Graph *g, std::vector<uint8_t> &out);
+void partition_features(Graph *g, const std::vector<uint32_t> &h_odeg,
+                        const std::vector<uint32_t> &h_work, const std::vector<uint32_t> &h_deg,
+                        const std::vector<uint32_t> &rowlist);
+
+}  // namespace pac
+
+// The opaque handles of include/pac.h.
+struct pac_ctx {
+  pac::Ctx c;
+};
+struct pac_graph {
+  pac::Graph g;
+};

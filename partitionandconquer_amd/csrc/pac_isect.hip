@@ -1,0 +1,781 @@
+// pac_isect.hip -- gfx950 intersection kernels for triangle counting, LCC and
+// per-edge support.
+//
+// Counterparts of the reference's hot loops (Byy-0024/PartitionAndConquer):
+//   Graph::cnt_common_neis_merge / _less_than_merge / _galloping (Graph.cpp:186-285)
+//   Graph::solve_triangle_counting_merge (Graph.cpp:355-368)
+//   Partitioned_Graph::solve_triangle_counting_merge (Graph.cpp:1346-1362)
+//   Graph::solve_lcc_merge (Graph.cpp:828-840)
+//
+// Work decomposition.  The graph is oriented by degree: u->v iff (d(u),u) < (d(v),v),
+// so every triangle {r,c,w} is found exactly once, at the oriented edge r->c whose
+// tail r has the smallest (degree, id), as w in N+(r) & N+(c).  A workgroup owns a
+// row r: N+(r) (the list A) is staged once in LDS and every middle vertex c = A[k] is
+// given to one wave, which streams N+(c) (the list B) from HBM with coalesced loads
+// and looks each element up in A.  The three strategies differ only in that lookup:
+//   kMerge  : wave64 merge-path merge of A (LDS) and B (HBM): each lane walks an
+//             equal share of the merged sequence after a diagonal binary search.
+//   kGallop : the shorter list's elements are binary-searched in the longer one
+//             (B elements in A held in LDS, or A elements in B in HBM).
+//   kBitmap : a hashed bitmap of A in LDS filters B elements with one LDS read; the
+//             few survivors are confirmed by binary search in A (exact).
+// Counting is exact by construction in all three: a hit is only recorded for an
+// element equal to a list entry.
+//
+// Per-vertex and per-edge counts.  For a triangle (r, c=A[k], w=A[i]) found at row r:
+// t(r) += 1 (held in a register per row), t(c) and t(w) go to the LDS counter of
+// positions k and i of A; after the row, each nonzero LDS counter is added to the
+// global count of A[i] -- one global atomic per oriented edge instead of one per
+// triangle.  The support of edge r->A[i] (SUP mode, for Jaccard) is flushed the same
+// way; the third edge c->w gets one atomic per triangle at its position in B.
+#include "pac_internal.hpp"
+
+namespace pac {
+
+// waves per row, LDS list capacity (uint32 entries), LDS bitmap bits
+const TileCfg kTiles[PAC_NUM_TILES] = {
+    {2, 1024, 16384}, {4, 2048, 32768}, {4, 4096, 65536}, {8, 4096, 65536}};
+
+namespace {
+
+constexpr uint32_t kHashMul = 0x9E3779B1u;  // Fibonacci hashing multiplier
+constexpr uint32_t kAllC = 0xFFFFFFFFu;
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
+
+template <int NBITS>
+__device__ __forceinline__ uint32_t bit_of(uint32_t w) {
+  return (w * kHashMul) >> (32 - ilog2(NBITS));
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Branch-free lower_bound: first index i in [0,n) with a[i] >= x (n if none).  The
+// trip count depends only on n, so lanes searching the same list stay converged.
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t x) {
+  if (n == 0) return 0;
+  uint32_t base = 0;
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+    base = (a[base + half] < x) ? base + half : base;
+    n -= half;
+  }
+  return base + (a[base] < x ? 1u : 0u);
+}
+
+template <int CAP, int NBITS, int MODE>
+struct Smem {
+  uint32_t A[CAP];
+  uint32_t bits[NBITS / 32];
+  uint32_t cnt[MODE == kModeTC ? 1 : CAP];
+  uint32_t item;
+};
+
+// Record a confirmed triangle (r, c, w): w = A[i] = B[j].
+template <int MODE>
+__device__ __forceinline__ void on_hit(uint32_t *cnt, uint32_t i, uint32_t *sup, uint32_t bj) {
+  if (MODE != kModeTC) atomicAdd(&cnt[i], 1u);
+  if (MODE == kModeSUP) atomicAdd(&sup[bj], 1u);
+}
+
+// ---- the three lookup engines: one wave, one middle vertex c ------------------------
+// A: sorted N+(r) in LDS (nA entries); B = oadj[b0 .. b0+nb): sorted N+(c) in HBM.
+// Each returns this lane's number of confirmed common neighbours.
+
+template <int MODE, int NBITS>
+__device__ __forceinline__ uint32_t probe_bitmap(const uint32_t *__restrict__ oadj,
+                                                 const uint32_t *A, uint32_t nA,
+                                                 const uint32_t *bits, uint32_t *cnt,
+                                                 uint32_t b0, uint32_t nb, uint32_t *sup,
+                                                 uint32_t lane) {
+  uint32_t hits = 0;
+  for (uint32_t j0 = lane; j0 < nb; j0 += 256) {
+    uint32_t w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t j = j0 + 64u * u;
+      w[u] = j < nb ? oadj[b0 + j] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t j = j0 + 64u * u;
+      if (j < nb) {
+        const uint32_t h = bit_of<NBITS>(w[u]);
+        if ((bits[h >> 5] >> (h & 31u)) & 1u) {
+          const uint32_t i = lower_bound_u32(A, nA, w[u]);
+          if (i < nA && A[i] == w[u]) {
+            ++hits;
+            on_hit<MODE>(cnt, i, sup, b0 + j);
+          }
+        }
+      }
+    }
+  }
+  return hits;
+}
+
+template <int MODE>
+__device__ __forceinline__ uint32_t probe_gallop(const uint32_t *__restrict__ oadj,
+                                                 const uint32_t *A, uint32_t nA, uint32_t *cnt,
+                                                 uint32_t b0, uint32_t nb, uint32_t *sup,
+                                                 uint32_t lane) {
+  uint32_t hits = 0;
+  if (nb <= nA) {
+    // B is the shorter list: each B element is binary-searched in A (LDS).
+    for (uint32_t j0 = lane; j0 < nb; j0 += 128) {
+      uint32_t w[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const uint32_t j = j0 + 64u * u;
+        w[u] = j < nb ? oadj[b0 + j] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const uint32_t j = j0 + 64u * u;
+        if (j < nb) {
+          const uint32_t i = lower_bound_u32(A, nA, w[u]);
+          if (i < nA && A[i] == w[u]) {
+            ++hits;
+            on_hit<MODE>(cnt, i, sup, b0 + j);
+          }
+        }
+      }
+    }
+  } else {
+    // A is the shorter list: each A element is binary-searched in B (HBM), skipping
+    // A elements outside B's value range.
+    const uint32_t *B = oadj + b0;
+    const uint32_t lo = B[0], hi = B[nb - 1];
+    for (uint32_t i = lane; i < nA; i += 64) {
+      const uint32_t x = A[i];
+      if (x < lo || x > hi) continue;
+      const uint32_t j = lower_bound_u32(B, nb, x);
+      if (j < nb && B[j] == x) {
+        ++hits;
+        on_hit<MODE>(cnt, i, sup, b0 + j);
+      }
+    }
+  }
+  return hits;
+}
+
+// Merge path: the merged order takes A[i] before B[j] when A[i] <= B[j].  Lane l
+// owns merged positions [l*L, (l+1)*L).  Its start (i,j) comes from a binary search
+// on its diagonal; then it walks.  When B[j] is taken, every A element <= B[j] has
+// already been taken, so B[j] is a common element iff A[i-1] == B[j].
+template <int MODE>
+__device__ __forceinline__ uint32_t probe_merge(const uint32_t *__restrict__ oadj,
+                                                const uint32_t *A, uint32_t nA, uint32_t *cnt,
+                                                uint32_t b0, uint32_t nb, uint32_t *sup,
+                                                uint32_t lane) {
+  const uint32_t *B = oadj + b0;
+  const uint32_t total = nA + nb;
+  const uint32_t L = (total + 63u) >> 6;
+  uint32_t d0 = lane * L;
+  if (d0 >= total) return 0;
+  const uint32_t d1 = min(d0 + L, total);
+  uint32_t lo = d0 > nb ? d0 - nb : 0u, hi = min(d0, nA);
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (A[mid] <= B[d0 - mid - 1]) lo = mid + 1;
+    else hi = mid;
+  }
+  uint32_t i = lo, j = d0 - lo;
+  uint32_t ai = i < nA ? A[i] : 0xFFFFFFFFu;
+  uint32_t bj = j < nb ? B[j] : 0xFFFFFFFFu;
+  uint32_t hits = 0;
+  for (uint32_t d = d0; d < d1; ++d) {
+    if (i < nA && (j >= nb || ai <= bj)) {
+      ++i;
+      ai = i < nA ? A[i] : 0xFFFFFFFFu;
+    } else {
+      if (i > 0 && A[i - 1] == bj) {
+        ++hits;
+        on_hit<MODE>(cnt, i - 1, sup, b0 + j);
+      }
+      ++j;
+      bj = j < nb ? B[j] : 0xFFFFFFFFu;
+    }
+  }
+  return hits;
+}
+
+// ---- one row with A staged in LDS -----------------------------------------------
+template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
+__device__ __forceinline__ void row_lds(const GraphDev &G, uint32_t r, uint32_t c_lo,
+                                        uint32_t c_hi, Smem<CAP, NBITS, MODE> &s,
+                                        unsigned long long &acc) {
+  constexpr uint32_t BS = 64u * WAVES;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
+  for (uint32_t i = tid; i < nA; i += BS) {
+    const uint32_t w = G.oadj[a0 + i];
+    s.A[i] = w;
+    if (STRAT == kBitmap) {
+      const uint32_t h = bit_of<NBITS>(w);
+      atomicOr(&s.bits[h >> 5], 1u << (h & 31u));
+    }
+    if (MODE != kModeTC) s.cnt[i] = 0;
+  }
+  __syncthreads();
+  if (c_hi > nA) c_hi = nA;
+  uint32_t row_hits = 0;
+  for (uint32_t k = c_lo + wave; k < c_hi; k += WAVES) {
+    const uint32_t c = s.A[k];
+    const uint32_t b0 = G.ooff[c], nb = G.ooff[c + 1] - b0;
+    uint32_t h = 0;
+    if (nb != 0) {
+      if (STRAT == kBitmap)
+        h = probe_bitmap<MODE, NBITS>(G.oadj, s.A, nA, s.bits, s.cnt, b0, nb, G.sup, lane);
+      else if (STRAT == kGallop)
+        h = probe_gallop<MODE>(G.oadj, s.A, nA, s.cnt, b0, nb, G.sup, lane);
+      else
+        h = probe_merge<MODE>(G.oadj, s.A, nA, s.cnt, b0, nb, G.sup, lane);
+    }
+    if (MODE != kModeTC) {
+      const uint32_t ch = wave_sum(h);
+      if (lane == 0 && ch) atomicAdd(&s.cnt[k], ch);
+    }
+    row_hits += h;
+  }
+  acc += row_hits;
+  if (MODE == kModeLCC) {
+    const uint32_t wh = wave_sum(row_hits);
+    if (lane == 0 && wh) atomicAdd(&G.tvert[r], (unsigned long long)wh);
+  }
+  __syncthreads();
+  if (MODE != kModeTC) {
+    for (uint32_t i = tid; i < nA; i += BS) {
+      const uint32_t v = s.cnt[i];
+      if (v) {
+        if (MODE == kModeLCC) atomicAdd(&G.tvert[s.A[i]], (unsigned long long)v);
+        if (MODE == kModeSUP) atomicAdd(&G.sup[a0 + i], v);
+      }
+    }
+  }
+  if (STRAT == kBitmap)
+    for (uint32_t i = tid; i < nA; i += BS) s.bits[bit_of<NBITS>(s.A[i]) >> 5] = 0u;
+  __syncthreads();
+}
+
+// ---- one row whose N+(r) exceeds the LDS capacity: lookups in HBM ------------------
+template <int MODE, int WAVES>
+__device__ __forceinline__ void row_hbm(const GraphDev &G, uint32_t r, uint32_t c_lo,
+                                        uint32_t c_hi, unsigned long long &acc) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
+  const uint32_t *A = G.oadj + a0;
+  if (c_hi > nA) c_hi = nA;
+  uint32_t row_hits = 0;
+  for (uint32_t k = c_lo + wave; k < c_hi; k += WAVES) {
+    const uint32_t c = A[k];
+    const uint32_t b0 = G.ooff[c], nb = G.ooff[c + 1] - b0;
+    uint32_t h = 0;
+    for (uint32_t j = lane; j < nb; j += 64) {
+      const uint32_t w = G.oadj[b0 + j];
+      const uint32_t i = lower_bound_u32(A, nA, w);
+      if (i < nA && A[i] == w) {
+        ++h;
+        if (MODE == kModeLCC) atomicAdd(&G.tvert[w], 1ull);
+        if (MODE == kModeSUP) {
+          atomicAdd(&G.sup[a0 + i], 1u);
+          atomicAdd(&G.sup[b0 + j], 1u);
+        }
+      }
+    }
+    if (MODE != kModeTC) {
+      const uint32_t ch = wave_sum(h);
+      if (lane == 0 && ch) {
+        if (MODE == kModeLCC) atomicAdd(&G.tvert[c], (unsigned long long)ch);
+        if (MODE == kModeSUP) atomicAdd(&G.sup[a0 + k], ch);
+      }
+    }
+    row_hits += h;
+  }
+  acc += row_hits;
+  if (MODE == kModeLCC) {
+    const uint32_t wh = wave_sum(row_hits);
+    if (lane == 0 && wh) atomicAdd(&G.tvert[r], (unsigned long long)wh);
+  }
+}
+
+template <int WAVES>
+__device__ __forceinline__ void flush_total(unsigned long long *total, unsigned long long acc) {
+  const unsigned long long s = wave_sum64(acc);
+  if ((threadIdx.x & 63u) == 0 && s) atomicAdd(total, s);
+}
+
+// ---- plain kernel: one workgroup per row, one launch per partition ---------------
+template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
+__global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev G, uint32_t row_begin) {
+  __shared__ Smem<CAP, NBITS, MODE> s;
+  if (STRAT == kBitmap)
+    for (uint32_t i = threadIdx.x; i < NBITS / 32; i += 64 * WAVES) s.bits[i] = 0;
+  __syncthreads();
+  const uint32_t r = G.rows[row_begin + blockIdx.x];
+  const uint32_t nA = G.ooff[r + 1] - G.ooff[r];
+  unsigned long long acc = 0;
+  if (nA > (uint32_t)CAP)
+    row_hbm<MODE, WAVES>(G, r, 0, kAllC, acc);
+  else
+    row_lds<STRAT, MODE, WAVES, CAP, NBITS>(G, r, 0, kAllC, s, acc);
+  flush_total<WAVES>(G.total, acc);
+}
+
+// ---- persistent kernel: every workgroup pulls work items from one atomic queue --------
+template <int MODE, int WAVES, int CAP, int NBITS>
+__global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__restrict__ Gs,
+                                                           const WorkItem *__restrict__ items,
+                                                           uint32_t nitems,
+                                                           const uint8_t *__restrict__ strat,
+                                                           uint32_t *queue) {
+  __shared__ Smem<CAP, NBITS, MODE> s;
+  for (uint32_t i = threadIdx.x; i < NBITS / 32; i += 64 * WAVES) s.bits[i] = 0;
+  unsigned long long acc = 0;
+  unsigned long long *cur_total = nullptr;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) s.item = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t it = s.item;
+    if (it >= nitems) break;
+    const WorkItem w = items[it];
+    const GraphDev G = Gs[w.g];
+    if (G.total != cur_total) {
+      if (cur_total) flush_total<WAVES>(cur_total, acc);
+      acc = 0;
+      cur_total = G.total;
+    }
+    const uint8_t st = strat[G.part_base + w.part];
+    const bool single = (w.hi == w.lo + 1);
+    const uint32_t c_lo = single ? w.c_lo : 0u, c_hi = single ? w.c_hi : kAllC;
+    for (uint32_t q = w.lo; q < w.hi; ++q) {
+      const uint32_t r = G.rows[q];
+      const uint32_t nA = G.ooff[r + 1] - G.ooff[r];
+      if (nA > (uint32_t)CAP) {
+        row_hbm<MODE, WAVES>(G, r, c_lo, c_hi, acc);
+      } else if (st == kBitmap) {
+        row_lds<kBitmap, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, s, acc);
+      } else if (st == kGallop) {
+        row_lds<kGallop, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, s, acc);
+      } else {
+        row_lds<kMerge, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, s, acc);
+      }
+    }
+  }
+  if (cur_total) flush_total<WAVES>(cur_total, acc);
+}
+
+__global__ void k_reset_queue(uint32_t *q) { *q = 0; }
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+// work[u] = sum over c in N+(u) of d+(c): the B-side length of row u (wave per row).
+__global__ void k_rowwork(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
+                          uint32_t V, uint32_t *__restrict__ work) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wpb = blockDim.x >> 6;
+  for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
+       u += (uint64_t)gridDim.x * wpb) {
+    const uint32_t a0 = ooff[u], a1 = ooff[u + 1];
+    uint64_t s = 0;
+    for (uint32_t i = a0 + lane; i < a1; i += 64) {
+      const uint32_t c = oadj[i];
+      s += ooff[c + 1] - ooff[c];
+    }
+    s = wave_sum64(s);
+    if (lane == 0) work[u] = s > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)s;
+  }
+}
+
+// ---- epilogues -----------------------------------------------------------------------
+// LCC of Graph::solve_lcc_merge (Graph.cpp:828-840): numerator 2t(u) and denominator
+// d*(d-1) are exact integers converted to double, one IEEE division, then one
+// rounding to float.  d = 0 keeps 0.0, d = 1 gives 0/0 = NaN, as the reference.
+__global__ void k_lcc(const unsigned long long *__restrict__ tvert,
+                      const uint32_t *__restrict__ off, uint32_t V, float *lcc32,
+                      double *lcc64) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < V;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t d = off[u + 1] - off[u];
+    double r = 0.0;
+    if (d >= 1) r = (double)(2ull * tvert[u]) / (double)(d * (d - 1));
+    if (lcc32) lcc32[u] = (float)r;
+    if (lcc64) lcc64[u] = r;
+  }
+}
+
+// Per-edge Jaccard (pair formula of Graph.cpp:808-809 on every oriented edge).
+__global__ void k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
+                          const uint32_t *__restrict__ sup, const uint32_t *__restrict__ off,
+                          uint32_t V, int32_t *src, int32_t *dst, uint32_t *inter,
+                          uint32_t *uni, float *ratio) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wpb = blockDim.x >> 6;
+  for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
+       u += (uint64_t)gridDim.x * wpb) {
+    const uint32_t a0 = ooff[u], a1 = ooff[u + 1];
+    const uint32_t du = off[u + 1] - off[u];
+    for (uint32_t e = a0 + lane; e < a1; e += 64) {
+      const uint32_t v = oadj[e];
+      const uint32_t c = sup[e];
+      const uint32_t un = du + (off[v + 1] - off[v]) - c;
+      if (src) src[e] = (int32_t)u;
+      if (dst) dst[e] = (int32_t)v;
+      if (inter) inter[e] = c;
+      if (uni) uni[e] = un;
+      if (ratio) ratio[e] = (float)((double)c / (double)un);
+    }
+  }
+}
+
+// |N(u) & N(v)| over the full symmetric lists, one wave per pair: the shorter list's
+// elements are binary-searched in the longer one (Graph::cnt_common_neis_galloping,
+// Graph.cpp:186-198).  Optional Jaccard ratio c / ((d_u + d_v) - c) in double
+// (Graph.cpp:808-809).
+__global__ void k_pairs(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+                        const int32_t *__restrict__ pu, const int32_t *__restrict__ pv,
+                        uint64_t n, unsigned long long *res, double *jac, const int32_t *q,
+                        uint32_t nq) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wpb = blockDim.x >> 6;
+  for (uint64_t p = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); p < n;
+       p += (uint64_t)gridDim.x * wpb) {
+    uint32_t u, v;
+    if (q) {
+      u = (uint32_t)q[p / nq];
+      v = (uint32_t)q[p % nq];
+    } else {
+      u = (uint32_t)pu[p];
+      v = (uint32_t)pv[p];
+    }
+    uint32_t a0 = off[u], na = off[u + 1] - a0, b0 = off[v], nb = off[v + 1] - b0;
+    const uint32_t du = na, dv = nb;
+    if (na > nb) {
+      uint32_t t = a0; a0 = b0; b0 = t;
+      t = na; na = nb; nb = t;
+    }
+    uint32_t h = 0;
+    if (nb) {
+      const uint32_t *B = adj + b0;
+      for (uint32_t i = lane; i < na; i += 64) {
+        const uint32_t x = adj[a0 + i];
+        const uint32_t j = lower_bound_u32(B, nb, x);
+        h += (j < nb && B[j] == x) ? 1u : 0u;
+      }
+    }
+    h = wave_sum(h);
+    if (lane == 0) {
+      if (res) res[p] = h;
+      if (jac) {
+        const double c = (double)h;
+        jac[p] = c / ((double)((uint64_t)du + dv) - c);
+      }
+    }
+  }
+}
+
+template <int MODE, int WAVES, int CAP, int NBITS>
+int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint64_t *launches) {
+  int per_cu = 0;
+  PAC_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(k_persistent<MODE, WAVES, CAP, NBITS>),
+      64 * WAVES, 0));
+  if (per_cu < 1) per_cu = 1;
+  uint32_t grid = (uint32_t)(per_cu * ctx->num_cus);
+  if (grid > nitems) grid = nitems ? nitems : 1;
+  hipLaunchKernelGGL((k_reset_queue), dim3(1), dim3(1), 0, ctx->stream, ctx->queue.as<uint32_t>());
+  hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, NBITS>), dim3(grid), dim3(64 * WAVES), 0,
+                     ctx->stream, ctx->gdesc.as<const GraphDev>(), items, nitems,
+                     ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>());
+  PAC_CHECK_HIP(hipGetLastError());
+  if (launches) *launches += 2;
+  return PAC_OK;
+}
+
+template <int MODE>
+int run_persistent_mode(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint64_t *launches) {
+  switch (ctx->tile) {
+    case 0: return run_persistent_cfg<MODE, 2, 1024, 16384>(ctx, items, nitems, launches);
+    case 1: return run_persistent_cfg<MODE, 4, 2048, 32768>(ctx, items, nitems, launches);
+    case 2: return run_persistent_cfg<MODE, 4, 4096, 65536>(ctx, items, nitems, launches);
+    default: return run_persistent_cfg<MODE, 8, 4096, 65536>(ctx, items, nitems, launches);
+  }
+}
+
+int run_persistent_any(Ctx *ctx, int mode, const WorkItem *items, uint32_t nitems,
+                       uint64_t *launches) {
+  if (mode == kModeLCC) return run_persistent_mode<kModeLCC>(ctx, items, nitems, launches);
+  if (mode == kModeSUP) return run_persistent_mode<kModeSUP>(ctx, items, nitems, launches);
+  return run_persistent_mode<kModeTC>(ctx, items, nitems, launches);
+}
+
+template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
+int run_plain_cfg(Ctx *ctx, const GraphDev &G, uint32_t row_begin, uint32_t nrows) {
+  hipLaunchKernelGGL((k_plain<STRAT, MODE, WAVES, CAP, NBITS>), dim3(nrows), dim3(64 * WAVES), 0,
+                     ctx->stream, G, row_begin);
+  PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
+}
+
+template <int STRAT, int MODE>
+int run_plain_tile(Ctx *ctx, const GraphDev &G, uint32_t row_begin, uint32_t nrows) {
+  switch (ctx->tile) {
+    case 0: return run_plain_cfg<STRAT, MODE, 2, 1024, 16384>(ctx, G, row_begin, nrows);
+    case 1: return run_plain_cfg<STRAT, MODE, 4, 2048, 32768>(ctx, G, row_begin, nrows);
+    case 2: return run_plain_cfg<STRAT, MODE, 4, 4096, 65536>(ctx, G, row_begin, nrows);
+    default: return run_plain_cfg<STRAT, MODE, 8, 4096, 65536>(ctx, G, row_begin, nrows);
+  }
+}
+
+template <int MODE>
+int run_plain(Ctx *ctx, const GraphDev &G, int strat, uint32_t row_begin, uint32_t nrows) {
+  if (strat == kBitmap) return run_plain_tile<kBitmap, MODE>(ctx, G, row_begin, nrows);
+  if (strat == kGallop) return run_plain_tile<kGallop, MODE>(ctx, G, row_begin, nrows);
+  return run_plain_tile<kMerge, MODE>(ctx, G, row_begin, nrows);
+}
+
+int run_plain_any(Ctx *ctx, const GraphDev &G, int mode, int strat, uint32_t row_begin,
+                  uint32_t nrows) {
+  if (nrows == 0) return PAC_OK;
+  if (mode == kModeLCC) return run_plain<kModeLCC>(ctx, G, strat, row_begin, nrows);
+  if (mode == kModeSUP) return run_plain<kModeSUP>(ctx, G, strat, row_begin, nrows);
+  return run_plain<kModeTC>(ctx, G, strat, row_begin, nrows);
+}
+
+GraphDev make_desc(Graph *g, uint32_t part_base) {
+  GraphDev d;
+  d.ooff = g->ooff.as<const uint32_t>();
+  d.oadj = g->oadj.as<const uint32_t>();
+  d.rows = g->rows.as<const uint32_t>();
+  d.total = g->total.as<unsigned long long>();
+  d.tvert = g->tvert.as<unsigned long long>();
+  d.sup = g->sup.as<uint32_t>();
+  d.V = g->V;
+  d.part_base = part_base;
+  return d;
+}
+
+uint32_t grid_for(uint64_t n, uint32_t per_block) {
+  uint64_t b = (n + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > (1u << 20)) b = 1u << 20;
+  return (uint32_t)b;
+}
+
+}  // namespace
+
+int fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t val) {
+  if (n == 0) return PAC_OK;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256 * 8)), dim3(256), 0, s, p, n, val);
+  PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
+}
+
+int alloc_outputs(Graph *const *gs, uint32_t G, int mode) {
+  for (uint32_t i = 0; i < G; ++i) {
+    Graph *g = gs[i];
+    PAC_TRY(g->total.reserve(sizeof(unsigned long long)));
+    if (mode == kModeLCC) PAC_TRY(g->tvert.reserve(sizeof(unsigned long long) * (size_t)(g->V ? g->V : 1)));
+    if (mode == kModeSUP) PAC_TRY(g->sup.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
+  }
+  return PAC_OK;
+}
+
+// Zero-fill kernels (not memset nodes, so the same sequence can be graph-captured).
+int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
+  PAC_TRY(alloc_outputs(gs, G, mode));
+  for (uint32_t i = 0; i < G; ++i) {
+    Graph *g = gs[i];
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream,
+                       g->total.as<unsigned long long>(), (uint64_t)1);
+    if (mode == kModeLCC && g->V)
+      hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
+                         g->tvert.as<unsigned long long>(), (uint64_t)g->V);
+    if (mode == kModeSUP) PAC_TRY(fill_u32(ctx->stream, g->sup.as<uint32_t>(), g->m, 0u));
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  return PAC_OK;
+}
+
+int compute_rowwork(Graph *g) {
+  PAC_TRY(g->rowwork.reserve(sizeof(uint32_t) * (size_t)(g->V ? g->V : 1)));
+  if (g->V == 0) return PAC_OK;
+  hipLaunchKernelGGL(k_rowwork, dim3(grid_for(g->V, 4)), dim3(256), 0, g->ctx->stream,
+                     g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(), g->V,
+                     g->rowwork.as<uint32_t>());
+  PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
+}
+
+int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
+                 uint64_t *launches) {
+  const GraphDev d = make_desc(g, 0);
+  for (uint32_t k = 0; k < g->P; ++k) {
+    const uint32_t b = g->h_pstart[k], e = g->h_pstart[k + 1];
+    if (e == b) continue;
+    PAC_TRY(run_plain_any(ctx, d, mode, strategy_per_part[k], b, e - b));
+    if (launches) ++*launches;
+  }
+  return PAC_OK;
+}
+
+int launch_partition(Ctx *ctx, Graph *g, int mode, uint32_t part, int strat) {
+  const GraphDev d = make_desc(g, 0);
+  const uint32_t b = g->h_pstart[part], e = g->h_pstart[part + 1];
+  return run_plain_any(ctx, d, mode, strat, b, e - b);
+}
+
+int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
+                      const uint8_t *strategy_per_part_concat, bool use_graph,
+                      uint64_t *launches) {
+  std::vector<GraphDev> descs(G);
+  uint64_t nparts = 0, nitems = 0;
+  for (uint32_t i = 0; i < G; ++i) {
+    descs[i] = make_desc(gs[i], (uint32_t)nparts);
+    nparts += gs[i]->P;
+    nitems += gs[i]->h_items.size();
+  }
+  if (nitems > 0xFFFFFFF0ull) {
+    set_error("too many work items (%llu)", (unsigned long long)nitems);
+    return PAC_ERR_UNSUPPORTED;
+  }
+  PAC_TRY(ctx->gdesc.reserve(sizeof(GraphDev) * (G ? G : 1)));
+  PAC_TRY(ctx->dispatch.reserve(nparts ? nparts : 1));
+  PAC_TRY(ctx->queue.reserve(64));
+  // Descriptors and the strategy table are plain uploads before the launch (and before
+  // any graph replay), so a captured graph always reads the current table.
+  PAC_CHECK_HIP(hipMemcpyAsync(ctx->gdesc.p, descs.data(), sizeof(GraphDev) * G,
+                               hipMemcpyHostToDevice, ctx->stream));
+  PAC_CHECK_HIP(hipMemcpyAsync(ctx->dispatch.p, strategy_per_part_concat, nparts,
+                               hipMemcpyHostToDevice, ctx->stream));
+  const WorkItem *items = nullptr;
+  std::vector<const void *> key;
+  for (uint32_t i = 0; i < G; ++i) key.push_back(gs[i]);
+  if (G == 1) {
+    items = gs[0]->items.as<const WorkItem>();
+  } else {
+    if (ctx->batch_key != key) {
+      PAC_TRY(ctx->batch_items.reserve(sizeof(WorkItem) * (nitems ? nitems : 1)));
+      std::vector<WorkItem> all;
+      all.reserve(nitems);
+      for (uint32_t i = 0; i < G; ++i)
+        for (WorkItem w : gs[i]->h_items) {
+          w.g = i;
+          all.push_back(w);
+        }
+      PAC_CHECK_HIP(hipMemcpyAsync(ctx->batch_items.p, all.data(), sizeof(WorkItem) * nitems,
+                                   hipMemcpyHostToDevice, ctx->stream));
+      PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->batch_key = key;
+    }
+    items = ctx->batch_items.as<const WorkItem>();
+  }
+  if (!use_graph) {
+    // the caller has zeroed the outputs
+    if (nitems == 0) return PAC_OK;
+    return run_persistent_any(ctx, mode, items, (uint32_t)nitems, launches);
+  }
+  // hipGraph: zero-fill kernels + queue reset + persistent kernel, captured once per
+  // (graph set, mode, tile, buffer addresses) and replayed.
+  PAC_TRY(alloc_outputs(gs, G, mode));  // allocate outside the capture
+  key.push_back(items);
+  key.push_back(ctx->gdesc.p);
+  key.push_back(ctx->dispatch.p);
+  key.push_back(ctx->queue.p);
+  for (uint32_t i = 0; i < G; ++i) {
+    key.push_back(gs[i]->tvert.p);
+    key.push_back(gs[i]->sup.p);
+    key.push_back(gs[i]->total.p);
+  }
+  if (!(ctx->hexec && ctx->hgraph_key == key && ctx->hgraph_mode == mode &&
+        ctx->hgraph_tile == ctx->tile)) {
+    if (ctx->hexec) (void)hipGraphExecDestroy(ctx->hexec);
+    if (ctx->hgraph) (void)hipGraphDestroy(ctx->hgraph);
+    ctx->hexec = nullptr;
+    ctx->hgraph = nullptr;
+    PAC_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = zero_outputs(ctx, gs, G, mode);
+    if (rc == PAC_OK && nitems) rc = run_persistent_any(ctx, mode, items, (uint32_t)nitems, nullptr);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != PAC_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    PAC_CHECK_HIP(e);
+    ctx->hgraph = graph;
+    PAC_CHECK_HIP(hipGraphInstantiate(&ctx->hexec, graph, nullptr, nullptr, 0));
+    ctx->hgraph_key = key;
+    ctx->hgraph_mode = mode;
+    ctx->hgraph_tile = ctx->tile;
+  }
+  PAC_CHECK_HIP(hipGraphLaunch(ctx->hexec, ctx->stream));
+  if (launches) *launches += 1;
+  return PAC_OK;
+}
+
+int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64) {
+  if (want32) PAC_TRY(g->lcc32.reserve(sizeof(float) * (size_t)(g->V ? g->V : 1)));
+  if (want64) PAC_TRY(g->lcc64.reserve(sizeof(double) * (size_t)(g->V ? g->V : 1)));
+  if (g->V == 0) return PAC_OK;
+  hipLaunchKernelGGL(k_lcc, dim3(grid_for(g->V, 256 * 4)), dim3(256), 0, ctx->stream,
+                     g->tvert.as<const unsigned long long>(), g->off.as<const uint32_t>(), g->V,
+                     want32 ? g->lcc32.as<float>() : nullptr,
+                     want64 ? g->lcc64.as<double>() : nullptr);
+  PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
+}
+
+int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *inter,
+                     uint32_t *uni, float *ratio) {
+  if (g->V == 0) return PAC_OK;
+  hipLaunchKernelGGL(k_jaccard, dim3(grid_for(g->V, 4)), dim3(256), 0, ctx->stream,
+                     g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
+                     g->sup.as<const uint32_t>(), g->off.as<const uint32_t>(), g->V, src, dst,
+                     inter, uni, ratio);
+  PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
+}
+
+int pair_counts(Ctx *ctx, Graph *g, const int32_t *d_u, const int32_t *d_v, uint64_t n,
+                uint64_t *d_res) {
+  if (n == 0) return PAC_OK;
+  hipLaunchKernelGGL(k_pairs, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream,
+                     g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(), d_u, d_v, n,
+                     reinterpret_cast<unsigned long long *>(d_res), nullptr, nullptr, 1u);
+  PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
+}
+
+int query_matrix(Ctx *ctx, Graph *g, const int32_t *d_q, uint32_t nq, double *d_res) {
+  const uint64_t n = (uint64_t)nq * nq;
+  if (n == 0) return PAC_OK;
+  hipLaunchKernelGGL(k_pairs, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream,
+                     g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(), nullptr, nullptr,
+                     n, nullptr, d_res, d_q, nq);
+  PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
+}
+
+}  // namespace pac

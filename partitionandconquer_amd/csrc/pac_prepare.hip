@@ -1,0 +1,228 @@
+// pac_prepare.hip -- symmetric CSR upload/validation and the degree-ordered orientation.
+//
+// Reference counterparts (Byy-0024/PartitionAndConquer):
+//   Graph::Graph(csrdir, bool) / Graph(adj_list)  Graph.cpp:92-149 (CSR members)
+//   orientation: the reference orients implicitly by vertex ID (the `u <= v` skip of
+//   Graph.cpp:361 and the threshold merge of Graph.cpp:272-285).  Here the orientation
+//   is by (degree, id), which bounds every out-list by sqrt(2m); triangle counts do
+//   not depend on the orientation.
+//
+// Orientation = histogram / exclusive scan / scatter into a uint32 CSR:
+//   1. d+(u) = |{v in N(u) : (d(u),u) < (d(v),v)}|    (one wave per row, ballot+popcount)
+//   2. ooff = exclusive_scan(d+)                        (rocPRIM device scan)
+//   3. each row's kept neighbours are written in input order (ballot prefix), so
+//      rows stay sorted by ID and no re-sort is needed.
+#include <rocprim/device/device_scan.hpp>
+
+#include "pac_internal.hpp"
+
+namespace pac {
+namespace {
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ bool before(uint32_t du, uint32_t u, uint32_t dv, uint32_t v) {
+  return du < dv || (du == dv && u < v);
+}
+
+uint32_t grid_waves(uint64_t n) {
+  uint64_t b = (n + 3) / 4;
+  if (b < 1) b = 1;
+  if (b > (1u << 20)) b = 1u << 20;
+  return (uint32_t)b;
+}
+
+__global__ void k_deg(const uint32_t *__restrict__ off, uint32_t V, uint32_t *__restrict__ deg,
+                      uint32_t *maxdeg) {
+  uint32_t mx = 0;
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < V;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = off[u + 1] - off[u];
+    deg[u] = d;
+    mx = max(mx, d);
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63u) == 0) atomicMax(maxdeg, mx);
+}
+
+// Error bits: 1 ID >= V, 2 row not strictly increasing, 4 self-loop, 8 asymmetric.
+__global__ void k_check_rows(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+                             uint32_t V, int check_sym, uint32_t *err) {
+  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
+  uint32_t bits = 0;
+  for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
+       u += (uint64_t)gridDim.x * wpb) {
+    const uint32_t a0 = off[u], a1 = off[u + 1];
+    for (uint32_t i = a0 + lane; i < a1; i += 64) {
+      const uint32_t v = adj[i];
+      if (v >= V) {
+        bits |= 1u;
+        continue;
+      }
+      if (i + 1 < a1 && adj[i + 1] <= v) bits |= 2u;
+      if (v == u) bits |= 4u;
+      if (check_sym) {
+        // binary search u in N(v)
+        uint32_t lo = off[v], n = off[v + 1] - lo;
+        bool found = false;
+        while (n > 0) {
+          const uint32_t half = n >> 1;
+          const uint32_t x = adj[lo + half];
+          if (x == u) {
+            found = true;
+            break;
+          }
+          if (x < u) {
+            lo += half + 1;
+            n -= half + 1;
+          } else {
+            n = half;
+          }
+        }
+        if (!found) bits |= 8u;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) bits |= (uint32_t)__shfl_xor(bits, o, 64);
+  if (lane == 0 && bits) atomicOr(err, bits);
+}
+
+// 1. out-degree histogram (one wave per row)
+__global__ void k_outdeg(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+                         const uint32_t *__restrict__ deg, uint32_t V,
+                         uint32_t *__restrict__ odeg) {
+  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
+  for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
+       u += (uint64_t)gridDim.x * wpb) {
+    const uint32_t a0 = off[u], a1 = off[u + 1], du = a1 - a0;
+    uint32_t cnt = 0;
+    for (uint32_t i0 = a0; i0 < a1; i0 += 64) {
+      const uint32_t i = i0 + lane;
+      bool keep = false;
+      if (i < a1) {
+        const uint32_t v = adj[i];
+        keep = before(du, (uint32_t)u, deg[v], v);
+      }
+      cnt += (uint32_t)__popcll(__ballot(keep));
+    }
+    if (lane == 0) odeg[u] = cnt;
+  }
+}
+
+// 3. order-preserving scatter of the kept neighbours
+__global__ void k_scatter(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+                          const uint32_t *__restrict__ deg, const uint32_t *__restrict__ ooff,
+                          uint32_t V, uint32_t *__restrict__ oadj) {
+  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
+  for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
+       u += (uint64_t)gridDim.x * wpb) {
+    const uint32_t a0 = off[u], a1 = off[u + 1], du = a1 - a0;
+    uint32_t out = ooff[u];
+    for (uint32_t i0 = a0; i0 < a1; i0 += 64) {
+      const uint32_t i = i0 + lane;
+      bool keep = false;
+      uint32_t v = 0;
+      if (i < a1) {
+        v = adj[i];
+        keep = before(du, (uint32_t)u, deg[v], v);
+      }
+      const uint64_t mask = __ballot(keep);
+      if (keep) oadj[out + lanes_below(mask)] = v;
+      out += (uint32_t)__popcll(mask);
+    }
+  }
+}
+
+}  // namespace
+
+int build_symmetric_from_host(Graph *g, const uint64_t *vlist, const int32_t *elist) {
+  Ctx *ctx = g->ctx;
+  const uint64_t V = g->V, nnz = g->nnz;
+  std::vector<uint32_t> off32(V + 1);
+  for (uint64_t u = 0; u <= V; ++u) off32[u] = (uint32_t)vlist[u];
+  PAC_TRY(g->off.reserve(sizeof(uint32_t) * (V + 1)));
+  PAC_TRY(g->adj.reserve(sizeof(uint32_t) * (nnz ? nnz : 1)));
+  PAC_CHECK_HIP(hipMemcpyAsync(g->off.p, off32.data(), sizeof(uint32_t) * (V + 1),
+                               hipMemcpyHostToDevice, ctx->stream));
+  if (nnz)
+    PAC_CHECK_HIP(hipMemcpyAsync(g->adj.p, elist, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice,
+                                 ctx->stream));
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return PAC_OK;
+}
+
+int graph_validate(const Graph *g, int level) {
+  Ctx *ctx = g->ctx;
+  if (g->V == 0) return PAC_OK;
+  PAC_TRY(ctx->scratch2.reserve(64));
+  uint32_t *err = ctx->scratch2.as<uint32_t>();
+  PAC_TRY(fill_u32(ctx->stream, err, 1, 0u));
+  hipLaunchKernelGGL(k_check_rows, dim3(grid_waves(g->V)), dim3(256), 0, ctx->stream,
+                     g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(), g->V,
+                     level >= 1 ? 1 : 0, err);
+  PAC_CHECK_HIP(hipGetLastError());
+  uint32_t h = 0;
+  PAC_CHECK_HIP(hipMemcpyAsync(&h, err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (h) {
+    set_error("malformed CSR:%s%s%s%s", (h & 1) ? " neighbour ID out of range" : "",
+              (h & 2) ? " row not strictly increasing (unsorted or duplicate)" : "",
+              (h & 4) ? " self-loop" : "", (h & 8) ? " asymmetric (u,v) without (v,u)" : "");
+    return PAC_ERR_FORMAT;
+  }
+  return PAC_OK;
+}
+
+int orient(Graph *g) {
+  Ctx *ctx = g->ctx;
+  const uint32_t V = g->V;
+  PAC_TRY(g->deg.reserve(sizeof(uint32_t) * (V ? V : 1)));
+  PAC_TRY(g->odeg.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  PAC_TRY(g->ooff.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  PAC_TRY(ctx->scratch2.reserve(64));
+  uint32_t *maxdeg = ctx->scratch2.as<uint32_t>();
+  PAC_TRY(fill_u32(ctx->stream, maxdeg, 1, 0u));
+  PAC_TRY(fill_u32(ctx->stream, g->odeg.as<uint32_t>(), (uint64_t)V + 1, 0u));
+  if (V) {
+    hipLaunchKernelGGL(k_deg, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
+                       g->off.as<const uint32_t>(), V, g->deg.as<uint32_t>(), maxdeg);
+    hipLaunchKernelGGL(k_outdeg, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
+                       g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
+                       g->deg.as<const uint32_t>(), V, g->odeg.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  size_t tmp_bytes = 0;
+  PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, g->odeg.as<uint32_t>(),
+                                        g->ooff.as<uint32_t>(), 0u, (size_t)V + 1,
+                                        rocprim::plus<uint32_t>(), ctx->stream));
+  PAC_TRY(ctx->scratch.reserve(tmp_bytes));
+  PAC_CHECK_HIP(rocprim::exclusive_scan(ctx->scratch.p, tmp_bytes, g->odeg.as<uint32_t>(),
+                                        g->ooff.as<uint32_t>(), 0u, (size_t)V + 1,
+                                        rocprim::plus<uint32_t>(), ctx->stream));
+  uint32_t hm[2] = {0, 0};
+  PAC_CHECK_HIP(hipMemcpyAsync(&hm[0], g->ooff.as<uint32_t>() + V, 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  PAC_CHECK_HIP(hipMemcpyAsync(&hm[1], maxdeg, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  g->m = hm[0];
+  g->max_deg = hm[1];
+  if (2 * g->m != g->nnz) {
+    set_error("orientation kept %llu of %llu entries: the CSR is not symmetric",
+              (unsigned long long)g->m, (unsigned long long)g->nnz);
+    return PAC_ERR_FORMAT;
+  }
+  PAC_TRY(g->oadj.reserve(sizeof(uint32_t) * (g->m ? g->m : 1)));
+  if (V) {
+    hipLaunchKernelGGL(k_scatter, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
+                       g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
+                       g->deg.as<const uint32_t>(), g->ooff.as<const uint32_t>(), V,
+                       g->oadj.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  return PAC_OK;
+}
+
+}  // namespace pac

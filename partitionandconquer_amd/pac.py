@@ -1,0 +1,475 @@
+"""Python binding of the C ABI in include/pac.h (ctypes over libpac.so).
+
+Mirrors the reference's C++ interface of Byy-0024/PartitionAndConquer for the
+intersection path: ``Graph`` (src/Graph.hpp:38-103) with
+``solve_triangle_counting_{merge,galloping}``, ``solve_lcc_{merge,galloping}``,
+``solve_jsim_{merge,galloping}``, and ``PartitionedGraph`` (Partitioned_Graph,
+src/Graph.hpp:105-155).  The GPU-native calls (``tc``, ``lcc``, ``jaccard_edges``,
+``tc_batch``) expose the dispatch and launch options.
+
+The library is required: importing works without it, but every call raises
+``PacError`` when libpac.so is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpac.so")
+
+PAC_OK = 0
+ERRORS = {-1: "ARG", -2: "HIP", -3: "OOM", -4: "FORMAT", -5: "IO", -6: "STATE",
+          -7: "UNSUPPORTED", -8: "TIMEOUT"}
+
+DISPATCH_FIXED, DISPATCH_LEARNED, FORCE_MERGE, FORCE_GALLOP, FORCE_BITMAP = range(5)
+LAUNCH_PLAIN, LAUNCH_PERSISTENT, LAUNCH_GRAPH = range(3)
+STRATEGIES = ("merge", "gallop", "bitmap")
+NUM_TILES = 4
+MLP_IN, MLP_HIDDEN, MLP_OUT = 16, 64, 3
+MLP_NPARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + \
+    MLP_OUT * MLP_HIDDEN + MLP_OUT
+
+
+class PacError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"PAC_ERR_{ERRORS.get(code, code)} ({code}): {msg}")
+        self.code = code
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint64), ("num_entries", C.c_uint64),
+                ("num_edges", C.c_uint64), ("max_degree", C.c_uint64),
+                ("max_out_degree", C.c_uint64), ("work_W", C.c_uint64),
+                ("num_partitions", C.c_uint64), ("num_items", C.c_uint64),
+                ("prepare_ms", C.c_double)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("isect_ms", C.c_double),
+                ("epilogue_ms", C.c_double), ("launches", C.c_uint64),
+                ("strategy_partitions", C.c_uint64 * 3)]
+
+
+_lib = None
+vp = C.c_void_p
+
+
+def lib():
+    """Load libpac.so (raises PacError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PacError(-7, f"{LIB_PATH} not built: run python -m partitionandconquer_amd.build")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "pac_last_error": ([], C.c_char_p),
+        "pac_ctx_create": ([C.c_int, C.POINTER(vp)], C.c_int),
+        "pac_ctx_destroy": ([vp], None),
+        "pac_ctx_last_timing": ([vp, C.POINTER(Timing)], C.c_int),
+        "pac_ctx_num_cus": ([vp], C.c_int),
+        "pac_graph_upload": ([vp, vp, C.c_uint64, vp, C.c_uint64, C.POINTER(vp)], C.c_int),
+        "pac_graph_load_csr_dir": ([vp, C.c_char_p, C.POINTER(vp)], C.c_int),
+        "pac_graph_load_text": ([vp, C.c_char_p, C.POINTER(vp)], C.c_int),
+        "pac_graph_save_csr_dir": ([vp, C.c_char_p], C.c_int),
+        "pac_graph_generate_rmat": ([vp, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                     C.c_double, C.c_uint64, C.POINTER(vp)], C.c_int),
+        "pac_graph_generate_er": ([vp, C.c_uint64, C.c_double, C.c_uint64, C.POINTER(vp)], C.c_int),
+        "pac_graph_download_csr": ([vp, vp, vp], C.c_int),
+        "pac_graph_get_info": ([vp, C.POINTER(GraphInfo)], C.c_int),
+        "pac_graph_validate": ([vp, C.c_int], C.c_int),
+        "pac_graph_free": ([vp], None),
+        "pac_graph_prepare": ([vp, C.c_uint32, vp], C.c_int),
+        "pac_graph_partition_sizes": ([vp, vp], C.c_int),
+        "pac_random_partition": ([C.c_uint64, C.c_uint32, C.c_uint64, vp], C.c_int),
+        "pac_greedy_partition": ([vp, C.c_uint32, C.c_uint64, vp], C.c_int),
+        "pac_evaluate_partition": ([vp, vp, C.c_uint32, C.POINTER(C.c_double)], C.c_int),
+        "pac_tc": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
+        "pac_lcc": ([vp, C.c_int, C.c_int, vp, vp, vp], C.c_int),
+        "pac_jaccard_edges": ([vp, C.c_int, C.c_int, vp, vp, vp, vp, vp], C.c_int),
+        "pac_jaccard_queries": ([vp, vp, C.c_uint32, vp], C.c_int),
+        "pac_cnt_common_pairs": ([vp, vp, vp, C.c_uint64, vp], C.c_int),
+        "pac_tc_batch": ([vp, vp, C.c_uint32, C.c_int, C.c_int, vp], C.c_int),
+        "pac_autotune": ([vp, vp, C.c_uint32, C.c_char_p, C.POINTER(C.c_int), vp], C.c_int),
+        "pac_set_tile": ([vp, C.c_int], C.c_int),
+        "pac_get_tile": ([vp, C.POINTER(C.c_int)], C.c_int),
+        "pac_partition_features": ([vp, vp], C.c_int),
+        "pac_profile_partitions": ([vp, C.c_int, vp], C.c_int),
+        "pac_fixed_choice": ([vp, vp], C.c_int),
+        "pac_mlp_choice": ([vp, vp], C.c_int),
+        "pac_mlp_set_params": ([vp, vp], C.c_int),
+        "pac_mlp_get_params": ([vp, vp], C.c_int),
+        "pac_mlp_init": ([vp, C.c_uint64], C.c_int),
+        "pac_mlp_forward": ([vp, vp, C.c_uint64, vp], C.c_int),
+        "pac_mlp_loss_grad": ([vp, vp, vp, vp, C.c_uint64, C.POINTER(C.c_float), vp], C.c_int),
+        "pac_mlp_train": ([vp, vp, vp, vp, C.c_uint64, C.c_uint32, C.c_float,
+                           C.POINTER(C.c_float)], C.c_int),
+        "pac_mlp_save": ([vp, C.c_char_p], C.c_int),
+        "pac_mlp_load": ([vp, C.c_char_p], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != PAC_OK:
+        raise PacError(rc, lib().pac_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+@dataclass
+class LastTiming:
+    total_ms: float
+    isect_ms: float
+    epilogue_ms: float
+    launches: int
+    strategy_partitions: tuple
+
+
+class Context:
+    """One HIP device + stream (pac_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = vp()
+        _check(lib().pac_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pac_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def num_cus(self) -> int:
+        return lib().pac_ctx_num_cus(self._h)
+
+    def last_timing(self) -> LastTiming:
+        t = Timing()
+        _check(lib().pac_ctx_last_timing(self._h, C.byref(t)))
+        return LastTiming(t.total_ms, t.isect_ms, t.epilogue_ms, t.launches,
+                          tuple(t.strategy_partitions))
+
+    # ---- tiles / autotuning ----
+    def set_tile(self, tile: int):
+        _check(lib().pac_set_tile(self._h, tile))
+
+    def get_tile(self) -> int:
+        t = C.c_int()
+        _check(lib().pac_get_tile(self._h, C.byref(t)))
+        return t.value
+
+    def autotune(self, graphs, cache_path=None, report=True):
+        arr = (vp * len(graphs))(*[g._h for g in graphs])
+        t = C.c_int()
+        ms = np.zeros(NUM_TILES, np.float64)
+        _check(lib().pac_autotune(self._h, arr, len(graphs),
+                                  cache_path.encode() if cache_path else None, C.byref(t),
+                                  _ptr(ms) if report else None))
+        return t.value, ms
+
+    def tc_batch(self, graphs, dispatch=DISPATCH_FIXED, launch=LAUNCH_PERSISTENT) -> np.ndarray:
+        arr = (vp * len(graphs))(*[g._h for g in graphs])
+        out = np.zeros(len(graphs), np.uint64)
+        _check(lib().pac_tc_batch(self._h, arr, len(graphs), dispatch, launch, _ptr(out)))
+        return out
+
+    # ---- MLP ----
+    def mlp_init(self, seed=1):
+        _check(lib().pac_mlp_init(self._h, seed))
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == MLP_NPARAMS
+        _check(lib().pac_mlp_set_params(self._h, _ptr(p)))
+
+    def mlp_get_params(self) -> np.ndarray:
+        p = np.zeros(MLP_NPARAMS, np.float32)
+        _check(lib().pac_mlp_get_params(self._h, _ptr(p)))
+        return p
+
+    def mlp_forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((x.shape[0], MLP_OUT), np.float32)
+        _check(lib().pac_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(out)))
+        return out
+
+    def mlp_loss_grad(self, x, labels, weights=None):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(labels, np.int32)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        g = np.zeros(MLP_NPARAMS, np.float32)
+        loss = C.c_float()
+        _check(lib().pac_mlp_loss_grad(self._h, _ptr(x), _ptr(y), _ptr(w), x.shape[0],
+                                       C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def mlp_train(self, x, labels, weights=None, epochs=500, lr=3e-3) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(labels, np.int32)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        loss = C.c_float()
+        _check(lib().pac_mlp_train(self._h, _ptr(x), _ptr(y), _ptr(w), x.shape[0], epochs, lr,
+                                   C.byref(loss)))
+        return loss.value
+
+    def mlp_save(self, path):
+        _check(lib().pac_mlp_save(self._h, path.encode()))
+
+    def mlp_load(self, path):
+        _check(lib().pac_mlp_load(self._h, path.encode()))
+
+
+class Graph:
+    """A symmetric graph on the device (pac_graph); reference Graph, Graph.hpp:38-103."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self._h = handle
+        self._prepared_P = 0
+
+    # ---- constructors ----
+    @classmethod
+    def from_csr(cls, ctx: Context, vlist, elist) -> "Graph":
+        vl = np.ascontiguousarray(vlist, np.uint64)
+        el = np.ascontiguousarray(elist, np.int32)
+        h = vp()
+        _check(lib().pac_graph_upload(ctx._h, _ptr(vl), len(vl) - 1, _ptr(el), len(el), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def load_csr_dir(cls, ctx: Context, path: str) -> "Graph":
+        h = vp()
+        _check(lib().pac_graph_load_csr_dir(ctx._h, path.encode(), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def load_text(cls, ctx: Context, path: str) -> "Graph":
+        h = vp()
+        _check(lib().pac_graph_load_text(ctx._h, path.encode(), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def rmat(cls, ctx: Context, scale: int, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):
+        h = vp()
+        _check(lib().pac_graph_generate_rmat(ctx._h, scale, edge_factor, a, b, c, seed,
+                                             C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def er(cls, ctx: Context, n: int, avg_degree: float, seed=1):
+        h = vp()
+        _check(lib().pac_graph_generate_er(ctx._h, n, avg_degree, seed, C.byref(h)))
+        return cls(ctx, h)
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().pac_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # ---- reference accessors (Graph.cpp:151-157, 288-298) ----
+    def info(self) -> GraphInfo:
+        i = GraphInfo()
+        _check(lib().pac_graph_get_info(self._h, C.byref(i)))
+        return i
+
+    def get_vertex_num(self) -> int:
+        return self.info().num_vertices
+
+    def get_edge_num(self) -> int:
+        """Directed entries 2m, as Graph::get_edge_num (Graph.cpp:288-290)."""
+        return self.info().num_entries
+
+    def csr(self):
+        i = self.info()
+        vl = np.zeros(i.num_vertices + 1, np.uint64)
+        el = np.zeros(i.num_entries, np.int32)
+        _check(lib().pac_graph_download_csr(self._h, _ptr(vl), _ptr(el)))
+        return vl, el
+
+    def save_csr_dir(self, path: str):
+        _check(lib().pac_graph_save_csr_dir(self._h, path.encode()))
+
+    def validate(self, level=1):
+        _check(lib().pac_graph_validate(self._h, level))
+
+    # ---- partitioning ----
+    def prepare(self, num_partitions=1, v2p=None):
+        a = None if v2p is None else np.ascontiguousarray(v2p, np.int32)
+        _check(lib().pac_graph_prepare(self._h, num_partitions, _ptr(a)))
+        self._prepared_P = self.info().num_partitions
+        return self
+
+    def _ensure_prepared(self):
+        if not self._prepared_P:
+            self.prepare(1)
+
+    def partition_sizes(self) -> np.ndarray:
+        out = np.zeros(self.info().num_partitions, np.uint64)
+        _check(lib().pac_graph_partition_sizes(self._h, _ptr(out)))
+        return out
+
+    def greedy_partition(self, P: int, seed=1) -> np.ndarray:
+        out = np.zeros(self.get_vertex_num(), np.int32)
+        _check(lib().pac_greedy_partition(self._h, P, seed, _ptr(out)))
+        return out
+
+    def evaluate_partition(self, v2p, P: int) -> float:
+        a = np.ascontiguousarray(v2p, np.int32)
+        r = C.c_double()
+        _check(lib().pac_evaluate_partition(self._h, _ptr(a), P, C.byref(r)))
+        return r.value
+
+    # ---- GPU-native solvers ----
+    def tc(self, dispatch=DISPATCH_FIXED, launch=LAUNCH_PERSISTENT, per_vertex=False):
+        self._ensure_prepared()
+        tot = np.zeros(1, np.uint64)
+        pv = np.zeros(self.get_vertex_num(), np.uint64) if per_vertex else None
+        _check(lib().pac_tc(self._h, dispatch, launch, _ptr(tot), _ptr(pv)))
+        return (int(tot[0]), pv) if per_vertex else int(tot[0])
+
+    def lcc(self, dispatch=DISPATCH_FIXED, launch=LAUNCH_PERSISTENT, want64=True):
+        self._ensure_prepared()
+        V = self.get_vertex_num()
+        l32 = np.zeros(V, np.float32)
+        l64 = np.zeros(V, np.float64) if want64 else None
+        tot = np.zeros(1, np.uint64)
+        _check(lib().pac_lcc(self._h, dispatch, launch, _ptr(l32), _ptr(l64), _ptr(tot)))
+        return l32, l64, int(tot[0])
+
+    def jaccard_edges(self, dispatch=DISPATCH_FIXED, launch=LAUNCH_PERSISTENT):
+        self._ensure_prepared()
+        m = self.info().num_edges
+        src = np.zeros(m, np.int32)
+        dst = np.zeros(m, np.int32)
+        inter = np.zeros(m, np.uint32)
+        uni = np.zeros(m, np.uint32)
+        ratio = np.zeros(m, np.float32)
+        _check(lib().pac_jaccard_edges(self._h, dispatch, launch, _ptr(src), _ptr(dst),
+                                       _ptr(inter), _ptr(uni), _ptr(ratio)))
+        return src, dst, inter, uni, ratio
+
+    def cnt_common_pairs(self, u, v) -> np.ndarray:
+        u = np.ascontiguousarray(u, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        out = np.zeros(len(u), np.uint64)
+        _check(lib().pac_cnt_common_pairs(self._h, _ptr(u), _ptr(v), len(u), _ptr(out)))
+        return out
+
+    def features(self) -> np.ndarray:
+        P = self.info().num_partitions
+        out = np.zeros((P, MLP_IN), np.float32)
+        _check(lib().pac_partition_features(self._h, _ptr(out)))
+        return out
+
+    def profile_partitions(self, reps=3) -> np.ndarray:
+        P = self.info().num_partitions
+        out = np.zeros((P, 3), np.float64)
+        _check(lib().pac_profile_partitions(self._h, reps, _ptr(out)))
+        return out
+
+    def fixed_choice(self) -> np.ndarray:
+        out = np.zeros(self.info().num_partitions, np.uint8)
+        _check(lib().pac_fixed_choice(self._h, _ptr(out)))
+        return out
+
+    def mlp_choice(self) -> np.ndarray:
+        out = np.zeros(self.info().num_partitions, np.uint8)
+        _check(lib().pac_mlp_choice(self._h, _ptr(out)))
+        return out
+
+    # ---- reference method names (Graph.hpp:66-67, 90-93) ----
+    def solve_triangle_counting_merge(self, num_threads: int = 0) -> int:
+        """Graph::solve_triangle_counting_merge (Graph.cpp:355-368): T."""
+        return self.tc()
+
+    def solve_triangle_counting_galloping(self, num_threads: int = 0) -> int:
+        """Graph::solve_triangle_counting_galloping (Graph.cpp:342-353): 6T."""
+        return 6 * self.tc(dispatch=FORCE_GALLOP)
+
+    def solve_lcc_merge(self, num_threads: int = 0) -> np.ndarray:
+        """Graph::solve_lcc_merge (Graph.cpp:828-840): double per vertex."""
+        return self.lcc()[1]
+
+    def solve_lcc_galloping(self, num_threads: int = 0) -> np.ndarray:
+        return self.lcc(dispatch=FORCE_GALLOP)[1]
+
+    def solve_jsim_merge(self, queries, num_threads: int = 0) -> np.ndarray:
+        """Graph::solve_jsim_merge (Graph.cpp:798-812): n*n doubles."""
+        q = np.ascontiguousarray(queries, np.int32)
+        out = np.zeros(len(q) * len(q), np.float64)
+        _check(lib().pac_jaccard_queries(self._h, _ptr(q), len(q), _ptr(out)))
+        return out
+
+    solve_jsim_galloping = solve_jsim_merge
+
+
+class PartitionedGraph:
+    """Partitioned_Graph(g, vertex2partition, thread_num), Graph.cpp:1017-1069: records
+    the partition map and prepares the device work queue for it."""
+
+    def __init__(self, g: Graph, vertex2partition=None, thread_num: int = 0, num_partitions=0):
+        self.g = g
+        v2p = None if vertex2partition is None else np.ascontiguousarray(vertex2partition, np.int32)
+        g.prepare(num_partitions, v2p)
+
+    def get_partition_num(self) -> int:
+        return self.g.info().num_partitions
+
+    def get_vertex_num(self) -> int:
+        return self.g.get_vertex_num()
+
+    def get_edge_num(self) -> int:
+        return self.g.get_edge_num()
+
+    def solve_triangle_counting_merge(self) -> int:
+        return self.g.tc()
+
+    def solve_triangle_counting_galloping(self) -> int:
+        return 6 * self.g.tc(dispatch=FORCE_GALLOP)
+
+    def solve_lcc_merge(self) -> np.ndarray:
+        return self.g.lcc()[1]
+
+    def solve_lcc_galloping(self) -> np.ndarray:
+        return self.g.lcc(dispatch=FORCE_GALLOP)[1]
+
+
+def random_partition(V: int, P: int, seed=1) -> np.ndarray:
+    out = np.zeros(V, np.int32)
+    _check(lib().pac_random_partition(V, P, seed, _ptr(out)))
+    return out
+
+
+def contiguous_partition(V: int, P: int) -> np.ndarray:
+    w = max(1, -(-V // P))
+    return (np.arange(V, dtype=np.int64) // w).astype(np.int32)

@@ -1,0 +1,293 @@
+"""CPU tests that pin the oracle (oracle/pac_oracle.cpp) before anything is checked
+against it: golden fixtures computed by dense linear algebra, closed forms,
+networkx, brute force on small inputs, and invariants."""
+import itertools
+import json
+import math
+import os
+
+import networkx as nx
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+NAMES = sorted(k for k in json.load(open(os.path.join(GOLDEN, "golden_index.json"))))
+
+
+def load(name, orc):
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    return orc.CSR(z["vlist"], z["elist"]), z
+
+
+def same_with_nan(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and np.array_equal(np.isnan(a), np.isnan(b)) and \
+        np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+
+
+def nx_graph(g):
+    G = nx.Graph()
+    G.add_nodes_from(range(g.V))
+    u, v = g.edges()
+    G.add_edges_from(zip(u.tolist(), v.tolist()))
+    return G
+
+
+# ---- golden fixtures -----------------------------------------------------------------
+
+@pytest.mark.parametrize("name", NAMES)
+def test_golden_tc(name, orc):
+    g, z = load(name, orc)
+    assert orc.tc_merge(g, 2) == int(z["T"])
+    assert orc.tc_galloping(g, 2) == 6 * int(z["T"])      # Graph.cpp:342-353 returns 6T
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_golden_per_vertex_and_lcc(name, orc):
+    g, z = load(name, orc)
+    assert np.array_equal(orc.per_vertex_triangles(g, 2), z["t"])
+    assert same_with_nan(orc.lcc_merge(g, 2), z["lcc"])
+    assert same_with_nan(orc.lcc_galloping(g, 2), z["lcc"])
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_golden_jaccard(name, orc):
+    g, z = load(name, orc)
+    inter, uni, ratio = orc.jaccard_edges(g, z["eu"], z["ev"], 2)
+    assert np.array_equal(inter, z["inter"])
+    assert np.array_equal(uni, z["uni"])
+    assert np.array_equal(ratio.view(np.uint32), z["ratio"].view(np.uint32))
+    assert same_with_nan(orc.jsim_merge(g, z["queries"], 2), z["jsim"])
+    assert same_with_nan(orc.jsim_galloping(g, z["queries"], 2), z["jsim"])
+
+
+def test_text_loader_fixtures(orc):
+    exp = json.load(open(os.path.join(GOLDEN, "text_expected.json")))
+    for f, e in exp.items():
+        if f.startswith("_"):
+            continue
+        g = orc.text_to_csr(os.path.join(GOLDEN, f))
+        assert g.V == e["V"]
+        assert g.vlist.tolist() == e["vlist"]
+        assert g.elist.tolist() == e["elist"]
+
+
+def test_text_loader_missing_file(orc, tmp_path):
+    with pytest.raises(OSError):
+        orc.text_to_csr(str(tmp_path / "nope.txt"))
+
+
+# ---- closed forms ---------------------------------------------------------------------
+
+def complete(n, orc):
+    return orc.edges_to_csr(n, *zip(*itertools.combinations(range(n), 2)))
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 9, 30])
+def test_complete_graph(n, orc):
+    g = complete(n, orc)
+    assert orc.tc_merge(g, 2) == math.comb(n, 3)
+    assert np.all(orc.lcc_merge(g, 2) == 1.0)
+    u, v = g.edges()
+    inter, uni, ratio = orc.jaccard_edges(g, u, v, 2)
+    assert np.all(inter == n - 2) and np.all(uni == n)
+    assert np.all(ratio == np.float32((n - 2) / n))
+
+
+@pytest.mark.parametrize("n", [4, 5, 8, 17])
+def test_wheel(n, orc):
+    edges = [(0, i) for i in range(1, n + 1)] + [(i, i % n + 1) for i in range(1, n + 1)]
+    g = orc.edges_to_csr(n + 1, *zip(*edges))
+    T = n if n > 3 else 4
+    assert orc.tc_merge(g, 2) == T
+    lcc = orc.lcc_merge(g, 2)
+    assert lcc[0] == 2.0 / (n - 1)
+    assert np.all(lcc[1:] == 2.0 / 3.0)
+
+
+@pytest.mark.parametrize("n", [3, 4, 10])
+def test_cycle(n, orc):
+    g = orc.edges_to_csr(n, list(range(n)), [(i + 1) % n for i in range(n)])
+    assert orc.tc_merge(g, 2) == (1 if n == 3 else 0)
+
+
+def test_bipartite_and_friendship(orc):
+    g = orc.edges_to_csr(9, *zip(*[(i, j) for i in range(4) for j in range(4, 9)]))
+    assert orc.tc_merge(g, 2) == 0
+    k = 7
+    e = [(0, 2 * i + 1) for i in range(k)] + [(0, 2 * i + 2) for i in range(k)] + \
+        [(2 * i + 1, 2 * i + 2) for i in range(k)]
+    g = orc.edges_to_csr(2 * k + 1, *zip(*e))
+    assert orc.tc_merge(g, 2) == k
+
+
+def test_degenerate(orc):
+    empty = orc.CSR(np.zeros(5, np.uint64), np.zeros(0, np.int32))
+    assert orc.tc_merge(empty, 1) == 0
+    lcc = orc.lcc_merge(empty, 1)
+    assert np.all(lcc == 0.0)                                  # d = 0 keeps 0.0 (Graph.cpp:833)
+    one = orc.edges_to_csr(3, [0], [1])
+    assert orc.tc_merge(one, 1) == 0
+    lcc = orc.lcc_merge(one, 1)
+    assert np.isnan(lcc[0]) and np.isnan(lcc[1]) and lcc[2] == 0.0   # d = 1 -> 0/0
+
+
+# ---- brute force and networkx ---------------------------------------------------------
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_dense_brute_force_rmat(seed, orc):
+    g = orc.gen_rmat(8, 8, 0.57, 0.19, 0.19, seed=seed)
+    A = g.dense()
+    A3 = A @ A @ A
+    assert orc.tc_merge(g, 2) == np.trace(A3) // 6
+    assert np.array_equal(orc.per_vertex_triangles(g, 2), np.diag(A3) // 2)
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_networkx(seed, orc):
+    g = orc.gen_er(300, 12.0, seed=seed)
+    G = nx_graph(g)
+    tri = nx.triangles(G)
+    assert orc.tc_merge(g, 2) == sum(tri.values()) // 3
+    assert np.array_equal(orc.per_vertex_triangles(g, 2), [tri[i] for i in range(g.V)])
+    cl = nx.clustering(G)
+    lcc = orc.lcc_merge(g, 2)
+    deg = g.deg()
+    for u in range(g.V):
+        if deg[u] >= 2:
+            assert lcc[u] == pytest.approx(cl[u], rel=1e-12)
+    u, v = g.edges()
+    _, _, ratio = orc.jaccard_edges(g, u, v, 2)
+    ref = {(a, b): r for a, b, r in nx.jaccard_coefficient(G, list(zip(u.tolist(), v.tolist())))}
+    for i in range(len(u)):
+        assert ratio[i] == np.float32(ref[(int(u[i]), int(v[i]))])
+
+
+def test_intersections_vs_sets(orc):
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        g = orc.gen_er(120, rng.uniform(2, 30), seed=int(rng.integers(1 << 30)))
+        for _ in range(30):
+            a, b = rng.integers(0, g.V, 2)
+            na = set(g.elist[g.vlist[a]:g.vlist[a + 1]].tolist())
+            nb = set(g.elist[g.vlist[b]:g.vlist[b + 1]].tolist())
+            assert orc.cnt_merge(g, a, b) == len(na & nb)
+            assert orc.cnt_galloping(g, a, b) == len(na & nb)
+            thr = int(rng.integers(0, g.V))
+            assert orc.cnt_less_than_merge(g, a, b, thr) == len({x for x in na & nb if x < thr})
+
+
+# ---- partitioned graph -----------------------------------------------------------------
+
+@pytest.mark.parametrize("P", [1, 4, 64])
+def test_partitioned_sum_invariance(P, orc):
+    g = orc.gen_rmat(10, 16, 0.57, 0.19, 0.19, seed=10)
+    T = orc.tc_merge(g, 4)
+    for v2p in (orc.contiguous_partition(g.V, P), orc.random_partition(g.V, P, seed=P)):
+        pg = orc.PGraph(g, v2p, 4)
+        assert pg.tc_merge() == T
+        assert pg.tc_galloping() == 6 * T
+        assert sum(pg.tc_merge_partial(i) for i in range(pg.P)) == T
+        assert same_with_nan(pg.lcc_merge(), orc.lcc_merge(g, 4))
+
+
+def test_partitioned_partial_is_smallest_vertex_partition(orc):
+    """Partial i counts the triangles whose smallest-ID vertex lies in partition i."""
+    g = orc.gen_rmat(8, 16, 0.57, 0.19, 0.19, seed=3)
+    P = 5
+    v2p = orc.random_partition(g.V, P, seed=11)
+    pg = orc.PGraph(g, v2p, 2)
+    A = g.dense()
+    want = np.zeros(P, np.int64)
+    for w in range(g.V):
+        nbr = np.nonzero(A[w])[0]
+        hi = nbr[nbr > w]
+        sub = A[np.ix_(hi, hi)]
+        want[v2p[w]] += int(sub.sum()) // 2
+    assert [pg.tc_merge_partial(i) for i in range(P)] == want.tolist()
+
+
+# ---- generators -----------------------------------------------------------------------
+
+def check_structure(g):
+    vl, el = g.vlist.astype(np.int64), g.elist.astype(np.int64)
+    assert vl[0] == 0 and np.all(np.diff(vl) >= 0) and vl[-1] == len(el)
+    src = np.repeat(np.arange(g.V), np.diff(vl))
+    assert np.all(el >= 0) and np.all(el < g.V)
+    assert not np.any(src == el)
+    same_row = src[1:] == src[:-1]
+    assert np.all(el[1:][same_row] > el[:-1][same_row])           # sorted, unique
+    fwd = set(zip(src.tolist(), el.tolist()))
+    assert all((b, a) in fwd for a, b in fwd)                      # symmetric
+
+
+def test_rmat_generator(orc):
+    g = orc.gen_rmat(10, 16, 0.57, 0.19, 0.19, seed=42)
+    check_structure(g)
+    g2 = orc.gen_rmat(10, 16, 0.57, 0.19, 0.19, seed=42)
+    assert np.array_equal(g.elist, g2.elist)
+    g3 = orc.gen_rmat(10, 16, 0.57, 0.19, 0.19, seed=43)
+    assert not np.array_equal(g.vlist, g3.vlist)
+    # power law: the low-ID corner is dense
+    assert g.deg()[0] > 20 * np.median(g.deg()[g.deg() > 0])
+
+
+def test_er_generator(orc):
+    n, k = 4000, 10.0
+    g = orc.gen_er(n, k, seed=9)
+    check_structure(g)
+    m = g.nnz // 2
+    mean = k * n / 2
+    sd = math.sqrt(mean)
+    assert abs(m - mean) < 5 * sd
+
+
+def test_oriented_stats_brute(orc):
+    g = orc.gen_rmat(7, 8, 0.57, 0.19, 0.19, seed=4)
+    d = g.deg()
+    s, t = g.edges()
+    dplus = np.zeros(g.V, np.int64)
+    ori = []
+    for a, b in zip(s.tolist(), t.tolist()):
+        lo, hi = (a, b) if (d[a], a) < (d[b], b) else (b, a)
+        dplus[lo] += 1
+        ori.append((lo, hi))
+    W = sum(dplus[a] + dplus[b] for a, b in ori)
+    st = orc.oriented_stats(g)
+    assert st["m"] == len(ori) and st["W"] == W and st["max_outdeg"] == dplus.max()
+
+
+# ---- MLP oracle vs finite differences of an independent float64 forward ---------------
+
+def np_forward(P, x):
+    P = P.astype(np.float64)
+    W1 = P[:1024].reshape(64, 16); b1 = P[1024:1088]
+    W2 = P[1088:5184].reshape(64, 64); b2 = P[5184:5248]
+    W3 = P[5248:5440].reshape(3, 64); b3 = P[5440:5443]
+    h1 = np.maximum(x @ W1.T + b1, 0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    return h2 @ W3.T + b3
+
+
+def np_loss(P, x, y, w):
+    z = np_forward(P, x.astype(np.float64))
+    z = z - z.max(1, keepdims=True)
+    lse = np.log(np.exp(z).sum(1))
+    return float(np.mean(w * (lse - z[np.arange(len(y)), y])))
+
+
+def test_mlp_oracle_forward_and_gradient(orc):
+    rng = np.random.default_rng(0)
+    P = (rng.standard_normal(orc.N_PARAMS) * 0.3).astype(np.float32)
+    x = rng.random((16, 16)).astype(np.float32)
+    y = rng.integers(0, 3, 16).astype(np.int32)
+    w = rng.uniform(0.5, 2.0, 16).astype(np.float32)
+    z = orc.mlp_forward(P, x)
+    assert np.allclose(z, np_forward(P, x.astype(np.float64)), rtol=1e-5, atol=1e-5)
+    loss, g = orc.mlp_backward(P, x, y, w)
+    assert loss == pytest.approx(np_loss(P, x, y, w), rel=1e-5)
+    idx = rng.choice(orc.N_PARAMS, 60, replace=False)
+    for k in idx:
+        e = np.zeros(orc.N_PARAMS, np.float64); e[k] = 1e-4
+        fd = (np_loss(P + e, x, y, w) - np_loss(P - e, x, y, w)) / 2e-4
+        assert g[k] == pytest.approx(fd, rel=2e-3, abs=2e-6)
