@@ -78,6 +78,8 @@ typedef struct {
   double total_ms;      /* device time of the last solve (zeroing+intersection+epilogue) */
   double isect_ms;      /* device time of the intersection launches alone               */
   double epilogue_ms;   /* device time of the epilogue kernel                           */
+  double kernel_ms;     /* device time of the dominant intersection kernel alone (events  *
+                         * around its launch; 0 when replayed from a hipGraph)            */
   uint64_t launches;    /* kernel launches in the intersection phase                    */
   uint64_t strategy_partitions[PAC_NUM_STRATEGIES]; /* partitions per strategy          */
 } pac_timing;
@@ -93,6 +95,10 @@ const char *pac_last_error(void);
 int pac_ctx_last_timing(const pac_ctx *ctx, pac_timing *out);
 /* Number of compute units of the context's device. */
 int pac_ctx_num_cus(const pac_ctx *ctx);
+/* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming kernel,
+ * `reps` times; *ms receives the median kernel time.  Used to measure the achievable
+ * HBM bandwidth and to calibrate hardware byte counters.  No reference counterpart. */
+int pac_stream_copy(pac_ctx *ctx, uint64_t bytes, int reps, double *ms);
 
 /* ---- graph construction (L1/L2 of the reference) ---------------------------------- */
 /* Replaces Graph::Graph(const std::vector<...>&) / the CSR members (Graph.hpp:99-100):

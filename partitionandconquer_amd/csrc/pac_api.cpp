@@ -214,8 +214,15 @@ int choose_strategies(Graph *g, int dispatch, std::vector<uint8_t> &out) {
         out = g->h_fixed;  // documented fallback: no model loaded
         return PAC_OK;
       }
-      pac_graph *h = reinterpret_cast<pac_graph *>(reinterpret_cast<char *>(g) - offsetof(pac_graph, g));
-      return pac_mlp_choice(h, out.data());
+      if (g->learned_version != g->ctx->mlp_version || g->h_learned.size() != g->P) {
+        pac_graph *h =
+            reinterpret_cast<pac_graph *>(reinterpret_cast<char *>(g) - offsetof(pac_graph, g));
+        g->h_learned.assign(g->P, (uint8_t)kBitmap);
+        PAC_TRY(pac_mlp_choice(h, g->h_learned.data()));
+        g->learned_version = g->ctx->mlp_version;
+      }
+      out = g->h_learned;
+      return PAC_OK;
     }
     default:
       set_error("unknown dispatch mode %d", dispatch);
@@ -253,16 +260,18 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 // The timed region of every solve: zero outputs, intersection, epilogue.
 struct Timed {
   Ctx *ctx;
+  bool kernel_timed = false;
   explicit Timed(Ctx *c) : ctx(c) { ctx->last = pac_timing{}; }
   int mark(int i) {
     PAC_CHECK_HIP(hipEventRecord(ctx->ev[i], ctx->stream));
     return PAC_OK;
   }
-  int finish() {
+  int finish(bool kernel_timed = false) {
     PAC_CHECK_HIP(hipEventSynchronize(ctx->ev[3]));
     ctx->last.total_ms = elapsed(ctx->ev[0], ctx->ev[3]);
     ctx->last.isect_ms = elapsed(ctx->ev[1], ctx->ev[2]);
     ctx->last.epilogue_ms = elapsed(ctx->ev[2], ctx->ev[3]);
+    ctx->last.kernel_ms = (kernel_timed || this->kernel_timed) ? elapsed(ctx->kev[0], ctx->kev[1]) : 0.0;
     return PAC_OK;
   }
 };
@@ -278,11 +287,16 @@ int run_intersection(Graph *g, int mode, int dispatch, int launch, Timed &t) {
   if (launch != PAC_LAUNCH_GRAPH) PAC_TRY(zero_outputs(ctx, gs, 1, mode));
   PAC_TRY(t.mark(1));
   uint64_t launches = 0;
-  if (launch == PAC_LAUNCH_PLAIN)
+  if (launch == PAC_LAUNCH_PLAIN) {
     PAC_TRY(launch_plain(ctx, g, mode, strat.data(), &launches));
-  else
-    PAC_TRY(launch_persistent(ctx, gs, 1, mode, strat.data(), launch == PAC_LAUNCH_GRAPH,
-                              &launches));
+  } else {
+    ctx->time_kernel = (launch == PAC_LAUNCH_PERSISTENT) && !g->h_items.empty();
+    const int rc = launch_persistent(ctx, gs, 1, mode, strat.data(), launch == PAC_LAUNCH_GRAPH,
+                                     &launches);
+    t.kernel_timed = ctx->time_kernel;
+    ctx->time_kernel = false;
+    PAC_TRY(rc);
+  }
   ctx->last.launches = launches;
   PAC_TRY(t.mark(2));
   return PAC_OK;
@@ -438,6 +452,12 @@ int pac_ctx_create(int device, pac_ctx **out) {
     return PAC_ERR_HIP;
   }
   c->c.num_cus = prop.multiProcessorCount;
+  for (auto &e : c->c.kev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      set_error("hipEventCreate failed");
+      pac_ctx_destroy(c);
+      return PAC_ERR_HIP;
+    }
   for (auto &e : c->c.ev)
     if (hipEventCreate(&e) != hipSuccess) {
       set_error("hipEventCreate failed");
@@ -458,6 +478,8 @@ void pac_ctx_destroy(pac_ctx *ctx) {
   if (ctx->c.hgraph) (void)hipGraphDestroy(ctx->c.hgraph);
   for (auto &e : ctx->c.ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : ctx->c.kev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }
@@ -472,6 +494,15 @@ int pac_ctx_last_timing(const pac_ctx *ctx, pac_timing *out) {
 }
 
 int pac_ctx_num_cus(const pac_ctx *ctx) { return ctx ? ctx->c.num_cus : 0; }
+
+int pac_stream_copy(pac_ctx *ctx, uint64_t bytes, int reps, double *ms) {
+  PAC_TRY(ensure_ctx(ctx));
+  if (!ms) {
+    set_error("null output");
+    return PAC_ERR_ARG;
+  }
+  return stream_copy(&ctx->c, bytes, reps, ms);
+}
 
 int pac_graph_upload(pac_ctx *ctx, const uint64_t *vlist, uint64_t num_vertices,
                      const int32_t *elist, uint64_t nnz, pac_graph **out) {
@@ -785,6 +816,8 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
     g->h_items.swap(sorted);
   }
   partition_features(g, odeg, work, deg, rowlist);
+  g->h_learned.clear();
+  g->learned_version = 0;
   PAC_TRY(g->rows.reserve(sizeof(uint32_t) * (rowlist.empty() ? 1 : rowlist.size())));
   PAC_TRY(g->items.reserve(sizeof(WorkItem) * (g->h_items.empty() ? 1 : g->h_items.size())));
   if (!rowlist.empty())
@@ -1019,8 +1052,12 @@ int pac_tc_batch(pac_ctx *ctx, pac_graph *const *hs, uint32_t G, int dispatch, i
   if (launch != PAC_LAUNCH_GRAPH) PAC_TRY(zero_outputs(c, gs.data(), G, kModeTC));
   PAC_TRY(t.mark(1));
   uint64_t launches = 0;
-  PAC_TRY(launch_persistent(c, gs.data(), G, kModeTC, strat.data(), launch == PAC_LAUNCH_GRAPH,
-                            &launches));
+  c->time_kernel = (launch == PAC_LAUNCH_PERSISTENT);
+  const int rc = launch_persistent(c, gs.data(), G, kModeTC, strat.data(),
+                                   launch == PAC_LAUNCH_GRAPH, &launches);
+  t.kernel_timed = c->time_kernel;
+  c->time_kernel = false;
+  PAC_TRY(rc);
   PAC_TRY(t.mark(2));
   PAC_TRY(t.mark(3));
   PAC_TRY(t.finish());

@@ -108,6 +108,8 @@ struct Graph {
   std::vector<uint64_t> h_pedges;      // P: oriented edges per partition
   std::vector<float> h_features;       // P x 16
   std::vector<uint8_t> h_fixed;        // P: fixed-heuristic strategy
+  std::vector<uint8_t> h_learned;      // P: MLP choice cached for mlp_version
+  uint64_t learned_version = 0;
   std::vector<WorkItem> h_items;
   DevBuf rows, items;
   DevBuf rowwork;  // V: sum over c in N+(u) of d+(c)
@@ -121,6 +123,8 @@ struct Ctx {
   int num_cus = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[6] = {};
+  hipEvent_t kev[2] = {};     // around the dominant kernel when time_kernel is set
+  bool time_kernel = false;
   pac_timing last{};
   int tile = 1;
   DevBuf scratch;     // generic temporary storage (library-owned)
@@ -133,6 +137,7 @@ struct Ctx {
   // MLP
   DevBuf mlp_params;  // bf16 parameters on device
   bool mlp_loaded = false;
+  uint64_t mlp_version = 0;   // bumped whenever the parameters change
   std::vector<float> mlp_host;  // float copy (bf16-rounded)
   // hipGraph cache for the batched persistent launch
   hipGraph_t hgraph = nullptr;
@@ -172,6 +177,7 @@ int pair_counts(Ctx *ctx, Graph *g, const int32_t *d_u, const int32_t *d_v, uint
                 uint64_t *d_res);
 int query_matrix(Ctx *ctx, Graph *g, const int32_t *d_q, uint32_t nq, double *d_res);
 int fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t val);
+int stream_copy(Ctx *ctx, uint64_t bytes, int reps, double *ms);
 
 // ---- mlp.hip ---------------------------------------------------------------------
 int mlp_forward_dev(Ctx *ctx, const float *d_x, uint64_t n, float *d_logits);

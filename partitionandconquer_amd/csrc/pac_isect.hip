@@ -28,6 +28,8 @@
 // global count of A[i] -- one global atomic per oriented edge instead of one per
 // triangle.  The support of edge r->A[i] (SUP mode, for Jaccard) is flushed the same
 // way; the third edge c->w gets one atomic per triangle at its position in B.
+#include <algorithm>
+
 #include "pac_internal.hpp"
 
 namespace pac {
@@ -378,6 +380,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__res
 
 __global__ void k_reset_queue(uint32_t *q) { *q = 0; }
 
+// Plain streaming copy (16-byte vectors, grid-stride): the bandwidth reference.
+__global__ void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -505,10 +514,12 @@ int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint64_
   uint32_t grid = (uint32_t)(per_cu * ctx->num_cus);
   if (grid > nitems) grid = nitems ? nitems : 1;
   hipLaunchKernelGGL((k_reset_queue), dim3(1), dim3(1), 0, ctx->stream, ctx->queue.as<uint32_t>());
+  if (ctx->time_kernel) PAC_CHECK_HIP(hipEventRecord(ctx->kev[0], ctx->stream));
   hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, NBITS>), dim3(grid), dim3(64 * WAVES), 0,
                      ctx->stream, ctx->gdesc.as<const GraphDev>(), items, nitems,
                      ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>());
   PAC_CHECK_HIP(hipGetLastError());
+  if (ctx->time_kernel) PAC_CHECK_HIP(hipEventRecord(ctx->kev[1], ctx->stream));
   if (launches) *launches += 2;
   return PAC_OK;
 }
@@ -584,6 +595,34 @@ uint32_t grid_for(uint64_t n, uint32_t per_block) {
 }
 
 }  // namespace
+
+int stream_copy(Ctx *ctx, uint64_t bytes, int reps, double *ms) {
+  if (bytes < 16 || bytes % 16 || reps < 1) {
+    set_error("stream copy needs a multiple of 16 bytes and reps >= 1");
+    return PAC_ERR_ARG;
+  }
+  DevBuf a, b;
+  PAC_TRY(a.reserve(bytes));
+  PAC_TRY(b.reserve(bytes));
+  PAC_TRY(fill_u32(ctx->stream, a.as<uint32_t>(), bytes / 4, 1u));
+  const uint64_t n = bytes / 16;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 64);
+  std::vector<double> t;
+  for (int r = 0; r <= reps; ++r) {
+    PAC_CHECK_HIP(hipEventRecord(ctx->kev[0], ctx->stream));
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, a.as<const uint4>(),
+                       b.as<uint4>(), n);
+    PAC_CHECK_HIP(hipGetLastError());
+    PAC_CHECK_HIP(hipEventRecord(ctx->kev[1], ctx->stream));
+    PAC_CHECK_HIP(hipEventSynchronize(ctx->kev[1]));
+    float x = 0.f;
+    PAC_CHECK_HIP(hipEventElapsedTime(&x, ctx->kev[0], ctx->kev[1]));
+    if (r) t.push_back(x);  // first launch is a warm-up
+  }
+  std::sort(t.begin(), t.end());
+  *ms = t[t.size() / 2];
+  return PAC_OK;
+}
 
 int fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t val) {
   if (n == 0) return PAC_OK;
@@ -715,8 +754,11 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
     ctx->hexec = nullptr;
     ctx->hgraph = nullptr;
     PAC_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const bool timing = ctx->time_kernel;
+    ctx->time_kernel = false;  // no event records inside the captured graph
     int rc = zero_outputs(ctx, gs, G, mode);
     if (rc == PAC_OK && nitems) rc = run_persistent_any(ctx, mode, items, (uint32_t)nitems, nullptr);
+    ctx->time_kernel = timing;
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (rc != PAC_OK) {

@@ -226,6 +226,7 @@ int upload_params(Ctx *ctx, const float *params) {
   PAC_CHECK_HIP(hipMemcpy(ctx->mlp_params.p, b.data(), sizeof(uint16_t) * kNP,
                           hipMemcpyHostToDevice));
   ctx->mlp_loaded = true;
+  ctx->mlp_version++;
   return PAC_OK;
 }
 
@@ -431,6 +432,7 @@ int pac_mlp_train(pac_ctx *ctx_, const float *x, const int32_t *labels, const fl
                           hipMemcpyDeviceToHost));
   for (int i = 0; i < kNP; ++i) ctx->mlp_host[i] = bf16_to_f32(b[i]);
   ctx->mlp_loaded = true;
+  ctx->mlp_version++;
   if (loss_out)
     PAC_CHECK_HIP(hipMemcpy(loss_out, dg.as<float>() + kNP, sizeof(float), hipMemcpyDeviceToHost));
   return PAC_OK;

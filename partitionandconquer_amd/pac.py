@@ -50,8 +50,8 @@ class GraphInfo(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("isect_ms", C.c_double),
-                ("epilogue_ms", C.c_double), ("launches", C.c_uint64),
-                ("strategy_partitions", C.c_uint64 * 3)]
+                ("epilogue_ms", C.c_double), ("kernel_ms", C.c_double),
+                ("launches", C.c_uint64), ("strategy_partitions", C.c_uint64 * 3)]
 
 
 _lib = None
@@ -72,6 +72,7 @@ def lib():
         "pac_ctx_destroy": ([vp], None),
         "pac_ctx_last_timing": ([vp, C.POINTER(Timing)], C.c_int),
         "pac_ctx_num_cus": ([vp], C.c_int),
+        "pac_stream_copy": ([vp, C.c_uint64, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "pac_graph_upload": ([vp, vp, C.c_uint64, vp, C.c_uint64, C.POINTER(vp)], C.c_int),
         "pac_graph_load_csr_dir": ([vp, C.c_char_p, C.POINTER(vp)], C.c_int),
         "pac_graph_load_text": ([vp, C.c_char_p, C.POINTER(vp)], C.c_int),
@@ -133,6 +134,7 @@ class LastTiming:
     total_ms: float
     isect_ms: float
     epilogue_ms: float
+    kernel_ms: float
     launches: int
     strategy_partitions: tuple
 
@@ -170,8 +172,14 @@ class Context:
     def last_timing(self) -> LastTiming:
         t = Timing()
         _check(lib().pac_ctx_last_timing(self._h, C.byref(t)))
-        return LastTiming(t.total_ms, t.isect_ms, t.epilogue_ms, t.launches,
+        return LastTiming(t.total_ms, t.isect_ms, t.epilogue_ms, t.kernel_ms, t.launches,
                           tuple(t.strategy_partitions))
+
+    def stream_copy(self, nbytes: int, reps: int = 5) -> float:
+        """Median time (ms) of a plain device-to-device streaming copy of nbytes."""
+        ms = C.c_double()
+        _check(lib().pac_stream_copy(self._h, nbytes, reps, C.byref(ms)))
+        return ms.value
 
     # ---- tiles / autotuning ----
     def set_tile(self, tile: int):
@@ -365,6 +373,13 @@ class Graph:
         tot = np.zeros(1, np.uint64)
         _check(lib().pac_lcc(self._h, dispatch, launch, _ptr(l32), _ptr(l64), _ptr(tot)))
         return l32, l64, int(tot[0])
+
+    def lcc_device(self, dispatch=DISPATCH_FIXED, launch=LAUNCH_PERSISTENT) -> int:
+        """TC + LCC with the per-vertex results left in device memory; returns T."""
+        self._ensure_prepared()
+        tot = np.zeros(1, np.uint64)
+        _check(lib().pac_lcc(self._h, dispatch, launch, None, None, _ptr(tot)))
+        return int(tot[0])
 
     def jaccard_edges(self, dispatch=DISPATCH_FIXED, launch=LAUNCH_PERSISTENT):
         self._ensure_prepared()
