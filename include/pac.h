@@ -57,7 +57,8 @@ enum {
 
 /* How the per-partition work is launched. */
 enum {
-  PAC_LAUNCH_PLAIN = 0,      /* one kernel launch per partition, one workgroup/row    */
+  PAC_LAUNCH_PLAIN = 0,      /* one launch per partition group (a run of consecutive  *
+                              * partitions with the same strategy), one workgroup/row */
   PAC_LAUNCH_PERSISTENT = 1, /* one persistent kernel over an atomic work queue       */
   PAC_LAUNCH_GRAPH = 2       /* PERSISTENT, captured once in a hipGraph and replayed  */
 };
@@ -172,7 +173,8 @@ int pac_lcc(pac_graph *g, int dispatch, int launch, float *lcc32, double *lcc64,
  * src/dst (original IDs, src has the smaller (degree, id)), inter = |N(u)&N(v)|,
  * uni = d(u)+d(v)-inter, ratio = (float)((double)inter/(double)uni) -- the pair
  * formula of Graph::solve_jsim_merge (Graph.cpp:808-809) applied to each edge.  Any
- * output pointer may be NULL. */
+ * output pointer may be NULL; when all are NULL the intersection counts (support),
+ * unions and ratios stay in device memory. */
 int pac_jaccard_edges(pac_graph *g, int dispatch, int launch, int32_t *src, int32_t *dst,
                       uint32_t *inter, uint32_t *uni, float *ratio);
 /* Replaces Graph::solve_jsim_merge / _galloping (Graph.cpp:782-812): the n x n matrix

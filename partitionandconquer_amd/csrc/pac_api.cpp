@@ -603,7 +603,10 @@ int pac_graph_generate_rmat(pac_ctx *ctx, uint32_t scale, uint32_t edge_factor, 
 }
 
 // G(n,p) by Batagelj-Brandes geometric skipping with one stream per row v (seeded by
-// mix64(key + v*gamma)): w advances by 1 + floor(log(r)/log(1-p)), r in (0,1].
+// mix64(key + v*gamma)): w advances by 1 + floor(log(r)/log(1-p)), r in (0,1].  Pairs
+// come out in increasing (v, w) order, so the symmetric CSR is built by count / scan /
+// fill with every row already sorted: row u holds its lower neighbours (from row u, in
+// increasing w) followed by its upper neighbours (from rows v > u, in increasing v).
 int pac_graph_generate_er(pac_ctx *ctx, uint64_t n, double avg_degree, uint64_t seed,
                           pac_graph **out) {
   PAC_TRY(ensure_ctx(ctx));
@@ -615,9 +618,11 @@ int pac_graph_generate_er(pac_ctx *ctx, uint64_t n, double avg_degree, uint64_t 
   if (p > 1.0) p = 1.0;
   const uint64_t key = mix64(seed + kGamma);
   const double lq = std::log1p(-p);
-  std::vector<uint64_t> keys;
-  keys.reserve((size_t)(avg_degree * (double)n * 1.1) + 16);
+  std::vector<uint32_t> lo;          // lower endpoints, grouped by upper endpoint v
+  std::vector<uint64_t> vstart(n + 1, 0);
+  lo.reserve((size_t)(avg_degree * (double)n * 0.55) + 16);
   for (uint64_t v = 1; v < n; ++v) {
+    vstart[v] = lo.size();
     uint64_t st = mix64(key + v * kGamma);
     int64_t w = -1;
     while (true) {
@@ -627,13 +632,23 @@ int pac_graph_generate_er(pac_ctx *ctx, uint64_t n, double avg_degree, uint64_t 
       if (skip >= (double)n) break;
       w += 1 + (int64_t)skip;
       if (w >= (int64_t)v) break;
-      keys.push_back((v << 32) | (uint64_t)w);
-      keys.push_back(((uint64_t)w << 32) | v);
+      lo.push_back((uint32_t)w);
     }
   }
-  std::vector<uint64_t> vl;
-  std::vector<int32_t> el;
-  keys_to_csr(n, keys, vl, el);
+  vstart[n] = lo.size();
+  vstart[0] = 0;
+  std::vector<uint64_t> vl(n + 1, 0);
+  for (uint64_t v = 1; v < n; ++v) {
+    vl[v + 1] += vstart[v + 1] - vstart[v];  // lower neighbours of v
+    for (uint64_t i = vstart[v]; i < vstart[v + 1]; ++i) vl[lo[i] + 1]++;  // v is upper of lo[i]
+  }
+  for (uint64_t u = 0; u < n; ++u) vl[u + 1] += vl[u];
+  std::vector<int32_t> el(vl[n]);
+  std::vector<uint64_t> cur(vl.begin(), vl.end() - 1);
+  for (uint64_t v = 1; v < n; ++v)
+    for (uint64_t i = vstart[v]; i < vstart[v + 1]; ++i) el[cur[v]++] = (int32_t)lo[i];
+  for (uint64_t v = 1; v < n; ++v)
+    for (uint64_t i = vstart[v]; i < vstart[v + 1]; ++i) el[cur[lo[i]]++] = (int32_t)v;
   return upload_csr(ctx, vl.data(), n, el.data(), el.size(), out);
 }
 
@@ -943,15 +958,24 @@ int pac_jaccard_edges(pac_graph *h, int dispatch, int launch, int32_t *src, int3
   Ctx *ctx = g->ctx;
   const uint64_t m = g->m ? g->m : 1;
   DevBuf d_src, d_dst, d_inter, d_uni, d_ratio;
+  const bool device_only = !src && !dst && !inter && !uni && !ratio;
   if (src) PAC_TRY(d_src.reserve(4 * m));
   if (dst) PAC_TRY(d_dst.reserve(4 * m));
   if (inter) PAC_TRY(d_inter.reserve(4 * m));
   if (uni) PAC_TRY(d_uni.reserve(4 * m));
   if (ratio) PAC_TRY(d_ratio.reserve(4 * m));
+  if (device_only) {  // results stay in graph-owned device buffers
+    PAC_TRY(g->juni.reserve(4 * m));
+    PAC_TRY(g->jratio.reserve(4 * m));
+  }
   Timed t(ctx);
   PAC_TRY(run_intersection(g, kModeSUP, dispatch, launch, t));
-  PAC_TRY(epilogue_jaccard(ctx, g, d_src.as<int32_t>(), d_dst.as<int32_t>(),
-                           d_inter.as<uint32_t>(), d_uni.as<uint32_t>(), d_ratio.as<float>()));
+  if (device_only)
+    PAC_TRY(epilogue_jaccard(ctx, g, nullptr, nullptr, nullptr, g->juni.as<uint32_t>(),
+                             g->jratio.as<float>()));
+  else
+    PAC_TRY(epilogue_jaccard(ctx, g, d_src.as<int32_t>(), d_dst.as<int32_t>(),
+                             d_inter.as<uint32_t>(), d_uni.as<uint32_t>(), d_ratio.as<float>()));
   PAC_TRY(t.mark(3));
   PAC_TRY(t.finish());
   if (g->m) {

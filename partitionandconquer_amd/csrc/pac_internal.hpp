@@ -114,7 +114,7 @@ struct Graph {
   DevBuf rows, items;
   DevBuf rowwork;  // V: sum over c in N+(u) of d+(c)
   // outputs
-  DevBuf total, tvert, sup, lcc32, lcc64;
+  DevBuf total, tvert, sup, lcc32, lcc64, juni, jratio;
   double prepare_ms = 0;
 };
 

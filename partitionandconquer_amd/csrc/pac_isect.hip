@@ -75,15 +75,57 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t 
   return base + (a[base] < x ? 1u : 0u);
 }
 
+constexpr uint32_t kTabMul = 0x85EBCA6Bu;   // hash-table multiplier (independent of kHashMul)
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;     // empty hash-table slot (IDs are < 2^31)
+
+// LDS of one workgroup.  The sorted-list strategies (merge, gallop) keep A and one
+// 32-bit counter per position; the bitmap strategy keeps a linear-probing hash table
+// of A (2*CAP slots) with packed 16-bit counters and, in SUP mode, 16-bit positions.
+// Both live in one union: a workgroup runs one strategy per row.  The filter bitmap
+// sits outside the union and is all-zero between bitmap rows.
 template <int CAP, int NBITS, int MODE>
 struct Smem {
-  uint32_t A[CAP];
+  union {
+    struct {
+      uint32_t A[CAP];
+      uint32_t cnt[MODE == kModeTC ? 1 : CAP];
+    } s;  // merge / gallop
+    struct {
+      uint32_t tab[2 * CAP];
+      uint32_t cnt16[MODE == kModeTC ? 1 : CAP];   // 2 x u16 per word
+      uint32_t pos16[MODE == kModeSUP ? CAP : 1];  // 2 x u16 per word
+    } h;  // bitmap
+  } u;
   uint32_t bits[NBITS / 32];
-  uint32_t cnt[MODE == kModeTC ? 1 : CAP];
   uint32_t item;
 };
 
-// Record a confirmed triangle (r, c, w): w = A[i] = B[j].
+__device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+
+// 16-bit counters packed two per word.  Counts stay below 2^16 (each is at most the
+// number of middle vertices of a row plus one, <= 2*CAP), so halves never carry.
+__device__ __forceinline__ void add16(uint32_t *w, uint32_t slot, uint32_t v) {
+  atomicAdd(&w[slot >> 1], v << (16u * (slot & 1u)));
+}
+__device__ __forceinline__ uint32_t get16(const uint32_t *w, uint32_t slot) {
+  return (w[slot >> 1] >> (16u * (slot & 1u))) & 0xFFFFu;
+}
+
+// Slot of x in the table (kEmpty if absent).
+__device__ __forceinline__ uint32_t tab_find(const uint32_t *tab, uint32_t tbits, uint32_t x) {
+  const uint32_t mask = (1u << tbits) - 1u;
+  uint32_t h = (x * kTabMul) >> (32u - tbits);
+  while (true) {
+    const uint32_t k = tab[h];
+    if (k == x) return h;
+    if (k == kEmpty) return kEmpty;
+    h = (h + 1u) & mask;
+  }
+}
+
+// Record a confirmed triangle (r, c, w) in the sorted-list strategies: w = A[i] = B[j].
 template <int MODE>
 __device__ __forceinline__ void on_hit(uint32_t *cnt, uint32_t i, uint32_t *sup, uint32_t bj) {
   if (MODE != kModeTC) atomicAdd(&cnt[i], 1u);
@@ -91,34 +133,38 @@ __device__ __forceinline__ void on_hit(uint32_t *cnt, uint32_t i, uint32_t *sup,
 }
 
 // ---- the three lookup engines: one wave, one middle vertex c ------------------------
-// A: sorted N+(r) in LDS (nA entries); B = oadj[b0 .. b0+nb): sorted N+(c) in HBM.
-// Each returns this lane's number of confirmed common neighbours.
+// B = oadj[b0 .. b0+nb): sorted N+(c) in HBM.  Each returns this lane's number of
+// confirmed common neighbours.
+
+// Bitmap: one LDS read of the hashed filter bit; a set bit is confirmed (and its
+// counter slot found) in the LDS hash table of A.  Lists are consumed in chunks of
+// 256 elements (4 per lane).
+__device__ __forceinline__ void load_chunk(const uint32_t *__restrict__ oadj, uint32_t b0,
+                                           uint32_t nb, uint32_t j0, uint32_t w[4]) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const uint32_t j = j0 + 64u * u;
+    w[u] = j < nb ? oadj[b0 + j] : 0u;
+  }
+}
 
 template <int MODE, int NBITS>
-__device__ __forceinline__ uint32_t probe_bitmap(const uint32_t *__restrict__ oadj,
-                                                 const uint32_t *A, uint32_t nA,
-                                                 const uint32_t *bits, uint32_t *cnt,
-                                                 uint32_t b0, uint32_t nb, uint32_t *sup,
-                                                 uint32_t lane) {
+__device__ __forceinline__ uint32_t probe_chunk(const uint32_t w[4], uint32_t j0, uint32_t nb,
+                                                const uint32_t *bits, const uint32_t *tab,
+                                                uint32_t tbits, uint32_t *cnt16, uint32_t b0,
+                                                uint32_t *sup) {
   uint32_t hits = 0;
-  for (uint32_t j0 = lane; j0 < nb; j0 += 256) {
-    uint32_t w[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t j = j0 + 64u * u;
-      w[u] = j < nb ? oadj[b0 + j] : 0u;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t j = j0 + 64u * u;
-      if (j < nb) {
-        const uint32_t h = bit_of<NBITS>(w[u]);
-        if ((bits[h >> 5] >> (h & 31u)) & 1u) {
-          const uint32_t i = lower_bound_u32(A, nA, w[u]);
-          if (i < nA && A[i] == w[u]) {
-            ++hits;
-            on_hit<MODE>(cnt, i, sup, b0 + j);
-          }
+  for (int u = 0; u < 4; ++u) {
+    const uint32_t j = j0 + 64u * u;
+    if (j < nb) {
+      const uint32_t hb = bit_of<NBITS>(w[u]);
+      if ((bits[hb >> 5] >> (hb & 31u)) & 1u) {
+        const uint32_t slot = tab_find(tab, tbits, w[u]);
+        if (slot != kEmpty) {
+          ++hits;
+          if (MODE != kModeTC) add16(cnt16, slot, 1u);
+          if (MODE == kModeSUP) atomicAdd(&sup[b0 + j], 1u);
         }
       }
     }
@@ -212,44 +258,63 @@ __device__ __forceinline__ uint32_t probe_merge(const uint32_t *__restrict__ oad
   return hits;
 }
 
-// ---- one row with A staged in LDS -----------------------------------------------
+// The middle vertices of a row are visited in batches of 64 per wave: lane l loads
+// c = A[k_l] and the bounds of N+(c) for the l-th middle vertex of the wave, so the
+// offsets of 64 lists arrive with one round trip; the wave then walks the batch,
+// broadcasting each list's bounds with readlane.  Wave w takes positions
+// c_lo + w, c_lo + w + WAVES, ...  body(k, c, b0, nb) runs once per middle vertex.
+template <int WAVES, class AAt, class Body>
+__device__ __forceinline__ void for_each_c(const GraphDev &G, uint32_t c_lo, uint32_t c_hi,
+                                           uint32_t wave, uint32_t lane, AAt a_at, Body body) {
+  for (uint32_t t0 = 0;; t0 += 64) {
+    const uint32_t kl = c_lo + wave + (uint32_t)WAVES * (t0 + lane);
+    uint32_t cl = 0, bl = 0, nl = 0;
+    if (kl < c_hi) {
+      cl = a_at(kl);
+      bl = G.ooff[cl];
+      nl = G.ooff[cl + 1] - bl;
+    }
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
+    for (uint32_t jj = 0; jj < nvalid; ++jj)
+      body(c_lo + wave + (uint32_t)WAVES * (t0 + jj), readlane(cl, jj), readlane(bl, jj),
+           readlane(nl, jj));
+    if (nvalid < 64) break;
+  }
+}
+
+// ---- one row, sorted-list strategies (merge, gallop): A staged in LDS ---------------
 template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
-__device__ __forceinline__ void row_lds(const GraphDev &G, uint32_t r, uint32_t c_lo,
-                                        uint32_t c_hi, Smem<CAP, NBITS, MODE> &s,
-                                        unsigned long long &acc) {
+__device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32_t c_lo,
+                                           uint32_t c_hi, Smem<CAP, NBITS, MODE> &sm,
+                                           unsigned long long &acc) {
   constexpr uint32_t BS = 64u * WAVES;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
+  uint32_t *A = sm.u.s.A;
+  uint32_t *cnt = sm.u.s.cnt;
   for (uint32_t i = tid; i < nA; i += BS) {
-    const uint32_t w = G.oadj[a0 + i];
-    s.A[i] = w;
-    if (STRAT == kBitmap) {
-      const uint32_t h = bit_of<NBITS>(w);
-      atomicOr(&s.bits[h >> 5], 1u << (h & 31u));
-    }
-    if (MODE != kModeTC) s.cnt[i] = 0;
+    A[i] = G.oadj[a0 + i];
+    if (MODE != kModeTC) cnt[i] = 0;
   }
   __syncthreads();
   if (c_hi > nA) c_hi = nA;
   uint32_t row_hits = 0;
-  for (uint32_t k = c_lo + wave; k < c_hi; k += WAVES) {
-    const uint32_t c = s.A[k];
-    const uint32_t b0 = G.ooff[c], nb = G.ooff[c + 1] - b0;
+  for_each_c<WAVES>(G, c_lo, c_hi, wave, lane, [&](uint32_t kl) { return A[kl]; },
+                    [&](uint32_t k, uint32_t c, uint32_t b0, uint32_t nb) {
+    (void)c;
     uint32_t h = 0;
     if (nb != 0) {
-      if (STRAT == kBitmap)
-        h = probe_bitmap<MODE, NBITS>(G.oadj, s.A, nA, s.bits, s.cnt, b0, nb, G.sup, lane);
-      else if (STRAT == kGallop)
-        h = probe_gallop<MODE>(G.oadj, s.A, nA, s.cnt, b0, nb, G.sup, lane);
+      if (STRAT == kGallop)
+        h = probe_gallop<MODE>(G.oadj, A, nA, cnt, b0, nb, G.sup, lane);
       else
-        h = probe_merge<MODE>(G.oadj, s.A, nA, s.cnt, b0, nb, G.sup, lane);
+        h = probe_merge<MODE>(G.oadj, A, nA, cnt, b0, nb, G.sup, lane);
     }
     if (MODE != kModeTC) {
       const uint32_t ch = wave_sum(h);
-      if (lane == 0 && ch) atomicAdd(&s.cnt[k], ch);
+      if (lane == 0 && ch) atomicAdd(&cnt[k], ch);
     }
     row_hits += h;
-  }
+  });
   acc += row_hits;
   if (MODE == kModeLCC) {
     const uint32_t wh = wave_sum(row_hits);
@@ -258,15 +323,97 @@ __device__ __forceinline__ void row_lds(const GraphDev &G, uint32_t r, uint32_t 
   __syncthreads();
   if (MODE != kModeTC) {
     for (uint32_t i = tid; i < nA; i += BS) {
-      const uint32_t v = s.cnt[i];
+      const uint32_t v = cnt[i];
       if (v) {
-        if (MODE == kModeLCC) atomicAdd(&G.tvert[s.A[i]], (unsigned long long)v);
+        if (MODE == kModeLCC) atomicAdd(&G.tvert[A[i]], (unsigned long long)v);
         if (MODE == kModeSUP) atomicAdd(&G.sup[a0 + i], v);
       }
     }
   }
-  if (STRAT == kBitmap)
-    for (uint32_t i = tid; i < nA; i += BS) s.bits[bit_of<NBITS>(s.A[i]) >> 5] = 0u;
+  __syncthreads();
+}
+
+// ---- one row, bitmap strategy: hashed filter bitmap + LDS hash table of A -------------
+template <int MODE, int WAVES, int CAP, int NBITS>
+__device__ __forceinline__ void row_bitmap(const GraphDev &G, uint32_t r, uint32_t c_lo,
+                                           uint32_t c_hi, Smem<CAP, NBITS, MODE> &sm,
+                                           unsigned long long &acc) {
+  constexpr uint32_t BS = 64u * WAVES;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
+  uint32_t *tab = sm.u.h.tab;
+  uint32_t *cnt16 = sm.u.h.cnt16;
+  uint32_t *pos16 = sm.u.h.pos16;
+  // table of 2^tbits >= 2*nA slots (load factor <= 1/2), at least 64; nA <= CAP
+  uint32_t tbits = 6;
+  while ((1u << tbits) < 2u * nA) ++tbits;
+  const uint32_t tsize = 1u << tbits, tmask = tsize - 1u;
+  for (uint32_t i = tid; i < tsize; i += BS) tab[i] = kEmpty;
+  if (MODE != kModeTC)
+    for (uint32_t i = tid; i < tsize / 2; i += BS) cnt16[i] = 0;
+  if (MODE == kModeSUP)
+    for (uint32_t i = tid; i < tsize / 2; i += BS) pos16[i] = 0;
+  __syncthreads();
+  for (uint32_t i = tid; i < nA; i += BS) {
+    const uint32_t w = G.oadj[a0 + i];
+    uint32_t h = (w * kTabMul) >> (32u - tbits);
+    while (atomicCAS(&tab[h], kEmpty, w) != kEmpty) h = (h + 1u) & tmask;
+    if (MODE == kModeSUP) add16(pos16, h, i);
+    const uint32_t hb = bit_of<NBITS>(w);
+    atomicOr(&sm.bits[hb >> 5], 1u << (hb & 31u));
+  }
+  __syncthreads();
+  if (c_hi > nA) c_hi = nA;
+  uint32_t row_hits = 0;
+  // Batches of 64 middle vertices per wave (offsets fetched in one round trip); within
+  // a batch the first chunk of the next list is loaded before the current one is
+  // probed, so two chunks per lane are in flight.
+  for (uint32_t t0 = 0;; t0 += 64) {
+    const uint32_t kl = c_lo + wave + (uint32_t)WAVES * (t0 + lane);
+    uint32_t cl = 0, bl = 0, nl = 0;
+    if (kl < c_hi) {
+      cl = G.oadj[a0 + kl];
+      bl = G.ooff[cl];
+      nl = G.ooff[cl + 1] - bl;
+    }
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
+    uint32_t wn[4] = {0u, 0u, 0u, 0u};
+    if (nvalid) load_chunk(G.oadj, readlane(bl, 0), readlane(nl, 0), lane, wn);
+    for (uint32_t jj = 0; jj < nvalid; ++jj) {
+      const uint32_t c = readlane(cl, jj), b0 = readlane(bl, jj), nb = readlane(nl, jj);
+      uint32_t w[4] = {wn[0], wn[1], wn[2], wn[3]};
+      if (jj + 1 < nvalid) load_chunk(G.oadj, readlane(bl, jj + 1), readlane(nl, jj + 1), lane, wn);
+      uint32_t h = probe_chunk<MODE, NBITS>(w, lane, nb, sm.bits, tab, tbits, cnt16, b0, G.sup);
+      for (uint32_t j0 = lane + 256u; j0 < nb; j0 += 256u) {
+        load_chunk(G.oadj, b0, nb, j0, w);
+        h += probe_chunk<MODE, NBITS>(w, j0, nb, sm.bits, tab, tbits, cnt16, b0, G.sup);
+      }
+      if (MODE != kModeTC) {
+        const uint32_t ch = wave_sum(h);
+        if (lane == 0 && ch) add16(cnt16, tab_find(tab, tbits, c), ch);
+      }
+      row_hits += h;
+    }
+    if (nvalid < 64) break;
+  }
+  acc += row_hits;
+  if (MODE == kModeLCC) {
+    const uint32_t wh = wave_sum(row_hits);
+    if (lane == 0 && wh) atomicAdd(&G.tvert[r], (unsigned long long)wh);
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < tsize; i += BS) {
+    const uint32_t key = tab[i];
+    if (key == kEmpty) continue;
+    if (MODE != kModeTC) {
+      const uint32_t v = get16(cnt16, i);
+      if (v) {
+        if (MODE == kModeLCC) atomicAdd(&G.tvert[key], (unsigned long long)v);
+        if (MODE == kModeSUP) atomicAdd(&G.sup[a0 + get16(pos16, i)], v);
+      }
+    }
+    sm.bits[bit_of<NBITS>(key) >> 5] = 0u;
+  }
   __syncthreads();
 }
 
@@ -279,9 +426,8 @@ __device__ __forceinline__ void row_hbm(const GraphDev &G, uint32_t r, uint32_t 
   const uint32_t *A = G.oadj + a0;
   if (c_hi > nA) c_hi = nA;
   uint32_t row_hits = 0;
-  for (uint32_t k = c_lo + wave; k < c_hi; k += WAVES) {
-    const uint32_t c = A[k];
-    const uint32_t b0 = G.ooff[c], nb = G.ooff[c + 1] - b0;
+  for_each_c<WAVES>(G, c_lo, c_hi, wave, lane, [&](uint32_t kl) { return A[kl]; },
+                    [&](uint32_t k, uint32_t c, uint32_t b0, uint32_t nb) {
     uint32_t h = 0;
     for (uint32_t j = lane; j < nb; j += 64) {
       const uint32_t w = G.oadj[b0 + j];
@@ -303,12 +449,28 @@ __device__ __forceinline__ void row_hbm(const GraphDev &G, uint32_t r, uint32_t 
       }
     }
     row_hits += h;
-  }
+  });
   acc += row_hits;
   if (MODE == kModeLCC) {
     const uint32_t wh = wave_sum(row_hits);
     if (lane == 0 && wh) atomicAdd(&G.tvert[r], (unsigned long long)wh);
   }
+}
+
+// One row (or a middle-vertex range of one) with the given strategy.
+template <int MODE, int WAVES, int CAP, int NBITS>
+__device__ __forceinline__ void do_row(const GraphDev &G, uint32_t r, uint32_t c_lo,
+                                       uint32_t c_hi, int st, Smem<CAP, NBITS, MODE> &sm,
+                                       unsigned long long &acc) {
+  const uint32_t nA = G.ooff[r + 1] - G.ooff[r];
+  if (nA > (uint32_t)CAP)
+    row_hbm<MODE, WAVES>(G, r, c_lo, c_hi, acc);
+  else if (st == kBitmap)
+    row_bitmap<MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
+  else if (st == kGallop)
+    row_sorted<kGallop, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
+  else
+    row_sorted<kMerge, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
 }
 
 template <int WAVES>
@@ -317,20 +479,16 @@ __device__ __forceinline__ void flush_total(unsigned long long *total, unsigned 
   if ((threadIdx.x & 63u) == 0 && s) atomicAdd(total, s);
 }
 
-// ---- plain kernel: one workgroup per row, one launch per partition ---------------
+// ---- plain kernel: one workgroup per row, one launch per partition group -----------
 template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
 __global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev G, uint32_t row_begin) {
-  __shared__ Smem<CAP, NBITS, MODE> s;
+  __shared__ Smem<CAP, NBITS, MODE> sm;
   if (STRAT == kBitmap)
-    for (uint32_t i = threadIdx.x; i < NBITS / 32; i += 64 * WAVES) s.bits[i] = 0;
+    for (uint32_t i = threadIdx.x; i < NBITS / 32; i += 64 * WAVES) sm.bits[i] = 0;
   __syncthreads();
   const uint32_t r = G.rows[row_begin + blockIdx.x];
-  const uint32_t nA = G.ooff[r + 1] - G.ooff[r];
   unsigned long long acc = 0;
-  if (nA > (uint32_t)CAP)
-    row_hbm<MODE, WAVES>(G, r, 0, kAllC, acc);
-  else
-    row_lds<STRAT, MODE, WAVES, CAP, NBITS>(G, r, 0, kAllC, s, acc);
+  do_row<MODE, WAVES, CAP, NBITS>(G, r, 0, kAllC, STRAT, sm, acc);
   flush_total<WAVES>(G.total, acc);
 }
 
@@ -361,19 +519,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__res
     const uint8_t st = strat[G.part_base + w.part];
     const bool single = (w.hi == w.lo + 1);
     const uint32_t c_lo = single ? w.c_lo : 0u, c_hi = single ? w.c_hi : kAllC;
-    for (uint32_t q = w.lo; q < w.hi; ++q) {
-      const uint32_t r = G.rows[q];
-      const uint32_t nA = G.ooff[r + 1] - G.ooff[r];
-      if (nA > (uint32_t)CAP) {
-        row_hbm<MODE, WAVES>(G, r, c_lo, c_hi, acc);
-      } else if (st == kBitmap) {
-        row_lds<kBitmap, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, s, acc);
-      } else if (st == kGallop) {
-        row_lds<kGallop, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, s, acc);
-      } else {
-        row_lds<kMerge, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, s, acc);
-      }
-    }
+    for (uint32_t q = w.lo; q < w.hi; ++q)
+      do_row<MODE, WAVES, CAP, NBITS>(G, G.rows[q], c_lo, c_hi, st, s, acc);
   }
   if (cur_total) flush_total<WAVES>(cur_total, acc);
 }
@@ -667,14 +814,21 @@ int compute_rowwork(Graph *g) {
   return PAC_OK;
 }
 
+// One launch per partition group: a maximal run of consecutive partitions with the
+// same strategy (their rows are contiguous in the row list), one workgroup per row.
 int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
                  uint64_t *launches) {
   const GraphDev d = make_desc(g, 0);
-  for (uint32_t k = 0; k < g->P; ++k) {
-    const uint32_t b = g->h_pstart[k], e = g->h_pstart[k + 1];
-    if (e == b) continue;
-    PAC_TRY(run_plain_any(ctx, d, mode, strategy_per_part[k], b, e - b));
-    if (launches) ++*launches;
+  uint32_t k = 0;
+  while (k < g->P) {
+    uint32_t e = k + 1;
+    while (e < g->P && strategy_per_part[e] == strategy_per_part[k]) ++e;
+    const uint32_t rb = g->h_pstart[k], re = g->h_pstart[e];
+    if (re > rb) {
+      PAC_TRY(run_plain_any(ctx, d, mode, strategy_per_part[k], rb, re - rb));
+      if (launches) ++*launches;
+    }
+    k = e;
   }
   return PAC_OK;
 }

@@ -171,8 +171,8 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const uint16_t *__restrict
 #pragma unroll
   for (int o = 0; o < kOut; ++o) out[oW3 + o * kH + lane] = gW3[o];
   if (lane < kOut) out[oB3 + lane] = gb3;
-  const float lsum = wsum(loss);
-  if (lane == 0) out[kNP] = lsum;
+  // every lane holds the same (wave-uniform) loss
+  if (lane == 0) out[kNP] = loss;
 }
 
 // grads[k] = sum over waves of partial[w][k], in wave order; k = kNP is the loss.

@@ -1,0 +1,194 @@
+"""Measurements of the non-headline configurations (one JSON line each).
+
+    python -m partitionandconquer_amd.experiments scale22   # plain vs batched, every strategy
+    python -m partitionandconquer_amd.experiments batch     # 256 uneven graphs, autotuned tile
+    python -m partitionandconquer_amd.experiments jaccard   # per-edge Jaccard at R-MAT scale 25
+
+scale22: R-MAT scale 22, ef 16, (0.57,0.19,0.19,0.05), seed 22, 1024 contiguous
+  partitions; TC + LCC device time of the plain launch (one launch per partition
+  group) and of the persistent/hipGraph launch for each strategy and dispatch.
+batch: 256 graphs, graph g seeded 3000+g; even g: Erdos-Renyi G(n,p) with n = 2^U(10,18)
+  and average degree U(8,64); odd g: R-MAT scale U{10..18}, ef 16; the draws come from
+  a SplitMix64 stream seeded 3.  Each graph gets ceil(V/4096) contiguous partitions.
+  The tile (waves per row, LDS sizes) is autotuned on the batch; reported: one batched
+  persistent launch (and its hipGraph replay) vs one plain solve per graph, and every
+  graph's count against the oracle.
+jaccard: R-MAT scale 25, ef 16, seed 25, 4096 partitions; per-edge |N(u)&N(v)|,
+  union and FP32 ratio for every edge with all results left in HBM; sampled edges are
+  checked against the oracle; the oracle's per-edge merge is timed on a sample of edges.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import partitionandconquer_amd as pac  # noqa: E402
+
+GAMMA = 0x9E3779B97F4A7C15
+MASK = (1 << 64) - 1
+
+
+def mix64(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return z ^ (z >> 31)
+
+
+class Stream:
+    def __init__(self, seed):
+        self.st = mix64((seed + GAMMA) & MASK)
+
+    def uniform(self):
+        self.st = (self.st + GAMMA) & MASK
+        return (mix64(self.st) >> 11) / float(1 << 53)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def median_ms(ctx, fn, reps=5, warm=1):
+    for _ in range(warm):
+        fn()
+    v = []
+    for _ in range(reps):
+        fn()
+        v.append(ctx.last_timing().total_ms)
+    return float(np.median(v))
+
+
+def exp_scale22(ctx, scale=22, partitions=1024):
+    g = pac.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed=scale)
+    g.prepare(partitions)
+    info = g.info()
+    T = g.tc(pac.FORCE_BITMAP)
+    res = {"config": f"rmat-s{scale}-ef16-P{partitions}", "m": info.num_edges, "W": info.work_W,
+           "triangles": T, "device_ms": {}}
+    for dname, d in (("fixed", pac.DISPATCH_FIXED), ("merge", pac.FORCE_MERGE),
+                     ("gallop", pac.FORCE_GALLOP), ("bitmap", pac.FORCE_BITMAP)):
+        for lname, l in (("plain", pac.LAUNCH_PLAIN), ("persistent", pac.LAUNCH_PERSISTENT),
+                         ("graph", pac.LAUNCH_GRAPH)):
+            assert g.lcc_device(d, l) == T
+            ms = median_ms(ctx, lambda: g.lcc_device(d, l))
+            res["device_ms"][f"{dname}/{lname}"] = round(ms, 3)
+    best = min(res["device_ms"]["fixed/graph"], res["device_ms"]["bitmap/graph"])
+    res["gteps_batched"] = round(info.num_edges / (best * 1e-3) / 1e9, 4)
+    res["roofline_frac"] = round(4 * info.work_W / (best * 1e-3) / 8e12, 4)
+    res["speedup_batched_vs_plain_fixed"] = round(res["device_ms"]["fixed/plain"] / best, 3)
+    return res
+
+
+def batch_specs(G=256, master=3):
+    s = Stream(master)
+    out = []
+    for g in range(G):
+        if g % 2 == 0:
+            n = int(round(2 ** (10 + 8 * s.uniform())))
+            k = 8 + 56 * s.uniform()
+            out.append({"kind": "er", "n": n, "k": k, "seed": 3000 + g})
+        else:
+            scale = 10 + min(8, int(9 * s.uniform()))
+            out.append({"kind": "rmat", "scale": scale, "seed": 3000 + g})
+    return out
+
+
+def exp_batch(ctx, G=256, check=True):
+    from oracle import oracle as O
+    specs = batch_specs(G)
+    gs = []
+    t0 = time.time()
+    for sp in specs:
+        if sp["kind"] == "er":
+            g = pac.Graph.er(ctx, sp["n"], sp["k"], seed=sp["seed"])
+        else:
+            g = pac.Graph.rmat(ctx, sp["scale"], 16, 0.57, 0.19, 0.19, seed=sp["seed"])
+        g.prepare(max(1, -(-g.get_vertex_num() // 4096)))
+        gs.append(g)
+    gen_s = time.time() - t0
+    m_total = sum(g.info().num_edges for g in gs)
+    tile, tile_ms = ctx.autotune(gs)
+    tot = ctx.tc_batch(gs, pac.DISPATCH_FIXED, pac.LAUNCH_PERSISTENT)
+    res = {"config": f"batch-{G}", "graphs": G, "m_total": m_total, "gen_s": round(gen_s, 1),
+           "tile": tile, "tile_ms": [round(x, 3) for x in tile_ms]}
+    for name, l in (("persistent", pac.LAUNCH_PERSISTENT), ("graph", pac.LAUNCH_GRAPH),
+                    ("plain_per_graph", pac.LAUNCH_PLAIN)):
+        v = []
+        for _ in range(5):
+            out = ctx.tc_batch(gs, pac.DISPATCH_FIXED, l)
+            assert np.array_equal(out, tot)
+            v.append(ctx.last_timing().total_ms)
+        res[f"{name}_ms"] = round(float(np.median(v)), 3)
+    res["speedup_batched_vs_per_graph"] = round(res["plain_per_graph_ms"] / res["graph_ms"], 3)
+    res["gteps_batched"] = round(m_total / (res["graph_ms"] * 1e-3) / 1e9, 4)
+    if check:
+        bad = 0
+        for g, t in zip(gs, tot):
+            vl, el = g.csr()
+            bad += int(O.tc_merge(O.CSR(vl, el), int(os.environ.get("OMP_NUM_THREADS", 8))) != int(t))
+        res["oracle_mismatches"] = bad
+    res["triangles_total"] = int(tot.sum())
+    return res
+
+
+def exp_jaccard(ctx, scale=25, partitions=4096, sample=200000, cpu_budget=15.0):
+    from oracle import oracle as O
+    g = pac.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed=scale)
+    g.prepare(partitions)
+    info = g.info()
+    res = {"config": f"rmat-s{scale}-ef16-P{partitions}-jaccard", "m": info.num_edges,
+           "W": info.work_W, "device_ms": {}}
+    for name, d, l in (("plain_fixed", pac.DISPATCH_FIXED, pac.LAUNCH_PLAIN),
+                       ("batched_fixed", pac.DISPATCH_FIXED, pac.LAUNCH_GRAPH),
+                       ("batched_bitmap", pac.FORCE_BITMAP, pac.LAUNCH_GRAPH)):
+        res["device_ms"][name] = round(median_ms(ctx, lambda: g.jaccard_edges_device(d, l), 3), 3)
+    best = min(res["device_ms"]["batched_fixed"], res["device_ms"]["batched_bitmap"])
+    res["gteps"] = round(info.num_edges / (best * 1e-3) / 1e9, 4)
+    res["speedup_batched_vs_plain_fixed"] = round(res["device_ms"]["plain_fixed"] / best, 3)
+    # parity on a sample of edges, and the CPU baseline on the same sample
+    src, dst, inter, uni, ratio = g.jaccard_edges(pac.DISPATCH_FIXED, pac.LAUNCH_PERSISTENT)
+    T = g.tc(pac.FORCE_BITMAP)
+    res["sum_inter_equals_3T"] = int(inter.sum(dtype=np.uint64)) == 3 * T
+    vl, el = g.csr()
+    ref = O.CSR(vl, el)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(src), size=min(sample, len(src)), replace=False)
+    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    ri, ru, rr = O.jaccard_edges(ref, src[idx], dst[idx], nt)
+    dt = time.perf_counter() - t0
+    res["sample_edges"] = int(len(idx))
+    res["sample_exact"] = bool(np.array_equal(ri, inter[idx]) and np.array_equal(ru, uni[idx])
+                               and np.array_equal(rr.view(np.uint32), ratio[idx].view(np.uint32)))
+    res["cpu_baseline"] = {"edges_per_s": round(len(idx) / dt, 1), "threads": nt,
+                           "est_full_s": round(dt * info.num_edges / len(idx), 1)}
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", choices=["scale22", "batch", "jaccard"])
+    ap.add_argument("--graphs", type=int, default=256)
+    ap.add_argument("--scale", type=int, default=None)
+    ap.add_argument("--no-check", action="store_true")
+    a = ap.parse_args(argv)
+    ctx = pac.Context(0)
+    if a.which == "scale22":
+        r = exp_scale22(ctx, a.scale or 22)
+    elif a.which == "batch":
+        r = exp_batch(ctx, a.graphs, not a.no_check)
+    else:
+        r = exp_jaccard(ctx, a.scale or 25)
+    print(json.dumps(r))
+
+
+if __name__ == "__main__":
+    main()

@@ -393,6 +393,11 @@ class Graph:
                                        _ptr(inter), _ptr(uni), _ptr(ratio)))
         return src, dst, inter, uni, ratio
 
+    def jaccard_edges_device(self, dispatch=DISPATCH_FIXED, launch=LAUNCH_PERSISTENT):
+        """Per-edge Jaccard with every result left in device memory."""
+        self._ensure_prepared()
+        _check(lib().pac_jaccard_edges(self._h, dispatch, launch, None, None, None, None, None))
+
     def cnt_common_pairs(self, u, v) -> np.ndarray:
         u = np.ascontiguousarray(u, np.int32)
         v = np.ascontiguousarray(v, np.int32)
