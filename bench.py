@@ -12,8 +12,8 @@ per-vertex counts), LCC epilogue -- the analogue of the reference's timed region
 
 The headline path is the batched persistent kernel replayed from a hipGraph with
 learned dispatch (the trained MLP in partitionandconquer_amd/dispatch_mlp.bin; the
-fixed heuristic when no model file exists).  The plain path (one launch per partition,
-fixed heuristic) is timed beside it for the speedup.  GTEPS = m / t / 1e9 with m the
+fixed heuristic when no model file exists).  The plain path (one launch per partition
+group, one workgroup per row, fixed heuristic) is timed beside it for the speedup.  GTEPS = m / t / 1e9 with m the
 undirected edge count.  The roofline entry prices the dominant kernel's algorithmic
 bytes 4*W, W = sum over oriented u->v of (d+(u) + d+(v)), against 8 TB/s.  The CPU
 baseline is the oracle's restatement of Graph::solve_triangle_counting_merge and
@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="under a profiler: a few non-captured headline launches and the "
+                         "stream-copy calibration kernel, nothing else")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {a.gpus}); "
@@ -120,6 +123,17 @@ def main():
     if learned:
         ctx.mlp_load(MODEL)
     dispatch = pac.DISPATCH_LEARNED if learned else pac.DISPATCH_FIXED
+
+    if a.profile_only:
+        for _ in range(max(1, a.steps)):
+            g.lcc_device(dispatch, pac.LAUNCH_PERSISTENT)
+            t = ctx.last_timing()
+            log(f"k_persistent kernel_ms={t.kernel_ms:.3f} total_ms={t.total_ms:.3f}")
+        copy_ms = ctx.stream_copy(1 << 30, 3)
+        log(f"k_stream_copy 1 GiB: {copy_ms:.3f} ms")
+        print(json.dumps({"profile_only": True, "W": W, "m": m,
+                          "kernel_ms": ctx.last_timing().kernel_ms}))
+        return
 
     # correctness guard on the timed configuration: all strategies agree
     T_ref = g.tc(pac.FORCE_BITMAP, pac.LAUNCH_PERSISTENT)

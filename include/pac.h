@@ -89,6 +89,8 @@ typedef struct {
 /* No reference counterpart (the reference is single-process OpenMP): binds a HIP
  * device and creates the context's stream. */
 int pac_ctx_create(int device, pac_ctx **out);
+/* Destroys the context and every graph still alive on it (their handles become
+ * invalid; do not pass them to pac_graph_free afterwards). */
 void pac_ctx_destroy(pac_ctx *ctx);
 /* Thread-local description of the last failure on this thread. */
 const char *pac_last_error(void);

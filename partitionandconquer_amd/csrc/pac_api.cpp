@@ -82,8 +82,17 @@ int new_graph(pac_ctx *ctx, pac_graph **out) {
     return PAC_ERR_OOM;
   }
   h->g.ctx = &ctx->c;
+  ctx->c.graphs.push_back(h);
   *out = h;
   return PAC_OK;
+}
+
+// Unregister and delete a graph (device buffers are released by its destructors).
+void delete_graph(pac_graph *h) {
+  Ctx *ctx = h->g.ctx;
+  auto &v = ctx->graphs;
+  v.erase(std::remove(v.begin(), v.end(), (void *)h), v.end());
+  delete h;
 }
 
 // Host validation of the boundary offsets, then upload, device validation and the
@@ -126,7 +135,7 @@ int upload_csr(pac_ctx *ctx, const uint64_t *vlist, uint64_t V, const int32_t *e
   if (rc == PAC_OK) rc = graph_validate(g, 0);
   if (rc == PAC_OK) rc = orient(g);
   if (rc != PAC_OK) {
-    delete h;
+    delete_graph(h);
     return rc;
   }
   *out = h;
@@ -136,7 +145,7 @@ int upload_csr(pac_ctx *ctx, const uint64_t *vlist, uint64_t V, const int32_t *e
 int finish_generated(pac_ctx *ctx, pac_graph *h, pac_graph **out) {
   int rc = orient(&h->g);
   if (rc != PAC_OK) {
-    delete h;
+    delete_graph(h);
     return rc;
   }
   (void)ctx;
@@ -474,6 +483,10 @@ void pac_ctx_destroy(pac_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->c.device);
   if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  // graphs still alive belong to this context: free them now (their handles die here)
+  std::vector<void *> live;
+  live.swap(ctx->c.graphs);
+  for (void *p : live) delete static_cast<pac_graph *>(p);
   if (ctx->c.hexec) (void)hipGraphExecDestroy(ctx->c.hexec);
   if (ctx->c.hgraph) (void)hipGraphDestroy(ctx->c.hgraph);
   for (auto &e : ctx->c.ev)
@@ -596,7 +609,7 @@ int pac_graph_generate_rmat(pac_ctx *ctx, uint32_t scale, uint32_t edge_factor, 
   PAC_TRY(new_graph(ctx, &h));
   int rc = generate_rmat(&h->g, scale, edge_factor, a, b, c, seed);
   if (rc != PAC_OK) {
-    delete h;
+    delete_graph(h);
     return rc;
   }
   return finish_generated(ctx, h, out);
@@ -713,7 +726,7 @@ void pac_graph_free(pac_graph *h) {
       ctx->hgraph_key.clear();
       break;
     }
-  delete h;
+  delete_graph(h);
 }
 
 int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p) {

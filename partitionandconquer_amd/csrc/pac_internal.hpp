@@ -139,6 +139,7 @@ struct Ctx {
   bool mlp_loaded = false;
   uint64_t mlp_version = 0;   // bumped whenever the parameters change
   std::vector<float> mlp_host;  // float copy (bf16-rounded)
+  std::vector<void *> graphs;  // live pac_graph handles, freed with the context
   // hipGraph cache for the batched persistent launch
   hipGraph_t hgraph = nullptr;
   hipGraphExec_t hexec = nullptr;

@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -147,9 +148,13 @@ class Context:
         _check(lib().pac_ctx_create(device, C.byref(h)))
         self._h = h
         self.device = device
+        self._graphs = weakref.WeakSet()
 
     def close(self):
+        """Destroy the context; graphs still alive on it are freed with it."""
         if getattr(self, "_h", None):
+            for g in list(self._graphs):
+                g._h = None
             lib().pac_ctx_destroy(self._h)
             self._h = None
 
@@ -258,6 +263,7 @@ class Graph:
         self.ctx = ctx
         self._h = handle
         self._prepared_P = 0
+        ctx._graphs.add(self)
 
     # ---- constructors ----
     @classmethod
@@ -294,9 +300,9 @@ class Graph:
         return cls(ctx, h)
 
     def free(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and getattr(self.ctx, "_h", None):
             lib().pac_graph_free(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
