@@ -136,24 +136,30 @@ def main():
         return
 
     # correctness guard on the timed configuration: all strategies agree
+    log("phase: reference count (bitmap, persistent)")
     T_ref = g.tc(pac.FORCE_BITMAP, pac.LAUNCH_PERSISTENT)
+    log("phase: headline path once (graph)")
     T_head = g.lcc_device(dispatch, pac.LAUNCH_GRAPH)
     assert T_head == T_ref, (T_head, T_ref)
 
     # headline: hipGraph-replayed persistent kernel, learned (or fixed) dispatch
+    log("phase: timed headline steps")
     wall, dev, _ = timed_steps(ctx, g, dispatch, pac.LAUNCH_GRAPH, a.steps, a.warmup)
     # same dispatch, non-captured persistent launch: kernel-only event timing
+    log("phase: kernel-only timing (persistent)")
     _, dev_p, ker = timed_steps(ctx, g, dispatch, pac.LAUNCH_PERSISTENT, max(3, a.steps // 2), 1)
     ms_step = wall
     gteps = m / (ms_step * 1e-3) / 1e9
     out = {}
     if not a.no_plain:
+        log("phase: plain fixed-heuristic path")
         wall_pf, dev_pf, _ = timed_steps(ctx, g, pac.DISPATCH_FIXED, pac.LAUNCH_PLAIN,
                                          max(3, a.steps // 2), 1)
         _, dev_bf, _ = timed_steps(ctx, g, pac.DISPATCH_FIXED, pac.LAUNCH_GRAPH, max(3, a.steps // 2), 1)
         out["plain_fixed_ms"] = round(wall_pf, 3)
         out["batched_fixed_device_ms"] = round(dev_bf, 3)
         out["speedup_vs_plain_fixed"] = round(wall_pf / ms_step, 3)
+    log("phase: stream copy")
     copy_bytes = 1 << 30
     copy_ms = ctx.stream_copy(copy_bytes, 5)
     stream_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
@@ -188,6 +194,7 @@ def main():
     }
     res.update(out)
     if not a.no_cpu:
+        log("phase: cpu baseline")
         vl, el = g.csr()
         est_s, rows, nt, spent = cpu_baseline(vl, el, a.cpu_budget)
         res["cpu_baseline"] = {

@@ -721,6 +721,7 @@ int run_plain_any(Ctx *ctx, const GraphDev &G, int mode, int strat, uint32_t row
   return run_plain<kModeTC>(ctx, G, strat, row_begin, nrows);
 }
 
+// Device descriptor of a graph; its output pointers must already be allocated.
 GraphDev make_desc(Graph *g, uint32_t part_base) {
   GraphDev d;
   d.ooff = g->ooff.as<const uint32_t>();
@@ -818,6 +819,8 @@ int compute_rowwork(Graph *g) {
 // same strategy (their rows are contiguous in the row list), one workgroup per row.
 int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
                  uint64_t *launches) {
+  Graph *gs[1] = {g};
+  PAC_TRY(alloc_outputs(gs, 1, mode));
   const GraphDev d = make_desc(g, 0);
   uint32_t k = 0;
   while (k < g->P) {
@@ -834,6 +837,8 @@ int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
 }
 
 int launch_partition(Ctx *ctx, Graph *g, int mode, uint32_t part, int strat) {
+  Graph *gs[1] = {g};
+  PAC_TRY(alloc_outputs(gs, 1, mode));
   const GraphDev d = make_desc(g, 0);
   const uint32_t b = g->h_pstart[part], e = g->h_pstart[part + 1];
   return run_plain_any(ctx, d, mode, strat, b, e - b);
@@ -842,6 +847,9 @@ int launch_partition(Ctx *ctx, Graph *g, int mode, uint32_t part, int strat) {
 int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
                       const uint8_t *strategy_per_part_concat, bool use_graph,
                       uint64_t *launches) {
+  // Output buffers must exist before the descriptors that point at them are built
+  // (on the hipGraph path nothing has allocated them yet).
+  PAC_TRY(alloc_outputs(gs, G, mode));
   std::vector<GraphDev> descs(G);
   uint64_t nparts = 0, nitems = 0;
   for (uint32_t i = 0; i < G; ++i) {
@@ -891,7 +899,6 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   }
   // hipGraph: zero-fill kernels + queue reset + persistent kernel, captured once per
   // (graph set, mode, tile, buffer addresses) and replayed.
-  PAC_TRY(alloc_outputs(gs, G, mode));  // allocate outside the capture
   key.push_back(items);
   key.push_back(ctx->gdesc.p);
   key.push_back(ctx->dispatch.p);

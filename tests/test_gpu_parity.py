@@ -296,3 +296,28 @@ def test_generator_properties_large(ctx, orc):
     tv = orc.lcc_merge_rows(ref, rows, NT)[1]
     l32, l64, _ = g.lcc()
     assert bits_equal(l64[rows], tv)
+
+
+@pytest.mark.parametrize("launch", [pac.LAUNCH_GRAPH, pac.LAUNCH_PERSISTENT, pac.LAUNCH_PLAIN])
+def test_first_solve_of_each_mode_on_a_fresh_graph(ctx, orc, launch):
+    """Regression: the first LCC / SUP solve of a graph must allocate its per-vertex and
+    per-edge outputs before the kernel descriptors that point at them are built (the
+    hipGraph path used to capture a descriptor with an unallocated output)."""
+    ref = orc.gen_rmat(12, 16, 0.57, 0.19, 0.19, seed=121)
+    T = orc.tc_merge(ref, NT)
+    lref = orc.lcc_merge(ref, NT)
+    g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    g.prepare(8)
+    assert g.lcc_device(pac.DISPATCH_FIXED, launch) == T
+    l32, l64, t = g.lcc(pac.DISPATCH_FIXED, launch)
+    assert t == T and bits_equal(l64, lref)
+    g2 = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    g2.prepare(8)
+    src, dst, inter, uni, ratio = g2.jaccard_edges(pac.DISPATCH_FIXED, launch)
+    ri, ru, rr = orc.jaccard_edges(ref, src, dst, NT)
+    assert np.array_equal(inter, ri) and bits_equal(ratio, rr)
+    g3 = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    g3.prepare(8)
+    g3.jaccard_edges_device(pac.FORCE_BITMAP, launch)
+    t3, pv = g3.tc(pac.FORCE_MERGE, launch, per_vertex=True)
+    assert t3 == T and int(pv.sum()) == 3 * T
