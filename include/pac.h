@@ -117,6 +117,12 @@ int pac_graph_load_csr_dir(pac_ctx *ctx, const char *csrdir, pac_graph **out);
  * list: '#'/'%' comments, one-character separator, 1-based IDs shifted when no 0
  * appears, self-loops dropped, symmetrized, rows sorted and deduplicated). */
 int pac_graph_load_text(pac_ctx *ctx, const char *path, pac_graph **out);
+/* Builds the graph from an undirected edge list on the device: both directions,
+ * self-loops dropped, duplicates removed, rows sorted (the row building of
+ * Graph.cpp:70-85, done by a device radix sort + unique + compaction).  num_vertices
+ * = 0 takes max ID + 1.  The text loader above parses on the host and uses this. */
+int pac_graph_from_edges(pac_ctx *ctx, const int32_t *src, const int32_t *dst, uint64_t ne,
+                         uint64_t num_vertices, pac_graph **out);
 /* Replaces Graph::save_as_csr_binfile, Graph.cpp:595-605 (same two files). */
 int pac_graph_save_csr_dir(const pac_graph *g, const char *csrdir);
 /* Seeded R-MAT generator on the GPU: ef*2^scale samples with quadrant probabilities

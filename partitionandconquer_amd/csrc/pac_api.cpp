@@ -59,18 +59,6 @@ int ensure_ctx(pac_ctx *ctx) {
   return PAC_OK;
 }
 
-// Build a symmetric CSR (uint64 offsets, int32 IDs) from directed keys (u<<32|v).
-void keys_to_csr(uint64_t V, std::vector<uint64_t> &keys, std::vector<uint64_t> &vl,
-                 std::vector<int32_t> &el) {
-  std::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  vl.assign(V + 1, 0);
-  el.resize(keys.size());
-  for (uint64_t k : keys) vl[(k >> 32) + 1]++;
-  for (uint64_t u = 0; u < V; ++u) vl[u + 1] += vl[u];
-  for (size_t i = 0; i < keys.size(); ++i) el[i] = (int32_t)(uint32_t)(keys[i] & 0xffffffffu);
-}
-
 int new_graph(pac_ctx *ctx, pac_graph **out) {
   if (!out) {
     set_error("null output handle");
@@ -134,6 +122,36 @@ int upload_csr(pac_ctx *ctx, const uint64_t *vlist, uint64_t V, const int32_t *e
   int rc = build_symmetric_from_host(g, vlist, elist);
   if (rc == PAC_OK) rc = graph_validate(g, 0);
   if (rc == PAC_OK) rc = orient(g);
+  if (rc != PAC_OK) {
+    delete_graph(h);
+    return rc;
+  }
+  *out = h;
+  return PAC_OK;
+}
+
+// Upload an undirected edge list and build the symmetric CSR on the device.
+int edges_to_graph(pac_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint64_t ne,
+                   uint64_t V, pac_graph **out) {
+  if (2 * ne >= (1ull << 32)) {
+    set_error("%llu edges exceed the uint32 CSR limit", (unsigned long long)ne);
+    return PAC_ERR_UNSUPPORTED;
+  }
+  pac_graph *h = nullptr;
+  PAC_TRY(new_graph(ctx, &h));
+  DevBuf ds, dd;
+  int rc = ds.reserve(4 * (ne ? ne : 1));
+  if (rc == PAC_OK) rc = dd.reserve(4 * (ne ? ne : 1));
+  if (rc == PAC_OK && ne) {
+    if (hipMemcpy(ds.p, src, 4 * ne, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dd.p, dst, 4 * ne, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("edge upload failed");
+      rc = PAC_ERR_HIP;
+    }
+  }
+  if (rc == PAC_OK)
+    rc = graph_from_device_edges(&h->g, ds.as<uint32_t>(), dd.as<uint32_t>(), ne, (uint32_t)V);
+  if (rc == PAC_OK) rc = orient(&h->g);
   if (rc != PAC_OK) {
     delete_graph(h);
     return rc;
@@ -573,18 +591,44 @@ int pac_graph_load_text(pac_ctx *ctx, const char *path, pac_graph **out) {
   }
   fclose(fp);
   const uint64_t V = zero_based ? maxid + 1 : maxid;
-  std::vector<uint64_t> keys;
-  keys.reserve(edges.size() * 2);
   const uint32_t shift = zero_based ? 0u : 1u;
-  for (auto &e : edges) {
-    const uint64_t a = e.first - shift, b = e.second - shift;
-    keys.push_back((a << 32) | b);
-    keys.push_back((b << 32) | a);
+  std::vector<uint32_t> src(edges.size()), dst(edges.size());
+  for (size_t i = 0; i < edges.size(); ++i) {
+    src[i] = edges[i].first - shift;
+    dst[i] = edges[i].second - shift;
   }
-  std::vector<uint64_t> vl;
-  std::vector<int32_t> el;
-  keys_to_csr(V, keys, vl, el);
-  return upload_csr(ctx, vl.data(), V, el.data(), el.size(), out);
+  std::vector<std::pair<uint32_t, uint32_t>>().swap(edges);
+  // symmetrize, sort and deduplicate on the device
+  return edges_to_graph(ctx, src.data(), dst.data(), src.size(), V, out);
+}
+
+int pac_graph_from_edges(pac_ctx *ctx, const int32_t *src, const int32_t *dst, uint64_t ne,
+                         uint64_t num_vertices, pac_graph **out) {
+  PAC_TRY(ensure_ctx(ctx));
+  if (ne && (!src || !dst)) {
+    set_error("null edge array");
+    return PAC_ERR_ARG;
+  }
+  uint64_t V = num_vertices;
+  int64_t mx = -1;
+  for (uint64_t i = 0; i < ne; ++i) {
+    if (src[i] < 0 || dst[i] < 0) {
+      set_error("negative vertex ID in edge %llu", (unsigned long long)i);
+      return PAC_ERR_FORMAT;
+    }
+    mx = std::max<int64_t>(mx, std::max(src[i], dst[i]));
+  }
+  if (V == 0) V = (uint64_t)(mx + 1);
+  if ((int64_t)V <= mx) {
+    set_error("edge endpoint %lld >= num_vertices %llu", (long long)mx, (unsigned long long)V);
+    return PAC_ERR_FORMAT;
+  }
+  if (V >= (1ull << 31)) {
+    set_error("%llu vertices exceed the int32 vertex ID range", (unsigned long long)V);
+    return PAC_ERR_UNSUPPORTED;
+  }
+  return edges_to_graph(ctx, reinterpret_cast<const uint32_t *>(src),
+                        reinterpret_cast<const uint32_t *>(dst), ne, V, out);
 }
 
 int pac_graph_save_csr_dir(const pac_graph *h, const char *csrdir) {

@@ -48,6 +48,29 @@ __global__ void k_rmat(uint64_t ns, uint32_t scale, uint64_t t1, uint64_t t2, ui
   }
 }
 
+// Directed keys (u << shift | v) of both directions of each input edge; self-loops get
+// the sentinel, IDs >= V set the error flag (and the sentinel).
+__global__ void k_edge_keys(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst,
+                            uint64_t ne, uint32_t shift, uint32_t V,
+                            uint64_t *__restrict__ keys, uint32_t *err) {
+  const uint64_t sentinel = 1ull << (2 * shift);
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ne;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = src[i], v = dst[i];
+    if (u >= V || v >= V) {
+      atomicOr(err, 1u);
+      keys[2 * i] = sentinel;
+      keys[2 * i + 1] = sentinel;
+    } else if (u == v) {
+      keys[2 * i] = sentinel;
+      keys[2 * i + 1] = sentinel;
+    } else {
+      keys[2 * i] = ((uint64_t)u << shift) | v;
+      keys[2 * i + 1] = ((uint64_t)v << shift) | u;
+    }
+  }
+}
+
 __global__ void k_unique_flags(const uint64_t *__restrict__ keys, uint64_t n, uint64_t sentinel,
                                uint32_t *__restrict__ flags) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -99,6 +122,63 @@ uint32_t grid1d(uint64_t n) {
 
 }  // namespace
 
+// Sort the n2 directed keys in keys_a (key = u << shift | v, sentinel 1 << 2*shift),
+// drop duplicates and sentinels, and build the symmetric CSR of V vertices in g.
+int csr_from_keys_device(Graph *g, DevBuf &keys_a, uint64_t n2, uint32_t shift, uint32_t V) {
+  Ctx *ctx = g->ctx;
+  const uint64_t sentinel = 1ull << (2 * shift);
+  DevBuf keys_b, flags, pos, tmp;
+  PAC_TRY(keys_b.reserve(sizeof(uint64_t) * (n2 ? n2 : 1)));
+  uint64_t nnz = 0;
+  if (n2) {
+    size_t tb = 0;
+    PAC_CHECK_HIP(rocprim::radix_sort_keys(nullptr, tb, keys_a.as<uint64_t>(),
+                                           keys_b.as<uint64_t>(), n2, 0u, 2 * shift + 1,
+                                           ctx->stream));
+    PAC_TRY(tmp.reserve(tb));
+    PAC_CHECK_HIP(rocprim::radix_sort_keys(tmp.p, tb, keys_a.as<uint64_t>(),
+                                           keys_b.as<uint64_t>(), n2, 0u, 2 * shift + 1,
+                                           ctx->stream));
+    PAC_TRY(flags.reserve(sizeof(uint32_t) * n2));
+    PAC_TRY(pos.reserve(sizeof(uint32_t) * n2));
+    hipLaunchKernelGGL(k_unique_flags, dim3(grid1d(n2)), dim3(256), 0, ctx->stream,
+                       keys_b.as<const uint64_t>(), n2, sentinel, flags.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+    size_t sb = 0;
+    PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, sb, flags.as<uint32_t>(), pos.as<uint32_t>(),
+                                          0u, (size_t)n2, rocprim::plus<uint32_t>(), ctx->stream));
+    if (sb > tmp.bytes) {
+      tmp.release();
+      PAC_TRY(tmp.reserve(sb));
+    }
+    PAC_CHECK_HIP(rocprim::exclusive_scan(tmp.p, sb, flags.as<uint32_t>(), pos.as<uint32_t>(),
+                                          0u, (size_t)n2, rocprim::plus<uint32_t>(), ctx->stream));
+    uint32_t last[2] = {0, 0};
+    PAC_CHECK_HIP(hipMemcpyAsync(&last[0], pos.as<uint32_t>() + (n2 - 1), 4,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    PAC_CHECK_HIP(hipMemcpyAsync(&last[1], flags.as<uint32_t>() + (n2 - 1), 4,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    nnz = (uint64_t)last[0] + last[1];
+  }
+  g->V = V;
+  g->nnz = nnz;
+  PAC_TRY(g->adj.reserve(sizeof(uint32_t) * (nnz ? nnz : 1)));
+  PAC_TRY(g->off.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  if (n2) {
+    // compacted keys go to keys_a (free after the sort)
+    hipLaunchKernelGGL(k_compact, dim3(grid1d(n2)), dim3(256), 0, ctx->stream,
+                       keys_b.as<const uint64_t>(), flags.as<const uint32_t>(),
+                       pos.as<const uint32_t>(), n2, (uint32_t)((1ull << shift) - 1),
+                       g->adj.as<uint32_t>(), keys_a.as<uint64_t>());
+  }
+  hipLaunchKernelGGL(k_offsets, dim3(grid1d((uint64_t)V + 1)), dim3(256), 0, ctx->stream,
+                     keys_a.as<const uint64_t>(), nnz, shift, V, g->off.as<uint32_t>());
+  PAC_CHECK_HIP(hipGetLastError());
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return PAC_OK;
+}
+
 int generate_rmat(Graph *g, uint32_t scale, uint32_t ef, double a, double b, double c,
                   uint64_t seed) {
   Ctx *ctx = g->ctx;
@@ -123,58 +203,38 @@ int generate_rmat(Graph *g, uint32_t scale, uint32_t ef, double a, double b, dou
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   const uint64_t key = z ^ (z >> 31);
-  const uint32_t V = 1u << scale;
-  const uint64_t sentinel = 1ull << (2 * scale);
-
-  DevBuf keys_a, keys_b, flags, pos, tmp;
+  DevBuf keys_a;
   PAC_TRY(keys_a.reserve(sizeof(uint64_t) * n2));
-  PAC_TRY(keys_b.reserve(sizeof(uint64_t) * n2));
   hipLaunchKernelGGL(k_rmat, dim3(grid1d(ns)), dim3(256), 0, ctx->stream, ns, scale, t1, t2, t3,
                      key, keys_a.as<uint64_t>());
   PAC_CHECK_HIP(hipGetLastError());
-  size_t tb = 0;
-  PAC_CHECK_HIP(rocprim::radix_sort_keys(nullptr, tb, keys_a.as<uint64_t>(),
-                                         keys_b.as<uint64_t>(), n2, 0u, 2 * scale + 1,
-                                         ctx->stream));
-  PAC_TRY(tmp.reserve(tb));
-  PAC_CHECK_HIP(rocprim::radix_sort_keys(tmp.p, tb, keys_a.as<uint64_t>(),
-                                         keys_b.as<uint64_t>(), n2, 0u, 2 * scale + 1,
-                                         ctx->stream));
-  PAC_TRY(flags.reserve(sizeof(uint32_t) * n2));
-  PAC_TRY(pos.reserve(sizeof(uint32_t) * n2));
-  hipLaunchKernelGGL(k_unique_flags, dim3(grid1d(n2)), dim3(256), 0, ctx->stream,
-                     keys_b.as<const uint64_t>(), n2, sentinel, flags.as<uint32_t>());
-  PAC_CHECK_HIP(hipGetLastError());
-  size_t sb = 0;
-  PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, sb, flags.as<uint32_t>(), pos.as<uint32_t>(),
-                                        0u, (size_t)n2, rocprim::plus<uint32_t>(), ctx->stream));
-  if (sb > tmp.bytes) {
-    tmp.release();
-    PAC_TRY(tmp.reserve(sb));
+  return csr_from_keys_device(g, keys_a, n2, scale, 1u << scale);
+}
+
+int graph_from_device_edges(Graph *g, const uint32_t *d_src, const uint32_t *d_dst, uint64_t ne,
+                            uint32_t V) {
+  Ctx *ctx = g->ctx;
+  uint32_t shift = 1;
+  while ((1ull << shift) < (uint64_t)V) ++shift;
+  const uint64_t n2 = 2 * ne;
+  DevBuf keys_a;
+  PAC_TRY(keys_a.reserve(sizeof(uint64_t) * (n2 ? n2 : 1)));
+  PAC_TRY(ctx->scratch2.reserve(64));
+  uint32_t *err = ctx->scratch2.as<uint32_t>();
+  PAC_TRY(fill_u32(ctx->stream, err, 1, 0u));
+  if (ne) {
+    hipLaunchKernelGGL(k_edge_keys, dim3(grid1d(ne)), dim3(256), 0, ctx->stream, d_src, d_dst, ne,
+                       shift, V, keys_a.as<uint64_t>(), err);
+    PAC_CHECK_HIP(hipGetLastError());
   }
-  PAC_CHECK_HIP(rocprim::exclusive_scan(tmp.p, sb, flags.as<uint32_t>(), pos.as<uint32_t>(), 0u,
-                                        (size_t)n2, rocprim::plus<uint32_t>(), ctx->stream));
-  uint32_t last[2] = {0, 0};
-  PAC_CHECK_HIP(hipMemcpyAsync(&last[0], pos.as<uint32_t>() + (n2 - 1), 4, hipMemcpyDeviceToHost,
-                               ctx->stream));
-  PAC_CHECK_HIP(hipMemcpyAsync(&last[1], flags.as<uint32_t>() + (n2 - 1), 4,
-                               hipMemcpyDeviceToHost, ctx->stream));
+  uint32_t h = 0;
+  PAC_CHECK_HIP(hipMemcpyAsync(&h, err, 4, hipMemcpyDeviceToHost, ctx->stream));
   PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  const uint64_t nnz = (uint64_t)last[0] + last[1];
-  g->V = V;
-  g->nnz = nnz;
-  PAC_TRY(g->adj.reserve(sizeof(uint32_t) * (nnz ? nnz : 1)));
-  PAC_TRY(g->off.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
-  // compacted keys go to keys_a (free after the sort)
-  hipLaunchKernelGGL(k_compact, dim3(grid1d(n2)), dim3(256), 0, ctx->stream,
-                     keys_b.as<const uint64_t>(), flags.as<const uint32_t>(),
-                     pos.as<const uint32_t>(), n2, (uint32_t)(V - 1), g->adj.as<uint32_t>(),
-                     keys_a.as<uint64_t>());
-  hipLaunchKernelGGL(k_offsets, dim3(grid1d((uint64_t)V + 1)), dim3(256), 0, ctx->stream,
-                     keys_a.as<const uint64_t>(), nnz, scale, V, g->off.as<uint32_t>());
-  PAC_CHECK_HIP(hipGetLastError());
-  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  return PAC_OK;
+  if (h) {
+    set_error("edge endpoint >= number of vertices (%u)", V);
+    return PAC_ERR_FORMAT;
+  }
+  return csr_from_keys_device(g, keys_a, n2, shift, V);
 }
 
 }  // namespace pac

@@ -155,6 +155,9 @@ int graph_validate(const Graph *g, int level);
 // ---- gen.hip ---------------------------------------------------------------------
 int generate_rmat(Graph *g, uint32_t scale, uint32_t ef, double a, double b, double c,
                   uint64_t seed);
+// Symmetric, deduplicated, self-loop-free CSR from an undirected edge list on the device.
+int graph_from_device_edges(Graph *g, const uint32_t *d_src, const uint32_t *d_dst, uint64_t ne,
+                            uint32_t V);
 
 // ---- isect.hip -------------------------------------------------------------------
 // Allocate / zero the outputs of `mode` for each graph.

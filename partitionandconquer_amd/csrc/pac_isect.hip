@@ -85,15 +85,16 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;     // empty hash-table slot (IDs are <
 // sits outside the union and is all-zero between bitmap rows.
 template <int CAP, int NBITS, int MODE>
 struct Smem {
+  static constexpr uint32_t kTabCap = CAP + CAP / 2;  // hash slots: load factor <= 2/3
   union {
     struct {
       uint32_t A[CAP];
       uint32_t cnt[MODE == kModeTC ? 1 : CAP];
     } s;  // merge / gallop
     struct {
-      uint32_t tab[2 * CAP];
-      uint32_t cnt16[MODE == kModeTC ? 1 : CAP];   // 2 x u16 per word
-      uint32_t pos16[MODE == kModeSUP ? CAP : 1];  // 2 x u16 per word
+      uint32_t tab[kTabCap];
+      uint32_t cnt16[MODE == kModeTC ? 1 : kTabCap / 2];   // 2 x u16 per word
+      uint32_t pos16[MODE == kModeSUP ? kTabCap / 2 : 1];  // 2 x u16 per word
     } h;  // bitmap
   } u;
   uint32_t bits[NBITS / 32];
@@ -113,15 +114,20 @@ __device__ __forceinline__ uint32_t get16(const uint32_t *w, uint32_t slot) {
   return (w[slot >> 1] >> (16u * (slot & 1u))) & 0xFFFFu;
 }
 
-// Slot of x in the table (kEmpty if absent).
-__device__ __forceinline__ uint32_t tab_find(const uint32_t *tab, uint32_t tbits, uint32_t x) {
-  const uint32_t mask = (1u << tbits) - 1u;
-  uint32_t h = (x * kTabMul) >> (32u - tbits);
+// Home slot of x in a table of tsize slots: high half of (hash * tsize).
+__device__ __forceinline__ uint32_t tab_home(uint32_t x, uint32_t tsize) {
+  return (uint32_t)(((uint64_t)(x * kTabMul) * tsize) >> 32);
+}
+
+// Slot of x in the table (kEmpty if absent); linear probing with wrap-around.  The
+// table always has empty slots (load factor <= 2/3), so the probe ends.
+__device__ __forceinline__ uint32_t tab_find(const uint32_t *tab, uint32_t tsize, uint32_t x) {
+  uint32_t h = tab_home(x, tsize);
   while (true) {
     const uint32_t k = tab[h];
     if (k == x) return h;
     if (k == kEmpty) return kEmpty;
-    h = (h + 1u) & mask;
+    h = (h + 1u == tsize) ? 0u : h + 1u;
   }
 }
 
@@ -135,42 +141,6 @@ __device__ __forceinline__ void on_hit(uint32_t *cnt, uint32_t i, uint32_t *sup,
 // ---- the three lookup engines: one wave, one middle vertex c ------------------------
 // B = oadj[b0 .. b0+nb): sorted N+(c) in HBM.  Each returns this lane's number of
 // confirmed common neighbours.
-
-// Bitmap: one LDS read of the hashed filter bit; a set bit is confirmed (and its
-// counter slot found) in the LDS hash table of A.  Lists are consumed in chunks of
-// 256 elements (4 per lane).
-__device__ __forceinline__ void load_chunk(const uint32_t *__restrict__ oadj, uint32_t b0,
-                                           uint32_t nb, uint32_t j0, uint32_t w[4]) {
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const uint32_t j = j0 + 64u * u;
-    w[u] = j < nb ? oadj[b0 + j] : 0u;
-  }
-}
-
-template <int MODE, int NBITS>
-__device__ __forceinline__ uint32_t probe_chunk(const uint32_t w[4], uint32_t j0, uint32_t nb,
-                                                const uint32_t *bits, const uint32_t *tab,
-                                                uint32_t tbits, uint32_t *cnt16, uint32_t b0,
-                                                uint32_t *sup) {
-  uint32_t hits = 0;
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const uint32_t j = j0 + 64u * u;
-    if (j < nb) {
-      const uint32_t hb = bit_of<NBITS>(w[u]);
-      if ((bits[hb >> 5] >> (hb & 31u)) & 1u) {
-        const uint32_t slot = tab_find(tab, tbits, w[u]);
-        if (slot != kEmpty) {
-          ++hits;
-          if (MODE != kModeTC) add16(cnt16, slot, 1u);
-          if (MODE == kModeSUP) atomicAdd(&sup[b0 + j], 1u);
-        }
-      }
-    }
-  }
-  return hits;
-}
 
 template <int MODE>
 __device__ __forceinline__ uint32_t probe_gallop(const uint32_t *__restrict__ oadj,
@@ -334,30 +304,37 @@ __device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32
 }
 
 // ---- one row, bitmap strategy: hashed filter bitmap + LDS hash table of A -------------
+// The table has tsize = max(64, min(2*nA, TABCAP)) slots (load factor <= 2/3), indexed
+// by the high half of (hash * tsize).  A wave takes its middle vertices in batches of
+// 64 (lane l loads c_l, the bounds of N+(c_l) and the table slot of c_l), then streams
+// the concatenation of the batch's 64 lists 256 elements at a time, dense across
+// lanes and coalesced; each lane finds the list its element belongs to by walking the
+// batch's inclusive prefix sums (held one per lane, read with ds_bpermute), which only
+// moves forward.
 template <int MODE, int WAVES, int CAP, int NBITS>
 __device__ __forceinline__ void row_bitmap(const GraphDev &G, uint32_t r, uint32_t c_lo,
                                            uint32_t c_hi, Smem<CAP, NBITS, MODE> &sm,
                                            unsigned long long &acc) {
   constexpr uint32_t BS = 64u * WAVES;
+  constexpr uint32_t TABCAP = Smem<CAP, NBITS, MODE>::kTabCap;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
   uint32_t *tab = sm.u.h.tab;
   uint32_t *cnt16 = sm.u.h.cnt16;
   uint32_t *pos16 = sm.u.h.pos16;
-  // table of 2^tbits >= 2*nA slots (load factor <= 1/2), at least 64; nA <= CAP
-  uint32_t tbits = 6;
-  while ((1u << tbits) < 2u * nA) ++tbits;
-  const uint32_t tsize = 1u << tbits, tmask = tsize - 1u;
+  uint32_t tsize = 2u * nA;
+  if (tsize > TABCAP) tsize = TABCAP;
+  if (tsize < 64u) tsize = 64u;
   for (uint32_t i = tid; i < tsize; i += BS) tab[i] = kEmpty;
   if (MODE != kModeTC)
-    for (uint32_t i = tid; i < tsize / 2; i += BS) cnt16[i] = 0;
+    for (uint32_t i = tid; i < (tsize + 1) / 2; i += BS) cnt16[i] = 0;
   if (MODE == kModeSUP)
-    for (uint32_t i = tid; i < tsize / 2; i += BS) pos16[i] = 0;
+    for (uint32_t i = tid; i < (tsize + 1) / 2; i += BS) pos16[i] = 0;
   __syncthreads();
   for (uint32_t i = tid; i < nA; i += BS) {
     const uint32_t w = G.oadj[a0 + i];
-    uint32_t h = (w * kTabMul) >> (32u - tbits);
-    while (atomicCAS(&tab[h], kEmpty, w) != kEmpty) h = (h + 1u) & tmask;
+    uint32_t h = tab_home(w, tsize);
+    while (atomicCAS(&tab[h], kEmpty, w) != kEmpty) h = (h + 1u == tsize) ? 0u : h + 1u;
     if (MODE == kModeSUP) add16(pos16, h, i);
     const uint32_t hb = bit_of<NBITS>(w);
     atomicOr(&sm.bits[hb >> 5], 1u << (hb & 31u));
@@ -365,35 +342,66 @@ __device__ __forceinline__ void row_bitmap(const GraphDev &G, uint32_t r, uint32
   __syncthreads();
   if (c_hi > nA) c_hi = nA;
   uint32_t row_hits = 0;
-  // Batches of 64 middle vertices per wave (offsets fetched in one round trip); within
-  // a batch the first chunk of the next list is loaded before the current one is
-  // probed, so two chunks per lane are in flight.
   for (uint32_t t0 = 0;; t0 += 64) {
     const uint32_t kl = c_lo + wave + (uint32_t)WAVES * (t0 + lane);
-    uint32_t cl = 0, bl = 0, nl = 0;
+    uint32_t bl = 0, nl = 0, sc = 0;
     if (kl < c_hi) {
-      cl = G.oadj[a0 + kl];
+      const uint32_t cl = G.oadj[a0 + kl];
       bl = G.ooff[cl];
       nl = G.ooff[cl + 1] - bl;
+      if (MODE != kModeTC) sc = tab_find(tab, tsize, cl);
     }
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
-    uint32_t wn[4] = {0u, 0u, 0u, 0u};
-    if (nvalid) load_chunk(G.oadj, readlane(bl, 0), readlane(nl, 0), lane, wn);
-    for (uint32_t jj = 0; jj < nvalid; ++jj) {
-      const uint32_t c = readlane(cl, jj), b0 = readlane(bl, jj), nb = readlane(nl, jj);
-      uint32_t w[4] = {wn[0], wn[1], wn[2], wn[3]};
-      if (jj + 1 < nvalid) load_chunk(G.oadj, readlane(bl, jj + 1), readlane(nl, jj + 1), lane, wn);
-      uint32_t h = probe_chunk<MODE, NBITS>(w, lane, nb, sm.bits, tab, tbits, cnt16, b0, G.sup);
-      for (uint32_t j0 = lane + 256u; j0 < nb; j0 += 256u) {
-        load_chunk(G.oadj, b0, nb, j0, w);
-        h += probe_chunk<MODE, NBITS>(w, j0, nb, sm.bits, tab, tbits, cnt16, b0, G.sup);
-      }
-      if (MODE != kModeTC) {
-        const uint32_t ch = wave_sum(h);
-        if (lane == 0 && ch) add16(cnt16, tab_find(tab, tbits, c), ch);
-      }
-      row_hits += h;
+    // inclusive prefix of the list lengths over the batch's lanes
+    uint32_t incl = nl;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o, 64);
+      if (lane >= (uint32_t)o) incl += v;
     }
+    const uint32_t total = readlane(incl, 63);
+    const uint32_t base = bl - (incl - nl);  // flat index q of list l sits at oadj[base_l + q]
+    uint32_t cur = 0;  // owner walk, monotone in q; always < 64 (see below)
+    uint32_t hits = 0;
+    for (uint32_t q0 = 0; q0 < total; q0 += 256u) {  // wave-uniform loop
+      uint32_t w[4], addr[4], so[4];
+      bool ok[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t q = q0 + 64u * u + lane;
+        ok[u] = q < total;
+        // The owner is the first lane whose inclusive prefix exceeds q.  Lanes past the
+        // end use q = total-1, so every walk stops at a lane <= 63 (incl[63] = total).
+        // Every shuffle runs with all 64 lanes active (bpermute reads of inactive lanes
+        // are not defined), and the walk continues while any lane still has to advance.
+        const uint32_t qc = ok[u] ? q : total - 1u;
+        while (true) {
+          const bool adv = (uint32_t)__shfl(incl, (int)cur, 64) <= qc;
+          if (!__any(adv)) break;
+          cur += adv ? 1u : 0u;
+        }
+        addr[u] = (uint32_t)__shfl(base, (int)cur, 64) + qc;
+        so[u] = (MODE != kModeTC) ? (uint32_t)__shfl(sc, (int)cur, 64) : 0u;
+        w[u] = ok[u] ? G.oadj[addr[u]] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!ok[u]) continue;
+        const uint32_t hb = bit_of<NBITS>(w[u]);
+        if ((sm.bits[hb >> 5] >> (hb & 31u)) & 1u) {
+          const uint32_t slot = tab_find(tab, tsize, w[u]);
+          if (slot != kEmpty) {
+            ++hits;
+            if (MODE != kModeTC) {
+              add16(cnt16, slot, 1u);   // t(w) / edge r->w
+              add16(cnt16, so[u], 1u);  // t(c) / edge r->c
+            }
+            if (MODE == kModeSUP) atomicAdd(&G.sup[addr[u]], 1u);  // edge c->w
+          }
+        }
+      }
+    }
+    row_hits += hits;
     if (nvalid < 64) break;
   }
   acc += row_hits;

@@ -14,6 +14,8 @@
 //      rows stay sorted by ID and no re-sort is needed.
 #include <rocprim/device/device_scan.hpp>
 
+#include <algorithm>
+
 #include "pac_internal.hpp"
 
 namespace pac {
@@ -45,7 +47,14 @@ __global__ void k_deg(const uint32_t *__restrict__ off, uint32_t V, uint32_t *__
     mx = max(mx, d);
   }
   for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, 64));
-  if ((threadIdx.x & 63u) == 0) atomicMax(maxdeg, mx);
+  // one global atomic per block (per-wave atomics on one address serialise)
+  __shared__ uint32_t wmax[16];
+  if ((threadIdx.x & 63u) == 0) wmax[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) mx = max(mx, wmax[w]);
+    if (mx) atomicMax(maxdeg, mx);
+  }
 }
 
 // Error bits: 1 ID >= V, 2 row not strictly increasing, 4 self-loop, 8 asymmetric.
@@ -187,7 +196,8 @@ int orient(Graph *g) {
   PAC_TRY(fill_u32(ctx->stream, maxdeg, 1, 0u));
   PAC_TRY(fill_u32(ctx->stream, g->odeg.as<uint32_t>(), (uint64_t)V + 1, 0u));
   if (V) {
-    hipLaunchKernelGGL(k_deg, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_deg, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(4096u, (V + 255) / 256))),
+                       dim3(256), 0, ctx->stream,
                        g->off.as<const uint32_t>(), V, g->deg.as<uint32_t>(), maxdeg);
     hipLaunchKernelGGL(k_outdeg, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
                        g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),

@@ -78,6 +78,7 @@ def lib():
         "pac_graph_load_csr_dir": ([vp, C.c_char_p, C.POINTER(vp)], C.c_int),
         "pac_graph_load_text": ([vp, C.c_char_p, C.POINTER(vp)], C.c_int),
         "pac_graph_save_csr_dir": ([vp, C.c_char_p], C.c_int),
+        "pac_graph_from_edges": ([vp, vp, vp, C.c_uint64, C.c_uint64, C.POINTER(vp)], C.c_int),
         "pac_graph_generate_rmat": ([vp, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                      C.c_double, C.c_uint64, C.POINTER(vp)], C.c_int),
         "pac_graph_generate_er": ([vp, C.c_uint64, C.c_double, C.c_uint64, C.POINTER(vp)], C.c_int),
@@ -272,6 +273,16 @@ class Graph:
         el = np.ascontiguousarray(elist, np.int32)
         h = vp()
         _check(lib().pac_graph_upload(ctx._h, _ptr(vl), len(vl) - 1, _ptr(el), len(el), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_edges(cls, ctx: Context, src, dst, num_vertices: int = 0) -> "Graph":
+        """Undirected edge list -> symmetric CSR, built on the device."""
+        s = np.ascontiguousarray(src, np.int32)
+        d = np.ascontiguousarray(dst, np.int32)
+        h = vp()
+        _check(lib().pac_graph_from_edges(ctx._h, _ptr(s), _ptr(d), len(s), num_vertices,
+                                          C.byref(h)))
         return cls(ctx, h)
 
     @classmethod

@@ -321,3 +321,27 @@ def test_first_solve_of_each_mode_on_a_fresh_graph(ctx, orc, launch):
     g3.jaccard_edges_device(pac.FORCE_BITMAP, launch)
     t3, pv = g3.tc(pac.FORCE_MERGE, launch, per_vertex=True)
     assert t3 == T and int(pv.sum()) == 3 * T
+
+
+def test_edge_list_ingestion_on_device(ctx, orc):
+    """Device symmetrize / sort / dedup of a raw edge list with duplicates, reversed
+    duplicates and self-loops equals the host construction (Graph.cpp:70-85)."""
+    rng = np.random.default_rng(9)
+    V = 5000
+    src = rng.integers(0, V, 60000).astype(np.int32)
+    dst = (src + rng.integers(0, 40, 60000)) % V          # includes self-loops (offset 0)
+    src = np.concatenate([src, dst[:5000], src[:3000]]).astype(np.int32)
+    dst = np.concatenate([dst, src[:5000], dst[:3000]]).astype(np.int32)
+    ref = orc.edges_to_csr(V, src.astype(np.uint32), dst.astype(np.uint32))
+    g = pac.Graph.from_edges(ctx, src, dst, V)
+    vl, el = g.csr()
+    assert np.array_equal(vl, ref.vlist) and np.array_equal(el, ref.elist)
+    g.prepare(4)
+    assert g.tc() == orc.tc_merge(ref, NT)
+    g2 = pac.Graph.from_edges(ctx, src, dst)                  # V inferred = max ID + 1
+    assert g2.get_vertex_num() == int(max(src.max(), dst.max())) + 1
+    with pytest.raises(pac.PacError) as e:
+        pac.Graph.from_edges(ctx, src, dst, 10)
+    assert e.value.code == -4
+    empty = pac.Graph.from_edges(ctx, np.zeros(0, np.int32), np.zeros(0, np.int32), 7)
+    assert empty.get_vertex_num() == 7 and empty.get_edge_num() == 0

@@ -1,0 +1,54 @@
+"""CPU tests of host-side logic that needs no GPU: the seeded workload specifications of
+the batch and learned-dispatch experiments, and the bench's one-GPU contract."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_batch_specs_follow_the_config():
+    from partitionandconquer_amd.experiments import batch_specs
+    s = batch_specs(256)
+    assert s == batch_specs(256)                       # deterministic
+    er = [x for x in s if x["kind"] == "er"]
+    rm = [x for x in s if x["kind"] == "rmat"]
+    assert len(er) == 128 and len(rm) == 128
+    assert all(2 ** 10 - 1 <= x["n"] <= 2 ** 18 + 1 for x in er)
+    assert all(8 <= x["k"] <= 64 for x in er)
+    assert all(10 <= x["scale"] <= 18 for x in rm)
+    assert [x["seed"] for x in s] == list(range(3000, 3256))
+    # log-uniform sizes spread over the whole range
+    logs = np.log2([x["n"] for x in er])
+    assert logs.min() < 11 and logs.max() > 17
+    assert len({x["scale"] for x in rm}) == 9
+
+
+def test_training_specs_follow_the_config():
+    from partitionandconquer_amd.train_dispatch import training_specs
+    s = training_specs(64)
+    assert s == training_specs(64)
+    assert [x["seed"] for x in s] == list(range(4000, 4064))
+    assert [x["scale"] for x in s] == [18 + i % 5 for i in range(64)]
+    for x in s:
+        assert 0.45 <= x["a"] <= 0.65
+        assert abs(x["a"] + x["b"] + x["c"] + 0.05 - 1.0) < 1e-12 and x["b"] == x["c"]
+
+
+def test_labels_and_scores():
+    from partitionandconquer_amd.train_dispatch import labels_weights, score
+    ms = np.array([[3.0, 2.0, 1.0], [1.0, 5.0, 2.0], [2.0, 2.0, 2.0]])
+    y, w = labels_weights(ms)
+    assert y.tolist() == [2, 0, 0]
+    assert np.isclose(w.mean(), 1.0) and w[2] == 0.0
+    sc = score(np.array([2, 1, 0]), ms)
+    assert sc["mispredict"] == 1 / 3
+    assert np.isclose(sc["regret"], (0 + 4 + 0) / (1 + 1 + 2))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "8"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
