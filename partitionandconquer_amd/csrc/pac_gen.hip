@@ -1,6 +1,12 @@
-// pac_gen.hip -- seeded R-MAT generation straight into the device CSR.
+// pac_gen.hip -- device construction of the symmetric CSR: from a seeded R-MAT
+// generator, or from an uploaded undirected edge list.
 //
-// No reference counterpart: the reference only reads graphs from files
+// Edge lists: both directions of every edge become 64-bit keys (u << shift | v,
+// self-loops -> sentinel), a device radix sort + unique + compaction yields the sorted,
+// duplicate-free rows -- the row building of Graph::Graph(graphfile), Graph.cpp:70-85,
+// done on the GPU.
+//
+// R-MAT has no reference counterpart: the reference only reads graphs from files
 // (Graph.cpp:27-137).  The generator is the synthetic-input side of the benchmarks:
 // ef*2^scale samples, each drawing `scale` quadrants with probabilities (a,b,c,d)
 // from a SplitMix64 stream (Steele, Lea, Flood 2014) seeded by a counter-based hash
