@@ -382,38 +382,43 @@ void greedy_partition_host(const std::vector<uint64_t> &vl, const std::vector<in
 }  // namespace
 
 // ---- fixed heuristic and features ---------------------------------------------------
-// Feature vector of a partition (16 floats, log-scaled then divided by 16).
+// Feature vector of a partition (16 floats, log-scaled then divided by 16).  Per row x:
+// a = d+(x) (the list staged in LDS), p = x's pairs, w = the total length streamed for
+// them; mean_b = w/p per pair.
 void partition_features(Graph *g, const std::vector<uint32_t> &h_odeg,
+                        const std::vector<uint32_t> &h_npairs,
                         const std::vector<uint32_t> &h_work, const std::vector<uint32_t> &h_deg,
                         const std::vector<uint32_t> &rowlist) {
   g->h_features.assign((size_t)g->P * PAC_MLP_IN, 0.f);
   g->h_fixed.assign(g->P, (uint8_t)kBitmap);
   for (uint32_t k = 0; k < g->P; ++k) {
-    double n = 0, sa = 0, sa2 = 0, maxa = 0, sd = 0, maxd = 0, sb = 0, maxw = 0, big = 0,
-           small = 0, gal = 0;
+    double n = 0, sa = 0, sa2 = 0, maxa = 0, sd = 0, maxd = 0, sp = 0, sap = 0, sb = 0,
+           maxw = 0, big = 0, small = 0, gal = 0;
     for (uint32_t q = g->h_pstart[k]; q < g->h_pstart[k + 1]; ++q) {
       const uint32_t r = rowlist[q];
-      const double a = h_odeg[r], w = h_work[r], d = h_deg[r];
+      const double a = h_odeg[r], p = h_npairs[r], w = h_work[r], d = h_deg[r];
       n += 1;
       sa += a;
       sa2 += a * a;
       maxa = std::max(maxa, a);
       sd += d;
       maxd = std::max(maxd, d);
+      sp += p;
+      sap += a * p;
       sb += w;
       maxw = std::max(maxw, w);
       big += a > 64 ? 1 : 0;
       small += a <= 4 ? 1 : 0;
-      const double mb = a > 0 ? w / a : 0;
-      gal += a * std::min(a, mb) * std::log2(2.0 + std::max(a, mb));
+      const double mb = p > 0 ? w / p : 0;
+      gal += p * std::min(a, mb) * std::log2(2.0 + std::max(a, mb));
     }
-    const double mean_a = n ? sa / n : 0, mean_b = sa ? sb / sa : 0;
+    const double mean_a = n ? sa / n : 0, mean_b = sp ? sb / sp : 0;
     const double var_a = n ? std::max(0.0, sa2 / n - mean_a * mean_a) : 0;
-    const double wmean_a = sa ? sa2 / sa : 0;
+    const double wmean_a = sp ? sap / sp : 0;  // staged length seen by the average pair
     auto L = [](double x) { return (float)(std::log2(1.0 + x) / 16.0); };
     float *f = &g->h_features[(size_t)k * PAC_MLP_IN];
     f[0] = L(n);
-    f[1] = L(sa);
+    f[1] = L(sp);
     f[2] = L(sb);
     f[3] = L(mean_a);
     f[4] = L(std::sqrt(var_a));
@@ -421,20 +426,20 @@ void partition_features(Graph *g, const std::vector<uint32_t> &h_odeg,
     f[6] = L(n ? sd / n : 0);
     f[7] = L(maxd);
     f[8] = L(mean_b);
-    f[9] = (float)((std::log2(1.0 + mean_b) - std::log2(1.0 + mean_a)) / 16.0);
+    f[9] = (float)((std::log2(1.0 + mean_b) - std::log2(1.0 + wmean_a)) / 16.0);
     f[10] = L(wmean_a);
     f[11] = n ? (float)(big / n) : 0.f;
     f[12] = n ? (float)(small / n) : 0.f;
-    f[13] = L(sa2 + sb);
+    f[13] = L(sap + sb);
     f[14] = L(gal);
     f[15] = L(maxw);
-    // Fixed degree-threshold heuristic: hub-heavy partitions (edge-weighted mean
-    // out-degree >= 32) use the LDS bitmap; strongly skewed ones (middle vertices'
-    // lists 8x longer than the rows') use galloping; the rest merge.
+    // Fixed degree-threshold heuristic: partitions whose pairs see long staged lists
+    // (pair-weighted mean out-degree >= 32) use the LDS bitmap; strongly skewed ones
+    // (staged and streamed lists 8x apart) use galloping; the rest merge.
     uint8_t s = kMerge;
     if (wmean_a >= 32.0)
       s = kBitmap;
-    else if (mean_b >= 8.0 * std::max(1.0, wmean_a))
+    else if (mean_b >= 8.0 * std::max(1.0, wmean_a) || wmean_a >= 8.0 * std::max(1.0, mean_b))
       s = kGallop;
     g->h_fixed[k] = s;
   }
@@ -800,27 +805,30 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
   PAC_CHECK_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
   PAC_TRY(compute_rowwork(g));
   PAC_CHECK_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
-  std::vector<uint32_t> odeg, work, off;
+  std::vector<uint32_t> odeg, work, off, poff;
   PAC_TRY(d2h(ctx, odeg, g->odeg, V));
   PAC_TRY(d2h(ctx, work, g->rowwork, V));
   PAC_TRY(d2h(ctx, off, g->off, (size_t)V + 1));
-  std::vector<uint32_t> deg(V);
+  PAC_TRY(d2h(ctx, poff, g->poff, (size_t)V + 1));
+  std::vector<uint32_t> deg(V), npairs(V);
   uint64_t W = 0, maxo = 0;
   for (uint32_t u = 0; u < V; ++u) {
     deg[u] = off[u + 1] - off[u];
-    W += (uint64_t)odeg[u] * odeg[u] + work[u];
+    npairs[u] = poff[u + 1] - poff[u];
+    // sum over oriented edges t->h of d+(t) + d+(h) = sum over u of d+(u) * d(u)
+    W += (uint64_t)odeg[u] * deg[u];
     maxo = std::max<uint64_t>(maxo, odeg[u]);
   }
   g->W = W;
   g->max_outdeg = maxo;
-  // rows grouped by partition (stable in ID), rows with d+ = 0 dropped
+  // rows (vertices owning pairs) grouped by partition, stable in ID
   const uint64_t width = ((uint64_t)V + P - 1) / P;
   auto part_of = [&](uint32_t u) -> uint32_t {
     return v2p ? (uint32_t)v2p[u] : (uint32_t)(u / (width ? width : 1));
   };
   std::vector<uint32_t> count(P + 1, 0);
   for (uint32_t u = 0; u < V; ++u)
-    if (odeg[u]) count[part_of(u) + 1]++;
+    if (npairs[u]) count[part_of(u) + 1]++;
   for (uint32_t k = 0; k < P; ++k) count[k + 1] += count[k];
   g->P = P;
   g->h_pstart = count;
@@ -828,26 +836,29 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
   {
     std::vector<uint32_t> cur(count.begin(), count.end() - 1);
     for (uint32_t u = 0; u < V; ++u)
-      if (odeg[u]) rowlist[cur[part_of(u)]++] = u;
+      if (npairs[u]) rowlist[cur[part_of(u)]++] = u;
   }
   g->h_pedges.assign(P, 0);
-  for (uint32_t k = 0; k < P; ++k)
-    for (uint32_t q = count[k]; q < count[k + 1]; ++q) g->h_pedges[k] += odeg[rowlist[q]];
-  // work items: pack small rows, split hub rows by middle-vertex range
+  for (uint32_t u = 0; u < V; ++u) g->h_pedges[part_of(u)] += odeg[u];
+  // Work items: pack small rows, split heavy rows by pair range.  Row cost = staged
+  // list + streamed lists + a per-pair and a per-row overhead.
   const uint64_t target = item_target();
+  auto row_cost = [&](uint32_t r) {
+    return (uint64_t)odeg[r] + work[r] + 4ull * npairs[r] + 64;
+  };
   g->h_items.clear();
   for (uint32_t k = 0; k < P; ++k) {
     uint32_t lo = count[k];
     uint64_t acc = 0;
     for (uint32_t q = count[k]; q < count[k + 1]; ++q) {
       const uint32_t r = rowlist[q];
-      const uint64_t cost = (uint64_t)odeg[r] + work[r] + 64;
+      const uint64_t cost = row_cost(r);
       if (cost > target) {
         if (q > lo) g->h_items.push_back({0, k, lo, q, 0, 0xFFFFFFFFu, 0, 0});
-        const uint64_t nchunk = std::min<uint64_t>(odeg[r], (cost + target - 1) / target);
+        const uint64_t nchunk = std::min<uint64_t>(npairs[r], (cost + target - 1) / target);
         for (uint64_t c = 0; c < nchunk; ++c) {
-          const uint32_t c_lo = (uint32_t)(odeg[r] * c / nchunk);
-          const uint32_t c_hi = (uint32_t)(odeg[r] * (c + 1) / nchunk);
+          const uint32_t c_lo = (uint32_t)(npairs[r] * c / nchunk);
+          const uint32_t c_hi = (uint32_t)(npairs[r] * (c + 1) / nchunk);
           if (c_hi > c_lo) g->h_items.push_back({0, k, q, q + 1, c_lo, c_hi, 0, 0});
         }
         lo = q + 1;
@@ -871,9 +882,9 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
       uint64_t c = 0;
       if (w.hi == w.lo + 1 && w.c_hi != 0xFFFFFFFFu) {
         const uint32_t r = rowlist[w.lo];
-        c = (uint64_t)(work[r] + odeg[r]) * (w.c_hi - w.c_lo) / std::max(1u, odeg[r]);
+        c = row_cost(r) * (w.c_hi - w.c_lo) / std::max(1u, npairs[r]);
       } else {
-        for (uint32_t q = w.lo; q < w.hi; ++q) c += (uint64_t)odeg[rowlist[q]] + work[rowlist[q]];
+        for (uint32_t q = w.lo; q < w.hi; ++q) c += row_cost(rowlist[q]);
       }
       cost[i] = {c, (uint32_t)i};
     }
@@ -887,7 +898,7 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
     }
     g->h_items.swap(sorted);
   }
-  partition_features(g, odeg, work, deg, rowlist);
+  partition_features(g, odeg, npairs, work, deg, rowlist);
   g->h_learned.clear();
   g->learned_version = 0;
   PAC_TRY(g->rows.reserve(sizeof(uint32_t) * (rowlist.empty() ? 1 : rowlist.size())));

@@ -69,7 +69,10 @@ struct DevBuf {
 struct GraphDev {
   const uint32_t *ooff;  // V+1 oriented row offsets
   const uint32_t *oadj;  // m oriented neighbours, rows sorted by ID
-  const uint32_t *rows;  // rows with d+>0, grouped by partition
+  const uint32_t *poff;  // V+1 offsets of the intersection pairs owned by each vertex
+  const uint32_t *ppar;  // partner of each pair (its out-list is the one streamed)
+  const uint32_t *pedge; // oriented-edge index of each pair (position in oadj)
+  const uint32_t *rows;  // rows owning at least one pair, grouped by partition
   unsigned long long *total;  // T accumulator
   unsigned long long *tvert;  // per-vertex triangle counts (LCC mode)
   uint32_t *sup;              // per-oriented-edge support (SUP mode)
@@ -78,7 +81,7 @@ struct GraphDev {
 };
 
 // A unit of the persistent work queue: rows rows[lo..hi) of graph g, partition part;
-// when hi == lo+1 only the middle vertices at positions [c_lo, c_hi) of the row.
+// when hi == lo+1 only the pairs at positions [c_lo, c_hi) of the row.
 struct WorkItem {
   uint32_t g, part, lo, hi, c_lo, c_hi, cost_hi, cost_lo;
 };
@@ -101,6 +104,12 @@ struct Graph {
   DevBuf off, adj, deg;
   // oriented CSR
   DevBuf odeg, ooff, oadj;
+  // Intersection pairs.  Each oriented edge t->h stands for N+(t) & N+(h); the pair is
+  // owned by the endpoint with the longer out-list (ties: the tail), which stages its
+  // list in LDS while the partner's shorter list is streamed.  Pairs whose partner has
+  // an empty out-list are dropped (they hold no triangle).
+  DevBuf poff, ppar, pedge;
+  bool pairs_built = false;
   // partitioning and work items
   bool prepared = false;
   uint32_t P = 0;
@@ -112,7 +121,7 @@ struct Graph {
   uint64_t learned_version = 0;
   std::vector<WorkItem> h_items;
   DevBuf rows, items;
-  DevBuf rowwork;  // V: sum over c in N+(u) of d+(c)
+  DevBuf rowwork;  // V: streamed length of row u = sum of d+(partner) over u's pairs
   // outputs
   DevBuf total, tvert, sup, lcc32, lcc64, juni, jratio;
   double prepare_ms = 0;
@@ -150,6 +159,8 @@ struct Ctx {
 // ---- prepare.hip -----------------------------------------------------------------
 int build_symmetric_from_host(Graph *g, const uint64_t *vlist, const int32_t *elist);
 int orient(Graph *g);
+// Build the pair lists of an oriented graph (once; later calls return at once).
+int build_pairs(Graph *g);
 int graph_validate(const Graph *g, int level);
 
 // ---- gen.hip ---------------------------------------------------------------------
@@ -194,6 +205,7 @@ int mlp_store_bf16(Ctx *ctx, const float *d_master);
 // ---- host helpers ----------------------------------------------------------------
 int fixed_heuristic(const Graph *g, std::vector<uint8_t> &out);
 void partition_features(Graph *g, const std::vector<uint32_t> &h_odeg,
+                        const std::vector<uint32_t> &h_npairs,
                         const std::vector<uint32_t> &h_work, const std::vector<uint32_t> &h_deg,
                         const std::vector<uint32_t> &rowlist);
 

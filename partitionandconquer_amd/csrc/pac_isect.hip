@@ -8,26 +8,32 @@
 //   Graph::solve_lcc_merge (Graph.cpp:828-840)
 //
 // Work decomposition.  The graph is oriented by degree: u->v iff (d(u),u) < (d(v),v),
-// so every triangle {r,c,w} is found exactly once, at the oriented edge r->c whose
-// tail r has the smallest (degree, id), as w in N+(r) & N+(c).  A workgroup owns a
-// row r: N+(r) (the list A) is staged once in LDS and every middle vertex c = A[k] is
-// given to one wave, which streams N+(c) (the list B) from HBM with coalesced loads
-// and looks each element up in A.  The three strategies differ only in that lookup:
+// so every triangle {t,h,w} is found exactly once, at the oriented edge t->h whose
+// tail t has the smallest (degree, id), as w in N+(t) & N+(h).  Each oriented edge is
+// an intersection pair owned by the endpoint whose out-list is longer (ties: the
+// tail); pairs whose other list is empty are dropped.  A workgroup owns a row x:
+// N+(x) (the list A) is staged once in LDS and each of x's pairs is given to one wave,
+// which streams the partner y's out-list N+(y) (the list B, never longer than A) from
+// HBM with coalesced loads and looks each element up in A.  Streaming the shorter list
+// of every pair moves sum(min(d+(t), d+(h))) elements from HBM rather than
+// sum(d+(h)).  The three strategies differ only in the lookup:
 //   kMerge  : wave64 merge-path merge of A (LDS) and B (HBM): each lane walks an
 //             equal share of the merged sequence after a diagonal binary search.
 //   kGallop : the shorter list's elements are binary-searched in the longer one
 //             (B elements in A held in LDS, or A elements in B in HBM).
 //   kBitmap : a hashed bitmap of A in LDS filters B elements with one LDS read; the
-//             few survivors are confirmed by binary search in A (exact).
+//             survivors are confirmed in an LDS hash table of A (exact).
 // Counting is exact by construction in all three: a hit is only recorded for an
 // element equal to a list entry.
 //
-// Per-vertex and per-edge counts.  For a triangle (r, c=A[k], w=A[i]) found at row r:
-// t(r) += 1 (held in a register per row), t(c) and t(w) go to the LDS counter of
-// positions k and i of A; after the row, each nonzero LDS counter is added to the
-// global count of A[i] -- one global atomic per oriented edge instead of one per
-// triangle.  The support of edge r->A[i] (SUP mode, for Jaccard) is flushed the same
-// way; the third edge c->w gets one atomic per triangle at its position in B.
+// Per-vertex and per-edge counts.  A hit w of pair (x, y) is the triangle {x, y, w}:
+// t(x) += 1 (a register per row); t(w) goes to the LDS counter of w's place in A;
+// t(y) goes to the LDS counter of y's place in A when x->y, otherwise (y->x) to one
+// global atomic per pair.  After the row each nonzero LDS counter is added to the
+// global count of its vertex -- about one global atomic per pair instead of one per
+// triangle.  SUP mode (per-edge support, for Jaccard) keeps the same counters per
+// edge: x->w and x->y at their places in A, y->x per pair, and y->w with one atomic
+// per triangle at w's position in B.
 #include <algorithm>
 
 #include "pac_internal.hpp"
@@ -42,6 +48,7 @@ namespace {
 
 constexpr uint32_t kHashMul = 0x9E3779B1u;  // Fibonacci hashing multiplier
 constexpr uint32_t kAllC = 0xFFFFFFFFu;
+constexpr uint32_t kPassPairs = 16384;  // pairs per bitmap pass (bounds the 16-bit counters)
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 
@@ -80,10 +87,11 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;     // empty hash-table slot (IDs are <
 
 // LDS of one workgroup.  The sorted-list strategies (merge, gallop) keep A and one
 // 32-bit counter per position; the bitmap strategy keeps a linear-probing hash table
-// of A (2*CAP slots) with packed 16-bit counters and, in SUP mode, 16-bit positions.
-// Both live in one union: a workgroup runs one strategy per row.  The filter bitmap
-// sits outside the union and is all-zero between bitmap rows.
-template <int CAP, int NBITS, int MODE>
+// of A (1.5*CAP slots) with packed 16-bit counters and, in SUP mode, 16-bit positions,
+// plus one counter per pair of each wave's current batch.  Both live in one union: a
+// workgroup runs one strategy per row.  The filter bitmap sits outside the union and
+// is all-zero between bitmap rows.
+template <int CAP, int NBITS, int MODE, int WAVES>
 struct Smem {
   static constexpr uint32_t kTabCap = CAP + CAP / 2;  // hash slots: load factor <= 2/3
   union {
@@ -95,6 +103,7 @@ struct Smem {
       uint32_t tab[kTabCap];
       uint32_t cnt16[MODE == kModeTC ? 1 : kTabCap / 2];   // 2 x u16 per word
       uint32_t pos16[MODE == kModeSUP ? kTabCap / 2 : 1];  // 2 x u16 per word
+      uint32_t pcnt[MODE == kModeTC ? 1 : 64 * WAVES];     // hits per pair of a batch
     } h;  // bitmap
   } u;
   uint32_t bits[NBITS / 32];
@@ -105,8 +114,9 @@ __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 
-// 16-bit counters packed two per word.  Counts stay below 2^16 (each is at most the
-// number of middle vertices of a row plus one, <= 2*CAP), so halves never carry.
+// 16-bit counters packed two per word.  A bitmap pass covers at most kPassPairs pairs,
+// so a counter stays below 2^16 (at most one hit per pair for an element of A plus at
+// most nA <= CAP hits of the pair x->y at y's place) and halves never carry.
 __device__ __forceinline__ void add16(uint32_t *w, uint32_t slot, uint32_t v) {
   atomicAdd(&w[slot >> 1], v << (16u * (slot & 1u)));
 }
@@ -138,8 +148,8 @@ __device__ __forceinline__ void on_hit(uint32_t *cnt, uint32_t i, uint32_t *sup,
   if (MODE == kModeSUP) atomicAdd(&sup[bj], 1u);
 }
 
-// ---- the three lookup engines: one wave, one middle vertex c ------------------------
-// B = oadj[b0 .. b0+nb): sorted N+(c) in HBM.  Each returns this lane's number of
+// ---- the three lookup engines: one wave, one pair (x, y) ---------------------------
+// B = oadj[b0 .. b0+nb): sorted N+(y) in HBM.  Each returns this lane's number of
 // confirmed common neighbours.
 
 template <int MODE>
@@ -228,38 +238,46 @@ __device__ __forceinline__ uint32_t probe_merge(const uint32_t *__restrict__ oad
   return hits;
 }
 
-// The middle vertices of a row are visited in batches of 64 per wave: lane l loads
-// c = A[k_l] and the bounds of N+(c) for the l-th middle vertex of the wave, so the
-// offsets of 64 lists arrive with one round trip; the wave then walks the batch,
-// broadcasting each list's bounds with readlane.  Wave w takes positions
-// c_lo + w, c_lo + w + WAVES, ...  body(k, c, b0, nb) runs once per middle vertex.
-template <int WAVES, class AAt, class Body>
-__device__ __forceinline__ void for_each_c(const GraphDev &G, uint32_t c_lo, uint32_t c_hi,
-                                           uint32_t wave, uint32_t lane, AAt a_at, Body body) {
+// The pairs of a row are visited in batches of 64 per wave: lane l loads the partner y
+// and edge index e of the wave's l-th pair and the bounds of N+(y), so the offsets of
+// 64 lists arrive with one round trip; the wave then walks the batch, broadcasting
+// each pair with readlane.  Wave w takes pair positions k_lo + w, k_lo + w + WAVES, ...
+// body(y, e, b0, nb) runs once per pair (e is only loaded when counts are kept).
+template <int WAVES, int MODE, class Body>
+__device__ __forceinline__ void for_each_pair(const GraphDev &G, uint32_t p0, uint32_t k_lo,
+                                              uint32_t k_hi, uint32_t wave, uint32_t lane,
+                                              Body body) {
   for (uint32_t t0 = 0;; t0 += 64) {
-    const uint32_t kl = c_lo + wave + (uint32_t)WAVES * (t0 + lane);
-    uint32_t cl = 0, bl = 0, nl = 0;
-    if (kl < c_hi) {
-      cl = a_at(kl);
-      bl = G.ooff[cl];
-      nl = G.ooff[cl + 1] - bl;
+    const uint32_t kl = k_lo + wave + (uint32_t)WAVES * (t0 + lane);
+    uint32_t yl = 0, el = 0, bl = 0, nl = 0;
+    if (kl < k_hi) {
+      yl = G.ppar[p0 + kl];
+      if (MODE != kModeTC) el = G.pedge[p0 + kl];
+      bl = G.ooff[yl];
+      nl = G.ooff[yl + 1] - bl;
     }
-    const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < k_hi));
     for (uint32_t jj = 0; jj < nvalid; ++jj)
-      body(c_lo + wave + (uint32_t)WAVES * (t0 + jj), readlane(cl, jj), readlane(bl, jj),
-           readlane(nl, jj));
+      body(readlane(yl, jj), readlane(el, jj), readlane(bl, jj), readlane(nl, jj));
     if (nvalid < 64) break;
   }
 }
 
+// Completes this wave's LDS atomics before its lanes read what other lanes added.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // ---- one row, sorted-list strategies (merge, gallop): A staged in LDS ---------------
 template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
-__device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32_t c_lo,
-                                           uint32_t c_hi, Smem<CAP, NBITS, MODE> &sm,
+__device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32_t k_lo,
+                                           uint32_t k_hi, Smem<CAP, NBITS, MODE, WAVES> &sm,
                                            unsigned long long &acc) {
   constexpr uint32_t BS = 64u * WAVES;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
+  const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
   uint32_t *A = sm.u.s.A;
   uint32_t *cnt = sm.u.s.cnt;
   for (uint32_t i = tid; i < nA; i += BS) {
@@ -267,11 +285,10 @@ __device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32
     if (MODE != kModeTC) cnt[i] = 0;
   }
   __syncthreads();
-  if (c_hi > nA) c_hi = nA;
+  if (k_hi > np) k_hi = np;
   uint32_t row_hits = 0;
-  for_each_c<WAVES>(G, c_lo, c_hi, wave, lane, [&](uint32_t kl) { return A[kl]; },
-                    [&](uint32_t k, uint32_t c, uint32_t b0, uint32_t nb) {
-    (void)c;
+  for_each_pair<WAVES, MODE>(G, p0, k_lo, k_hi, wave, lane,
+                             [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb) {
     uint32_t h = 0;
     if (nb != 0) {
       if (STRAT == kGallop)
@@ -281,7 +298,14 @@ __device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32
     }
     if (MODE != kModeTC) {
       const uint32_t ch = wave_sum(h);
-      if (lane == 0 && ch) atomicAdd(&cnt[k], ch);
+      if (lane == 0 && ch) {
+        if (e - a0 < nA)
+          atomicAdd(&cnt[e - a0], ch);  // r->y: y = A[e - a0]
+        else if (MODE == kModeLCC)
+          atomicAdd(&G.tvert[y], (unsigned long long)ch);
+        else
+          atomicAdd(&G.sup[e], ch);
+      }
     }
     row_hits += h;
   });
@@ -305,29 +329,33 @@ __device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32
 
 // ---- one row, bitmap strategy: hashed filter bitmap + LDS hash table of A -------------
 // The table has tsize = max(64, min(2*nA, TABCAP)) slots (load factor <= 2/3), indexed
-// by the high half of (hash * tsize).  A wave takes its middle vertices in batches of
-// 64 (lane l loads c_l, the bounds of N+(c_l) and the table slot of c_l), then streams
-// the concatenation of the batch's 64 lists 256 elements at a time, dense across
-// lanes and coalesced; each lane finds the list its element belongs to by walking the
-// batch's inclusive prefix sums (held one per lane, read with ds_bpermute), which only
-// moves forward.
+// by the high half of (hash * tsize).  A wave takes its pairs in batches of 64 (lane l
+// loads partner y_l, the bounds of N+(y_l) and, when y_l is in A, its table slot), then
+// streams the concatenation of the batch's 64 lists 256 elements at a time, dense
+// across lanes and coalesced; each lane finds the list its element belongs to by
+// walking the batch's inclusive prefix sums (held one per lane, read with ds_bpermute),
+// which only moves forward.
 template <int MODE, int WAVES, int CAP, int NBITS>
 __device__ __forceinline__ void row_bitmap(const GraphDev &G, uint32_t r, uint32_t c_lo,
-                                           uint32_t c_hi, Smem<CAP, NBITS, MODE> &sm,
+                                           uint32_t c_hi, Smem<CAP, NBITS, MODE, WAVES> &sm,
                                            unsigned long long &acc) {
   constexpr uint32_t BS = 64u * WAVES;
-  constexpr uint32_t TABCAP = Smem<CAP, NBITS, MODE>::kTabCap;
+  constexpr uint32_t TABCAP = Smem<CAP, NBITS, MODE, WAVES>::kTabCap;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
+  const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
   uint32_t *tab = sm.u.h.tab;
   uint32_t *cnt16 = sm.u.h.cnt16;
   uint32_t *pos16 = sm.u.h.pos16;
+  uint32_t *pc = sm.u.h.pcnt + 64u * wave;
   uint32_t tsize = 2u * nA;
   if (tsize > TABCAP) tsize = TABCAP;
   if (tsize < 64u) tsize = 64u;
   for (uint32_t i = tid; i < tsize; i += BS) tab[i] = kEmpty;
-  if (MODE != kModeTC)
+  if (MODE != kModeTC) {
     for (uint32_t i = tid; i < (tsize + 1) / 2; i += BS) cnt16[i] = 0;
+    for (uint32_t i = tid; i < BS; i += BS) sm.u.h.pcnt[i] = 0;
+  }
   if (MODE == kModeSUP)
     for (uint32_t i = tid; i < (tsize + 1) / 2; i += BS) pos16[i] = 0;
   __syncthreads();
@@ -340,16 +368,19 @@ __device__ __forceinline__ void row_bitmap(const GraphDev &G, uint32_t r, uint32
     atomicOr(&sm.bits[hb >> 5], 1u << (hb & 31u));
   }
   __syncthreads();
-  if (c_hi > nA) c_hi = nA;
+  if (c_hi > np) c_hi = np;
   uint32_t row_hits = 0;
   for (uint32_t t0 = 0;; t0 += 64) {
     const uint32_t kl = c_lo + wave + (uint32_t)WAVES * (t0 + lane);
-    uint32_t bl = 0, nl = 0, sc = 0;
+    uint32_t yl = 0, el = 0, bl = 0, nl = 0, sc = kEmpty;
     if (kl < c_hi) {
-      const uint32_t cl = G.oadj[a0 + kl];
-      bl = G.ooff[cl];
-      nl = G.ooff[cl + 1] - bl;
-      if (MODE != kModeTC) sc = tab_find(tab, tsize, cl);
+      yl = G.ppar[p0 + kl];
+      bl = G.ooff[yl];
+      nl = G.ooff[yl + 1] - bl;
+      if (MODE != kModeTC) {
+        el = G.pedge[p0 + kl];
+        if (el - a0 < nA) sc = tab_find(tab, tsize, yl);  // r->y: y is in A
+      }
     }
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
     // inclusive prefix of the list lengths over the batch's lanes
@@ -364,7 +395,7 @@ __device__ __forceinline__ void row_bitmap(const GraphDev &G, uint32_t r, uint32
     uint32_t cur = 0;  // owner walk, monotone in q; always < 64 (see below)
     uint32_t hits = 0;
     for (uint32_t q0 = 0; q0 < total; q0 += 256u) {  // wave-uniform loop
-      uint32_t w[4], addr[4], so[4];
+      uint32_t w[4], addr[4], so[4], own[4];
       bool ok[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -380,6 +411,7 @@ __device__ __forceinline__ void row_bitmap(const GraphDev &G, uint32_t r, uint32
           if (!__any(adv)) break;
           cur += adv ? 1u : 0u;
         }
+        own[u] = cur;
         addr[u] = (uint32_t)__shfl(base, (int)cur, 64) + qc;
         so[u] = (MODE != kModeTC) ? (uint32_t)__shfl(sc, (int)cur, 64) : 0u;
         w[u] = ok[u] ? G.oadj[addr[u]] : 0u;
@@ -393,13 +425,29 @@ __device__ __forceinline__ void row_bitmap(const GraphDev &G, uint32_t r, uint32
           if (slot != kEmpty) {
             ++hits;
             if (MODE != kModeTC) {
-              add16(cnt16, slot, 1u);   // t(w) / edge r->w
-              add16(cnt16, so[u], 1u);  // t(c) / edge r->c
+              add16(cnt16, slot, 1u);  // t(w) / edge r->w
+              if (so[u] != kEmpty)
+                add16(cnt16, so[u], 1u);  // t(y) / edge r->y, y in A
+              else
+                atomicAdd(&pc[own[u]], 1u);  // t(y) / edge y->r
             }
-            if (MODE == kModeSUP) atomicAdd(&G.sup[addr[u]], 1u);  // edge c->w
+            if (MODE == kModeSUP) atomicAdd(&G.sup[addr[u]], 1u);  // edge y->w
           }
         }
       }
+    }
+    if (MODE != kModeTC) {
+      // pairs y->r: one global atomic per pair with hits, then the counter is reset
+      wave_lds_sync();
+      if (kl < c_hi && sc == kEmpty) {
+        const uint32_t v = pc[lane];
+        if (v) {
+          pc[lane] = 0;
+          if (MODE == kModeLCC) atomicAdd(&G.tvert[yl], (unsigned long long)v);
+          if (MODE == kModeSUP) atomicAdd(&G.sup[el], v);
+        }
+      }
+      wave_lds_sync();
     }
     row_hits += hits;
     if (nvalid < 64) break;
@@ -431,11 +479,12 @@ __device__ __forceinline__ void row_hbm(const GraphDev &G, uint32_t r, uint32_t 
                                         uint32_t c_hi, unsigned long long &acc) {
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
+  const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
   const uint32_t *A = G.oadj + a0;
-  if (c_hi > nA) c_hi = nA;
+  if (c_hi > np) c_hi = np;
   uint32_t row_hits = 0;
-  for_each_c<WAVES>(G, c_lo, c_hi, wave, lane, [&](uint32_t kl) { return A[kl]; },
-                    [&](uint32_t k, uint32_t c, uint32_t b0, uint32_t nb) {
+  for_each_pair<WAVES, MODE>(G, p0, c_lo, c_hi, wave, lane,
+                             [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb) {
     uint32_t h = 0;
     for (uint32_t j = lane; j < nb; j += 64) {
       const uint32_t w = G.oadj[b0 + j];
@@ -452,8 +501,8 @@ __device__ __forceinline__ void row_hbm(const GraphDev &G, uint32_t r, uint32_t 
     if (MODE != kModeTC) {
       const uint32_t ch = wave_sum(h);
       if (lane == 0 && ch) {
-        if (MODE == kModeLCC) atomicAdd(&G.tvert[c], (unsigned long long)ch);
-        if (MODE == kModeSUP) atomicAdd(&G.sup[a0 + k], ch);
+        if (MODE == kModeLCC) atomicAdd(&G.tvert[y], (unsigned long long)ch);
+        if (MODE == kModeSUP) atomicAdd(&G.sup[e], ch);
       }
     }
     row_hits += h;
@@ -465,20 +514,29 @@ __device__ __forceinline__ void row_hbm(const GraphDev &G, uint32_t r, uint32_t 
   }
 }
 
-// One row (or a middle-vertex range of one) with the given strategy.
+// One row (or a pair range of one) with the given strategy.
 template <int MODE, int WAVES, int CAP, int NBITS>
 __device__ __forceinline__ void do_row(const GraphDev &G, uint32_t r, uint32_t c_lo,
-                                       uint32_t c_hi, int st, Smem<CAP, NBITS, MODE> &sm,
+                                       uint32_t c_hi, int st, Smem<CAP, NBITS, MODE, WAVES> &sm,
                                        unsigned long long &acc) {
   const uint32_t nA = G.ooff[r + 1] - G.ooff[r];
-  if (nA > (uint32_t)CAP)
+  if (nA > (uint32_t)CAP) {
     row_hbm<MODE, WAVES>(G, r, c_lo, c_hi, acc);
-  else if (st == kBitmap)
-    row_bitmap<MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
-  else if (st == kGallop)
+  } else if (st == kBitmap) {
+    if (MODE == kModeTC) {
+      row_bitmap<MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
+    } else {
+      const uint32_t np = G.poff[r + 1] - G.poff[r];
+      const uint32_t hi = c_hi < np ? c_hi : np;
+      for (uint32_t lo = c_lo; lo < hi; lo = (hi - lo > kPassPairs) ? lo + kPassPairs : hi)
+        row_bitmap<MODE, WAVES, CAP, NBITS>(
+            G, r, lo, (hi - lo > kPassPairs) ? lo + kPassPairs : hi, sm, acc);
+    }
+  } else if (st == kGallop) {
     row_sorted<kGallop, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
-  else
+  } else {
     row_sorted<kMerge, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
+  }
 }
 
 template <int WAVES>
@@ -490,7 +548,7 @@ __device__ __forceinline__ void flush_total(unsigned long long *total, unsigned 
 // ---- plain kernel: one workgroup per row, one launch per partition group -----------
 template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
 __global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev G, uint32_t row_begin) {
-  __shared__ Smem<CAP, NBITS, MODE> sm;
+  __shared__ Smem<CAP, NBITS, MODE, WAVES> sm;
   if (STRAT == kBitmap)
     for (uint32_t i = threadIdx.x; i < NBITS / 32; i += 64 * WAVES) sm.bits[i] = 0;
   __syncthreads();
@@ -507,7 +565,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__res
                                                            uint32_t nitems,
                                                            const uint8_t *__restrict__ strat,
                                                            uint32_t *queue) {
-  __shared__ Smem<CAP, NBITS, MODE> s;
+  __shared__ Smem<CAP, NBITS, MODE, WAVES> s;
   for (uint32_t i = threadIdx.x; i < NBITS / 32; i += 64 * WAVES) s.bits[i] = 0;
   unsigned long long acc = 0;
   unsigned long long *cur_total = nullptr;
@@ -554,19 +612,17 @@ __global__ void k_fill_u64(unsigned long long *p, uint64_t n) {
     p[i] = 0ull;
 }
 
-// work[u] = sum over c in N+(u) of d+(c): the B-side length of row u (wave per row).
-__global__ void k_rowwork(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
-                          uint32_t V, uint32_t *__restrict__ work) {
+// work[u] = sum of d+(partner) over u's pairs: the streamed length of row u (wave per row).
+__global__ void k_rowwork(const uint32_t *__restrict__ poff, const uint32_t *__restrict__ ppar,
+                          const uint32_t *__restrict__ odeg, uint32_t V,
+                          uint32_t *__restrict__ work) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wpb = blockDim.x >> 6;
   for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
        u += (uint64_t)gridDim.x * wpb) {
-    const uint32_t a0 = ooff[u], a1 = ooff[u + 1];
+    const uint32_t p0 = poff[u], p1 = poff[u + 1];
     uint64_t s = 0;
-    for (uint32_t i = a0 + lane; i < a1; i += 64) {
-      const uint32_t c = oadj[i];
-      s += ooff[c + 1] - ooff[c];
-    }
+    for (uint32_t i = p0 + lane; i < p1; i += 64) s += odeg[ppar[i]];
     s = wave_sum64(s);
     if (lane == 0) work[u] = s > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)s;
   }
@@ -734,6 +790,9 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   GraphDev d;
   d.ooff = g->ooff.as<const uint32_t>();
   d.oadj = g->oadj.as<const uint32_t>();
+  d.poff = g->poff.as<const uint32_t>();
+  d.ppar = g->ppar.as<const uint32_t>();
+  d.pedge = g->pedge.as<const uint32_t>();
   d.rows = g->rows.as<const uint32_t>();
   d.total = g->total.as<unsigned long long>();
   d.tvert = g->tvert.as<unsigned long long>();
@@ -814,11 +873,12 @@ int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
 }
 
 int compute_rowwork(Graph *g) {
+  PAC_TRY(build_pairs(g));
   PAC_TRY(g->rowwork.reserve(sizeof(uint32_t) * (size_t)(g->V ? g->V : 1)));
   if (g->V == 0) return PAC_OK;
   hipLaunchKernelGGL(k_rowwork, dim3(grid_for(g->V, 4)), dim3(256), 0, g->ctx->stream,
-                     g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(), g->V,
-                     g->rowwork.as<uint32_t>());
+                     g->poff.as<const uint32_t>(), g->ppar.as<const uint32_t>(),
+                     g->odeg.as<const uint32_t>(), g->V, g->rowwork.as<uint32_t>());
   PAC_CHECK_HIP(hipGetLastError());
   return PAC_OK;
 }
