@@ -39,15 +39,19 @@ inline uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// Target work (probe count) of one persistent work item.  Rows are packed until an
-// item reaches it; rows above it are split by middle-vertex range.
-uint64_t item_target() {
+// Target work (streamed elements plus overheads) of one persistent work item.  Rows
+// are packed until an item reaches it; rows above it are split by pair range.  About
+// eight items per resident workgroup (six per CU) balance the queue's tail while
+// keeping rows whole: a whole row writes its per-edge counts with plain stores, a split
+// row with atomics.
+uint64_t item_target(uint64_t total_cost, int num_cus) {
   const char *e = getenv("PAC_ITEM_WORK");
   if (e) {
     const long long v = atoll(e);
     if (v >= 256) return (uint64_t)v;
   }
-  return 1ull << 16;
+  const uint64_t slots = (uint64_t)(num_cus > 0 ? num_cus : 1) * 6 * 8;
+  return std::max<uint64_t>(1ull << 16, total_cost / slots);
 }
 
 int ensure_ctx(pac_ctx *ctx) {
@@ -324,6 +328,7 @@ int run_intersection(Graph *g, int mode, int dispatch, int launch, Timed &t) {
     ctx->time_kernel = false;
     PAC_TRY(rc);
   }
+  if (mode == kModeLCC) PAC_TRY(vertex_counts(ctx, g));
   ctx->last.launches = launches;
   PAC_TRY(t.mark(2));
   return PAC_OK;
@@ -842,10 +847,21 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
   for (uint32_t u = 0; u < V; ++u) g->h_pedges[part_of(u)] += odeg[u];
   // Work items: pack small rows, split heavy rows by pair range.  Row cost = staged
   // list + streamed lists + a per-pair and a per-row overhead.
-  const uint64_t target = item_target();
   auto row_cost = [&](uint32_t r) {
     return (uint64_t)odeg[r] + work[r] + 4ull * npairs[r] + 64;
   };
+  uint64_t total_cost = 0;
+  for (uint32_t r : rowlist) total_cost += row_cost(r);
+  const uint64_t target = item_target(total_cost, ctx->num_cus);
+  // vertices whose in-degree needs the split credit sum
+  g->h_hubs.clear();
+  for (uint32_t u = 0; u < V; ++u)
+    if (deg[u] - odeg[u] > kHubIn) g->h_hubs.push_back(u);
+  if (!g->h_hubs.empty()) {
+    PAC_TRY(g->hubs.reserve(sizeof(uint32_t) * g->h_hubs.size()));
+    PAC_CHECK_HIP(hipMemcpy(g->hubs.p, g->h_hubs.data(), sizeof(uint32_t) * g->h_hubs.size(),
+                            hipMemcpyHostToDevice));
+  }
   g->h_items.clear();
   for (uint32_t k = 0; k < P; ++k) {
     uint32_t lo = count[k];

@@ -74,7 +74,9 @@ struct GraphDev {
   const uint32_t *pedge; // oriented-edge index of each pair (position in oadj)
   const uint32_t *rows;  // rows owning at least one pair, grouped by partition
   unsigned long long *total;  // T accumulator
-  unsigned long long *tvert;  // per-vertex triangle counts (LCC mode)
+  unsigned long long *tvert;  // per-vertex triangle counts (LCC mode: the row totals)
+  uint32_t *ehead;            // LCC mode: per oriented edge, triangles credited to its head
+  uint32_t *etail;            // LCC mode: per oriented edge, triangles credited to its tail
   uint32_t *sup;              // per-oriented-edge support (SUP mode)
   uint32_t V;
   uint32_t part_base;         // offset of this graph's partitions in the dispatch table
@@ -86,9 +88,9 @@ struct WorkItem {
   uint32_t g, part, lo, hi, c_lo, c_hi, cost_hi, cost_lo;
 };
 
-// Tile configurations: waves per row, LDS list capacity (entries), LDS bitmap bits.
+// Tile configurations: waves per row, LDS list capacity (entries).
 struct TileCfg {
-  int waves, cap, nbits;
+  int waves, cap;
 };
 extern const TileCfg kTiles[PAC_NUM_TILES];
 
@@ -109,6 +111,10 @@ struct Graph {
   // list in LDS while the partner's shorter list is streamed.  Pairs whose partner has
   // an empty out-list are dropped (they hold no triangle).
   DevBuf poff, ppar, pedge;
+  // In-edges: for each vertex v, the oadj positions of the edges u->v (V+1 offsets).
+  DevBuf inoff, inidx;
+  std::vector<uint32_t> h_hubs;  // vertices whose in-degree exceeds kHubIn
+  DevBuf hubs;
   bool pairs_built = false;
   // partitioning and work items
   bool prepared = false;
@@ -123,7 +129,7 @@ struct Graph {
   DevBuf rows, items;
   DevBuf rowwork;  // V: streamed length of row u = sum of d+(partner) over u's pairs
   // outputs
-  DevBuf total, tvert, sup, lcc32, lcc64, juni, jratio;
+  DevBuf total, tvert, ehead, etail, sup, lcc32, lcc64, juni, jratio;
   double prepare_ms = 0;
 };
 
@@ -159,8 +165,11 @@ struct Ctx {
 // ---- prepare.hip -----------------------------------------------------------------
 int build_symmetric_from_host(Graph *g, const uint64_t *vlist, const int32_t *elist);
 int orient(Graph *g);
-// Build the pair lists of an oriented graph (once; later calls return at once).
+// Build the pair lists and in-edge lists of an oriented graph (once; later calls
+// return at once).
 int build_pairs(Graph *g);
+// In-degree above which a vertex's credit sum is split over several workgroups.
+constexpr uint32_t kHubIn = 16384;
 int graph_validate(const Graph *g, int level);
 
 // ---- gen.hip ---------------------------------------------------------------------
@@ -185,6 +194,8 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
 // One partition alone with one strategy (profiling).
 int launch_partition(Ctx *ctx, Graph *g, int mode, uint32_t part, int strat);
 int compute_rowwork(Graph *g);
+// LCC mode: per-vertex triangle counts = row totals + per-edge credits (after a launch).
+int vertex_counts(Ctx *ctx, Graph *g);
 int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64);
 int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *inter,
                      uint32_t *uni, float *ratio);

@@ -40,22 +40,53 @@
 
 namespace pac {
 
-// waves per row, LDS list capacity (uint32 entries), LDS bitmap bits
-const TileCfg kTiles[PAC_NUM_TILES] = {
-    {2, 1024, 16384}, {4, 2048, 32768}, {4, 4096, 65536}, {8, 4096, 65536}};
+// waves per row, LDS list capacity (uint32 entries)
+const TileCfg kTiles[PAC_NUM_TILES] = {{2, 1024}, {4, 2048}, {4, 4096}, {8, 4096}};
 
 namespace {
 
-constexpr uint32_t kHashMul = 0x9E3779B1u;  // Fibonacci hashing multiplier
-constexpr uint32_t kAllC = 0xFFFFFFFFu;
-constexpr uint32_t kPassPairs = 16384;  // pairs per bitmap pass (bounds the 16-bit counters)
+// Device view of a GraphDev with its pointers in the global address space.  The
+// descriptors are read from memory, so without this the compiler cannot prove where
+// the pointers point and emits flat loads, which also count against the LDS wait
+// counter: every wait for an LDS result would then wait for the outstanding HBM loads.
+#define PAC_GLOBAL __attribute__((address_space(1)))
+struct GDev {
+  const PAC_GLOBAL uint32_t *ooff, *oadj, *poff, *ppar, *pedge, *rows;
+  PAC_GLOBAL unsigned long long *total, *tvert;
+  PAC_GLOBAL uint32_t *ehead, *etail, *sup;
+  uint32_t V, part_base;
+};
 
-constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
-
-template <int NBITS>
-__device__ __forceinline__ uint32_t bit_of(uint32_t w) {
-  return (w * kHashMul) >> (32 - ilog2(NBITS));
+__device__ __forceinline__ GDev to_dev(const GraphDev &d) {
+  GDev g;
+  g.ooff = (const PAC_GLOBAL uint32_t *)d.ooff;
+  g.oadj = (const PAC_GLOBAL uint32_t *)d.oadj;
+  g.poff = (const PAC_GLOBAL uint32_t *)d.poff;
+  g.ppar = (const PAC_GLOBAL uint32_t *)d.ppar;
+  g.pedge = (const PAC_GLOBAL uint32_t *)d.pedge;
+  g.rows = (const PAC_GLOBAL uint32_t *)d.rows;
+  g.total = (PAC_GLOBAL unsigned long long *)d.total;
+  g.tvert = (PAC_GLOBAL unsigned long long *)d.tvert;
+  g.ehead = (PAC_GLOBAL uint32_t *)d.ehead;
+  g.etail = (PAC_GLOBAL uint32_t *)d.etail;
+  g.sup = (PAC_GLOBAL uint32_t *)d.sup;
+  g.V = d.V;
+  g.part_base = d.part_base;
+  return g;
 }
+
+// Relaxed device-scope atomic adds on global memory.
+__device__ __forceinline__ void gadd(PAC_GLOBAL uint32_t *p, uint32_t v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gadd(PAC_GLOBAL unsigned long long *p, unsigned long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr uint32_t kAllC = 0xFFFFFFFFu;
+constexpr uint32_t kPassPairs = 57344;  // pairs per hash pass (bounds the 16-bit counters)
+
+constexpr uint32_t pow2_at_least(uint32_t x) { return x <= 1 ? 1u : 2u * pow2_at_least((x + 1) / 2); }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
@@ -71,7 +102,8 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
 
 // Branch-free lower_bound: first index i in [0,n) with a[i] >= x (n if none).  The
 // trip count depends only on n, so lanes searching the same list stay converged.
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t x) {
+template <class P>
+__device__ __forceinline__ uint32_t lower_bound_u32(P a, uint32_t n, uint32_t x) {
   if (n == 0) return 0;
   uint32_t base = 0;
   while (n > 1) {
@@ -82,41 +114,48 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t 
   return base + (a[base] < x ? 1u : 0u);
 }
 
-constexpr uint32_t kTabMul = 0x85EBCA6Bu;   // hash-table multiplier (independent of kHashMul)
-constexpr uint32_t kEmpty = 0xFFFFFFFFu;     // empty hash-table slot (IDs are < 2^31)
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // empty hash-table slot (IDs are < 2^31)
+constexpr uint32_t kStash = 32;           // overflow entries of the hash table
 
 // LDS of one workgroup.  The sorted-list strategies (merge, gallop) keep A and one
-// 32-bit counter per position; the bitmap strategy keeps a linear-probing hash table
-// of A (1.5*CAP slots) with packed 16-bit counters and, in SUP mode, 16-bit positions,
-// plus one counter per pair of each wave's current batch.  Both live in one union: a
-// workgroup runs one strategy per row.  The filter bitmap sits outside the union and
-// is all-zero between bitmap rows.
-template <int CAP, int NBITS, int MODE, int WAVES>
+// 32-bit counter per position; the hash strategy keeps the cuckoo table of A (plus
+// its stash) with packed 16-bit counters per slot and, in SUP mode, 16-bit positions.
+// Both live in one union: a workgroup runs one strategy per row.
+template <int CAP, int MODE, int WAVES>
 struct Smem {
-  static constexpr uint32_t kTabCap = CAP + CAP / 2;  // hash slots: load factor <= 2/3
+  static constexpr uint32_t kSlots = 2u * pow2_at_least((5u * CAP + 7u) / 8u);
   union {
     struct {
       uint32_t A[CAP];
       uint32_t cnt[MODE == kModeTC ? 1 : CAP];
     } s;  // merge / gallop
     struct {
-      uint32_t tab[kTabCap];
-      uint32_t cnt16[MODE == kModeTC ? 1 : kTabCap / 2];   // 2 x u16 per word
-      uint32_t pos16[MODE == kModeSUP ? kTabCap / 2 : 1];  // 2 x u16 per word
-      uint32_t pcnt[MODE == kModeTC ? 1 : 64 * WAVES];     // hits per pair of a batch
-    } h;  // bitmap
+      uint32_t tab[kSlots + kStash];
+      uint32_t cnt16[MODE == kModeTC ? 1 : (kSlots + kStash) / 2];   // 2 x u16 per word
+      uint32_t pos16[MODE == kModeSUP ? (kSlots + kStash) / 2 : 1];  // 2 x u16 per word
+    } h;  // hash
   } u;
-  uint32_t bits[NBITS / 32];
   uint32_t item;
+  uint32_t nstash;
+  uint32_t rowtot;  // the row's triangle count (LCC mode)
 };
+
+// LCC-mode credits.  A row call that covers all of the row's pairs is the only writer of
+// the row's edges and total, so it stores; a partial call (a split row) adds atomically.
+__device__ __forceinline__ void credit_edge(PAC_GLOBAL uint32_t *p, uint32_t v, bool whole) {
+  if (whole)
+    *p = v;
+  else
+    gadd(p, v);
+}
 
 __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 
-// 16-bit counters packed two per word.  A bitmap pass covers at most kPassPairs pairs,
+// 16-bit counters packed two per word.  A hash pass covers at most kPassPairs pairs,
 // so a counter stays below 2^16 (at most one hit per pair for an element of A plus at
-// most nA <= CAP hits of the pair x->y at y's place) and halves never carry.
+// most nA <= CAP <= 4096 hits of the pair x->y at y's place) and halves never carry.
 __device__ __forceinline__ void add16(uint32_t *w, uint32_t slot, uint32_t v) {
   atomicAdd(&w[slot >> 1], v << (16u * (slot & 1u)));
 }
@@ -124,28 +163,12 @@ __device__ __forceinline__ uint32_t get16(const uint32_t *w, uint32_t slot) {
   return (w[slot >> 1] >> (16u * (slot & 1u))) & 0xFFFFu;
 }
 
-// Home slot of x in a table of tsize slots: high half of (hash * tsize).
-__device__ __forceinline__ uint32_t tab_home(uint32_t x, uint32_t tsize) {
-  return (uint32_t)(((uint64_t)(x * kTabMul) * tsize) >> 32);
-}
-
-// Slot of x in the table (kEmpty if absent); linear probing with wrap-around.  The
-// table always has empty slots (load factor <= 2/3), so the probe ends.
-__device__ __forceinline__ uint32_t tab_find(const uint32_t *tab, uint32_t tsize, uint32_t x) {
-  uint32_t h = tab_home(x, tsize);
-  while (true) {
-    const uint32_t k = tab[h];
-    if (k == x) return h;
-    if (k == kEmpty) return kEmpty;
-    h = (h + 1u == tsize) ? 0u : h + 1u;
-  }
-}
-
 // Record a confirmed triangle (r, c, w) in the sorted-list strategies: w = A[i] = B[j].
 template <int MODE>
-__device__ __forceinline__ void on_hit(uint32_t *cnt, uint32_t i, uint32_t *sup, uint32_t bj) {
+__device__ __forceinline__ void on_hit(uint32_t *cnt, uint32_t i, PAC_GLOBAL uint32_t *sup,
+                                       uint32_t bj) {
   if (MODE != kModeTC) atomicAdd(&cnt[i], 1u);
-  if (MODE == kModeSUP) atomicAdd(&sup[bj], 1u);
+  if (MODE == kModeSUP) gadd(&sup[bj], 1u);
 }
 
 // ---- the three lookup engines: one wave, one pair (x, y) ---------------------------
@@ -153,10 +176,10 @@ __device__ __forceinline__ void on_hit(uint32_t *cnt, uint32_t i, uint32_t *sup,
 // confirmed common neighbours.
 
 template <int MODE>
-__device__ __forceinline__ uint32_t probe_gallop(const uint32_t *__restrict__ oadj,
+__device__ __forceinline__ uint32_t probe_gallop(const PAC_GLOBAL uint32_t *__restrict__ oadj,
                                                  const uint32_t *A, uint32_t nA, uint32_t *cnt,
-                                                 uint32_t b0, uint32_t nb, uint32_t *sup,
-                                                 uint32_t lane) {
+                                                 uint32_t b0, uint32_t nb,
+                                                 PAC_GLOBAL uint32_t *sup, uint32_t lane) {
   uint32_t hits = 0;
   if (nb <= nA) {
     // B is the shorter list: each B element is binary-searched in A (LDS).
@@ -182,7 +205,7 @@ __device__ __forceinline__ uint32_t probe_gallop(const uint32_t *__restrict__ oa
   } else {
     // A is the shorter list: each A element is binary-searched in B (HBM), skipping
     // A elements outside B's value range.
-    const uint32_t *B = oadj + b0;
+    const PAC_GLOBAL uint32_t *B = oadj + b0;
     const uint32_t lo = B[0], hi = B[nb - 1];
     for (uint32_t i = lane; i < nA; i += 64) {
       const uint32_t x = A[i];
@@ -202,11 +225,11 @@ __device__ __forceinline__ uint32_t probe_gallop(const uint32_t *__restrict__ oa
 // on its diagonal; then it walks.  When B[j] is taken, every A element <= B[j] has
 // already been taken, so B[j] is a common element iff A[i-1] == B[j].
 template <int MODE>
-__device__ __forceinline__ uint32_t probe_merge(const uint32_t *__restrict__ oadj,
+__device__ __forceinline__ uint32_t probe_merge(const PAC_GLOBAL uint32_t *__restrict__ oadj,
                                                 const uint32_t *A, uint32_t nA, uint32_t *cnt,
-                                                uint32_t b0, uint32_t nb, uint32_t *sup,
-                                                uint32_t lane) {
-  const uint32_t *B = oadj + b0;
+                                                uint32_t b0, uint32_t nb,
+                                                PAC_GLOBAL uint32_t *sup, uint32_t lane) {
+  const PAC_GLOBAL uint32_t *B = oadj + b0;
   const uint32_t total = nA + nb;
   const uint32_t L = (total + 63u) >> 6;
   uint32_t d0 = lane * L;
@@ -244,7 +267,7 @@ __device__ __forceinline__ uint32_t probe_merge(const uint32_t *__restrict__ oad
 // each pair with readlane.  Wave w takes pair positions k_lo + w, k_lo + w + WAVES, ...
 // body(y, e, b0, nb) runs once per pair (e is only loaded when counts are kept).
 template <int WAVES, int MODE, class Body>
-__device__ __forceinline__ void for_each_pair(const GraphDev &G, uint32_t p0, uint32_t k_lo,
+__device__ __forceinline__ void for_each_pair(const GDev &G, uint32_t p0, uint32_t k_lo,
                                               uint32_t k_hi, uint32_t wave, uint32_t lane,
                                               Body body) {
   for (uint32_t t0 = 0;; t0 += 64) {
@@ -263,16 +286,10 @@ __device__ __forceinline__ void for_each_pair(const GraphDev &G, uint32_t p0, ui
   }
 }
 
-// Completes this wave's LDS atomics before its lanes read what other lanes added.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-}
-
 // ---- one row, sorted-list strategies (merge, gallop): A staged in LDS ---------------
-template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
-__device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32_t k_lo,
-                                           uint32_t k_hi, Smem<CAP, NBITS, MODE, WAVES> &sm,
+template <int STRAT, int MODE, int WAVES, int CAP>
+__device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k_lo,
+                                           uint32_t k_hi, Smem<CAP, MODE, WAVES> &sm,
                                            unsigned long long &acc) {
   constexpr uint32_t BS = 64u * WAVES;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -284,8 +301,10 @@ __device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32
     A[i] = G.oadj[a0 + i];
     if (MODE != kModeTC) cnt[i] = 0;
   }
+  if (tid == 0) sm.rowtot = 0;
   __syncthreads();
   if (k_hi > np) k_hi = np;
+  const bool whole = k_lo == 0 && k_hi == np;
   uint32_t row_hits = 0;
   for_each_pair<WAVES, MODE>(G, p0, k_lo, k_hi, wave, lane,
                              [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb) {
@@ -302,9 +321,9 @@ __device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32
         if (e - a0 < nA)
           atomicAdd(&cnt[e - a0], ch);  // r->y: y = A[e - a0]
         else if (MODE == kModeLCC)
-          atomicAdd(&G.tvert[y], (unsigned long long)ch);
+          G.etail[e] = ch;  // y->r: credited to y (each pair is visited once)
         else
-          atomicAdd(&G.sup[e], ch);
+          gadd(&G.sup[e], ch);
       }
     }
     row_hits += h;
@@ -312,175 +331,177 @@ __device__ __forceinline__ void row_sorted(const GraphDev &G, uint32_t r, uint32
   acc += row_hits;
   if (MODE == kModeLCC) {
     const uint32_t wh = wave_sum(row_hits);
-    if (lane == 0 && wh) atomicAdd(&G.tvert[r], (unsigned long long)wh);
+    if (lane == 0 && wh) atomicAdd(&sm.rowtot, wh);
   }
   __syncthreads();
   if (MODE != kModeTC) {
     for (uint32_t i = tid; i < nA; i += BS) {
       const uint32_t v = cnt[i];
       if (v) {
-        if (MODE == kModeLCC) atomicAdd(&G.tvert[A[i]], (unsigned long long)v);
-        if (MODE == kModeSUP) atomicAdd(&G.sup[a0 + i], v);
+        if (MODE == kModeLCC) credit_edge(&G.ehead[a0 + i], v, whole);
+        if (MODE == kModeSUP) gadd(&G.sup[a0 + i], v);
       }
+    }
+    if (MODE == kModeLCC && tid == 0 && sm.rowtot) {
+      if (whole)
+        G.tvert[r] = sm.rowtot;
+      else
+        gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
     }
   }
   __syncthreads();
 }
 
-// ---- one row, bitmap strategy: hashed filter bitmap + LDS hash table of A -------------
-// The table has tsize = max(64, min(2*nA, TABCAP)) slots (load factor <= 2/3), indexed
-// by the high half of (hash * tsize).  A wave takes its pairs in batches of 64 (lane l
-// loads partner y_l, the bounds of N+(y_l) and, when y_l is in A, its table slot), then
-// streams the concatenation of the batch's 64 lists 256 elements at a time, dense
-// across lanes and coalesced; each lane finds the list its element belongs to by
-// walking the batch's inclusive prefix sums (held one per lane, read with ds_bpermute),
-// which only moves forward.
-template <int MODE, int WAVES, int CAP, int NBITS>
-__device__ __forceinline__ void row_bitmap(const GraphDev &G, uint32_t r, uint32_t c_lo,
-                                           uint32_t c_hi, Smem<CAP, NBITS, MODE, WAVES> &sm,
-                                           unsigned long long &acc) {
+// ---- one row, hash strategy: two-choice cuckoo table of A in LDS ---------------------
+// A is held in a table of 2^lg buckets of two slots (2^lg >= 5/8 nA, so the load
+// factor is at most 0.8).  Key x may sit in bucket h1(x) or h2(x); a lookup reads both
+// buckets (two 8-byte LDS reads) and compares four slots -- a fixed cost with no loop
+// and no divergence.  Insertion is a bounded random walk: a key takes a free slot of
+// its two buckets, else evicts a resident and carries it on; a key still homeless
+// after kMaxKicks evictions goes to a small stash that lookups also scan.  If the
+// stash overflows the row is done by the sorted-list path instead (exact either way).
+// Each pair's list N+(y) is streamed by one wave, 4 x 64 elements per step with
+// coalesced loads; the pair's hit count is a ballot popcount held in scalar registers.
+constexpr uint32_t kMaxKicks = 48;
+__device__ __forceinline__ uint32_t hash1(uint32_t x) { return x * 0x9E3779B1u; }
+__device__ __forceinline__ uint32_t hash2(uint32_t x) { return (x ^ (x >> 15)) * 0x85EBCA6Bu; }
+
+template <int MODE, int WAVES, int CAP>
+__device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_lo,
+                                         uint32_t c_hi, Smem<CAP, MODE, WAVES> &sm,
+                                         unsigned long long &acc) {
   constexpr uint32_t BS = 64u * WAVES;
-  constexpr uint32_t TABCAP = Smem<CAP, NBITS, MODE, WAVES>::kTabCap;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
   const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
-  uint32_t *tab = sm.u.h.tab;
-  uint32_t *cnt16 = sm.u.h.cnt16;
-  uint32_t *pos16 = sm.u.h.pos16;
-  uint32_t *pc = sm.u.h.pcnt + 64u * wave;
-  uint32_t tsize = 2u * nA;
-  if (tsize > TABCAP) tsize = TABCAP;
-  if (tsize < 64u) tsize = 64u;
-  for (uint32_t i = tid; i < tsize; i += BS) tab[i] = kEmpty;
-  if (MODE != kModeTC) {
-    for (uint32_t i = tid; i < (tsize + 1) / 2; i += BS) cnt16[i] = 0;
-    for (uint32_t i = tid; i < BS; i += BS) sm.u.h.pcnt[i] = 0;
-  }
+  auto &H = sm.u.h;
+  uint32_t lg = 4;
+  while ((8u << lg) < 5u * nA) ++lg;  // 2^lg >= 5/8 nA
+  const uint32_t nslot = 2u << lg, sh = 32u - lg;
+  for (uint32_t i = tid; i < nslot; i += BS) H.tab[i] = kEmpty;
+  if (MODE != kModeTC)
+    for (uint32_t i = tid; i < (nslot + kStash) / 2; i += BS) H.cnt16[i] = 0;
   if (MODE == kModeSUP)
-    for (uint32_t i = tid; i < (tsize + 1) / 2; i += BS) pos16[i] = 0;
+    for (uint32_t i = tid; i < (nslot + kStash) / 2; i += BS) H.pos16[i] = 0;
+  if (tid == 0) {
+    sm.nstash = 0;
+    sm.rowtot = 0;
+  }
   __syncthreads();
   for (uint32_t i = tid; i < nA; i += BS) {
-    const uint32_t w = G.oadj[a0 + i];
-    uint32_t h = tab_home(w, tsize);
-    while (atomicCAS(&tab[h], kEmpty, w) != kEmpty) h = (h + 1u == tsize) ? 0u : h + 1u;
-    if (MODE == kModeSUP) add16(pos16, h, i);
-    const uint32_t hb = bit_of<NBITS>(w);
-    atomicOr(&sm.bits[hb >> 5], 1u << (hb & 31u));
+    uint32_t x = G.oadj[a0 + i];
+    for (uint32_t kick = 0;; ++kick) {
+      const uint32_t b1 = hash1(x) >> sh, b2 = hash2(x) >> sh;
+      if (atomicCAS(&H.tab[2u * b1], kEmpty, x) == kEmpty) break;
+      if (atomicCAS(&H.tab[2u * b1 + 1u], kEmpty, x) == kEmpty) break;
+      if (atomicCAS(&H.tab[2u * b2], kEmpty, x) == kEmpty) break;
+      if (atomicCAS(&H.tab[2u * b2 + 1u], kEmpty, x) == kEmpty) break;
+      if (kick == kMaxKicks) {
+        const uint32_t k = atomicAdd(&sm.nstash, 1u);
+        if (k < kStash) H.tab[nslot + k] = x;
+        break;
+      }
+      // evict from the two buckets in turn, alternating slots
+      x = atomicExch(&H.tab[((kick & 1u) ? 2u * b2 : 2u * b1) + ((kick >> 1) & 1u)], x);
+      if (x == kEmpty) break;
+    }
   }
   __syncthreads();
+  const uint32_t nst = sm.nstash;  // workgroup-uniform
+  if (nst > kStash) return false;
+  // slot of w in the table (kEmpty if absent)
+  auto find = [&](uint32_t w) -> uint32_t {
+    const uint32_t b1 = hash1(w) >> sh, b2 = hash2(w) >> sh;
+    const uint32_t k10 = H.tab[2u * b1], k11 = H.tab[2u * b1 + 1u];
+    const uint32_t k20 = H.tab[2u * b2], k21 = H.tab[2u * b2 + 1u];
+    uint32_t slot = kEmpty;
+    slot = k21 == w ? 2u * b2 + 1u : slot;
+    slot = k20 == w ? 2u * b2 : slot;
+    slot = k11 == w ? 2u * b1 + 1u : slot;
+    slot = k10 == w ? 2u * b1 : slot;
+    for (uint32_t k = 0; k < nst; ++k) slot = H.tab[nslot + k] == w ? nslot + k : slot;
+    return slot;
+  };
+  if (MODE == kModeSUP) {
+    for (uint32_t i = tid; i < nA; i += BS) add16(H.pos16, find(G.oadj[a0 + i]), i);
+    __syncthreads();
+  }
   if (c_hi > np) c_hi = np;
-  uint32_t row_hits = 0;
-  for (uint32_t t0 = 0;; t0 += 64) {
-    const uint32_t kl = c_lo + wave + (uint32_t)WAVES * (t0 + lane);
-    uint32_t yl = 0, el = 0, bl = 0, nl = 0, sc = kEmpty;
-    if (kl < c_hi) {
-      yl = G.ppar[p0 + kl];
-      bl = G.ooff[yl];
-      nl = G.ooff[yl + 1] - bl;
-      if (MODE != kModeTC) {
-        el = G.pedge[p0 + kl];
-        if (el - a0 < nA) sc = tab_find(tab, tsize, yl);  // r->y: y is in A
-      }
-    }
-    const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
-    // inclusive prefix of the list lengths over the batch's lanes
-    uint32_t incl = nl;
+  const bool whole = c_lo == 0 && c_hi == np;
+  uint32_t row_hits = 0;  // wave-uniform
+  for_each_pair<WAVES, MODE>(G, p0, c_lo, c_hi, wave, lane,
+                             [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb) {
+    // N+(y) in blocks of 4 x 64 elements; 64-blocks past its end are skipped
+    uint32_t ph = 0;  // hits of this pair, wave-uniform
+    for (uint32_t j0 = 0; j0 < nb; j0 += 256u) {
+      uint32_t w[4];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t v = __shfl_up(incl, o, 64);
-      if (lane >= (uint32_t)o) incl += v;
-    }
-    const uint32_t total = readlane(incl, 63);
-    const uint32_t base = bl - (incl - nl);  // flat index q of list l sits at oadj[base_l + q]
-    uint32_t cur = 0;  // owner walk, monotone in q; always < 64 (see below)
-    uint32_t hits = 0;
-    for (uint32_t q0 = 0; q0 < total; q0 += 256u) {  // wave-uniform loop
-      uint32_t w[4], addr[4], so[4], own[4];
-      bool ok[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t q = q0 + 64u * u + lane;
-        ok[u] = q < total;
-        // The owner is the first lane whose inclusive prefix exceeds q.  Lanes past the
-        // end use q = total-1, so every walk stops at a lane <= 63 (incl[63] = total).
-        // Every shuffle runs with all 64 lanes active (bpermute reads of inactive lanes
-        // are not defined), and the walk continues while any lane still has to advance.
-        const uint32_t qc = ok[u] ? q : total - 1u;
-        while (true) {
-          const bool adv = (uint32_t)__shfl(incl, (int)cur, 64) <= qc;
-          if (!__any(adv)) break;
-          cur += adv ? 1u : 0u;
-        }
-        own[u] = cur;
-        addr[u] = (uint32_t)__shfl(base, (int)cur, 64) + qc;
-        so[u] = (MODE != kModeTC) ? (uint32_t)__shfl(sc, (int)cur, 64) : 0u;
-        w[u] = ok[u] ? G.oadj[addr[u]] : 0u;
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t j = j0 + 64u * u + lane;
+        w[u] = j < nb ? G.oadj[b0 + j] : 0u;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (!ok[u]) continue;
-        const uint32_t hb = bit_of<NBITS>(w[u]);
-        if ((sm.bits[hb >> 5] >> (hb & 31u)) & 1u) {
-          const uint32_t slot = tab_find(tab, tsize, w[u]);
-          if (slot != kEmpty) {
-            ++hits;
-            if (MODE != kModeTC) {
-              add16(cnt16, slot, 1u);  // t(w) / edge r->w
-              if (so[u] != kEmpty)
-                add16(cnt16, so[u], 1u);  // t(y) / edge r->y, y in A
-              else
-                atomicAdd(&pc[own[u]], 1u);  // t(y) / edge y->r
-            }
-            if (MODE == kModeSUP) atomicAdd(&G.sup[addr[u]], 1u);  // edge y->w
-          }
+      for (uint32_t u = 0; u < 4; ++u) {
+        if (j0 + 64u * u >= nb) break;  // wave-uniform
+        const uint32_t j = j0 + 64u * u + lane;
+        const uint32_t slot = j < nb ? find(w[u]) : kEmpty;
+        const bool hit = slot != kEmpty;
+        ph += (uint32_t)__popcll(__ballot(hit));
+        if (hit) {
+          if (MODE != kModeTC) add16(H.cnt16, slot, 1u);    // t(w) / edge r->w
+          if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);  // edge y->w
         }
       }
     }
-    if (MODE != kModeTC) {
-      // pairs y->r: one global atomic per pair with hits, then the counter is reset
-      wave_lds_sync();
-      if (kl < c_hi && sc == kEmpty) {
-        const uint32_t v = pc[lane];
-        if (v) {
-          pc[lane] = 0;
-          if (MODE == kModeLCC) atomicAdd(&G.tvert[yl], (unsigned long long)v);
-          if (MODE == kModeSUP) atomicAdd(&G.sup[el], v);
-        }
-      }
-      wave_lds_sync();
+    row_hits += ph;
+    // t(y) / the pair's edge: r->y adds to y's slot in the table, y->r to the global count
+    if (MODE != kModeTC && ph != 0 && lane == 0) {
+      if (e - a0 < nA)
+        add16(H.cnt16, find(y), ph);
+      else if (MODE == kModeLCC)
+        G.etail[e] = ph;  // y->r: credited to y (each pair is visited once)
+      else
+        gadd(&G.sup[e], ph);
     }
-    row_hits += hits;
-    if (nvalid < 64) break;
-  }
-  acc += row_hits;
+  });
+  // row_hits is the wave's total (uniform): counted once into acc, which is summed over
+  // the wave's lanes at the end, so lane 0 carries it
+  if (lane == 0) acc += row_hits;
+  if (MODE == kModeLCC && lane == 0 && row_hits) atomicAdd(&sm.rowtot, row_hits);
+  __syncthreads();
   if (MODE == kModeLCC) {
-    const uint32_t wh = wave_sum(row_hits);
-    if (lane == 0 && wh) atomicAdd(&G.tvert[r], (unsigned long long)wh);
-  }
-  __syncthreads();
-  for (uint32_t i = tid; i < tsize; i += BS) {
-    const uint32_t key = tab[i];
-    if (key == kEmpty) continue;
-    if (MODE != kModeTC) {
-      const uint32_t v = get16(cnt16, i);
-      if (v) {
-        if (MODE == kModeLCC) atomicAdd(&G.tvert[key], (unsigned long long)v);
-        if (MODE == kModeSUP) atomicAdd(&G.sup[a0 + get16(pos16, i)], v);
-      }
+    // per edge r->A[i], in A order (coalesced): the credits of A[i]'s slot
+    for (uint32_t i = tid; i < nA; i += BS) {
+      const uint32_t v = get16(H.cnt16, find(G.oadj[a0 + i]));
+      if (v) credit_edge(&G.ehead[a0 + i], v, whole);
     }
-    sm.bits[bit_of<NBITS>(key) >> 5] = 0u;
+    if (tid == 0 && sm.rowtot) {
+      if (whole)
+        G.tvert[r] = sm.rowtot;
+      else
+        gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
+    }
+  }
+  if (MODE == kModeSUP) {
+    for (uint32_t i = tid; i < nslot + nst; i += BS) {
+      const uint32_t key = H.tab[i];
+      if (key == kEmpty) continue;
+      const uint32_t v = get16(H.cnt16, i);
+      if (v) gadd(&G.sup[a0 + get16(H.pos16, i)], v);
+    }
   }
   __syncthreads();
+  return true;
 }
 
 // ---- one row whose N+(r) exceeds the LDS capacity: lookups in HBM ------------------
 template <int MODE, int WAVES>
-__device__ __forceinline__ void row_hbm(const GraphDev &G, uint32_t r, uint32_t c_lo,
+__device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo,
                                         uint32_t c_hi, unsigned long long &acc) {
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
   const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
-  const uint32_t *A = G.oadj + a0;
+  const PAC_GLOBAL uint32_t *A = G.oadj + a0;
   if (c_hi > np) c_hi = np;
   uint32_t row_hits = 0;
   for_each_pair<WAVES, MODE>(G, p0, c_lo, c_hi, wave, lane,
@@ -491,18 +512,23 @@ __device__ __forceinline__ void row_hbm(const GraphDev &G, uint32_t r, uint32_t 
       const uint32_t i = lower_bound_u32(A, nA, w);
       if (i < nA && A[i] == w) {
         ++h;
-        if (MODE == kModeLCC) atomicAdd(&G.tvert[w], 1ull);
+        if (MODE == kModeLCC) gadd(&G.ehead[a0 + i], 1u);
         if (MODE == kModeSUP) {
-          atomicAdd(&G.sup[a0 + i], 1u);
-          atomicAdd(&G.sup[b0 + j], 1u);
+          gadd(&G.sup[a0 + i], 1u);
+          gadd(&G.sup[b0 + j], 1u);
         }
       }
     }
     if (MODE != kModeTC) {
       const uint32_t ch = wave_sum(h);
       if (lane == 0 && ch) {
-        if (MODE == kModeLCC) atomicAdd(&G.tvert[y], (unsigned long long)ch);
-        if (MODE == kModeSUP) atomicAdd(&G.sup[e], ch);
+        if (MODE == kModeLCC) {
+          if (e - a0 < nA)
+            gadd(&G.ehead[e], ch);  // r->y
+          else
+            G.etail[e] = ch;        // y->r, visited once
+        }
+        if (MODE == kModeSUP) gadd(&G.sup[e], ch);
       }
     }
     row_hits += h;
@@ -510,65 +536,63 @@ __device__ __forceinline__ void row_hbm(const GraphDev &G, uint32_t r, uint32_t 
   acc += row_hits;
   if (MODE == kModeLCC) {
     const uint32_t wh = wave_sum(row_hits);
-    if (lane == 0 && wh) atomicAdd(&G.tvert[r], (unsigned long long)wh);
+    if (lane == 0 && wh) gadd(&G.tvert[r], (unsigned long long)wh);
   }
 }
 
 // One row (or a pair range of one) with the given strategy.
-template <int MODE, int WAVES, int CAP, int NBITS>
-__device__ __forceinline__ void do_row(const GraphDev &G, uint32_t r, uint32_t c_lo,
-                                       uint32_t c_hi, int st, Smem<CAP, NBITS, MODE, WAVES> &sm,
+template <int MODE, int WAVES, int CAP>
+__device__ __forceinline__ void do_row(const GDev &G, uint32_t r, uint32_t c_lo,
+                                       uint32_t c_hi, int st, Smem<CAP, MODE, WAVES> &sm,
                                        unsigned long long &acc) {
   const uint32_t nA = G.ooff[r + 1] - G.ooff[r];
   if (nA > (uint32_t)CAP) {
     row_hbm<MODE, WAVES>(G, r, c_lo, c_hi, acc);
   } else if (st == kBitmap) {
-    if (MODE == kModeTC) {
-      row_bitmap<MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
-    } else {
-      const uint32_t np = G.poff[r + 1] - G.poff[r];
-      const uint32_t hi = c_hi < np ? c_hi : np;
-      for (uint32_t lo = c_lo; lo < hi; lo = (hi - lo > kPassPairs) ? lo + kPassPairs : hi)
-        row_bitmap<MODE, WAVES, CAP, NBITS>(
-            G, r, lo, (hi - lo > kPassPairs) ? lo + kPassPairs : hi, sm, acc);
+    // passes of at most kPassPairs pairs keep the 16-bit slot counters from carrying
+    const uint32_t np = G.poff[r + 1] - G.poff[r];
+    const uint32_t hi = c_hi < np ? c_hi : np;
+    for (uint32_t lo = c_lo; lo < hi;) {
+      const uint32_t e = (MODE == kModeTC || hi - lo <= kPassPairs) ? hi : lo + kPassPairs;
+      if (!row_hash<MODE, WAVES, CAP>(G, r, lo, e, sm, acc))
+        row_sorted<kGallop, MODE, WAVES, CAP>(G, r, lo, e, sm, acc);
+      lo = e;
     }
   } else if (st == kGallop) {
-    row_sorted<kGallop, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
+    row_sorted<kGallop, MODE, WAVES, CAP>(G, r, c_lo, c_hi, sm, acc);
   } else {
-    row_sorted<kMerge, MODE, WAVES, CAP, NBITS>(G, r, c_lo, c_hi, sm, acc);
+    row_sorted<kMerge, MODE, WAVES, CAP>(G, r, c_lo, c_hi, sm, acc);
   }
 }
 
 template <int WAVES>
-__device__ __forceinline__ void flush_total(unsigned long long *total, unsigned long long acc) {
+__device__ __forceinline__ void flush_total(PAC_GLOBAL unsigned long long *total,
+                                            unsigned long long acc) {
   const unsigned long long s = wave_sum64(acc);
-  if ((threadIdx.x & 63u) == 0 && s) atomicAdd(total, s);
+  if ((threadIdx.x & 63u) == 0 && s) gadd(total, s);
 }
 
 // ---- plain kernel: one workgroup per row, one launch per partition group -----------
-template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
-__global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev G, uint32_t row_begin) {
-  __shared__ Smem<CAP, NBITS, MODE, WAVES> sm;
-  if (STRAT == kBitmap)
-    for (uint32_t i = threadIdx.x; i < NBITS / 32; i += 64 * WAVES) sm.bits[i] = 0;
-  __syncthreads();
+template <int STRAT, int MODE, int WAVES, int CAP>
+__global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev Gd, uint32_t row_begin) {
+  __shared__ Smem<CAP, MODE, WAVES> sm;
+  const GDev G = to_dev(Gd);
   const uint32_t r = G.rows[row_begin + blockIdx.x];
   unsigned long long acc = 0;
-  do_row<MODE, WAVES, CAP, NBITS>(G, r, 0, kAllC, STRAT, sm, acc);
+  do_row<MODE, WAVES, CAP>(G, r, 0, kAllC, STRAT, sm, acc);
   flush_total<WAVES>(G.total, acc);
 }
 
 // ---- persistent kernel: every workgroup pulls work items from one atomic queue --------
-template <int MODE, int WAVES, int CAP, int NBITS>
+template <int MODE, int WAVES, int CAP>
 __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__restrict__ Gs,
                                                            const WorkItem *__restrict__ items,
                                                            uint32_t nitems,
                                                            const uint8_t *__restrict__ strat,
                                                            uint32_t *queue) {
-  __shared__ Smem<CAP, NBITS, MODE, WAVES> s;
-  for (uint32_t i = threadIdx.x; i < NBITS / 32; i += 64 * WAVES) s.bits[i] = 0;
+  __shared__ Smem<CAP, MODE, WAVES> s;
   unsigned long long acc = 0;
-  unsigned long long *cur_total = nullptr;
+  PAC_GLOBAL unsigned long long *cur_total = nullptr;
   while (true) {
     __syncthreads();
     if (threadIdx.x == 0) s.item = atomicAdd(queue, 1u);
@@ -576,7 +600,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__res
     const uint32_t it = s.item;
     if (it >= nitems) break;
     const WorkItem w = items[it];
-    const GraphDev G = Gs[w.g];
+    const GDev G = to_dev(Gs[w.g]);
     if (G.total != cur_total) {
       if (cur_total) flush_total<WAVES>(cur_total, acc);
       acc = 0;
@@ -586,7 +610,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__res
     const bool single = (w.hi == w.lo + 1);
     const uint32_t c_lo = single ? w.c_lo : 0u, c_hi = single ? w.c_hi : kAllC;
     for (uint32_t q = w.lo; q < w.hi; ++q)
-      do_row<MODE, WAVES, CAP, NBITS>(G, G.rows[q], c_lo, c_hi, st, s, acc);
+      do_row<MODE, WAVES, CAP>(G, G.rows[q], c_lo, c_hi, st, s, acc);
   }
   if (cur_total) flush_total<WAVES>(cur_total, acc);
 }
@@ -629,6 +653,55 @@ __global__ void k_rowwork(const uint32_t *__restrict__ poff, const uint32_t *__r
 }
 
 // ---- epilogues -----------------------------------------------------------------------
+// LCC per-vertex triangle counts: t(v) = (v's row totals, already in tvert)
+//   + sum over out-edges e of v of etail[e] + sum over in-edges e of v of ehead[e].
+// One wave per vertex; a hub's in-edges beyond the first kHubIn are left to
+// k_vertex_hub, which splits them over several workgroups.
+__global__ void k_vertex_tri(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ etail,
+                             const uint32_t *__restrict__ inoff, const uint32_t *__restrict__ inidx,
+                             const uint32_t *__restrict__ ehead, uint32_t V,
+                             unsigned long long *__restrict__ tvert) {
+  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
+  for (uint64_t v = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); v < V;
+       v += (uint64_t)gridDim.x * wpb) {
+    unsigned long long s = 0;
+    for (uint32_t e = ooff[v] + lane; e < ooff[v + 1]; e += 64) s += etail[e];
+    const uint32_t k0 = inoff[v], k1 = min(inoff[v + 1], k0 + kHubIn);
+    for (uint32_t k = k0 + lane; k < k1; k += 256) {
+      uint32_t x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[u] = k + 64u * u < k1 ? ehead[inidx[k + 64u * u]] : 0u;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s += x[u];
+    }
+    s = wave_sum64(s);
+    if (lane == 0 && s) tvert[v] += s;
+  }
+}
+
+// The in-edges of hub h past its first kHubIn, in slices of kHubIn per workgroup.
+__global__ void k_vertex_hub(const uint32_t *__restrict__ hubs, const uint32_t *__restrict__ inoff,
+                             const uint32_t *__restrict__ inidx,
+                             const uint32_t *__restrict__ ehead,
+                             unsigned long long *__restrict__ tvert) {
+  const uint32_t h = hubs[blockIdx.x];
+  const uint64_t kbeg = inoff[h], kend = inoff[h + 1];
+  unsigned long long s = 0;
+  for (uint64_t k0 = kbeg + (uint64_t)kHubIn * (1u + blockIdx.y); k0 < kend;
+       k0 += (uint64_t)kHubIn * gridDim.y) {
+    const uint64_t k1 = k0 + kHubIn < kend ? k0 + kHubIn : kend;
+    for (uint64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) s += ehead[inidx[k]];
+  }
+  s = wave_sum64(s);
+  __shared__ unsigned long long ws[16];
+  if ((threadIdx.x & 63u) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) s += ws[w];
+    if (s) atomicAdd(&tvert[h], s);
+  }
+}
+
 // LCC of Graph::solve_lcc_merge (Graph.cpp:828-840): numerator 2t(u) and denominator
 // d*(d-1) are exact integers converted to double, one IEEE division, then one
 // rounding to float.  d = 0 keeps 0.0, d = 1 gives 0/0 = NaN, as the reference.
@@ -715,18 +788,18 @@ __global__ void k_pairs(const uint32_t *__restrict__ off, const uint32_t *__rest
   }
 }
 
-template <int MODE, int WAVES, int CAP, int NBITS>
+template <int MODE, int WAVES, int CAP>
 int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint64_t *launches) {
   int per_cu = 0;
   PAC_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void *>(k_persistent<MODE, WAVES, CAP, NBITS>),
+      &per_cu, reinterpret_cast<const void *>(k_persistent<MODE, WAVES, CAP>),
       64 * WAVES, 0));
   if (per_cu < 1) per_cu = 1;
   uint32_t grid = (uint32_t)(per_cu * ctx->num_cus);
   if (grid > nitems) grid = nitems ? nitems : 1;
   hipLaunchKernelGGL((k_reset_queue), dim3(1), dim3(1), 0, ctx->stream, ctx->queue.as<uint32_t>());
   if (ctx->time_kernel) PAC_CHECK_HIP(hipEventRecord(ctx->kev[0], ctx->stream));
-  hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, NBITS>), dim3(grid), dim3(64 * WAVES), 0,
+  hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP>), dim3(grid), dim3(64 * WAVES), 0,
                      ctx->stream, ctx->gdesc.as<const GraphDev>(), items, nitems,
                      ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>());
   PAC_CHECK_HIP(hipGetLastError());
@@ -738,10 +811,10 @@ int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint64_
 template <int MODE>
 int run_persistent_mode(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint64_t *launches) {
   switch (ctx->tile) {
-    case 0: return run_persistent_cfg<MODE, 2, 1024, 16384>(ctx, items, nitems, launches);
-    case 1: return run_persistent_cfg<MODE, 4, 2048, 32768>(ctx, items, nitems, launches);
-    case 2: return run_persistent_cfg<MODE, 4, 4096, 65536>(ctx, items, nitems, launches);
-    default: return run_persistent_cfg<MODE, 8, 4096, 65536>(ctx, items, nitems, launches);
+    case 0: return run_persistent_cfg<MODE, 2, 1024>(ctx, items, nitems, launches);
+    case 1: return run_persistent_cfg<MODE, 4, 2048>(ctx, items, nitems, launches);
+    case 2: return run_persistent_cfg<MODE, 4, 4096>(ctx, items, nitems, launches);
+    default: return run_persistent_cfg<MODE, 8, 4096>(ctx, items, nitems, launches);
   }
 }
 
@@ -752,9 +825,9 @@ int run_persistent_any(Ctx *ctx, int mode, const WorkItem *items, uint32_t nitem
   return run_persistent_mode<kModeTC>(ctx, items, nitems, launches);
 }
 
-template <int STRAT, int MODE, int WAVES, int CAP, int NBITS>
+template <int STRAT, int MODE, int WAVES, int CAP>
 int run_plain_cfg(Ctx *ctx, const GraphDev &G, uint32_t row_begin, uint32_t nrows) {
-  hipLaunchKernelGGL((k_plain<STRAT, MODE, WAVES, CAP, NBITS>), dim3(nrows), dim3(64 * WAVES), 0,
+  hipLaunchKernelGGL((k_plain<STRAT, MODE, WAVES, CAP>), dim3(nrows), dim3(64 * WAVES), 0,
                      ctx->stream, G, row_begin);
   PAC_CHECK_HIP(hipGetLastError());
   return PAC_OK;
@@ -763,10 +836,10 @@ int run_plain_cfg(Ctx *ctx, const GraphDev &G, uint32_t row_begin, uint32_t nrow
 template <int STRAT, int MODE>
 int run_plain_tile(Ctx *ctx, const GraphDev &G, uint32_t row_begin, uint32_t nrows) {
   switch (ctx->tile) {
-    case 0: return run_plain_cfg<STRAT, MODE, 2, 1024, 16384>(ctx, G, row_begin, nrows);
-    case 1: return run_plain_cfg<STRAT, MODE, 4, 2048, 32768>(ctx, G, row_begin, nrows);
-    case 2: return run_plain_cfg<STRAT, MODE, 4, 4096, 65536>(ctx, G, row_begin, nrows);
-    default: return run_plain_cfg<STRAT, MODE, 8, 4096, 65536>(ctx, G, row_begin, nrows);
+    case 0: return run_plain_cfg<STRAT, MODE, 2, 1024>(ctx, G, row_begin, nrows);
+    case 1: return run_plain_cfg<STRAT, MODE, 4, 2048>(ctx, G, row_begin, nrows);
+    case 2: return run_plain_cfg<STRAT, MODE, 4, 4096>(ctx, G, row_begin, nrows);
+    default: return run_plain_cfg<STRAT, MODE, 8, 4096>(ctx, G, row_begin, nrows);
   }
 }
 
@@ -796,6 +869,8 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   d.rows = g->rows.as<const uint32_t>();
   d.total = g->total.as<unsigned long long>();
   d.tvert = g->tvert.as<unsigned long long>();
+  d.ehead = g->ehead.as<uint32_t>();
+  d.etail = g->etail.as<uint32_t>();
   d.sup = g->sup.as<uint32_t>();
   d.V = g->V;
   d.part_base = part_base;
@@ -850,7 +925,11 @@ int alloc_outputs(Graph *const *gs, uint32_t G, int mode) {
   for (uint32_t i = 0; i < G; ++i) {
     Graph *g = gs[i];
     PAC_TRY(g->total.reserve(sizeof(unsigned long long)));
-    if (mode == kModeLCC) PAC_TRY(g->tvert.reserve(sizeof(unsigned long long) * (size_t)(g->V ? g->V : 1)));
+    if (mode == kModeLCC) {
+      PAC_TRY(g->tvert.reserve(sizeof(unsigned long long) * (size_t)(g->V ? g->V : 1)));
+      PAC_TRY(g->ehead.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
+      PAC_TRY(g->etail.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
+    }
     if (mode == kModeSUP) PAC_TRY(g->sup.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
   }
   return PAC_OK;
@@ -866,6 +945,10 @@ int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
     if (mode == kModeLCC && g->V)
       hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
                          g->tvert.as<unsigned long long>(), (uint64_t)g->V);
+    if (mode == kModeLCC && g->m) {
+      PAC_TRY(fill_u32(ctx->stream, g->ehead.as<uint32_t>(), g->m, 0u));
+      PAC_TRY(fill_u32(ctx->stream, g->etail.as<uint32_t>(), g->m, 0u));
+    }
     if (mode == kModeSUP) PAC_TRY(fill_u32(ctx->stream, g->sup.as<uint32_t>(), g->m, 0u));
     PAC_CHECK_HIP(hipGetLastError());
   }
@@ -1003,6 +1086,26 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   }
   PAC_CHECK_HIP(hipGraphLaunch(ctx->hexec, ctx->stream));
   if (launches) *launches += 1;
+  return PAC_OK;
+}
+
+int vertex_counts(Ctx *ctx, Graph *g) {
+  if (g->V == 0) return PAC_OK;
+  hipLaunchKernelGGL(k_vertex_tri, dim3(grid_for(g->V, 4)), dim3(256), 0, ctx->stream,
+                     g->ooff.as<const uint32_t>(), g->etail.as<const uint32_t>(),
+                     g->inoff.as<const uint32_t>(), g->inidx.as<const uint32_t>(),
+                     g->ehead.as<const uint32_t>(), g->V, g->tvert.as<unsigned long long>());
+  PAC_CHECK_HIP(hipGetLastError());
+  if (!g->h_hubs.empty()) {
+    uint64_t slices = (g->max_deg + kHubIn - 1) / kHubIn;  // in-degree <= max degree
+    if (slices < 1) slices = 1;
+    if (slices > 4096) slices = 4096;  // the kernel strides over further slices
+    hipLaunchKernelGGL(k_vertex_hub, dim3((uint32_t)g->h_hubs.size(), (uint32_t)slices), dim3(256),
+                       0, ctx->stream, g->hubs.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
+                       g->inidx.as<const uint32_t>(), g->ehead.as<const uint32_t>(),
+                       g->tvert.as<unsigned long long>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
   return PAC_OK;
 }
 

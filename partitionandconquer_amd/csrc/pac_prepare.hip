@@ -157,10 +157,11 @@ __device__ __forceinline__ bool owns(uint32_t dx, uint32_t x, uint32_t ox, uint3
   return *fwd ? (ou <= ox && ou > 0u) : (ox > ou);
 }
 
-// pairs owned per vertex (one wave per vertex, over its symmetric row)
+// pairs owned per vertex (one wave per vertex, over its symmetric row) and in-degree
 __global__ void k_pair_count(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                              const uint32_t *__restrict__ deg, const uint32_t *__restrict__ odeg,
-                             uint32_t V, uint32_t *__restrict__ npairs) {
+                             uint32_t V, uint32_t *__restrict__ npairs,
+                             uint32_t *__restrict__ nin) {
   const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
   for (uint64_t x = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); x < V;
        x += (uint64_t)gridDim.x * wpb) {
@@ -176,23 +177,28 @@ __global__ void k_pair_count(const uint32_t *__restrict__ off, const uint32_t *_
       }
       cnt += (uint32_t)__popcll(__ballot(own));
     }
-    if (lane == 0) npairs[x] = cnt;
+    if (lane == 0) {
+      npairs[x] = cnt;
+      nin[x] = dx - ox;
+    }
   }
 }
 
 // Order-preserving fill: x's pairs come out sorted by partner ID.  The edge index of a
 // pair x->u is ooff[x] + (rank of u among x's out-neighbours); of u->x it is the
-// position of x in N+(u), found by binary search.
+// position of x in N+(u), found by binary search.  The in-edge list of x (the edge
+// indices of all u->x, in u order) is written alongside.
 __global__ void k_pair_fill(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                             const uint32_t *__restrict__ deg, const uint32_t *__restrict__ odeg,
                             const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
-                            const uint32_t *__restrict__ poff, uint32_t V,
-                            uint32_t *__restrict__ ppar, uint32_t *__restrict__ pedge) {
+                            const uint32_t *__restrict__ poff, const uint32_t *__restrict__ inoff,
+                            uint32_t V, uint32_t *__restrict__ ppar, uint32_t *__restrict__ pedge,
+                            uint32_t *__restrict__ inidx) {
   const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
   for (uint64_t x = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); x < V;
        x += (uint64_t)gridDim.x * wpb) {
     const uint32_t a0 = off[x], a1 = off[x + 1], dx = a1 - a0, ox = odeg[x];
-    uint32_t out = poff[x], rank = ooff[x];
+    uint32_t out = poff[x], rank = ooff[x], in = inoff[x];
     for (uint32_t i0 = a0; i0 < a1; i0 += 64) {
       const uint32_t i = i0 + lane;
       bool own = false, fwd = false;
@@ -201,32 +207,36 @@ __global__ void k_pair_fill(const uint32_t *__restrict__ off, const uint32_t *__
         u = adj[i];
         own = owns(dx, (uint32_t)x, ox, deg[u], u, odeg[u], &fwd);
       }
+      const bool back = i < a1 && !fwd;
       const uint64_t fmask = __ballot(fwd);
       const uint64_t omask = __ballot(own);
-      if (own) {
-        uint32_t e;
-        if (fwd) {
-          e = rank + lanes_below(fmask);
-        } else {
-          const uint32_t *L = oadj + ooff[u];
-          uint32_t lo = 0, n = ooff[u + 1] - ooff[u];
-          while (n > 0) {  // lower_bound of x in N+(u); x is present (u->x)
-            const uint32_t half = n >> 1;
-            if (L[lo + half] < (uint32_t)x) {
-              lo += half + 1;
-              n -= half + 1;
-            } else {
-              n = half;
-            }
+      const uint64_t bmask = __ballot(back);
+      uint32_t e = 0;
+      if (fwd) {
+        e = rank + lanes_below(fmask);
+      } else if (back) {
+        const uint32_t *L = oadj + ooff[u];
+        uint32_t lo = 0, n = ooff[u + 1] - ooff[u];
+        while (n > 0) {  // lower_bound of x in N+(u); x is present (u->x)
+          const uint32_t half = n >> 1;
+          if (L[lo + half] < (uint32_t)x) {
+            lo += half + 1;
+            n -= half + 1;
+          } else {
+            n = half;
           }
-          e = ooff[u] + lo;
         }
+        e = ooff[u] + lo;
+        inidx[in + lanes_below(bmask)] = e;
+      }
+      if (own) {
         const uint32_t k = out + lanes_below(omask);
         ppar[k] = u;
         pedge[k] = e;
       }
       rank += (uint32_t)__popcll(fmask);
       out += (uint32_t)__popcll(omask);
+      in += (uint32_t)__popcll(bmask);
     }
   }
 }
@@ -327,17 +337,29 @@ int build_pairs(Graph *g) {
   if (g->pairs_built) return PAC_OK;
   Ctx *ctx = g->ctx;
   const uint32_t V = g->V;
-  DevBuf npairs;
+  DevBuf npairs, nin;
   PAC_TRY(npairs.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  PAC_TRY(nin.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
   PAC_TRY(g->poff.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  PAC_TRY(g->inoff.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  PAC_TRY(g->inidx.reserve(sizeof(uint32_t) * (g->m ? g->m : 1)));
   PAC_TRY(fill_u32(ctx->stream, npairs.as<uint32_t>(), (uint64_t)V + 1, 0u));
+  PAC_TRY(fill_u32(ctx->stream, nin.as<uint32_t>(), (uint64_t)V + 1, 0u));
   if (V) {
     hipLaunchKernelGGL(k_pair_count, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
                        g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
                        g->deg.as<const uint32_t>(), g->odeg.as<const uint32_t>(), V,
-                       npairs.as<uint32_t>());
+                       npairs.as<uint32_t>(), nin.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
+  size_t tmp_in = 0;
+  PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp_in, nin.as<uint32_t>(),
+                                        g->inoff.as<uint32_t>(), 0u, (size_t)V + 1,
+                                        rocprim::plus<uint32_t>(), ctx->stream));
+  PAC_TRY(ctx->scratch.reserve(tmp_in));
+  PAC_CHECK_HIP(rocprim::exclusive_scan(ctx->scratch.p, tmp_in, nin.as<uint32_t>(),
+                                        g->inoff.as<uint32_t>(), 0u, (size_t)V + 1,
+                                        rocprim::plus<uint32_t>(), ctx->stream));
   size_t tmp_bytes = 0;
   PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, npairs.as<uint32_t>(),
                                         g->poff.as<uint32_t>(), 0u, (size_t)V + 1,
@@ -357,16 +379,16 @@ int build_pairs(Graph *g) {
   }
   PAC_TRY(g->ppar.reserve(sizeof(uint32_t) * (np ? np : 1)));
   PAC_TRY(g->pedge.reserve(sizeof(uint32_t) * (np ? np : 1)));
-  if (V && np) {
+  if (V && g->m) {
     hipLaunchKernelGGL(k_pair_fill, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
                        g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
                        g->deg.as<const uint32_t>(), g->odeg.as<const uint32_t>(),
                        g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
-                       g->poff.as<const uint32_t>(), V, g->ppar.as<uint32_t>(),
-                       g->pedge.as<uint32_t>());
+                       g->poff.as<const uint32_t>(), g->inoff.as<const uint32_t>(), V,
+                       g->ppar.as<uint32_t>(), g->pedge.as<uint32_t>(), g->inidx.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
-  // npairs is freed on return: finish its readers first
+  // npairs and nin are freed on return: finish their readers first
   PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   g->pairs_built = true;
   return PAC_OK;
