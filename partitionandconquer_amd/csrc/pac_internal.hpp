@@ -114,7 +114,8 @@ struct Graph {
   // In-edges: for each vertex v, the oadj positions of the edges u->v (V+1 offsets).
   DevBuf inoff, inidx;
   std::vector<uint32_t> h_hubs;  // vertices whose in-degree exceeds kHubIn
-  DevBuf hubs;
+  std::vector<uint32_t> h_big;   // vertices whose degree exceeds kSmallDeg
+  DevBuf hubs, big;
   bool pairs_built = false;
   // partitioning and work items
   bool prepared = false;
@@ -170,6 +171,8 @@ int orient(Graph *g);
 int build_pairs(Graph *g);
 // In-degree above which a vertex's credit sum is split over several workgroups.
 constexpr uint32_t kHubIn = 16384;
+// Degree up to which a vertex's credit sum is done by one thread (else by one wave).
+constexpr uint32_t kSmallDeg = 32;
 int graph_validate(const Graph *g, int level);
 
 // ---- gen.hip ---------------------------------------------------------------------

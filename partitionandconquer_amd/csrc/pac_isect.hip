@@ -655,15 +655,34 @@ __global__ void k_rowwork(const uint32_t *__restrict__ poff, const uint32_t *__r
 // ---- epilogues -----------------------------------------------------------------------
 // LCC per-vertex triangle counts: t(v) = (v's row totals, already in tvert)
 //   + sum over out-edges e of v of etail[e] + sum over in-edges e of v of ehead[e].
-// One wave per vertex; a hub's in-edges beyond the first kHubIn are left to
-// k_vertex_hub, which splits them over several workgroups.
-__global__ void k_vertex_tri(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ etail,
+// Vertices of degree <= kSmallDeg (most of them) take one thread each; the others are
+// listed in `big` and take one wave each; a hub's in-edges beyond the first kHubIn are
+// left to k_vertex_hub, which splits them over several workgroups.
+__global__ void k_vertex_small(const uint32_t *__restrict__ off, const uint32_t *__restrict__ ooff,
+                               const uint32_t *__restrict__ etail,
+                               const uint32_t *__restrict__ inoff,
+                               const uint32_t *__restrict__ inidx,
+                               const uint32_t *__restrict__ ehead, uint32_t V,
+                               unsigned long long *__restrict__ tvert) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    if (off[v + 1] - off[v] > kSmallDeg) continue;
+    uint32_t s = 0;  // at most kSmallDeg edges, each credited with fewer than d(v) triangles
+    for (uint32_t e = ooff[v]; e < ooff[v + 1]; ++e) s += etail[e];
+    for (uint32_t k = inoff[v]; k < inoff[v + 1]; ++k) s += ehead[inidx[k]];
+    if (s) tvert[v] += s;
+  }
+}
+
+__global__ void k_vertex_big(const uint32_t *__restrict__ big, uint32_t nbig,
+                             const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ etail,
                              const uint32_t *__restrict__ inoff, const uint32_t *__restrict__ inidx,
-                             const uint32_t *__restrict__ ehead, uint32_t V,
+                             const uint32_t *__restrict__ ehead,
                              unsigned long long *__restrict__ tvert) {
   const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
-  for (uint64_t v = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); v < V;
-       v += (uint64_t)gridDim.x * wpb) {
+  for (uint64_t b = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); b < nbig;
+       b += (uint64_t)gridDim.x * wpb) {
+    const uint32_t v = big[b];
     unsigned long long s = 0;
     for (uint32_t e = ooff[v] + lane; e < ooff[v + 1]; e += 64) s += etail[e];
     const uint32_t k0 = inoff[v], k1 = min(inoff[v + 1], k0 + kHubIn);
@@ -1091,11 +1110,21 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
 
 int vertex_counts(Ctx *ctx, Graph *g) {
   if (g->V == 0) return PAC_OK;
-  hipLaunchKernelGGL(k_vertex_tri, dim3(grid_for(g->V, 4)), dim3(256), 0, ctx->stream,
-                     g->ooff.as<const uint32_t>(), g->etail.as<const uint32_t>(),
-                     g->inoff.as<const uint32_t>(), g->inidx.as<const uint32_t>(),
-                     g->ehead.as<const uint32_t>(), g->V, g->tvert.as<unsigned long long>());
+  hipLaunchKernelGGL(k_vertex_small, dim3(grid_for(g->V, 256)), dim3(256), 0, ctx->stream,
+                     g->off.as<const uint32_t>(), g->ooff.as<const uint32_t>(),
+                     g->etail.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
+                     g->inidx.as<const uint32_t>(), g->ehead.as<const uint32_t>(), g->V,
+                     g->tvert.as<unsigned long long>());
   PAC_CHECK_HIP(hipGetLastError());
+  const uint32_t nbig = (uint32_t)g->h_big.size();
+  if (nbig) {
+    hipLaunchKernelGGL(k_vertex_big, dim3(grid_for(nbig, 4)), dim3(256), 0, ctx->stream,
+                       g->big.as<const uint32_t>(), nbig, g->ooff.as<const uint32_t>(),
+                       g->etail.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
+                       g->inidx.as<const uint32_t>(), g->ehead.as<const uint32_t>(),
+                       g->tvert.as<unsigned long long>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
   if (!g->h_hubs.empty()) {
     uint64_t slices = (g->max_deg + kHubIn - 1) / kHubIn;  // in-degree <= max degree
     if (slices < 1) slices = 1;

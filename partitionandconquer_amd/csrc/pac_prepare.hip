@@ -12,6 +12,7 @@
 //   2. ooff = exclusive_scan(d+)                        (rocPRIM device scan)
 //   3. each row's kept neighbours are written in input order (ballot prefix), so
 //      rows stay sorted by ID and no re-sort is needed.
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -157,6 +158,12 @@ __device__ __forceinline__ bool owns(uint32_t dx, uint32_t x, uint32_t ox, uint3
   return *fwd ? (ou <= ox && ou > 0u) : (ox > ou);
 }
 
+__global__ void k_iota(uint32_t *__restrict__ p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = (uint32_t)i;
+}
+
 // pairs owned per vertex (one wave per vertex, over its symmetric row) and in-degree
 __global__ void k_pair_count(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                              const uint32_t *__restrict__ deg, const uint32_t *__restrict__ odeg,
@@ -185,15 +192,14 @@ __global__ void k_pair_count(const uint32_t *__restrict__ off, const uint32_t *_
 }
 
 // Order-preserving fill: x's pairs come out sorted by partner ID.  The edge index of a
-// pair x->u is ooff[x] + (rank of u among x's out-neighbours); of u->x it is the
-// position of x in N+(u), found by binary search.  The in-edge list of x (the edge
-// indices of all u->x, in u order) is written alongside.
+// pair x->u is ooff[x] + (rank of u among x's out-neighbours); of u->x it is the next
+// entry of x's in-edge list, which holds the edges u->x in u order.
 __global__ void k_pair_fill(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                             const uint32_t *__restrict__ deg, const uint32_t *__restrict__ odeg,
                             const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
                             const uint32_t *__restrict__ poff, const uint32_t *__restrict__ inoff,
-                            uint32_t V, uint32_t *__restrict__ ppar, uint32_t *__restrict__ pedge,
-                            uint32_t *__restrict__ inidx) {
+                            const uint32_t *__restrict__ inidx, uint32_t V,
+                            uint32_t *__restrict__ ppar, uint32_t *__restrict__ pedge) {
   const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
   for (uint64_t x = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); x < V;
        x += (uint64_t)gridDim.x * wpb) {
@@ -215,19 +221,7 @@ __global__ void k_pair_fill(const uint32_t *__restrict__ off, const uint32_t *__
       if (fwd) {
         e = rank + lanes_below(fmask);
       } else if (back) {
-        const uint32_t *L = oadj + ooff[u];
-        uint32_t lo = 0, n = ooff[u + 1] - ooff[u];
-        while (n > 0) {  // lower_bound of x in N+(u); x is present (u->x)
-          const uint32_t half = n >> 1;
-          if (L[lo + half] < (uint32_t)x) {
-            lo += half + 1;
-            n -= half + 1;
-          } else {
-            n = half;
-          }
-        }
-        e = ooff[u] + lo;
-        inidx[in + lanes_below(bmask)] = e;
+        e = inidx[in + lanes_below(bmask)];
       }
       if (own) {
         const uint32_t k = out + lanes_below(omask);
@@ -360,6 +354,30 @@ int build_pairs(Graph *g) {
   PAC_CHECK_HIP(rocprim::exclusive_scan(ctx->scratch.p, tmp_in, nin.as<uint32_t>(),
                                         g->inoff.as<uint32_t>(), 0u, (size_t)V + 1,
                                         rocprim::plus<uint32_t>(), ctx->stream));
+  // In-edge lists = the oriented edges sorted by head (a CSR transpose).  The radix sort
+  // is stable and the edges enter in tail order, so each in-list is in tail order.
+  if (g->m) {
+    DevBuf keys_out, vals_in;
+    PAC_TRY(keys_out.reserve(sizeof(uint32_t) * g->m));
+    PAC_TRY(vals_in.reserve(sizeof(uint32_t) * g->m));
+    hipLaunchKernelGGL(k_iota, dim3(std::min<uint64_t>(1u << 16, (g->m + 255) / 256)), dim3(256), 0,
+                       ctx->stream, vals_in.as<uint32_t>(), g->m);
+    PAC_CHECK_HIP(hipGetLastError());
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < V) ++bits;
+    size_t tmp_sort = 0;
+    PAC_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tmp_sort, g->oadj.as<const uint32_t>(),
+                                            keys_out.as<uint32_t>(), vals_in.as<const uint32_t>(),
+                                            g->inidx.as<uint32_t>(), (size_t)g->m, 0u, bits,
+                                            ctx->stream));
+    PAC_TRY(ctx->scratch.reserve(tmp_sort));
+    PAC_CHECK_HIP(rocprim::radix_sort_pairs(ctx->scratch.p, tmp_sort, g->oadj.as<const uint32_t>(),
+                                            keys_out.as<uint32_t>(), vals_in.as<const uint32_t>(),
+                                            g->inidx.as<uint32_t>(), (size_t)g->m, 0u, bits,
+                                            ctx->stream));
+    // keys_out and vals_in are freed on return
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  }
   size_t tmp_bytes = 0;
   PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, npairs.as<uint32_t>(),
                                         g->poff.as<uint32_t>(), 0u, (size_t)V + 1,
@@ -384,8 +402,9 @@ int build_pairs(Graph *g) {
                        g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
                        g->deg.as<const uint32_t>(), g->odeg.as<const uint32_t>(),
                        g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
-                       g->poff.as<const uint32_t>(), g->inoff.as<const uint32_t>(), V,
-                       g->ppar.as<uint32_t>(), g->pedge.as<uint32_t>(), g->inidx.as<uint32_t>());
+                       g->poff.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
+                       g->inidx.as<const uint32_t>(), V, g->ppar.as<uint32_t>(),
+                       g->pedge.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
   // npairs and nin are freed on return: finish their readers first
