@@ -361,7 +361,8 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
 // after kMaxKicks evictions goes to a small stash that lookups also scan.  If the
 // stash overflows the row is done by the sorted-list path instead (exact either way).
 // Each pair's list N+(y) is streamed by one wave, 4 x 64 elements per step with
-// coalesced loads; the pair's hit count is a ballot popcount held in scalar registers.
+// coalesced loads, software-pipelined one step ahead across the wave's pairs; the
+// pair's hit count is a ballot popcount held in scalar registers.
 constexpr uint32_t kMaxKicks = 48;
 __device__ __forceinline__ uint32_t hash1(uint32_t x) { return x * 0x9E3779B1u; }
 __device__ __forceinline__ uint32_t hash2(uint32_t x) { return (x ^ (x >> 15)) * 0x85EBCA6Bu; }
@@ -375,6 +376,15 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
   const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
   auto &H = sm.u.h;
+  if (c_hi > np) c_hi = np;
+  const bool whole = c_lo == 0 && c_hi == np;
+  // The wave's first batch of pairs: partners and edges are loaded before the table is
+  // built, the bounds of their lists after the insertions, so both latencies overlap it.
+  uint32_t kl = c_lo + wave + (uint32_t)WAVES * lane, yl = 0, el = 0, bl = 0, nl = 0;
+  if (kl < c_hi) {
+    yl = G.ppar[p0 + kl];
+    if (MODE != kModeTC) el = G.pedge[p0 + kl];
+  }
   uint32_t lg = 4;
   while ((8u << lg) < 5u * nA) ++lg;  // 2^lg >= 5/8 nA
   const uint32_t nslot = 2u << lg, sh = 32u - lg;
@@ -406,6 +416,10 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       if (x == kEmpty) break;
     }
   }
+  if (kl < c_hi) {
+    bl = G.ooff[yl];
+    nl = G.ooff[yl + 1] - bl;
+  }
   __syncthreads();
   const uint32_t nst = sm.nstash;  // workgroup-uniform
   if (nst > kStash) return false;
@@ -426,25 +440,53 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     for (uint32_t i = tid; i < nA; i += BS) add16(H.pos16, find(G.oadj[a0 + i]), i);
     __syncthreads();
   }
-  if (c_hi > np) c_hi = np;
-  const bool whole = c_lo == 0 && c_hi == np;
   uint32_t row_hits = 0;  // wave-uniform
-  for_each_pair<WAVES, MODE>(G, p0, c_lo, c_hi, wave, lane,
-                             [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb) {
-    // N+(y) in blocks of 4 x 64 elements; 64-blocks past its end are skipped
-    uint32_t ph = 0;  // hits of this pair, wave-uniform
-    for (uint32_t j0 = 0; j0 < nb; j0 += 256u) {
-      uint32_t w[4];
+  // Blocks of up to 4 x 64 elements of one pair's list N+(y), in pair order; the loads
+  // of the next block are issued before the current one is probed.  The loads are
+  // branch-free (indices past the list end are clamped to its last element and the
+  // values masked at the probe) and the two register sets swap roles every step, so
+  // the waits cover only the block being probed.
+  auto load_blk = [&](uint32_t b0, uint32_t nb, uint32_t j0, uint32_t *w) {
 #pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) {
-        const uint32_t j = j0 + 64u * u + lane;
-        w[u] = j < nb ? G.oadj[b0 + j] : 0u;
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t j = j0 + 64u * u + lane;
+      w[u] = G.oadj[b0 + (j < nb ? j : nb - 1u)];  // nb >= 1
+    }
+  };
+  for (uint32_t t0 = 0;; t0 += 64) {
+    if (t0 != 0) {
+      kl = c_lo + wave + (uint32_t)WAVES * (t0 + lane);
+      yl = el = bl = nl = 0;
+      if (kl < c_hi) {
+        yl = G.ppar[p0 + kl];
+        if (MODE != kModeTC) el = G.pedge[p0 + kl];
+        bl = G.ooff[yl];
+        nl = G.ooff[yl + 1] - bl;
       }
+    }
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
+    if (nvalid == 0) break;
+    uint32_t jj = 0, j0 = 0, b0 = readlane(bl, 0), nb = readlane(nl, 0), ph = 0;
+    uint32_t wa[4], wb[4];
+    load_blk(b0, nb, 0, wa);
+    auto step = [&](const uint32_t *cur, uint32_t *nxt) {
+      uint32_t jn = jj, j0n = j0 + 256u, b0n = b0, nbn = nb;
+      if (j0n >= nb) {
+        jn = jj + 1;
+        j0n = 0;
+        if (jn < nvalid) {
+          b0n = readlane(bl, jn);
+          nbn = readlane(nl, jn);
+        }
+      }
+      // past the batch's last block this reloads the current one (in bounds, unused)
+      load_blk(jn < nvalid ? b0n : b0, jn < nvalid ? nbn : nb, jn < nvalid ? j0n : j0, nxt);
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
         if (j0 + 64u * u >= nb) break;  // wave-uniform
         const uint32_t j = j0 + 64u * u + lane;
-        const uint32_t slot = j < nb ? find(w[u]) : kEmpty;
+        const uint32_t f = find(cur[u]);
+        const uint32_t slot = j < nb ? f : kEmpty;
         const bool hit = slot != kEmpty;
         ph += (uint32_t)__popcll(__ballot(hit));
         if (hit) {
@@ -452,18 +494,34 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
           if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);  // edge y->w
         }
       }
+      if (jn != jj) {
+        // pair jj is done.  t(y) / the pair's edge: r->y adds to y's slot in the
+        // table, y->r is credited to y (each pair is visited once)
+        row_hits += ph;
+        if (MODE != kModeTC && ph != 0 && lane == 0) {
+          const uint32_t y = readlane(yl, jj), e = readlane(el, jj);
+          if (e - a0 < nA)
+            add16(H.cnt16, find(y), ph);
+          else if (MODE == kModeLCC)
+            G.etail[e] = ph;
+          else
+            gadd(&G.sup[e], ph);
+        }
+        ph = 0;
+      }
+      jj = jn;
+      j0 = j0n;
+      b0 = b0n;
+      nb = nbn;
+    };
+    while (true) {  // wave-uniform
+      step(wa, wb);
+      if (jj >= nvalid) break;
+      step(wb, wa);
+      if (jj >= nvalid) break;
     }
-    row_hits += ph;
-    // t(y) / the pair's edge: r->y adds to y's slot in the table, y->r to the global count
-    if (MODE != kModeTC && ph != 0 && lane == 0) {
-      if (e - a0 < nA)
-        add16(H.cnt16, find(y), ph);
-      else if (MODE == kModeLCC)
-        G.etail[e] = ph;  // y->r: credited to y (each pair is visited once)
-      else
-        gadd(&G.sup[e], ph);
-    }
-  });
+    if (nvalid < 64) break;
+  }
   // row_hits is the wave's total (uniform): counted once into acc, which is summed over
   // the wave's lanes at the end, so lane 0 carries it
   if (lane == 0) acc += row_hits;
