@@ -123,7 +123,7 @@ constexpr uint32_t kStash = 32;           // overflow entries of the hash table
 // Both live in one union: a workgroup runs one strategy per row.
 template <int CAP, int MODE, int WAVES>
 struct Smem {
-  static constexpr uint32_t kSlots = 2u * pow2_at_least((5u * CAP + 7u) / 8u);
+  static constexpr uint32_t kSlots = 2u * pow2_at_least(CAP);
   union {
     struct {
       uint32_t A[CAP];
@@ -353,10 +353,12 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
 }
 
 // ---- one row, hash strategy: two-choice cuckoo table of A in LDS ---------------------
-// A is held in a table of 2^lg buckets of two slots (2^lg >= 5/8 nA, so the load
-// factor is at most 0.8).  Key x may sit in bucket h1(x) or h2(x); a lookup reads both
-// buckets (two 8-byte LDS reads) and compares four slots -- a fixed cost with no loop
-// and no divergence.  Insertion is a bounded random walk: a key takes a free slot of
+// A is held in a table of 2^lg buckets of two slots (2^lg >= nA, so the load factor is
+// at most 1/2, far below the ~0.89 limit of this layout: insertions almost never reach
+// the stash, whose scan every lookup of the row would pay).  Key x may sit in bucket
+// h1(x) or h2(x); a lookup reads both buckets (two 8-byte LDS reads) and compares four
+// slots -- a fixed cost with no loop and no divergence.  Insertion is a bounded random
+// walk: a key takes a free slot of
 // its two buckets, else evicts a resident and carries it on; a key still homeless
 // after kMaxKicks evictions goes to a small stash that lookups also scan.  If the
 // stash overflows the row is done by the sorted-list path instead (exact either way).
@@ -386,7 +388,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     if (MODE != kModeTC) el = G.pedge[p0 + kl];
   }
   uint32_t lg = 4;
-  while ((8u << lg) < 5u * nA) ++lg;  // 2^lg >= 5/8 nA
+  while ((1u << lg) < nA) ++lg;  // 2^lg >= nA: load factor <= 1/2
   const uint32_t nslot = 2u << lg, sh = 32u - lg;
   for (uint32_t i = tid; i < nslot; i += BS) H.tab[i] = kEmpty;
   if (MODE != kModeTC)
