@@ -79,6 +79,7 @@ struct GraphDev {
   uint32_t *etail;            // LCC mode: per oriented edge, triangles credited to its tail
   uint32_t *sup;              // per-oriented-edge support (SUP mode)
   uint32_t V;
+  uint32_t m;                 // oriented edges; oadj[m] holds the sentinel kListEnd
   uint32_t part_base;         // offset of this graph's partitions in the dispatch table
 };
 
@@ -169,6 +170,9 @@ int orient(Graph *g);
 // Build the pair lists and in-edge lists of an oriented graph (once; later calls
 // return at once).
 int build_pairs(Graph *g);
+// Value of the entry past the last oriented edge (oadj[m]): neither a vertex ID (IDs
+// are < 2^31) nor the empty hash-table slot, so a lookup of it never hits.
+constexpr uint32_t kListEnd = 0xFFFFFFFEu;
 // In-degree above which a vertex's credit sum is split over several workgroups.
 constexpr uint32_t kHubIn = 16384;
 // Degree up to which a vertex's credit sum is done by one thread (else by one wave).

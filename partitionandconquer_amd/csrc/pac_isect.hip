@@ -54,7 +54,7 @@ struct GDev {
   const PAC_GLOBAL uint32_t *ooff, *oadj, *poff, *ppar, *pedge, *rows;
   PAC_GLOBAL unsigned long long *total, *tvert;
   PAC_GLOBAL uint32_t *ehead, *etail, *sup;
-  uint32_t V, part_base;
+  uint32_t V, m, part_base;
 };
 
 __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
@@ -71,6 +71,7 @@ __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
   g.etail = (PAC_GLOBAL uint32_t *)d.etail;
   g.sup = (PAC_GLOBAL uint32_t *)d.sup;
   g.V = d.V;
+  g.m = d.m;
   g.part_base = d.part_base;
   return g;
 }
@@ -156,9 +157,6 @@ __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
 // 16-bit counters packed two per word.  A hash pass covers at most kPassPairs pairs,
 // so a counter stays below 2^16 (at most one hit per pair for an element of A plus at
 // most nA <= CAP <= 4096 hits of the pair x->y at y's place) and halves never carry.
-__device__ __forceinline__ void add16(uint32_t *w, uint32_t slot, uint32_t v) {
-  atomicAdd(&w[slot >> 1], v << (16u * (slot & 1u)));
-}
 __device__ __forceinline__ uint32_t get16(const uint32_t *w, uint32_t slot) {
   return (w[slot >> 1] >> (16u * (slot & 1u))) & 0xFFFFu;
 }
@@ -366,8 +364,19 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
 // coalesced loads, software-pipelined one step ahead across the wave's pairs; the
 // pair's hit count is a ballot popcount held in scalar registers.
 constexpr uint32_t kMaxKicks = 48;
-__device__ __forceinline__ uint32_t hash1(uint32_t x) { return x * 0x9E3779B1u; }
-__device__ __forceinline__ uint32_t hash2(uint32_t x) { return (x ^ (x >> 15)) * 0x85EBCA6Bu; }
+// The two buckets of x among 2^lg (sh = 32 - lg), from full-rate 24-bit multiplies: the
+// top lg bits of (y * 0x9E3779) and of ((h ^ (y >> 5)) * 0xC2B2AE) with y = x ^ (x >> 11),
+// forced distinct so that every key has two choices.
+// (__umul24 is declared returning int: its results are taken as unsigned before any
+// shift, so the bucket indices stay below 2^lg.)
+__device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t &b1, uint32_t &b2) {
+  const uint32_t y = x ^ (x >> 11);
+  const uint32_t h = (uint32_t)__umul24(y, 0x9E3779u);
+  const uint32_t g = (uint32_t)__umul24(h ^ (y >> 5), 0xC2B2AEu);
+  b1 = h >> sh;
+  b2 = g >> sh;
+  b2 = b2 == b1 ? b1 ^ 1u : b2;
+}
 
 template <int MODE, int WAVES, int CAP>
 __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_lo,
@@ -403,7 +412,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   for (uint32_t i = tid; i < nA; i += BS) {
     uint32_t x = G.oadj[a0 + i];
     for (uint32_t kick = 0;; ++kick) {
-      const uint32_t b1 = hash1(x) >> sh, b2 = hash2(x) >> sh;
+      uint32_t b1, b2;
+      buckets(x, sh, b1, b2);
       if (atomicCAS(&H.tab[2u * b1], kEmpty, x) == kEmpty) break;
       if (atomicCAS(&H.tab[2u * b1 + 1u], kEmpty, x) == kEmpty) break;
       if (atomicCAS(&H.tab[2u * b2], kEmpty, x) == kEmpty) break;
@@ -425,34 +435,45 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   __syncthreads();
   const uint32_t nst = sm.nstash;  // workgroup-uniform
   if (nst > kStash) return false;
-  // slot of w in the table (kEmpty if absent)
-  auto find = [&](uint32_t w) -> uint32_t {
-    const uint32_t b1 = hash1(w) >> sh, b2 = hash2(w) >> sh;
+  // Lookup of w.  A bucket's two slots share one word of the packed 16-bit counters, so
+  // a hit is reported as that word (the bucket) and the increment for w's half of it
+  // (1 or 1 << 16); inc = 0 when w is absent.  Stash entry k sits at slot nslot + k.
+  auto find = [&](uint32_t w, uint32_t &word) -> uint32_t {
+    uint32_t b1, b2;
+    buckets(w, sh, b1, b2);
     const uint32_t k10 = H.tab[2u * b1], k11 = H.tab[2u * b1 + 1u];
     const uint32_t k20 = H.tab[2u * b2], k21 = H.tab[2u * b2 + 1u];
-    uint32_t slot = kEmpty;
-    slot = k21 == w ? 2u * b2 + 1u : slot;
-    slot = k20 == w ? 2u * b2 : slot;
-    slot = k11 == w ? 2u * b1 + 1u : slot;
-    slot = k10 == w ? 2u * b1 : slot;
-    for (uint32_t k = 0; k < nst; ++k) slot = H.tab[nslot + k] == w ? nslot + k : slot;
-    return slot;
+    // bitwise | (no short-circuit): all four slots are read up front, no branches
+    const bool in1 = (k10 == w) | (k11 == w), hi = (k11 == w) | (k21 == w);
+    uint32_t inc = (in1 | (k20 == w) | hi) ? (hi ? 0x10000u : 1u) : 0u;
+    word = in1 ? b1 : b2;
+    for (uint32_t k = 0; k < nst; ++k) {
+      if (H.tab[nslot + k] == w) {
+        word = (nslot + k) >> 1;
+        inc = (k & 1u) ? 0x10000u : 1u;  // nslot is even
+      }
+    }
+    return inc;
   };
   if (MODE == kModeSUP) {
-    for (uint32_t i = tid; i < nA; i += BS) add16(H.pos16, find(G.oadj[a0 + i]), i);
+    for (uint32_t i = tid; i < nA; i += BS) {
+      uint32_t word;
+      const uint32_t inc = find(G.oadj[a0 + i], word);
+      atomicAdd(&H.pos16[word], inc * i);  // i < CAP < 2^16
+    }
     __syncthreads();
   }
   uint32_t row_hits = 0;  // wave-uniform
   // Blocks of up to 4 x 64 elements of one pair's list N+(y), in pair order; the loads
   // of the next block are issued before the current one is probed.  The loads are
-  // branch-free (indices past the list end are clamped to its last element and the
-  // values masked at the probe) and the two register sets swap roles every step, so
-  // the waits cover only the block being probed.
+  // branch-free (lanes past the list's end read the sentinel oadj[m], which never hits)
+  // and the two register sets swap roles every step, so the waits cover only the block
+  // being probed.
   auto load_blk = [&](uint32_t b0, uint32_t nb, uint32_t j0, uint32_t *w) {
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) {
       const uint32_t j = j0 + 64u * u + lane;
-      w[u] = G.oadj[b0 + (j < nb ? j : nb - 1u)];  // nb >= 1
+      w[u] = G.oadj[j < nb ? b0 + j : G.m];
     }
   };
   for (uint32_t t0 = 0;; t0 += 64) {
@@ -487,13 +508,12 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       for (uint32_t u = 0; u < 4; ++u) {
         if (j0 + 64u * u >= nb) break;  // wave-uniform
         const uint32_t j = j0 + 64u * u + lane;
-        const uint32_t f = find(cur[u]);
-        const uint32_t slot = j < nb ? f : kEmpty;
-        const bool hit = slot != kEmpty;
-        ph += (uint32_t)__popcll(__ballot(hit));
-        if (hit) {
-          if (MODE != kModeTC) add16(H.cnt16, slot, 1u);    // t(w) / edge r->w
-          if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);  // edge y->w
+        uint32_t word;
+        const uint32_t inc = find(cur[u], word);  // 0 for the sentinel past the list's end
+        ph += (uint32_t)__popcll(__ballot(inc != 0u));
+        if (inc != 0u) {
+          if (MODE != kModeTC) atomicAdd(&H.cnt16[word], inc);  // t(w) / edge r->w
+          if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);      // edge y->w
         }
       }
       if (jn != jj) {
@@ -502,9 +522,11 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         row_hits += ph;
         if (MODE != kModeTC && ph != 0 && lane == 0) {
           const uint32_t y = readlane(yl, jj), e = readlane(el, jj);
-          if (e - a0 < nA)
-            add16(H.cnt16, find(y), ph);
-          else if (MODE == kModeLCC)
+          if (e - a0 < nA) {
+            uint32_t word;
+            const uint32_t inc = find(y, word);
+            atomicAdd(&H.cnt16[word], inc * ph);  // ph <= nb <= CAP < 2^16
+          } else if (MODE == kModeLCC)
             G.etail[e] = ph;
           else
             gadd(&G.sup[e], ph);
@@ -532,7 +554,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   if (MODE == kModeLCC) {
     // per edge r->A[i], in A order (coalesced): the credits of A[i]'s slot
     for (uint32_t i = tid; i < nA; i += BS) {
-      const uint32_t v = get16(H.cnt16, find(G.oadj[a0 + i]));
+      uint32_t word;
+      const uint32_t inc = find(G.oadj[a0 + i], word);
+      const uint32_t v = (H.cnt16[word] >> (inc == 1u ? 0u : 16u)) & 0xFFFFu;
       if (v) credit_edge(&G.ehead[a0 + i], v, whole);
     }
     if (tid == 0 && sm.rowtot) {
@@ -952,6 +976,7 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   d.etail = g->etail.as<uint32_t>();
   d.sup = g->sup.as<uint32_t>();
   d.V = g->V;
+  d.m = (uint32_t)g->m;
   d.part_base = part_base;
   return d;
 }

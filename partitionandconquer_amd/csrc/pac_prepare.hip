@@ -314,7 +314,8 @@ int orient(Graph *g) {
               (unsigned long long)g->m, (unsigned long long)g->nnz);
     return PAC_ERR_FORMAT;
   }
-  PAC_TRY(g->oadj.reserve(sizeof(uint32_t) * (g->m ? g->m : 1)));
+  // m entries plus one sentinel (kListEnd) that list streams past their end read
+  PAC_TRY(g->oadj.reserve(sizeof(uint32_t) * (g->m + 1)));
   if (V) {
     hipLaunchKernelGGL(k_scatter, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
                        g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
@@ -322,6 +323,7 @@ int orient(Graph *g) {
                        g->oadj.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
+  PAC_TRY(fill_u32(ctx->stream, g->oadj.as<uint32_t>() + g->m, 1, kListEnd));
   g->pairs_built = false;
   return PAC_OK;
 }
