@@ -15,7 +15,9 @@ learned dispatch (the trained MLP in partitionandconquer_amd/dispatch_mlp.bin; t
 fixed heuristic when no model file exists).  The plain path (one launch per partition
 group, one workgroup per row, fixed heuristic) is timed beside it for the speedup.  GTEPS = m / t / 1e9 with m the
 undirected edge count.  The roofline entry prices the dominant kernel's algorithmic
-bytes 4*W, W = sum over oriented u->v of (d+(u) + d+(v)), against 8 TB/s.  The CPU
+bytes 4*W, W = sum over oriented u->v of (d+(u) + d+(v)), against 8 TB/s; `traffic` is
+the kernel's measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, committed
+under profiles/) and `traffic_GBps` that traffic over the live kernel time.  The CPU
 baseline is the oracle's restatement of Graph::solve_triangle_counting_merge and
 Graph::solve_lcc_merge (Graph.cpp:355-368, 828-840) on a bounded random sample of the
 same graph's rows, extrapolated to all rows.
@@ -39,6 +41,21 @@ MODEL = os.path.join(ROOT, "partitionandconquer_amd", "dispatch_mlp.bin")
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_persistent_rmat24_hbm_traffic.md")
+
+
+def measured_traffic(scale, partitions):
+    """HBM bytes per launch of the intersection kernel, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs (calibrated against a 1 GiB copy); those runs cover the
+    default workload only, so other workloads report None."""
+    import re
+    if (scale, partitions) != (24, 4096) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    m = re.search(r"HBM traffic per launch of the intersection kernel: (\d+) bytes",
+                  open(TRAFFIC_PROFILE).read())
+    return int(m.group(1)) if m else None
 
 
 def timed_steps(ctx, g, dispatch, launch, steps, warmup):
@@ -164,7 +181,9 @@ def main():
     copy_ms = ctx.stream_copy(copy_bytes, 5)
     stream_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
 
-    achieved = 4.0 * W / (ker * 1e-3) if ker > 0 else 4.0 * W / (dev_p * 1e-3)
+    kms = ker if ker > 0 else dev_p
+    achieved = 4.0 * W / (kms * 1e-3)
+    traffic = measured_traffic(scale, a.partitions)
     res = {
         "metric": "TC/LCC GTEPS at RMAT scale 24 (exact); learned vs fixed dispatch speedup",
         "value": round(gteps, 4),
@@ -184,9 +203,10 @@ def main():
                    "dispatch": "learned" if learned else "fixed",
                    "launch": "persistent+hipGraph", "tile": ctx.get_tile()},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "kernel_ms": round(ker, 3), "kernel": "k_persistent (LCC mode)",
-                     "bytes_per_launch": 4 * W},
+                     "bytes_per_launch": 4 * W,
+                     "traffic_GBps": round(traffic / (kms * 1e-3) / 1e9, 1) if traffic else None},
         "device_ms_per_step": round(dev, 3),
         "stream_copy_GBps": round(stream_gbs, 1),
         "preprocess_ms": round(prep_s * 1e3, 1),

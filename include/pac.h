@@ -51,7 +51,7 @@ enum {
                              * when no model is loaded)                              */
   PAC_FORCE_MERGE = 2,      /* wave64 merge-path sorted merge everywhere             */
   PAC_FORCE_GALLOP = 3,     /* binary search of the shorter list in the longer one   */
-  PAC_FORCE_BITMAP = 4      /* LDS hashed bitmap of N+(u) + exact LDS confirmation   */
+  PAC_FORCE_BITMAP = 4      /* LDS hash of N+(u): two-choice cuckoo table, exact      */
 };
 #define PAC_NUM_STRATEGIES 3 /* strategy ids: 0 merge, 1 gallop, 2 bitmap */
 
@@ -201,7 +201,7 @@ int pac_tc_batch(pac_ctx *ctx, pac_graph *const *gs, uint32_t G, int dispatch, i
                  uint64_t *totals);
 
 /* ---- autotuning ------------------------------------------------------------------ */
-/* Tile configurations (waves per row, LDS list capacity, LDS bitmap bits). */
+/* Tile configurations (waves per row, LDS list capacity). */
 #define PAC_NUM_TILES 4
 /* Times every tile configuration of the persistent kernel on the given graphs and
  * keeps the fastest for this context; ms_out (PAC_NUM_TILES, may be NULL) receives

@@ -439,7 +439,7 @@ void partition_features(Graph *g, const std::vector<uint32_t> &h_odeg,
     f[14] = L(gal);
     f[15] = L(maxw);
     // Fixed degree-threshold heuristic: partitions whose pairs see long staged lists
-    // (pair-weighted mean out-degree >= 32) use the LDS bitmap; strongly skewed ones
+    // (pair-weighted mean out-degree >= 32) use the LDS hash; strongly skewed ones
     // (staged and streamed lists 8x apart) use galloping; the rest merge.
     uint8_t s = kMerge;
     if (wmean_a >= 32.0)

@@ -52,3 +52,11 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "8"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_bench_traffic_comes_from_the_committed_profile():
+    sys.path.insert(0, ROOT)
+    import bench
+    t = bench.measured_traffic(24, 4096)
+    assert t is not None and t > 1e11        # hundreds of GB per launch at scale 24
+    assert bench.measured_traffic(22, 1024) is None
