@@ -193,6 +193,27 @@ int pac_jaccard_queries(pac_graph *g, const int32_t *q, uint32_t n, double *res)
 int pac_cnt_common_pairs(pac_graph *g, const int32_t *u, const int32_t *v, uint64_t n,
                          uint64_t *res);
 
+/* ---- multi-way intersections ----------------------------------------------------- */
+/* |N(q_0) & N(q_1) & ... & N(q_{k-1})| for nq queries in CSR form: query i is
+ * qv[qoff[i] .. qoff[i+1]) (qoff[0] = 0, every query non-empty), res[i] its count.
+ * Replaces Graph::batch_cnt_common_neis_{merge,galloping}(const vector<vector<VID>>&,
+ * vector<size_t>&, int) (Graph.cpp:423-429, 441-447) and the single-query
+ * Graph::cnt_common_neis_{merge,galloping}(const vector<VID>&) (Graph.cpp:200-225,
+ * 255-270); both reference variants return the same count. */
+int pac_cnt_common_kway(pac_graph *g, const uint64_t *qoff, const int32_t *qv, uint64_t nq,
+                        uint64_t *res);
+/* Sum over index pairs i < j of nodes (neither equal to root) of
+ * |N(root) & N(nodes[i]) & N(nodes[j])|.  Replaces Graph::solve_friend_triangle_{merge,
+ * galloping} (Graph.cpp:309-338) and Partitioned_Graph::solve_friend_triangle_*
+ * (Graph.cpp:1222-1264), whose per-partition partials sum to the same value. */
+int pac_friend_triangle(pac_graph *g, int32_t root, const int32_t *nodes, uint64_t n,
+                        uint64_t *res);
+/* Partitioned_Graph::solve_square_counting (Graph.cpp:1291-1324): the sum over every u
+ * and neighbours w < v < u of u of #{x < w : x ~ v, x ~ w} -- each 4-cycle whose
+ * largest and smallest vertex IDs are opposite corners, counted once.  Independent of
+ * the partitioning; needs no pac_graph_prepare. */
+int pac_square_count(pac_graph *g, uint64_t *res);
+
 /* ---- batching ------------------------------------------------------------------- */
 /* Triangle counts of G prepared graphs in one persistent launch (or one hipGraph
  * replay when launch == PAC_LAUNCH_GRAPH): totals[i] = T of gs[i].  No reference

@@ -83,6 +83,13 @@ def lib():
         L.orc_mlp_forward.argtypes = [f32p, f32p, C.c_uint64, f32p]
         L.orc_mlp_backward.argtypes = [f32p, f32p, i32p, C.c_void_p, C.c_uint64, f32p]
         L.orc_mlp_backward.restype = C.c_float
+        for f in ("orc_kway_merge", "orc_kway_galloping"):
+            getattr(L, f).argtypes = [u64p, i32p, i32p, C.c_uint64]
+            getattr(L, f).restype = C.c_uint64
+        L.orc_friend_triangle_merge.argtypes = [u64p, i32p, C.c_int32, i32p, C.c_uint64, C.c_int]
+        L.orc_friend_triangle_merge.restype = C.c_uint64
+        L.orc_pgraph_square_counting.argtypes = [C.c_void_p]
+        L.orc_pgraph_square_counting.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -278,6 +285,24 @@ class PGraph:
         out = np.zeros(self.V, np.float64)
         lib().orc_pgraph_lcc_merge(self._h, out)
         return out
+
+    def square_counting(self) -> int:
+        return lib().orc_pgraph_square_counting(self._h)
+
+
+def kway_merge(g: CSR, q) -> int:
+    q = np.ascontiguousarray(q, dtype=np.int32)
+    return lib().orc_kway_merge(g.vlist, g.elist, q, len(q))
+
+
+def kway_galloping(g: CSR, q) -> int:
+    q = np.ascontiguousarray(q, dtype=np.int32)
+    return lib().orc_kway_galloping(g.vlist, g.elist, q, len(q))
+
+
+def friend_triangle_merge(g: CSR, root, nodes, nt=0) -> int:
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    return lib().orc_friend_triangle_merge(g.vlist, g.elist, int(root), nodes, len(nodes), _nt(nt))
 
 
 def random_partition(V, P, seed=1) -> np.ndarray:

@@ -695,4 +695,100 @@ float orc_mlp_backward(const float *params, const float *x, const int32_t *label
   return (float)loss;
 }
 
+
+// ---- k-way intersections, friend triangles, squares -------------------------------
+
+// Graph::cnt_common_neis_merge(const vector<VID>&), Graph.cpp:255-270: the running
+// intersection starts as N(q[0]), is merged with N(q[1]) .. N(q[k-2]), and the last
+// list is only counted against it.
+uint64_t orc_kway_merge(const uint64_t *vl, const int32_t *el, const int32_t *q, uint64_t k) {
+  std::vector<int32_t> cur(el + vl[q[0]], el + vl[q[0] + 1]), nxt;
+  for (uint64_t i = 1; i + 1 < k; ++i) {
+    nxt.clear();
+    const int32_t *b = el + vl[q[i]];
+    const uint64_t nb = deg(vl, q[i]);
+    uint64_t x = 0, y = 0;
+    while (x < cur.size() && y < nb) {
+      if (cur[x] == b[y]) {
+        nxt.push_back(cur[x]);
+        ++x;
+        ++y;
+      } else if (cur[x] < b[y]) {
+        ++x;
+      } else {
+        ++y;
+      }
+    }
+    cur.swap(nxt);
+  }
+  return merge_count(cur.data(), cur.size(), el + vl[q[k - 1]], deg(vl, q[k - 1]));
+}
+
+// Graph::cnt_common_neis_galloping(const vector<VID>&), Graph.cpp:200-225: the
+// minimum-degree list drives; each of its elements is looked up with lower_bound in
+// every other list.
+uint64_t orc_kway_galloping(const uint64_t *vl, const int32_t *el, const int32_t *q, uint64_t k) {
+  int32_t u = q[0];
+  for (uint64_t i = 0; i < k; ++i)
+    if (deg(vl, q[i]) < deg(vl, u)) u = q[i];
+  uint64_t res = 0;
+  for (uint64_t i = vl[u]; i < vl[u + 1]; ++i) {
+    const int32_t t = el[i];
+    bool found = true;
+    for (uint64_t j = 0; j < k && found; ++j) {
+      const int32_t v = q[j];
+      if (v == u) continue;
+      const int32_t *b = el + vl[v], *e = el + vl[v + 1];
+      const int32_t *p = std::lower_bound(b, e, t);
+      found = p < e && *p == t;
+    }
+    if (found) ++res;
+  }
+  return res;
+}
+
+// Graph::solve_friend_triangle_merge, Graph.cpp:323-338: over all index pairs i < j of
+// `nodes` with neither equal to root, the 3-way common-neighbour count of
+// {root, nodes[i], nodes[j]}.
+uint64_t orc_friend_triangle_merge(const uint64_t *vl, const int32_t *el, int32_t root,
+                                   const int32_t *nodes, uint64_t n, int nt) {
+  uint64_t res = 0;
+#pragma omp parallel for num_threads(nt) reduction(+ : res) schedule(dynamic, 4)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    if (nodes[i] == root) continue;
+    for (uint64_t j = (uint64_t)i + 1; j < n; ++j) {
+      if (nodes[j] == root) continue;
+      const int32_t q[3] = {root, nodes[i], nodes[j]};
+      res += orc_kway_merge(vl, el, q, 3);
+    }
+  }
+  return res;
+}
+
+// Partitioned_Graph::solve_square_counting, Graph.cpp:1291-1324: work items (v, w) for
+// every u and neighbours w < v < u (sub-rows scanned partition by partition, each
+// stopping at the first neighbour >= the bound), then for every partition k the
+// common neighbours x < w of v and w in N_k.
+uint64_t orc_pgraph_square_counting(const orc_pgraph *g) {
+  const uint64_t V = g->V, P = g->P;
+  std::vector<std::pair<int32_t, int32_t>> items;
+  for (uint64_t u = 0; u < V; ++u)
+    for (uint64_t i = 0; i < P; ++i)
+      for (uint64_t vp = g->vl[i * V + u]; vp < g->vl[i * V + u + 1]; ++vp) {
+        const int32_t v = g->el[vp];
+        if ((int64_t)u <= v) break;
+        for (uint64_t j = 0; j < P; ++j)
+          for (uint64_t wp = g->vl[j * V + u]; wp < g->vl[j * V + u + 1]; ++wp) {
+            const int32_t w = g->el[wp];
+            if (v <= w) break;
+            items.emplace_back(v, w);
+          }
+      }
+  uint64_t res = 0;
+#pragma omp parallel for reduction(+ : res) schedule(dynamic, 1)
+  for (int64_t k = 0; k < (int64_t)P; ++k)
+    for (const auto &e : items) res += pg_below(g, e.first, e.second, e.second, (uint64_t)k);
+  return res;
+}
+
 }  // extern "C"

@@ -20,6 +20,11 @@
  *   orc_jaccard_edges    (A^2)_uv dense on the edge set
  *   orc_pgraph_*         partition-sum invariance against orc_tc_merge and
  *                        brute force, for random and contiguous vertex2partition
+ *   orc_kway_*           Python set intersections of k neighbourhoods
+ *   orc_friend_*         brute-force pairs over Python sets; closed form on K_n
+ *   orc_pgraph_square_*  brute-force enumeration of 4-cycles (u,v,x,w) with
+ *                        u > v > w > x on tiny graphs; K_n -> C(n,4); K_{a,b}
+ *                        labelled side by side -> 0
  *   orc_gen_*            structural invariants (symmetric, sorted, unique, no
  *                        self-loops), edge-count expectations; the GPU generator
  *                        must reproduce orc_gen_rmat bit-exactly
@@ -140,6 +145,18 @@ void orc_mlp_forward(const float *params, const float *x, uint64_t n, float *log
  * (same layout as params) and returns the loss. Sample weights may be NULL. */
 float orc_mlp_backward(const float *params, const float *x, const int32_t *labels,
                        const float *weights, uint64_t n, float *grads);
+
+/* ---- k-way intersections, friend triangles, squares ------------------------- */
+
+/* Graph::cnt_common_neis_merge(const vector<VID>&), Graph.cpp:255-270 (k >= 1). */
+uint64_t orc_kway_merge(const uint64_t *vl, const int32_t *el, const int32_t *q, uint64_t k);
+/* Graph::cnt_common_neis_galloping(const vector<VID>&), Graph.cpp:200-225. */
+uint64_t orc_kway_galloping(const uint64_t *vl, const int32_t *el, const int32_t *q, uint64_t k);
+/* Graph::solve_friend_triangle_merge, Graph.cpp:323-338. */
+uint64_t orc_friend_triangle_merge(const uint64_t *vl, const int32_t *el, int32_t root,
+                                   const int32_t *nodes, uint64_t n, int nt);
+/* Partitioned_Graph::solve_square_counting, Graph.cpp:1291-1324. */
+uint64_t orc_pgraph_square_counting(const orc_pgraph *g);
 
 #ifdef __cplusplus
 }

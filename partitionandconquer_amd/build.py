@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PAC_OFFLOAD_ARCH", "gfx950")
 
 LIB_SOURCES = ["pac_api.cpp", "pac_tune.cpp", "pac_prepare.hip", "pac_gen.hip",
-               "pac_isect.hip", "pac_mlp.hip"]
+               "pac_isect.hip", "pac_mlp.hip", "pac_multi.hip"]
 CLI_SOURCES = ["pac_graph.cpp", "pac_main.cpp"]
 HEADERS = ["pac_internal.hpp", "pac_graph.hpp"]
 

@@ -1130,6 +1130,84 @@ int pac_cnt_common_pairs(pac_graph *h, const int32_t *u, const int32_t *v, uint6
   return PAC_OK;
 }
 
+int pac_cnt_common_kway(pac_graph *h, const uint64_t *qoff, const int32_t *qv, uint64_t nq,
+                        uint64_t *res) {
+  if (!h || (nq && (!qoff || !qv || !res))) {
+    set_error("null argument");
+    return PAC_ERR_ARG;
+  }
+  Graph *g = &h->g;
+  PAC_CHECK_HIP(hipSetDevice(g->ctx->device));
+  if (nq == 0) return PAC_OK;
+  if (qoff[0] != 0) {
+    set_error("query offsets must start at 0");
+    return PAC_ERR_ARG;
+  }
+  for (uint64_t i = 0; i < nq; ++i)
+    if (qoff[i + 1] <= qoff[i]) {
+      set_error("query %llu is empty or its offsets decrease", (unsigned long long)i);
+      return PAC_ERR_ARG;
+    }
+  const uint64_t nv = qoff[nq];
+  for (uint64_t i = 0; i < nv; ++i)
+    if (qv[i] < 0 || (uint32_t)qv[i] >= g->V) {
+      set_error("query vertex %llu (= %d) out of range", (unsigned long long)i, qv[i]);
+      return PAC_ERR_ARG;
+    }
+  DevBuf dq, dv, dr;
+  PAC_TRY(dq.reserve(8 * (nq + 1)));
+  PAC_TRY(dv.reserve(4 * nv));
+  PAC_TRY(dr.reserve(8 * nq));
+  PAC_CHECK_HIP(hipMemcpy(dq.p, qoff, 8 * (nq + 1), hipMemcpyHostToDevice));
+  PAC_CHECK_HIP(hipMemcpy(dv.p, qv, 4 * nv, hipMemcpyHostToDevice));
+  PAC_TRY(kway_counts(g->ctx, g, dq.as<uint64_t>(), dv.as<int32_t>(), nq, dr.as<uint64_t>()));
+  PAC_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
+  PAC_CHECK_HIP(hipMemcpy(res, dr.p, 8 * nq, hipMemcpyDeviceToHost));
+  return PAC_OK;
+}
+
+int pac_friend_triangle(pac_graph *h, int32_t root, const int32_t *nodes, uint64_t n,
+                        uint64_t *res) {
+  if (!h || !res || (n && !nodes)) {
+    set_error("null argument");
+    return PAC_ERR_ARG;
+  }
+  Graph *g = &h->g;
+  PAC_CHECK_HIP(hipSetDevice(g->ctx->device));
+  if (root < 0 || (uint32_t)root >= g->V) {
+    set_error("root %d out of range", root);
+    return PAC_ERR_ARG;
+  }
+  for (uint64_t i = 0; i < n; ++i)
+    if (nodes[i] < 0 || (uint32_t)nodes[i] >= g->V) {
+      set_error("node %llu (= %d) out of range", (unsigned long long)i, nodes[i]);
+      return PAC_ERR_ARG;
+    }
+  uint32_t rb[2];
+  PAC_CHECK_HIP(hipMemcpy(rb, g->off.as<uint32_t>() + root, 8, hipMemcpyDeviceToHost));
+  const uint32_t rdeg = rb[1] - rb[0];
+  DevBuf dn, dc, dr;
+  PAC_TRY(dn.reserve(4 * (n ? n : 1)));
+  PAC_TRY(dc.reserve(4 * (uint64_t)(rdeg ? rdeg : 1)));
+  PAC_TRY(dr.reserve(8));
+  if (n) PAC_CHECK_HIP(hipMemcpy(dn.p, nodes, 4 * n, hipMemcpyHostToDevice));
+  PAC_TRY(friend_triangles(g->ctx, g, (uint32_t)root, dn.as<int32_t>(), n, rdeg,
+                           dc.as<uint32_t>(), dr.as<uint64_t>()));
+  PAC_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
+  PAC_CHECK_HIP(hipMemcpy(res, dr.p, 8, hipMemcpyDeviceToHost));
+  return PAC_OK;
+}
+
+int pac_square_count(pac_graph *h, uint64_t *res) {
+  if (!h || !res) {
+    set_error("null argument");
+    return PAC_ERR_ARG;
+  }
+  Graph *g = &h->g;
+  PAC_CHECK_HIP(hipSetDevice(g->ctx->device));
+  return square_count(g->ctx, g, res);
+}
+
 int pac_tc_batch(pac_ctx *ctx, pac_graph *const *hs, uint32_t G, int dispatch, int launch,
                  uint64_t *totals) {
   PAC_TRY(ensure_ctx(ctx));

@@ -177,6 +177,46 @@ void Graph::batch_cnt_common_neis_galloping(const std::vector<std::pair<VID, VID
   batch_cnt_common_neis_merge(queries, res, nt);
 }
 
+void Graph::batch_cnt_common_neis_merge(const std::vector<std::vector<VID>> &queries,
+                                        std::vector<size_t> &res, int) const {
+  std::vector<uint64_t> qoff(queries.size() + 1, 0);
+  std::vector<int32_t> qv;
+  for (size_t i = 0; i < queries.size(); ++i) {
+    qv.insert(qv.end(), queries[i].begin(), queries[i].end());
+    qoff[i + 1] = qv.size();
+  }
+  std::vector<uint64_t> r(queries.size());
+  check(pac_cnt_common_kway(g_, qoff.data(), qv.data(), queries.size(), r.data()));
+  res.assign(r.begin(), r.end());
+}
+
+void Graph::batch_cnt_common_neis_galloping(const std::vector<std::vector<VID>> &queries,
+                                            std::vector<size_t> &res, int nt) const {
+  batch_cnt_common_neis_merge(queries, res, nt);
+}
+
+size_t Graph::cnt_common_neis_merge(const std::vector<VID> &q) const {
+  std::vector<size_t> r;
+  batch_cnt_common_neis_merge(std::vector<std::vector<VID>>{q}, r, 1);
+  return r[0];
+}
+
+size_t Graph::cnt_common_neis_galloping(const std::vector<VID> &q) const {
+  return cnt_common_neis_merge(q);
+}
+
+size_t Graph::solve_friend_triangle_merge(const VID root, const std::vector<VID> &nodes,
+                                          const int) const {
+  uint64_t r = 0;
+  check(pac_friend_triangle(g_, root, nodes.data(), nodes.size(), &r));
+  return r;
+}
+
+size_t Graph::solve_friend_triangle_galloping(const VID root, const std::vector<VID> &nodes,
+                                              const int nt) const {
+  return solve_friend_triangle_merge(root, nodes, nt);
+}
+
 void Graph::greedy_partition(const size_t P, std::vector<int> &v2p) {
   v2p.assign(get_vertex_num(), 0);
   check(pac_greedy_partition(g_, (uint32_t)P, 1, v2p.data()));
@@ -291,6 +331,25 @@ void Partitioned_Graph::evaluate_partition() const {
   double ratio = 0;
   check(pac_evaluate_partition(g_, v2p_.data(), (uint32_t)partition_num_, &ratio));
   printf("balance ratio: %f\n", ratio);
+}
+
+// The per-partition partials of the reference sum to the unpartitioned quantity.
+size_t Partitioned_Graph::solve_friend_triangle_merge(const VID root,
+                                                      const std::vector<VID> &nodes) const {
+  uint64_t r = 0;
+  check(pac_friend_triangle(g_, root, nodes.data(), nodes.size(), &r));
+  return r;
+}
+
+size_t Partitioned_Graph::solve_friend_triangle_galloping(const VID root,
+                                                          const std::vector<VID> &nodes) const {
+  return solve_friend_triangle_merge(root, nodes);
+}
+
+size_t Partitioned_Graph::solve_square_counting() const {
+  uint64_t r = 0;
+  check(pac_square_count(g_, &r));
+  return r;
 }
 
 }  // namespace pac_ref

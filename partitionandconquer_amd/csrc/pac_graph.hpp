@@ -61,6 +61,17 @@ class Graph {
                                    std::vector<size_t> &res, int num_threads) const;
   void batch_cnt_common_neis_galloping(const std::vector<std::pair<VID, VID>> &queries,
                                        std::vector<size_t> &res, int num_threads) const;
+  // k-way intersections and friend triangles (Graph.cpp:200-225, 255-270, 309-338, 423-447)
+  size_t cnt_common_neis_merge(const std::vector<VID> &q) const;
+  size_t cnt_common_neis_galloping(const std::vector<VID> &q) const;
+  void batch_cnt_common_neis_merge(const std::vector<std::vector<VID>> &queries,
+                                   std::vector<size_t> &res, int num_threads) const;
+  void batch_cnt_common_neis_galloping(const std::vector<std::vector<VID>> &queries,
+                                       std::vector<size_t> &res, int num_threads) const;
+  size_t solve_friend_triangle_merge(const VID root, const std::vector<VID> &nodes,
+                                     const int num_threads) const;
+  size_t solve_friend_triangle_galloping(const VID root, const std::vector<VID> &nodes,
+                                         const int num_threads) const;
   void greedy_partition(const size_t partition_number, std::vector<int> &vertex2partition);
   void random_partition(const size_t partition_num, std::vector<int> &vertex2partition);
   void save_as_csr_binfile(const std::string csr_vlist_binfile,
@@ -102,6 +113,9 @@ class Partitioned_Graph {
   void solve_lcc_galloping(std::vector<double> &res) const;
   void solve_lcc_merge(std::vector<double> &res) const;
   void evaluate_partition() const;                   // Graph.cpp:1460-1473
+  size_t solve_friend_triangle_merge(const VID root, const std::vector<VID> &nodes) const;
+  size_t solve_friend_triangle_galloping(const VID root, const std::vector<VID> &nodes) const;
+  size_t solve_square_counting() const;              // Graph.cpp:1291-1324
   uint64_t triangle_count(int dispatch, int launch) const;
   void per_edge_jaccard(std::vector<int32_t> &src, std::vector<int32_t> &dst,
                         std::vector<uint32_t> &inter, std::vector<uint32_t> &uni,

@@ -210,6 +210,16 @@ int pair_counts(Ctx *ctx, Graph *g, const int32_t *d_u, const int32_t *d_v, uint
                 uint64_t *d_res);
 int query_matrix(Ctx *ctx, Graph *g, const int32_t *d_q, uint32_t nq, double *d_res);
 int fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t val);
+
+// ---- multi.hip -------------------------------------------------------------------
+// |N(q_0) & ... & N(q_k-1)| per query (query i = qv[qoff[i] .. qoff[i+1]), non-empty).
+int kway_counts(Ctx *ctx, Graph *g, const uint64_t *d_qoff, const int32_t *d_qv, uint64_t nq,
+                uint64_t *d_res);
+// Friend triangles of root over nodes; d_c holds root_deg counters, d_res one u64.
+int friend_triangles(Ctx *ctx, Graph *g, uint32_t root, const int32_t *d_nodes, uint64_t n,
+                     uint32_t root_deg, uint32_t *d_c, uint64_t *d_res);
+// The reference's square count (4-cycles with the extreme IDs at opposite corners).
+int square_count(Ctx *ctx, Graph *g, uint64_t *h_res);
 int stream_copy(Ctx *ctx, uint64_t bytes, int reps, double *ms);
 
 // ---- mlp.hip ---------------------------------------------------------------------

@@ -96,6 +96,9 @@ def lib():
         "pac_jaccard_edges": ([vp, C.c_int, C.c_int, vp, vp, vp, vp, vp], C.c_int),
         "pac_jaccard_queries": ([vp, vp, C.c_uint32, vp], C.c_int),
         "pac_cnt_common_pairs": ([vp, vp, vp, C.c_uint64, vp], C.c_int),
+        "pac_cnt_common_kway": ([vp, vp, vp, C.c_uint64, vp], C.c_int),
+        "pac_friend_triangle": ([vp, C.c_int32, vp, C.c_uint64, vp], C.c_int),
+        "pac_square_count": ([vp, vp], C.c_int),
         "pac_tc_batch": ([vp, vp, C.c_uint32, C.c_int, C.c_int, vp], C.c_int),
         "pac_autotune": ([vp, vp, C.c_uint32, C.c_char_p, C.POINTER(C.c_int), vp], C.c_int),
         "pac_set_tile": ([vp, C.c_int], C.c_int),
@@ -421,6 +424,39 @@ class Graph:
         out = np.zeros(len(u), np.uint64)
         _check(lib().pac_cnt_common_pairs(self._h, _ptr(u), _ptr(v), len(u), _ptr(out)))
         return out
+
+    def cnt_common_kway(self, queries) -> np.ndarray:
+        """|N(q_0) & ... & N(q_k-1)| for each query (a sequence of vertex IDs)."""
+        qoff = np.zeros(len(queries) + 1, np.uint64)
+        qoff[1:] = np.cumsum([len(q) for q in queries])
+        qv = np.ascontiguousarray(np.concatenate([np.asarray(q, np.int32) for q in queries])
+                                  if queries else np.zeros(0, np.int32), np.int32)
+        out = np.zeros(len(queries), np.uint64)
+        _check(lib().pac_cnt_common_kway(self._h, _ptr(qoff), _ptr(qv), len(queries), _ptr(out)))
+        return out
+
+    def friend_triangle(self, root, nodes) -> int:
+        nodes = np.ascontiguousarray(nodes, np.int32)
+        out = np.zeros(1, np.uint64)
+        _check(lib().pac_friend_triangle(self._h, int(root), _ptr(nodes), len(nodes), _ptr(out)))
+        return int(out[0])
+
+    def square_count(self) -> int:
+        out = np.zeros(1, np.uint64)
+        _check(lib().pac_square_count(self._h, _ptr(out)))
+        return int(out[0])
+
+    # reference method names (Graph.cpp:200-225, 255-270, 309-338, 1291-1324)
+    def cnt_common_neis_merge_k(self, q) -> int:
+        return int(self.cnt_common_kway([q])[0])
+
+    def solve_friend_triangle_merge(self, root, nodes, num_threads=0) -> int:
+        return self.friend_triangle(root, nodes)
+
+    solve_friend_triangle_galloping = solve_friend_triangle_merge
+
+    def solve_square_counting(self) -> int:
+        return self.square_count()
 
     def features(self) -> np.ndarray:
         P = self.info().num_partitions

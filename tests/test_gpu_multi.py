@@ -1,0 +1,85 @@
+"""GPU parity of the multi-way intersections (k-way common neighbours, friend triangles,
+square counting) through the C ABI against the CPU oracle on the same inputs: every
+count bit-exact."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import partitionandconquer_amd as pac
+
+pytestmark = pytest.mark.gpu
+NT = min(16, os.cpu_count() or 1)
+
+
+def gpu_graph(ctx, g):
+    return pac.Graph.from_csr(ctx, g.vlist, g.elist)
+
+
+def complete(n, orc):
+    return orc.edges_to_csr(n, *zip(*[(i, j) for i in range(n) for j in range(i + 1, n)]))
+
+
+@pytest.mark.parametrize("scale,seed", [(8, 1), (12, 2), (14, 3)])
+def test_kway_queries(ctx, orc, scale, seed):
+    ref = orc.gen_rmat(scale, 16, seed=seed)
+    g = gpu_graph(ctx, ref)
+    rng = np.random.default_rng(seed)
+    deg = np.diff(ref.vlist.astype(np.int64))
+    hubs = np.argsort(-deg)[:64]
+    queries = []
+    for k in (1, 2, 3, 4, 5, 8, 17):
+        for _ in range(60):
+            pool = hubs if rng.random() < 0.6 else np.arange(ref.V)
+            queries.append(rng.choice(pool, size=k).astype(np.int32))
+    queries.append(np.array([hubs[0]] * 3, np.int32))          # a vertex repeated
+    got = g.cnt_common_kway(queries)
+    want = [orc.kway_merge(ref, q) for q in queries]
+    assert got.tolist() == want
+    assert [orc.kway_galloping(ref, q) for q in queries[:80]] == want[:80]
+    pairs = [q for q in queries if len(q) == 2]                  # 2-way agrees with the pair API
+    u = np.array([q[0] for q in pairs], np.int32)
+    v = np.array([q[1] for q in pairs], np.int32)
+    assert g.cnt_common_pairs(u, v).tolist() == g.cnt_common_kway(pairs).tolist()
+
+
+def test_friend_triangles(ctx, orc):
+    n = 12
+    ref = complete(n, orc)
+    g = gpu_graph(ctx, ref)
+    assert g.friend_triangle(0, list(range(n))) == math.comb(n - 1, 2) * (n - 3)
+    ref = orc.gen_rmat(13, 16, seed=7)
+    g = gpu_graph(ctx, ref)
+    rng = np.random.default_rng(7)
+    deg = np.diff(ref.vlist.astype(np.int64))
+    for root in (int(np.argmax(deg)), 5, int(np.argsort(deg)[len(deg) // 2])):
+        nodes = np.concatenate([rng.choice(ref.V, size=150), np.argsort(-deg)[:40],
+                                [root, root]]).astype(np.int32)
+        assert g.friend_triangle(root, nodes) == orc.friend_triangle_merge(ref, root, nodes, NT)
+    assert g.friend_triangle(3, []) == 0
+
+
+def test_square_counting(ctx, orc):
+    for n in (4, 6, 9):
+        g = gpu_graph(ctx, complete(n, orc))
+        assert g.square_count() == math.comb(n, 4)
+    bip = orc.edges_to_csr(9, *zip(*[(i, j) for i in range(4) for j in range(4, 9)]))
+    assert gpu_graph(ctx, bip).square_count() == 0
+    for scale, seed in ((8, 1), (11, 2)):
+        ref = orc.gen_rmat(scale, 8, seed=seed)
+        want = orc.PGraph(ref, orc.contiguous_partition(ref.V, 4)).square_counting()
+        assert gpu_graph(ctx, ref).square_count() == want
+    er = orc.gen_er(3000, 20.0, seed=4)
+    want = orc.PGraph(er, orc.random_partition(er.V, 7, 2)).square_counting()
+    assert gpu_graph(ctx, er).square_count() == want
+
+
+def test_multiway_argument_errors(ctx, orc):
+    g = gpu_graph(ctx, complete(5, orc))
+    with pytest.raises(pac.PacError):
+        g.cnt_common_kway([np.array([0, 9], np.int32)])     # vertex out of range
+    with pytest.raises(pac.PacError):
+        g.cnt_common_kway([np.zeros(0, np.int32)])          # empty query
+    with pytest.raises(pac.PacError):
+        g.friend_triangle(7, [0, 1])                         # root out of range

@@ -291,3 +291,65 @@ def test_mlp_oracle_forward_and_gradient(orc):
         e = np.zeros(orc.N_PARAMS, np.float64); e[k] = 1e-4
         fd = (np_loss(P + e, x, y, w) - np_loss(P - e, x, y, w)) / 2e-4
         assert g[k] == pytest.approx(fd, rel=2e-3, abs=2e-6)
+
+
+# ---- k-way intersections, friend triangles, squares (independent definitions) --------
+
+def neighbour_sets(g):
+    return [set(g.elist[g.vlist[u]:g.vlist[u + 1]].tolist()) for u in range(g.V)]
+
+
+def brute_squares(g):
+    """4-tuples u > v > w > x with u~v, u~w, x~v, x~w, enumerated directly."""
+    N = neighbour_sets(g)
+    res = 0
+    for u in range(g.V):
+        for v in N[u]:
+            if v >= u:
+                continue
+            for w in N[u]:
+                if w >= v:
+                    continue
+                res += sum(1 for x in N[v] & N[w] if x < w)
+    return res
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_kway_vs_sets(seed, orc):
+    g = orc.gen_rmat(9, 8, seed=seed)
+    N = neighbour_sets(g)
+    rng = np.random.default_rng(seed)
+    hubs = np.argsort(-np.diff(g.vlist.astype(np.int64)))[:40]
+    for k in (1, 2, 3, 4, 6):
+        for _ in range(40):
+            q = rng.choice(hubs if rng.random() < 0.7 else np.arange(g.V), size=k)
+            want = len(set.intersection(*[N[int(v)] for v in q]))
+            assert orc.kway_merge(g, q) == want
+            assert orc.kway_galloping(g, q) == want
+
+
+def test_friend_triangle(orc):
+    n = 9                                    # K_n: every pair sees n - 3 common neighbours
+    g = complete(n, orc)
+    assert orc.friend_triangle_merge(g, 0, list(range(n)), 2) == math.comb(n - 1, 2) * (n - 3)
+    g = orc.gen_rmat(9, 8, seed=5)
+    N = neighbour_sets(g)
+    rng = np.random.default_rng(5)
+    for root in (0, 3, 17):
+        nodes = rng.choice(g.V, size=30).tolist() + [root, root]
+        want = sum(len(N[root] & N[a] & N[b])
+                   for i, a in enumerate(nodes) for b in nodes[i + 1:]
+                   if a != root and b != root)
+        assert orc.friend_triangle_merge(g, root, nodes, 2) == want
+
+
+@pytest.mark.parametrize("P", [1, 3, 8])
+def test_square_counting(P, orc):
+    for n in (4, 5, 7):                      # K_n: one arrangement in three per 4-set
+        g = complete(n, orc)
+        assert orc.PGraph(g, orc.random_partition(n, P, 3)).square_counting() == math.comb(n, 4)
+    g = orc.edges_to_csr(9, *zip(*[(i, j) for i in range(4) for j in range(4, 9)]))
+    assert orc.PGraph(g, orc.random_partition(9, P, 4)).square_counting() == 0   # K_{4,5}
+    for seed in (1, 2):
+        g = orc.gen_rmat(7, 6, seed=seed)
+        assert orc.PGraph(g, orc.random_partition(g.V, P, seed)).square_counting() == brute_squares(g)
