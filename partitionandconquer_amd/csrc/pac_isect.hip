@@ -438,10 +438,32 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // Lookup of w.  A bucket's two slots share one word of the packed 16-bit counters, so
   // a hit is reported as that word (the bucket) and the increment for w's half of it
   // (1 or 1 << 16); inc = 0 when w is absent.  Stash entry k sits at slot nslot + k.
+  //
+  // Slots never become empty again (insertion fills them, evictions exchange keys), and
+  // a key is placed in its second bucket only when its first is full.  So a key whose
+  // first bucket still has a free slot cannot be in the second bucket or the stash: in
+  // TC mode the second read is skipped for those lookups, which saves LDS bandwidth (in
+  // LCC mode, at lower occupancy, the dependent read costs more than it saves).
   auto find = [&](uint32_t w, uint32_t &word) -> uint32_t {
     uint32_t b1, b2;
     buckets(w, sh, b1, b2);
     const uint32_t k10 = H.tab[2u * b1], k11 = H.tab[2u * b1 + 1u];
+    if (MODE == kModeTC) {
+      uint32_t inc = (k10 == w) ? 1u : ((k11 == w) ? 0x10000u : 0u);
+      word = b1;
+      if (inc == 0u && k11 != kEmpty) {
+        const uint32_t k20 = H.tab[2u * b2], k21 = H.tab[2u * b2 + 1u];
+        inc = (k20 == w) ? 1u : ((k21 == w) ? 0x10000u : 0u);
+        word = b2;
+        for (uint32_t k = 0; k < nst; ++k) {
+          if (H.tab[nslot + k] == w) {
+            word = (nslot + k) >> 1;
+            inc = (k & 1u) ? 0x10000u : 1u;  // nslot is even
+          }
+        }
+      }
+      return inc;
+    }
     const uint32_t k20 = H.tab[2u * b2], k21 = H.tab[2u * b2 + 1u];
     // bitwise | (no short-circuit): all four slots are read up front, no branches
     const bool in1 = (k10 == w) | (k11 == w), hi = (k11 == w) | (k21 == w);
