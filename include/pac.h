@@ -192,6 +192,28 @@ int pac_jaccard_queries(pac_graph *g, const int32_t *q, uint32_t n, double *res)
  * res[i] = |N(u[i]) & N(v[i])| over the full symmetric lists. */
 int pac_cnt_common_pairs(pac_graph *g, const int32_t *u, const int32_t *v, uint64_t n,
                          uint64_t *res);
+/* Filtered pair counts: res[i] = #{w in N(u[i]) & N(v[i]) : w < thr[i], and part(w) ==
+ * partition}.  thr may be NULL (no bound); partition = -1 means every partition, else the
+ * graph must be prepared and 0 <= partition < P, part() being the map of the last
+ * pac_graph_prepare.  Replaces Graph::cnt_common_neis_less_than_merge (Graph.cpp:272-285),
+ * Partitioned_Graph::cnt_common_neis_{merge,galloping}(u, v, k) (Graph.cpp:1155-1173,
+ * 1071-1085), Partitioned_Graph::cnt_common_neis_less_than_merge(u, v, thr, k)
+ * (Graph.cpp:1132-1153) and, summed over k, Partitioned_Graph::batch_cnt_common_neis_*
+ * (Graph.cpp:1382-1458). */
+int pac_cnt_common_pairs_filtered(pac_graph *g, const int32_t *u, const int32_t *v,
+                                  const int32_t *thr, uint64_t n, int32_t partition,
+                                  uint64_t *res);
+/* Common-neighbour lists, ascending, each value once: N(u) & N(v) when nodes is NULL,
+ * else N(u) & nodes[0..n) (nodes sorted ascending, else PAC_ERR_ARG; values outside
+ * [0, V) never match), restricted to `partition` as in pac_cnt_common_pairs_filtered.
+ * *count receives the number found; out (capacity cap entries, may be NULL with cap 0
+ * to ask for the count only) receives the list, PAC_ERR_ARG when it does not fit.
+ * Replaces Graph::get_common_neis(u, v) and (u, nodes) (Graph.cpp:468-493),
+ * Graph::cnt_common_neis_merge(u, nodes) (Graph.cpp:241-253), and the per-partition
+ * Partitioned_Graph::get_common_neis, cnt_common_neis_merge(u, nodes, k) and
+ * get_neis(u, k) (= u with v = u) (Graph.cpp:1175-1195, 1475-1498). */
+int pac_common_neis(pac_graph *g, int32_t u, int32_t v, const int32_t *nodes, uint64_t n,
+                    int32_t partition, int32_t *out, uint64_t cap, uint64_t *count);
 
 /* ---- multi-way intersections ----------------------------------------------------- */
 /* |N(q_0) & N(q_1) & ... & N(q_{k-1})| for nq queries in CSR form: query i is
@@ -202,6 +224,12 @@ int pac_cnt_common_pairs(pac_graph *g, const int32_t *u, const int32_t *v, uint6
  * 255-270); both reference variants return the same count. */
 int pac_cnt_common_kway(pac_graph *g, const uint64_t *qoff, const int32_t *qv, uint64_t nq,
                         uint64_t *res);
+/* pac_cnt_common_kway restricted to the common neighbours in `partition` (-1: all), as in
+ * pac_cnt_common_pairs_filtered.  Replaces Partitioned_Graph::cnt_common_neis_{merge,
+ * galloping}(q, k) (Graph.cpp:1087-1130) and, summed over k, the k-way
+ * Partitioned_Graph::batch_cnt_common_neis_* (Graph.cpp:1395-1406, 1434-1445). */
+int pac_cnt_common_kway_filtered(pac_graph *g, const uint64_t *qoff, const int32_t *qv,
+                                 uint64_t nq, int32_t partition, uint64_t *res);
 /* Sum over index pairs i < j of nodes (neither equal to root) of
  * |N(root) & N(nodes[i]) & N(nodes[j])|.  Replaces Graph::solve_friend_triangle_{merge,
  * galloping} (Graph.cpp:309-338) and Partitioned_Graph::solve_friend_triangle_*

@@ -90,6 +90,24 @@ def lib():
         L.orc_friend_triangle_merge.restype = C.c_uint64
         L.orc_pgraph_square_counting.argtypes = [C.c_void_p]
         L.orc_pgraph_square_counting.restype = C.c_uint64
+        L.orc_pgraph_cnt.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, C.c_int]
+        L.orc_pgraph_cnt.restype = C.c_uint64
+        L.orc_pgraph_cnt_less_than.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                               C.c_uint64]
+        L.orc_pgraph_cnt_less_than.restype = C.c_uint64
+        L.orc_pgraph_cnt_nodes.argtypes = [C.c_void_p, C.c_int32, i32p, C.c_uint64, C.c_uint64]
+        L.orc_pgraph_cnt_nodes.restype = C.c_uint64
+        L.orc_pgraph_common_neis.argtypes = [C.c_void_p, C.c_int32, i32p, C.c_uint64, C.c_uint64,
+                                             i32p]
+        L.orc_pgraph_common_neis.restype = C.c_uint64
+        L.orc_pgraph_get_neis.argtypes = [C.c_void_p, C.c_int32, C.c_uint64, i32p]
+        L.orc_pgraph_get_neis.restype = C.c_uint64
+        L.orc_pgraph_kway.argtypes = [C.c_void_p, i32p, C.c_uint64, C.c_uint64, C.c_int]
+        L.orc_pgraph_kway.restype = C.c_uint64
+        L.orc_cnt_merge_nodes.argtypes = [u64p, i32p, C.c_int32, i32p, C.c_uint64]
+        L.orc_cnt_merge_nodes.restype = C.c_uint64
+        L.orc_common_neis_nodes.argtypes = [u64p, i32p, C.c_int32, i32p, C.c_uint64, i32p]
+        L.orc_common_neis_nodes.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -174,11 +192,11 @@ def gen_er(n, avg_deg, seed=1) -> CSR:
 
 
 def cnt_merge(g: CSR, u, v) -> int:
-    return lib().orc_cnt_merge(g.vlist, g.elist, u, v)
+    return lib().orc_cnt_merge(g.vlist, g.elist, int(u), int(v))
 
 
 def cnt_less_than_merge(g: CSR, u, v, thr) -> int:
-    return lib().orc_cnt_less_than_merge(g.vlist, g.elist, u, v, thr)
+    return lib().orc_cnt_less_than_merge(g.vlist, g.elist, int(u), int(v), int(thr))
 
 
 def cnt_galloping(g: CSR, u, v) -> int:
@@ -262,6 +280,7 @@ class PGraph:
         self.v2p = np.ascontiguousarray(v2p, dtype=np.int32)
         self._h = lib().orc_pgraph_build(g.vlist, g.elist, g.V, self.v2p, _nt(nt))
         self.V = g.V
+        self.max_deg = int(g.deg().max()) if g.V else 0
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -289,6 +308,39 @@ class PGraph:
     def square_counting(self) -> int:
         return lib().orc_pgraph_square_counting(self._h)
 
+    # per-partition queries, Graph.cpp:1071-1195, 1475-1498
+    def cnt_merge(self, u, v, k) -> int:
+        return lib().orc_pgraph_cnt(self._h, int(u), int(v), int(k), 0)
+
+    def cnt_galloping(self, u, v, k) -> int:
+        return lib().orc_pgraph_cnt(self._h, int(u), int(v), int(k), 1)
+
+    def cnt_less_than_merge(self, u, v, thr, k) -> int:
+        return lib().orc_pgraph_cnt_less_than(self._h, int(u), int(v), int(thr), int(k))
+
+    def cnt_merge_nodes(self, u, nodes, k) -> int:
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        return lib().orc_pgraph_cnt_nodes(self._h, int(u), nodes, len(nodes), int(k))
+
+    def common_neis(self, u, nodes, k) -> np.ndarray:
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        out = np.zeros(max(1, len(nodes)), np.int32)
+        c = lib().orc_pgraph_common_neis(self._h, int(u), nodes, len(nodes), int(k), out)
+        return out[:c].copy()
+
+    def get_neis(self, u, k) -> np.ndarray:
+        out = np.zeros(max(1, self.max_deg), np.int32)
+        c = lib().orc_pgraph_get_neis(self._h, int(u), int(k), out)
+        return out[:c].copy()
+
+    def kway_merge(self, q, k) -> int:
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        return lib().orc_pgraph_kway(self._h, q, len(q), int(k), 0)
+
+    def kway_galloping(self, q, k) -> int:
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        return lib().orc_pgraph_kway(self._h, q, len(q), int(k), 1)
+
 
 def kway_merge(g: CSR, q) -> int:
     q = np.ascontiguousarray(q, dtype=np.int32)
@@ -298,6 +350,20 @@ def kway_merge(g: CSR, q) -> int:
 def kway_galloping(g: CSR, q) -> int:
     q = np.ascontiguousarray(q, dtype=np.int32)
     return lib().orc_kway_galloping(g.vlist, g.elist, q, len(q))
+
+
+def cnt_merge_nodes(g: CSR, u, nodes) -> int:
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    return lib().orc_cnt_merge_nodes(g.vlist, g.elist, int(u), nodes, len(nodes))
+
+
+def common_neis(g: CSR, u, nodes) -> np.ndarray:
+    """Graph::get_common_neis(u, nodes) (Graph.cpp:481-493); with nodes = N(v) it is
+    Graph::get_common_neis(u, v) (Graph.cpp:468-479)."""
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    out = np.zeros(max(1, len(nodes)), np.int32)
+    c = lib().orc_common_neis_nodes(g.vlist, g.elist, int(u), nodes, len(nodes), out)
+    return out[:c].copy()
 
 
 def friend_triangle_merge(g: CSR, root, nodes, nt=0) -> int:

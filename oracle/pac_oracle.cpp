@@ -413,6 +413,97 @@ static inline uint64_t pg_merge(const orc_pgraph *g, int32_t u, int32_t v, uint6
   return merge_count(g->el.data() + ub, ue - ub, g->el.data() + vb, ve - vb);
 }
 
+// Per-partition pair counts, Graph.cpp:1071-1085 (galloping), 1155-1173 (merge).
+uint64_t orc_pgraph_cnt(const orc_pgraph *g, int32_t u, int32_t v, uint64_t k, int galloping) {
+  return galloping ? pg_gallop(g, u, v, k) : pg_merge(g, u, v, k);
+}
+
+// Partitioned_Graph::cnt_common_neis_less_than_merge, Graph.cpp:1132-1153.
+uint64_t orc_pgraph_cnt_less_than(const orc_pgraph *g, int32_t u, int32_t v, int32_t thr,
+                                  uint64_t k) {
+  return pg_below(g, u, v, thr, k);
+}
+
+// Partitioned_Graph::cnt_common_neis_merge(u, nodes, k), Graph.cpp:1175-1195: N_k(u)
+// merged against a sorted node list.
+uint64_t orc_pgraph_cnt_nodes(const orc_pgraph *g, int32_t u, const int32_t *nodes, uint64_t n,
+                              uint64_t k) {
+  const uint64_t V = g->V;
+  const uint64_t ub = g->vl[k * V + u], ue = g->vl[k * V + u + 1];
+  return merge_count(g->el.data() + ub, ue - ub, nodes, n);
+}
+
+// Partitioned_Graph::get_common_neis(u, nodes, k, res), Graph.cpp:1475-1493: the merge
+// of N_k(u) and a sorted node list, emitting each match.  Returns the count.
+uint64_t orc_pgraph_common_neis(const orc_pgraph *g, int32_t u, const int32_t *nodes,
+                                uint64_t n, uint64_t k, int32_t *out) {
+  const uint64_t V = g->V;
+  uint64_t i = g->vl[k * V + u], j = 0, c = 0;
+  const uint64_t ie = g->vl[k * V + u + 1];
+  while (i < ie && j < n) {
+    const int32_t a = g->el[i];
+    if (a == nodes[j]) {
+      out[c++] = nodes[j];
+      ++i;
+      ++j;
+    } else if (a < nodes[j]) {
+      ++i;
+    } else {
+      ++j;
+    }
+  }
+  return c;
+}
+
+// Partitioned_Graph::get_neis(u, k, res), Graph.cpp:1495-1498.  Returns the count.
+uint64_t orc_pgraph_get_neis(const orc_pgraph *g, int32_t u, uint64_t k, int32_t *out) {
+  const uint64_t V = g->V;
+  const uint64_t b = g->vl[k * V + u], e = g->vl[k * V + u + 1];
+  for (uint64_t i = b; i < e; ++i) out[i - b] = g->el[i];
+  return e - b;
+}
+
+// Partitioned_Graph::cnt_common_neis_merge(q, k), Graph.cpp:1115-1130: N_k(q[0]) is
+// narrowed by N_k(q[1]) .. N_k(q[n-2]) and the last sub-list is counted against it.
+// Partitioned_Graph::cnt_common_neis_galloping(q, k), Graph.cpp:1087-1113: the
+// sub-list of smallest size drives and each element is looked up in the others.
+uint64_t orc_pgraph_kway(const orc_pgraph *g, const int32_t *q, uint64_t n, uint64_t k,
+                         int galloping) {
+  const uint64_t V = g->V;
+  const int32_t *el = g->el.data();
+  auto lo = [&](int32_t x) { return g->vl[k * V + x]; };
+  auto hi = [&](int32_t x) { return g->vl[k * V + x + 1]; };
+  if (galloping) {
+    uint64_t min_deg = V;
+    int32_t u = q[0];
+    for (uint64_t i = 0; i < n; ++i)
+      if (hi(q[i]) - lo(q[i]) < min_deg) {
+        min_deg = hi(q[i]) - lo(q[i]);
+        u = q[i];
+      }
+    uint64_t res = 0;
+    for (uint64_t i = lo(u); i < hi(u); ++i) {
+      const int32_t t = el[i];
+      bool found = true;
+      for (uint64_t j = 0; j < n && found; ++j) {
+        if (q[j] == u) continue;
+        const int32_t *b = el + lo(q[j]), *e = el + hi(q[j]);
+        const int32_t *p = std::lower_bound(b, e, t);
+        found = p < e && *p == t;
+      }
+      if (found) ++res;
+    }
+    return res;
+  }
+  std::vector<int32_t> cur(el + lo(q[0]), el + hi(q[0])), nxt;
+  for (uint64_t i = 1; i + 1 < n; ++i) {
+    nxt.resize(cur.size());
+    nxt.resize(orc_pgraph_common_neis(g, q[i], cur.data(), cur.size(), k, nxt.data()));
+    cur.swap(nxt);
+  }
+  return merge_count(el + lo(q[n - 1]), hi(q[n - 1]) - lo(q[n - 1]), cur.data(), cur.size());
+}
+
 uint64_t orc_pgraph_tc_merge_partial(const orc_pgraph *g, uint64_t i) {
   const uint64_t V = g->V, P = g->P;
   uint64_t res = 0;
@@ -722,6 +813,33 @@ uint64_t orc_kway_merge(const uint64_t *vl, const int32_t *el, const int32_t *q,
     cur.swap(nxt);
   }
   return merge_count(cur.data(), cur.size(), el + vl[q[k - 1]], deg(vl, q[k - 1]));
+}
+
+// Graph::cnt_common_neis_merge(u, nodes), Graph.cpp:241-253: N(u) merged against a
+// sorted node list.
+uint64_t orc_cnt_merge_nodes(const uint64_t *vl, const int32_t *el, int32_t u,
+                             const int32_t *nodes, uint64_t n) {
+  return merge_count(el + vl[u], deg(vl, u), nodes, n);
+}
+
+// Graph::get_common_neis(u, v, res), Graph.cpp:468-479 (with v's list as `nodes`) and
+// Graph::get_common_neis(u, nodes, res), Graph.cpp:481-493: the merge emits each match.
+// Returns the count.
+uint64_t orc_common_neis_nodes(const uint64_t *vl, const int32_t *el, int32_t u,
+                               const int32_t *nodes, uint64_t n, int32_t *out) {
+  uint64_t i = vl[u], j = 0, c = 0;
+  while (i < vl[u + 1] && j < n) {
+    if (el[i] == nodes[j]) {
+      out[c++] = nodes[j];
+      ++i;
+      ++j;
+    } else if (el[i] < nodes[j]) {
+      ++i;
+    } else {
+      ++j;
+    }
+  }
+  return c;
 }
 
 // Graph::cnt_common_neis_galloping(const vector<VID>&), Graph.cpp:200-225: the

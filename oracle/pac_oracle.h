@@ -109,6 +109,24 @@ uint64_t orc_pgraph_tc_galloping(const orc_pgraph *g);
 void orc_pgraph_lcc_merge(const orc_pgraph *g, double *res);
 /* Per-partition partial of Graph.cpp:1346-1362 for intersection partition i. */
 uint64_t orc_pgraph_tc_merge_partial(const orc_pgraph *g, uint64_t i);
+/* Per-partition pair counts |N_k(u) & N_k(v)|: Graph.cpp:1155-1173 (merge) and
+ * 1071-1085 (galloping = nonzero). */
+uint64_t orc_pgraph_cnt(const orc_pgraph *g, int32_t u, int32_t v, uint64_t k, int galloping);
+/* Partitioned_Graph::cnt_common_neis_less_than_merge, Graph.cpp:1132-1153. */
+uint64_t orc_pgraph_cnt_less_than(const orc_pgraph *g, int32_t u, int32_t v, int32_t thr,
+                                  uint64_t k);
+/* Partitioned_Graph::cnt_common_neis_merge(u, nodes, k), Graph.cpp:1175-1195. */
+uint64_t orc_pgraph_cnt_nodes(const orc_pgraph *g, int32_t u, const int32_t *nodes, uint64_t n,
+                              uint64_t k);
+/* Partitioned_Graph::get_common_neis(u, nodes, k), Graph.cpp:1475-1493 -> count; out holds
+ * at least min(n, |N_k(u)|) entries. */
+uint64_t orc_pgraph_common_neis(const orc_pgraph *g, int32_t u, const int32_t *nodes,
+                                uint64_t n, uint64_t k, int32_t *out);
+/* Partitioned_Graph::get_neis(u, k), Graph.cpp:1495-1498 -> count. */
+uint64_t orc_pgraph_get_neis(const orc_pgraph *g, int32_t u, uint64_t k, int32_t *out);
+/* Partitioned_Graph::cnt_common_neis_{merge,galloping}(q, k), Graph.cpp:1087-1130 (n >= 1). */
+uint64_t orc_pgraph_kway(const orc_pgraph *g, const int32_t *q, uint64_t n, uint64_t k,
+                         int galloping);
 void orc_pgraph_free(orc_pgraph *g);
 
 /* ---- random_partition (Graph.cpp:564-570) with a seeded stream instead of rand() */
@@ -150,6 +168,13 @@ float orc_mlp_backward(const float *params, const float *x, const int32_t *label
 
 /* Graph::cnt_common_neis_merge(const vector<VID>&), Graph.cpp:255-270 (k >= 1). */
 uint64_t orc_kway_merge(const uint64_t *vl, const int32_t *el, const int32_t *q, uint64_t k);
+/* Graph::cnt_common_neis_merge(u, nodes), Graph.cpp:241-253 (nodes sorted). */
+uint64_t orc_cnt_merge_nodes(const uint64_t *vl, const int32_t *el, int32_t u,
+                             const int32_t *nodes, uint64_t n);
+/* Graph::get_common_neis(u, nodes), Graph.cpp:468-493 -> count; out holds at least
+ * min(n, d(u)) entries. */
+uint64_t orc_common_neis_nodes(const uint64_t *vl, const int32_t *el, int32_t u,
+                               const int32_t *nodes, uint64_t n, int32_t *out);
 /* Graph::cnt_common_neis_galloping(const vector<VID>&), Graph.cpp:200-225. */
 uint64_t orc_kway_galloping(const uint64_t *vl, const int32_t *el, const int32_t *q, uint64_t k);
 /* Graph::solve_friend_triangle_merge, Graph.cpp:323-338. */

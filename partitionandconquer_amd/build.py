@@ -19,6 +19,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpac.so")
 CLI = os.path.join(HERE, "pac_main")
+# test driver of the C++ facade (built only where the tests directory is present)
+FACADE_CHECK_SRC = os.path.join(HERE, "..", "tests", "facade_check.cpp")
+FACADE_CHECK = os.path.join(HERE, "..", "tests", "_bin", "facade_check")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PAC_OFFLOAD_ARCH", "gfx950")
 
@@ -76,6 +79,18 @@ def build(force: bool = False, jobs: int | None = None, cli: bool = True) -> str
             if r.returncode != 0:
                 raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
             os.replace(CLI + ".tmp", CLI)
+    if cli and os.path.exists(FACADE_CHECK_SRC):
+        graph_obj = os.path.join(BUILD, "pac_graph.cpp.o")
+        if force or _newer(FACADE_CHECK_SRC, FACADE_CHECK,
+                           [LIB, graph_obj, os.path.join(CSRC, "pac_graph.hpp")]):
+            os.makedirs(os.path.dirname(FACADE_CHECK), exist_ok=True)
+            obj = _compile(FACADE_CHECK_SRC, True)
+            cmd = [HIPCC, f"--offload-arch={ARCH}", "-o", FACADE_CHECK + ".tmp", obj, graph_obj,
+                   "-L" + HERE, "-lpac", "-Wl,-rpath,$ORIGIN/../../partitionandconquer_amd"]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+            os.replace(FACADE_CHECK + ".tmp", FACADE_CHECK)
     return LIB
 
 

@@ -837,6 +837,12 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
   for (uint32_t k = 0; k < P; ++k) count[k + 1] += count[k];
   g->P = P;
   g->h_pstart = count;
+  g->part_width = width ? width : 1;
+  if (v2p)
+    g->h_v2p.assign(v2p, v2p + V);
+  else
+    g->h_v2p.clear();
+  g->part_ready = false;
   std::vector<uint32_t> rowlist(count[P]);
   {
     std::vector<uint32_t> cur(count.begin(), count.end() - 1);
@@ -1106,6 +1112,36 @@ int pac_jaccard_queries(pac_graph *h, const int32_t *q, uint32_t n, double *res)
 
 int pac_cnt_common_pairs(pac_graph *h, const int32_t *u, const int32_t *v, uint64_t n,
                          uint64_t *res) {
+  return pac_cnt_common_pairs_filtered(h, u, v, nullptr, n, -1, res);
+}
+
+namespace {
+// Resolves a partition argument: -1 = no filter (*d_part = null); else the graph must be
+// prepared and 0 <= partition < P.
+int partition_filter(Graph *g, int32_t partition, const uint32_t **d_part) {
+  *d_part = nullptr;
+  if (partition < 0) {
+    if (partition != -1) {
+      set_error("partition %d: use -1 for no partition filter", partition);
+      return PAC_ERR_ARG;
+    }
+    return PAC_OK;
+  }
+  if (!g->prepared) {
+    set_error("a partition filter needs a prepared graph");
+    return PAC_ERR_STATE;
+  }
+  if ((uint32_t)partition >= g->P) {
+    set_error("partition %d out of range (P = %u)", partition, g->P);
+    return PAC_ERR_ARG;
+  }
+  return partition_map(g, d_part);
+}
+}  // namespace
+
+int pac_cnt_common_pairs_filtered(pac_graph *h, const int32_t *u, const int32_t *v,
+                                  const int32_t *thr, uint64_t n, int32_t partition,
+                                  uint64_t *res) {
   if (!h || (n && (!u || !v || !res))) {
     set_error("null argument");
     return PAC_ERR_ARG;
@@ -1117,27 +1153,101 @@ int pac_cnt_common_pairs(pac_graph *h, const int32_t *u, const int32_t *v, uint6
       set_error("query pair %llu out of range", (unsigned long long)i);
       return PAC_ERR_ARG;
     }
+  const uint32_t *d_part = nullptr;
+  PAC_TRY(partition_filter(g, partition, &d_part));
   if (n == 0) return PAC_OK;
-  DevBuf du, dv, dr;
+  DevBuf du, dv, dt, dr;
   PAC_TRY(du.reserve(4 * n));
   PAC_TRY(dv.reserve(4 * n));
   PAC_TRY(dr.reserve(8 * n));
   PAC_CHECK_HIP(hipMemcpy(du.p, u, 4 * n, hipMemcpyHostToDevice));
   PAC_CHECK_HIP(hipMemcpy(dv.p, v, 4 * n, hipMemcpyHostToDevice));
-  PAC_TRY(pair_counts(g->ctx, g, du.as<int32_t>(), dv.as<int32_t>(), n, dr.as<uint64_t>()));
+  if (thr) {
+    PAC_TRY(dt.reserve(4 * n));
+    PAC_CHECK_HIP(hipMemcpy(dt.p, thr, 4 * n, hipMemcpyHostToDevice));
+  }
+  PAC_TRY(pair_counts(g->ctx, g, du.as<int32_t>(), dv.as<int32_t>(), n, dr.as<uint64_t>(),
+                      thr ? dt.as<int32_t>() : nullptr, d_part, (uint32_t)partition));
   PAC_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
   PAC_CHECK_HIP(hipMemcpy(res, dr.p, 8 * n, hipMemcpyDeviceToHost));
   return PAC_OK;
 }
 
+int pac_common_neis(pac_graph *h, int32_t u, int32_t v, const int32_t *nodes, uint64_t n,
+                    int32_t partition, int32_t *out, uint64_t cap, uint64_t *count) {
+  if (!h || !count || (n && !nodes) || (cap && !out)) {
+    set_error("null argument");
+    return PAC_ERR_ARG;
+  }
+  Graph *g = &h->g;
+  PAC_CHECK_HIP(hipSetDevice(g->ctx->device));
+  if (u < 0 || (uint32_t)u >= g->V || (!nodes && (v < 0 || (uint32_t)v >= g->V))) {
+    set_error("vertex out of range (u = %d, v = %d, V = %u)", u, v, g->V);
+    return PAC_ERR_ARG;
+  }
+  for (uint64_t i = 1; nodes && i < n; ++i)
+    if (nodes[i] < nodes[i - 1]) {
+      set_error("nodes must be sorted ascending (nodes[%llu] < nodes[%llu])",
+                (unsigned long long)i, (unsigned long long)(i - 1));
+      return PAC_ERR_ARG;
+    }
+  const uint32_t *d_part = nullptr;
+  PAC_TRY(partition_filter(g, partition, &d_part));
+  uint32_t ru[2], rv[2] = {0, 0};
+  PAC_CHECK_HIP(hipMemcpy(ru, g->off.as<uint32_t>() + u, 8, hipMemcpyDeviceToHost));
+  // the list that is walked: `nodes`, or the shorter of N(u), N(v) (the other is searched)
+  uint32_t s = (uint32_t)u;
+  uint64_t len = n;
+  const int32_t *d_list = nullptr;
+  DevBuf dn;
+  if (nodes) {
+    if (n) {
+      PAC_TRY(dn.reserve(4 * n));
+      PAC_CHECK_HIP(hipMemcpy(dn.p, nodes, 4 * n, hipMemcpyHostToDevice));
+    }
+    d_list = dn.as<int32_t>();
+  } else {
+    PAC_CHECK_HIP(hipMemcpy(rv, g->off.as<uint32_t>() + v, 8, hipMemcpyDeviceToHost));
+    const bool walk_u = ru[1] - ru[0] < rv[1] - rv[0];
+    s = walk_u ? (uint32_t)v : (uint32_t)u;
+    const uint32_t *lo = walk_u ? ru : rv;
+    len = lo[1] - lo[0];
+    d_list = reinterpret_cast<const int32_t *>(g->adj.as<uint32_t>() + lo[0]);
+  }
+  const uint64_t most = std::min<uint64_t>(len, ru[1] - ru[0]);
+  DevBuf dout, dc;
+  PAC_TRY(dout.reserve(4 * (most ? most : 1)));
+  PAC_TRY(dc.reserve(8));
+  PAC_TRY(common_list(g->ctx, g, s, d_list, len, d_part, partition < 0 ? 0u : (uint32_t)partition,
+                      dout.as<int32_t>(), reinterpret_cast<unsigned long long *>(dc.p)));
+  PAC_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
+  uint64_t c = 0;
+  PAC_CHECK_HIP(hipMemcpy(&c, dc.p, 8, hipMemcpyDeviceToHost));
+  *count = c;
+  if (out && c > cap) {
+    set_error("%llu common neighbours, output holds %llu", (unsigned long long)c,
+              (unsigned long long)cap);
+    return PAC_ERR_ARG;
+  }
+  if (out && c) PAC_CHECK_HIP(hipMemcpy(out, dout.p, 4 * c, hipMemcpyDeviceToHost));
+  return PAC_OK;
+}
+
 int pac_cnt_common_kway(pac_graph *h, const uint64_t *qoff, const int32_t *qv, uint64_t nq,
                         uint64_t *res) {
+  return pac_cnt_common_kway_filtered(h, qoff, qv, nq, -1, res);
+}
+
+int pac_cnt_common_kway_filtered(pac_graph *h, const uint64_t *qoff, const int32_t *qv,
+                                 uint64_t nq, int32_t partition, uint64_t *res) {
   if (!h || (nq && (!qoff || !qv || !res))) {
     set_error("null argument");
     return PAC_ERR_ARG;
   }
   Graph *g = &h->g;
   PAC_CHECK_HIP(hipSetDevice(g->ctx->device));
+  const uint32_t *d_part = nullptr;
+  PAC_TRY(partition_filter(g, partition, &d_part));
   if (nq == 0) return PAC_OK;
   if (qoff[0] != 0) {
     set_error("query offsets must start at 0");
@@ -1160,7 +1270,8 @@ int pac_cnt_common_kway(pac_graph *h, const uint64_t *qoff, const int32_t *qv, u
   PAC_TRY(dr.reserve(8 * nq));
   PAC_CHECK_HIP(hipMemcpy(dq.p, qoff, 8 * (nq + 1), hipMemcpyHostToDevice));
   PAC_CHECK_HIP(hipMemcpy(dv.p, qv, 4 * nv, hipMemcpyHostToDevice));
-  PAC_TRY(kway_counts(g->ctx, g, dq.as<uint64_t>(), dv.as<int32_t>(), nq, dr.as<uint64_t>()));
+  PAC_TRY(kway_counts(g->ctx, g, dq.as<uint64_t>(), dv.as<int32_t>(), nq, dr.as<uint64_t>(),
+                      d_part, partition < 0 ? 0u : (uint32_t)partition));
   PAC_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
   PAC_CHECK_HIP(hipMemcpy(res, dr.p, 8 * nq, hipMemcpyDeviceToHost));
   return PAC_OK;

@@ -123,6 +123,75 @@ size_t Graph::cnt_common_neis_merge(VID u, VID v) const {
 
 size_t Graph::cnt_common_neis_galloping(VID u, VID v) const { return cnt_common_neis_merge(u, v); }
 
+bool Graph::has_directed_edge(VID u, VID v) const {
+  ensure_host();
+  const VID *b = elist_.data() + vlist_[u], *e = elist_.data() + vlist_[u + 1];
+  return std::binary_search(b, e, v);
+}
+
+VID Graph::get_nei(VID u, EID offset) const {
+  ensure_host();
+  return elist_[vlist_[u] + offset];
+}
+
+std::pair<VID, VID> Graph::get_edge(EID offset) const {
+  ensure_host();
+  const VID u = (VID)(std::upper_bound(vlist_.begin(), vlist_.end(), offset) - vlist_.begin()) - 1;
+  return {u, elist_[offset]};
+}
+
+unsigned long long Graph::size_in_bytes() const {
+  // Graph.cpp:409-411 counts the offsets at sizeof(int)
+  return (unsigned long long)(get_vertex_num() + 1) * sizeof(int) +
+         (unsigned long long)get_edge_num() * sizeof(VID);
+}
+
+void Graph::get_degs(std::vector<int> &degs) const {
+  ensure_host();
+  degs.resize(vlist_.size() - 1);
+  for (size_t u = 0; u + 1 < vlist_.size(); ++u) degs[u] = (int)(vlist_[u + 1] - vlist_[u]);
+}
+
+void Graph::get_neis(VID u, std::vector<VID> &neis) const {
+  ensure_host();
+  neis.assign(elist_.begin() + vlist_[u], elist_.begin() + vlist_[u + 1]);
+}
+
+namespace {
+// pac_common_neis into a vector (count first, then the list)
+std::vector<VID> common_list(pac_graph *g, VID u, VID v, const std::vector<VID> *nodes,
+                             int partition) {
+  const int32_t *nd = nodes ? nodes->data() : nullptr;
+  const uint64_t n = nodes ? nodes->size() : 0;
+  uint64_t c = 0;
+  check(pac_common_neis(g, u, v, nd, n, partition, nullptr, 0, &c));
+  std::vector<VID> out(c);
+  if (c) check(pac_common_neis(g, u, v, nd, n, partition, out.data(), c, &c));
+  return out;
+}
+}  // namespace
+
+size_t Graph::cnt_common_neis_merge(VID u, const std::vector<VID> &nodes) const {
+  uint64_t c = 0;
+  check(pac_common_neis(g_, u, -1, nodes.data(), nodes.size(), -1, nullptr, 0, &c));
+  return c;
+}
+
+size_t Graph::cnt_common_neis_less_than_merge(VID u, VID v, VID threshold) const {
+  uint64_t r = 0;
+  check(pac_cnt_common_pairs_filtered(g_, &u, &v, &threshold, 1, -1, &r));
+  return r;
+}
+
+void Graph::get_common_neis(VID u, VID v, std::vector<VID> &res) const {
+  const std::vector<VID> c = common_list(g_, u, v, nullptr, -1);
+  res.insert(res.end(), c.begin(), c.end());
+}
+
+void Graph::get_common_neis(VID u, const std::vector<VID> &nodes, std::vector<VID> &res) const {
+  res = common_list(g_, u, -1, &nodes, -1);
+}
+
 uint64_t Graph::triangle_count(int dispatch, int launch) const {
   ensure_prepared();
   uint64_t t = 0;
@@ -350,6 +419,119 @@ size_t Partitioned_Graph::solve_square_counting() const {
   uint64_t r = 0;
   check(pac_square_count(g_, &r));
   return r;
+}
+
+size_t Partitioned_Graph::cnt_common_neis_merge(const VID u, const VID v,
+                                                const int partition_id) const {
+  uint64_t r = 0;
+  check(pac_cnt_common_pairs_filtered(g_, &u, &v, nullptr, 1, partition_id, &r));
+  return r;
+}
+
+size_t Partitioned_Graph::cnt_common_neis_galloping(const VID u, const VID v,
+                                                    const int partition_id) const {
+  return cnt_common_neis_merge(u, v, partition_id);
+}
+
+size_t Partitioned_Graph::cnt_common_neis_less_than_merge(const VID u, const VID v,
+                                                          const VID threshold,
+                                                          const int partition_id) const {
+  uint64_t r = 0;
+  check(pac_cnt_common_pairs_filtered(g_, &u, &v, &threshold, 1, partition_id, &r));
+  return r;
+}
+
+size_t Partitioned_Graph::cnt_common_neis_merge(const std::vector<VID> &q,
+                                                const int partition_id) const {
+  const uint64_t qoff[2] = {0, q.size()};
+  uint64_t r = 0;
+  check(pac_cnt_common_kway_filtered(g_, qoff, q.data(), 1, partition_id, &r));
+  return r;
+}
+
+size_t Partitioned_Graph::cnt_common_neis_galloping(const std::vector<VID> &q,
+                                                    const int partition_id) const {
+  return cnt_common_neis_merge(q, partition_id);
+}
+
+size_t Partitioned_Graph::cnt_common_neis_merge(const VID u, const std::vector<VID> &nodes,
+                                                const int partition_id) const {
+  uint64_t c = 0;
+  check(pac_common_neis(g_, u, -1, nodes.data(), nodes.size(), partition_id, nullptr, 0, &c));
+  return c;
+}
+
+void Partitioned_Graph::get_common_neis(const VID u, const std::vector<VID> &nodes,
+                                        const int partition_id, std::vector<VID> &res) const {
+  res = common_list(g_, u, -1, &nodes, partition_id);
+}
+
+void Partitioned_Graph::get_neis(const VID u, const int partition_id,
+                                 std::vector<VID> &res) const {
+  res = common_list(g_, u, u, nullptr, partition_id);  // N(u) & N(u) = N(u)
+}
+
+namespace {
+void add_counts(std::vector<size_t> &res, const std::vector<uint64_t> &r) {
+  res.resize(r.size(), 0);
+  for (size_t i = 0; i < r.size(); ++i) res[i] += r[i];
+}
+}  // namespace
+
+void Partitioned_Graph::batch_cnt_common_neis_merge(
+    const std::vector<std::pair<VID, VID>> &queries, std::vector<size_t> &res) const {
+  std::vector<int32_t> u(queries.size()), v(queries.size());
+  for (size_t i = 0; i < queries.size(); ++i) {
+    u[i] = queries[i].first;
+    v[i] = queries[i].second;
+  }
+  std::vector<uint64_t> r(queries.size());
+  check(pac_cnt_common_pairs(g_, u.data(), v.data(), u.size(), r.data()));
+  add_counts(res, r);
+}
+
+void Partitioned_Graph::batch_cnt_common_neis_merge(const std::vector<std::vector<VID>> &queries,
+                                                    std::vector<size_t> &res) const {
+  std::vector<uint64_t> qoff(queries.size() + 1, 0);
+  std::vector<int32_t> qv;
+  for (size_t i = 0; i < queries.size(); ++i) {
+    qv.insert(qv.end(), queries[i].begin(), queries[i].end());
+    qoff[i + 1] = qv.size();
+  }
+  std::vector<uint64_t> r(queries.size());
+  check(pac_cnt_common_kway(g_, qoff.data(), qv.data(), queries.size(), r.data()));
+  add_counts(res, r);
+}
+
+void Partitioned_Graph::batch_cnt_common_neis_galloping(
+    const std::vector<std::pair<VID, VID>> &queries, std::vector<size_t> &res) const {
+  batch_cnt_common_neis_merge(queries, res);
+}
+
+void Partitioned_Graph::batch_cnt_common_neis_galloping(
+    const std::vector<std::vector<VID>> &queries, std::vector<size_t> &res) const {
+  batch_cnt_common_neis_merge(queries, res);
+}
+
+void Partitioned_Graph::batch_cnt_common_neis_merge_fine_grained(
+    const std::vector<std::pair<VID, VID>> &queries, std::vector<size_t> &res) const {
+  batch_cnt_common_neis_merge(queries, res);
+}
+
+void Partitioned_Graph::batch_cnt_common_neis_galloping_fine_grained(
+    const std::vector<std::pair<VID, VID>> &queries, std::vector<size_t> &res) const {
+  batch_cnt_common_neis_merge(queries, res);
+}
+
+void Partitioned_Graph::solve_jsim_merge(const std::vector<VID> &q,
+                                         std::vector<double> &res) const {
+  res.assign(q.size() * q.size(), 0.0);
+  check(pac_jaccard_queries(g_, q.data(), (uint32_t)q.size(), res.data()));
+}
+
+void Partitioned_Graph::solve_jsim_galloping(const std::vector<VID> &q,
+                                             std::vector<double> &res) const {
+  solve_jsim_merge(q, res);
 }
 
 }  // namespace pac_ref

@@ -49,6 +49,18 @@ class Graph {
   bool has_edge(VID u, VID v) const;
   size_t cnt_common_neis_merge(VID u, VID v) const;
   size_t cnt_common_neis_galloping(VID u, VID v) const;
+  // neighbour-list queries (Graph.cpp:163-184, 241-253, 272-285, 405-411, 468-508)
+  bool has_directed_edge(VID u, VID v) const;
+  bool is_weighted() const { return false; }
+  VID get_nei(VID u, EID offset) const;
+  std::pair<VID, VID> get_edge(EID offset) const;
+  unsigned long long size_in_bytes() const;
+  void get_degs(std::vector<int> &degs) const;
+  void get_neis(VID u, std::vector<VID> &neis) const;
+  size_t cnt_common_neis_merge(VID u, const std::vector<VID> &nodes) const;
+  size_t cnt_common_neis_less_than_merge(VID u, VID v, VID threshold) const;
+  void get_common_neis(VID u, VID v, std::vector<VID> &res) const;  // appends, as the reference
+  void get_common_neis(VID u, const std::vector<VID> &nodes, std::vector<VID> &res) const;
   size_t solve_triangle_counting_galloping(const int num_threads) const;
   size_t solve_triangle_counting_merge(const int num_threads) const;
   void solve_lcc_galloping(std::vector<double> &res, int num_threads) const;
@@ -116,6 +128,37 @@ class Partitioned_Graph {
   size_t solve_friend_triangle_merge(const VID root, const std::vector<VID> &nodes) const;
   size_t solve_friend_triangle_galloping(const VID root, const std::vector<VID> &nodes) const;
   size_t solve_square_counting() const;              // Graph.cpp:1291-1324
+  // Per-partition intersections: only common neighbours in partition_id count
+  // (Graph.cpp:1071-1195).
+  size_t cnt_common_neis_galloping(const VID u, const VID v, const int partition_id) const;
+  size_t cnt_common_neis_galloping(const std::vector<VID> &q, const int partition_id) const;
+  size_t cnt_common_neis_less_than_merge(const VID u, const VID v, const VID threshold,
+                                         const int partition_id) const;
+  size_t cnt_common_neis_merge(const VID u, const VID v, const int partition_id) const;
+  size_t cnt_common_neis_merge(const std::vector<VID> &q, const int partition_id) const;
+  size_t cnt_common_neis_merge(const VID u, const std::vector<VID> &nodes,
+                               const int partition_id) const;
+  void get_common_neis(const VID u, const std::vector<VID> &nodes, const int partition_id,
+                       std::vector<VID> &res) const;          // Graph.cpp:1475-1493
+  void get_neis(const VID u, const int partition_id, std::vector<VID> &res) const;
+  // Batched queries summed over all partitions (Graph.cpp:1382-1458).  As in the
+  // reference, res is resized to the query count and the counts are added to it.
+  void batch_cnt_common_neis_galloping(const std::vector<std::pair<VID, VID>> &queries,
+                                       std::vector<size_t> &res) const;
+  void batch_cnt_common_neis_galloping(const std::vector<std::vector<VID>> &queries,
+                                       std::vector<size_t> &res) const;
+  void batch_cnt_common_neis_galloping_fine_grained(
+      const std::vector<std::pair<VID, VID>> &queries, std::vector<size_t> &res) const;
+  void batch_cnt_common_neis_merge(const std::vector<std::pair<VID, VID>> &queries,
+                                   std::vector<size_t> &res) const;
+  void batch_cnt_common_neis_merge(const std::vector<std::vector<VID>> &queries,
+                                   std::vector<size_t> &res) const;
+  void batch_cnt_common_neis_merge_fine_grained(const std::vector<std::pair<VID, VID>> &queries,
+                                                std::vector<size_t> &res) const;
+  // Jaccard query matrix with the Graph:: formula (Graph.cpp:798-812); the partitioned
+  // reference's mis-indexed union (Graph.cpp:2157, 2192) is not reproduced.
+  void solve_jsim_galloping(const std::vector<VID> &queries, std::vector<double> &res) const;
+  void solve_jsim_merge(const std::vector<VID> &queries, std::vector<double> &res) const;
   uint64_t triangle_count(int dispatch, int launch) const;
   void per_edge_jaccard(std::vector<int32_t> &src, std::vector<int32_t> &dst,
                         std::vector<uint32_t> &inter, std::vector<uint32_t> &uni,

@@ -129,6 +129,12 @@ struct Graph {
   uint64_t learned_version = 0;
   std::vector<WorkItem> h_items;
   DevBuf rows, items;
+  // Partition map of the last prepare: the caller's vertex2partition, or (empty)
+  // contiguous ranges of part_width IDs.  Its device copy is built by partition_map().
+  std::vector<int32_t> h_v2p;
+  uint64_t part_width = 1;
+  DevBuf part;
+  bool part_ready = false;
   DevBuf rowwork;  // V: streamed length of row u = sum of d+(partner) over u's pairs
   // outputs
   DevBuf total, tvert, ehead, etail, sup, lcc32, lcc64, juni, jratio;
@@ -206,15 +212,26 @@ int vertex_counts(Ctx *ctx, Graph *g);
 int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64);
 int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *inter,
                      uint32_t *uni, float *ratio);
+// Common-neighbour counts of (u[i], v[i]); a common neighbour w counts only when
+// w < thr[i] (d_thr null: no bound) and, with d_part not null, d_part[w] == pid.
 int pair_counts(Ctx *ctx, Graph *g, const int32_t *d_u, const int32_t *d_v, uint64_t n,
-                uint64_t *d_res);
+                uint64_t *d_res, const int32_t *d_thr = nullptr, const uint32_t *d_part = nullptr,
+                uint32_t pid = 0);
 int query_matrix(Ctx *ctx, Graph *g, const int32_t *d_q, uint32_t nq, double *d_res);
 int fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t val);
 
 // ---- multi.hip -------------------------------------------------------------------
 // |N(q_0) & ... & N(q_k-1)| per query (query i = qv[qoff[i] .. qoff[i+1]), non-empty).
 int kway_counts(Ctx *ctx, Graph *g, const uint64_t *d_qoff, const int32_t *d_qv, uint64_t nq,
-                uint64_t *d_res);
+                uint64_t *d_res, const uint32_t *d_part = nullptr, uint32_t pid = 0);
+// The common neighbours of u and the sorted list d_list[0..n) (each value once, in
+// ascending order) whose partition is pid when d_part is not null: written to d_out,
+// their number to *d_count.
+int common_list(Ctx *ctx, Graph *g, uint32_t u, const int32_t *d_list, uint64_t n,
+                const uint32_t *d_part, uint32_t pid, int32_t *d_out,
+                unsigned long long *d_count);
+// The partition of every vertex under the map of the last prepare (built on first use).
+int partition_map(Graph *g, const uint32_t **d_part);
 // Friend triangles of root over nodes; d_c holds root_deg counters, d_res one u64.
 int friend_triangles(Ctx *ctx, Graph *g, uint32_t root, const int32_t *d_nodes, uint64_t n,
                      uint32_t root_deg, uint32_t *d_c, uint64_t *d_res);

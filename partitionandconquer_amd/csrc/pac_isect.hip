@@ -871,10 +871,13 @@ __global__ void k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__r
 // elements are binary-searched in the longer one (Graph::cnt_common_neis_galloping,
 // Graph.cpp:186-198).  Optional Jaccard ratio c / ((d_u + d_v) - c) in double
 // (Graph.cpp:808-809).
+// A common neighbour w counts when w < thr[p] (thr null: no bound) and, when part is
+// not null, part[w] == pid.
 __global__ void k_pairs(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                         const int32_t *__restrict__ pu, const int32_t *__restrict__ pv,
                         uint64_t n, unsigned long long *res, double *jac, const int32_t *q,
-                        uint32_t nq) {
+                        uint32_t nq, const int32_t *__restrict__ thr,
+                        const uint32_t *__restrict__ part, uint32_t pid) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wpb = blockDim.x >> 6;
   for (uint64_t p = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); p < n;
@@ -893,13 +896,16 @@ __global__ void k_pairs(const uint32_t *__restrict__ off, const uint32_t *__rest
       uint32_t t = a0; a0 = b0; b0 = t;
       t = na; na = nb; nb = t;
     }
+    const int32_t lim = thr ? thr[p] : 0x7FFFFFFF;
     uint32_t h = 0;
     if (nb) {
       const uint32_t *B = adj + b0;
       for (uint32_t i = lane; i < na; i += 64) {
         const uint32_t x = adj[a0 + i];
         const uint32_t j = lower_bound_u32(B, nb, x);
-        h += (j < nb && B[j] == x) ? 1u : 0u;
+        bool hit = j < nb && B[j] == x && (int32_t)x < lim;
+        if (part && hit) hit = part[x] == pid;
+        h += hit ? 1u : 0u;
       }
     }
     h = wave_sum(h);
@@ -1269,11 +1275,12 @@ int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *i
 }
 
 int pair_counts(Ctx *ctx, Graph *g, const int32_t *d_u, const int32_t *d_v, uint64_t n,
-                uint64_t *d_res) {
+                uint64_t *d_res, const int32_t *d_thr, const uint32_t *d_part, uint32_t pid) {
   if (n == 0) return PAC_OK;
   hipLaunchKernelGGL(k_pairs, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream,
                      g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(), d_u, d_v, n,
-                     reinterpret_cast<unsigned long long *>(d_res), nullptr, nullptr, 1u);
+                     reinterpret_cast<unsigned long long *>(d_res), nullptr, nullptr, 1u, d_thr,
+                     d_part, pid);
   PAC_CHECK_HIP(hipGetLastError());
   return PAC_OK;
 }
@@ -1283,7 +1290,7 @@ int query_matrix(Ctx *ctx, Graph *g, const int32_t *d_q, uint32_t nq, double *d_
   if (n == 0) return PAC_OK;
   hipLaunchKernelGGL(k_pairs, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream,
                      g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(), nullptr, nullptr,
-                     n, nullptr, d_res, d_q, nq);
+                     n, nullptr, d_res, d_q, nq, nullptr, nullptr, 0u);
   PAC_CHECK_HIP(hipGetLastError());
   return PAC_OK;
 }

@@ -10,6 +10,13 @@
 //   pac_main jaccard <graphdir> <is_partition> <method> <P> <threads> [outdir]
 //   pac_main preprocess <graphdir> <graphfile>             preprocess.cpp:39-45 (no weights)
 //   pac_main gen-rmat <scale> <ef> <a> <b> <c> <seed> <graphdir>
+//   pac_main generate_queries <query_num> <outputfile> [num_vertices]
+//       test.cpp:321-330: 2 * query_num values of the C library's rand() (unseeded) as
+//       int32, written raw; with num_vertices each value is taken modulo it, so that the
+//       pairs are vertex IDs of a graph of that size.
+//   pac_main common <graphdir> <queryfile> <merge|galloping> <is_partition> <method> <P> [out]
+//       batch_cnt_common_neis_* over the (u, v) pairs of a query file (Graph.cpp:413-447,
+//       1382-1458); prints the sum of the counts and writes them as uint64 to [out].
 // With PAC_JSON=1 every solve also prints one JSON line with the device timing.
 #include <chrono>
 #include <cstdio>
@@ -76,7 +83,10 @@ int usage() {
           "       pac_main lcc <graphdir> <merge|galloping> <is_partition> <method> <P> <threads> [out]\n"
           "       pac_main jaccard <graphdir> <is_partition> <method> <P> <threads> [outdir]\n"
           "       pac_main preprocess <graphdir> <graphfile>\n"
-          "       pac_main gen-rmat <scale> <ef> <a> <b> <c> <seed> <graphdir>\n");
+          "       pac_main gen-rmat <scale> <ef> <a> <b> <c> <seed> <graphdir>\n"
+          "       pac_main generate_queries <query_num> <outputfile> [num_vertices]\n"
+          "       pac_main common <graphdir> <queryfile> <merge|galloping> <is_partition> "
+          "<method> <P> [out]\n");
   return 2;
 }
 
@@ -223,6 +233,64 @@ int run(int argc, char **argv) {
     Graph g(dir + file);  // graphdir + graphfile, no separator (preprocess.cpp:43)
     mkdirs(dir + "/origin");
     g.save_as_csr_binfile(dir + "/origin/csr_vlist.bin", dir + "/origin/csr_elist.bin");
+    return 0;
+  }
+  if (task == "generate_queries") {
+    if (argc < 4) return usage();
+    const size_t n = std::stoul(argv[2]);
+    const long long nv = argc > 4 ? std::stoll(argv[4]) : 0;
+    std::vector<int> q;
+    q.reserve(2 * n);
+    for (size_t i = 0; i < 2 * n; ++i) q.push_back(nv > 0 ? (int)(rand() % nv) : rand());
+    std::ofstream out(argv[3], std::ios::binary);
+    out.write(reinterpret_cast<const char *>(q.data()), (std::streamsize)(q.size() * sizeof(int)));
+    return 0;
+  }
+  if (task == "common") {
+    if (argc < 8) return usage();
+    const std::string dir = argv[2], method = argv[4], pmethod = argv[6];
+    const bool is_partition = std::stod(argv[5]) != 0;
+    const int P = (int)std::stod(argv[7]);
+    const bool gallop = method == "galloping";
+    if (!gallop && method != "merge") {
+      fprintf(stderr, "unknown intersection method %s\n", method.c_str());
+      return 2;
+    }
+    const std::vector<int> raw = read_partition(argv[3]);  // raw int32 file
+    std::vector<std::pair<VID, VID>> queries(raw.size() / 2);
+    for (size_t i = 0; i < queries.size(); ++i) queries[i] = {raw[2 * i], raw[2 * i + 1]};
+    Graph g(dir + "/origin", false);
+    std::vector<size_t> cnt;
+    unsigned long long ns = 0;
+    if (is_partition) {
+      const std::vector<int> v2p =
+          read_partition(dir + "/partition/" + pmethod + "/P" + std::to_string(P) + ".bin");
+      Partitioned_Graph G(g, v2p, P);
+      ns = timed([&] {
+        if (gallop)
+          G.batch_cnt_common_neis_galloping(queries, cnt);
+        else
+          G.batch_cnt_common_neis_merge(queries, cnt);
+      });
+      print_time("partitioned " + method + " queries", ns);
+    } else {
+      cnt.assign(queries.size(), 0);
+      ns = timed([&] {
+        if (gallop)
+          g.batch_cnt_common_neis_galloping(queries, cnt, 0);
+        else
+          g.batch_cnt_common_neis_merge(queries, cnt, 0);
+      });
+      print_time("paralle " + method + " queries", ns);
+    }
+    printf("#Queries: %zu, #Common neighbours: %llu.\n", cnt.size(),
+           (unsigned long long)std::accumulate(cnt.begin(), cnt.end(), 0ull));
+    if (argc > 8) {
+      std::vector<uint64_t> c64(cnt.begin(), cnt.end());
+      std::ofstream out(argv[8], std::ios::binary);
+      out.write(reinterpret_cast<const char *>(c64.data()),
+                (std::streamsize)(c64.size() * sizeof(uint64_t)));
+    }
     return 0;
   }
   if (task == "gen-rmat") {

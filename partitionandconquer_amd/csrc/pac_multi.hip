@@ -7,6 +7,9 @@
 //   Graph::solve_friend_triangle_{galloping,merge}                Graph.cpp:309-338
 //   Partitioned_Graph::solve_friend_triangle_*                    Graph.cpp:1222-1264
 //   Partitioned_Graph::solve_square_counting                      Graph.cpp:1291-1324
+//   Graph::get_common_neis / Partitioned_Graph::get_common_neis, get_neis
+//                                                   Graph.cpp:468-493, 1475-1498
+//   per-partition k-way counts (Partitioned_Graph)                 Graph.cpp:1087-1130
 //
 // All three count exactly; their results do not depend on the partitioning (the
 // partitioned reference versions sum per-partition partials of the same quantity).
@@ -49,9 +52,12 @@ __device__ __forceinline__ void st_l2(uint32_t *p, uint32_t v) {
 // The minimum-degree list of the query drives (as in the galloping reference); each of
 // its elements is binary-searched in every other list, and the wave stops searching
 // once no lane's element is still present in all lists seen so far.
+// With part not null only common neighbours w with part[w] == pid count (the per-partition
+// form, Graph.cpp:1087-1130).
 __global__ void k_kway(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                        const uint64_t *__restrict__ qoff, const int32_t *__restrict__ qv,
-                       uint64_t nq, unsigned long long *__restrict__ res) {
+                       uint64_t nq, unsigned long long *__restrict__ res,
+                       const uint32_t *__restrict__ part, uint32_t pid) {
   const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
   for (uint64_t q = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); q < nq;
        q += (uint64_t)gridDim.x * wpb) {
@@ -78,6 +84,7 @@ __global__ void k_kway(const uint32_t *__restrict__ off, const uint32_t *__restr
       const uint32_t j = j0 + lane;
       bool alive = j < du;
       const uint32_t x = alive ? adj[ub + j] : 0u;
+      if (part && alive) alive = part[x] == pid;
       for (uint64_t i = k0; i < k1; ++i) {  // wave-uniform
         if (!__any(alive)) break;
         const uint32_t v = (uint32_t)qv[i];
@@ -90,6 +97,56 @@ __global__ void k_kway(const uint32_t *__restrict__ off, const uint32_t *__restr
     }
     if (lane == 0) res[q] = cnt;
   }
+}
+
+// ---- common-neighbour lists (one workgroup of 256) ---------------------------------
+// The values of the sorted list L that are neighbours of u (each value once; adjacent
+// duplicates of L are skipped) and, with part not null, lie in partition pid, written in
+// ascending order: a block-wide ballot compaction per 256 elements of L.  Matches the
+// merges of Graph.cpp:468-493 and 1475-1493 for a sorted L.
+__global__ void __launch_bounds__(256)
+k_common_list(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj, uint32_t u,
+              const int32_t *__restrict__ L, uint64_t n, const uint32_t *__restrict__ part,
+              uint32_t pid, int32_t *__restrict__ out, unsigned long long *__restrict__ count) {
+  __shared__ uint32_t wcnt[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t ub = off[u], du = off[u + 1] - ub;
+  uint64_t base = 0;
+  for (uint64_t i0 = 0; i0 < n; i0 += 256) {
+    const uint64_t i = i0 + tid;
+    bool keep = false;
+    uint32_t x = 0;
+    if (i < n) {
+      x = (uint32_t)L[i];
+      keep = i == 0 || (uint32_t)L[i - 1] != x;
+      if (keep) {
+        const uint32_t p = lbound(adj + ub, du, x);
+        keep = p < du && adj[ub + p] == x;  // x is then a vertex ID
+      }
+      if (keep && part) keep = part[x] == pid;
+    }
+    const uint64_t bal = __ballot(keep);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      const uint32_t c = wcnt[k];
+      pre += k < w ? c : 0u;
+      tot += c;
+    }
+    if (keep) out[base + pre + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)x;
+    base += tot;
+    __syncthreads();  // wcnt is rewritten by the next chunk
+  }
+  if (tid == 0) *count = base;
+}
+
+// Contiguous partitions: part[v] = v / width.
+__global__ void k_part_contig(uint32_t *__restrict__ part, uint32_t V, uint64_t width) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    part[v] = (uint32_t)(v / width);
 }
 
 // ---- friend triangles --------------------------------------------------------------
@@ -218,12 +275,40 @@ __global__ void k_zero_u32(uint32_t *p, uint64_t n) {
 }  // namespace
 
 int kway_counts(Ctx *ctx, Graph *g, const uint64_t *d_qoff, const int32_t *d_qv, uint64_t nq,
-                uint64_t *d_res) {
+                uint64_t *d_res, const uint32_t *d_part, uint32_t pid) {
   if (nq == 0) return PAC_OK;
   hipLaunchKernelGGL(k_kway, dim3(blocks_for(nq, 4)), dim3(256), 0, ctx->stream,
                      g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(), d_qoff, d_qv, nq,
-                     reinterpret_cast<unsigned long long *>(d_res));
+                     reinterpret_cast<unsigned long long *>(d_res), d_part, pid);
   PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
+}
+
+int common_list(Ctx *ctx, Graph *g, uint32_t u, const int32_t *d_list, uint64_t n,
+                const uint32_t *d_part, uint32_t pid, int32_t *d_out,
+                unsigned long long *d_count) {
+  hipLaunchKernelGGL(k_common_list, dim3(1), dim3(256), 0, ctx->stream,
+                     g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(), u, d_list, n,
+                     d_part, pid, d_out, d_count);
+  PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
+}
+
+int partition_map(Graph *g, const uint32_t **d_part) {
+  if (!g->part_ready) {
+    PAC_TRY(g->part.reserve(4ull * (g->V ? g->V : 1)));
+    if (!g->h_v2p.empty()) {
+      PAC_CHECK_HIP(hipMemcpyAsync(g->part.p, g->h_v2p.data(), 4ull * g->V,
+                                   hipMemcpyHostToDevice, g->ctx->stream));
+      PAC_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
+    } else if (g->V) {
+      hipLaunchKernelGGL(k_part_contig, dim3(blocks_for(g->V, 256 * 8)), dim3(256), 0,
+                         g->ctx->stream, g->part.as<uint32_t>(), g->V, g->part_width);
+      PAC_CHECK_HIP(hipGetLastError());
+    }
+    g->part_ready = true;
+  }
+  *d_part = g->part.as<const uint32_t>();
   return PAC_OK;
 }
 

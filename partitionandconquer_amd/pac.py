@@ -97,6 +97,10 @@ def lib():
         "pac_jaccard_queries": ([vp, vp, C.c_uint32, vp], C.c_int),
         "pac_cnt_common_pairs": ([vp, vp, vp, C.c_uint64, vp], C.c_int),
         "pac_cnt_common_kway": ([vp, vp, vp, C.c_uint64, vp], C.c_int),
+        "pac_cnt_common_pairs_filtered": ([vp, vp, vp, vp, C.c_uint64, C.c_int32, vp], C.c_int),
+        "pac_cnt_common_kway_filtered": ([vp, vp, vp, C.c_uint64, C.c_int32, vp], C.c_int),
+        "pac_common_neis": ([vp, C.c_int32, C.c_int32, vp, C.c_uint64, C.c_int32, vp,
+                             C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
         "pac_friend_triangle": ([vp, C.c_int32, vp, C.c_uint64, vp], C.c_int),
         "pac_square_count": ([vp, vp], C.c_int),
         "pac_tc_batch": ([vp, vp, C.c_uint32, C.c_int, C.c_int, vp], C.c_int),
@@ -418,22 +422,74 @@ class Graph:
         self._ensure_prepared()
         _check(lib().pac_jaccard_edges(self._h, dispatch, launch, None, None, None, None, None))
 
-    def cnt_common_pairs(self, u, v) -> np.ndarray:
+    def cnt_common_pairs(self, u, v, thr=None, partition=-1) -> np.ndarray:
+        """|N(u[i]) & N(v[i])|, optionally only common neighbours w < thr[i] and/or in
+        `partition` of the map of the last prepare (-1: all)."""
         u = np.ascontiguousarray(u, np.int32)
         v = np.ascontiguousarray(v, np.int32)
+        if len(v) != len(u):
+            raise ValueError("u and v differ in length")
         out = np.zeros(len(u), np.uint64)
-        _check(lib().pac_cnt_common_pairs(self._h, _ptr(u), _ptr(v), len(u), _ptr(out)))
+        if thr is None and partition == -1:
+            _check(lib().pac_cnt_common_pairs(self._h, _ptr(u), _ptr(v), len(u), _ptr(out)))
+            return out
+        t = None
+        if thr is not None:
+            t = np.ascontiguousarray(np.broadcast_to(np.asarray(thr, np.int32), u.shape))
+        _check(lib().pac_cnt_common_pairs_filtered(self._h, _ptr(u), _ptr(v), _ptr(t), len(u),
+                                                   int(partition), _ptr(out)))
         return out
 
-    def cnt_common_kway(self, queries) -> np.ndarray:
-        """|N(q_0) & ... & N(q_k-1)| for each query (a sequence of vertex IDs)."""
+    def cnt_common_kway(self, queries, partition=-1) -> np.ndarray:
+        """|N(q_0) & ... & N(q_k-1)| for each query (a sequence of vertex IDs), optionally
+        only the common neighbours in `partition`."""
         qoff = np.zeros(len(queries) + 1, np.uint64)
         qoff[1:] = np.cumsum([len(q) for q in queries])
         qv = np.ascontiguousarray(np.concatenate([np.asarray(q, np.int32) for q in queries])
                                   if queries else np.zeros(0, np.int32), np.int32)
         out = np.zeros(len(queries), np.uint64)
-        _check(lib().pac_cnt_common_kway(self._h, _ptr(qoff), _ptr(qv), len(queries), _ptr(out)))
+        _check(lib().pac_cnt_common_kway_filtered(self._h, _ptr(qoff), _ptr(qv), len(queries),
+                                                  int(partition), _ptr(out)))
         return out
+
+    def common_neis(self, u, v=None, nodes=None, partition=-1) -> np.ndarray:
+        """The common neighbours of u and v (or of u and the sorted list `nodes`), ascending,
+        optionally restricted to `partition`."""
+        if (v is None) == (nodes is None):
+            raise ValueError("give exactly one of v and nodes")
+        nd = None if nodes is None else np.ascontiguousarray(nodes, np.int32)
+        n = 0 if nd is None else len(nd)
+        cnt = C.c_uint64(0)
+        _check(lib().pac_common_neis(self._h, int(u), -1 if v is None else int(v), _ptr(nd), n,
+                                     int(partition), None, 0, C.byref(cnt)))
+        out = np.zeros(max(1, cnt.value), np.int32)
+        _check(lib().pac_common_neis(self._h, int(u), -1 if v is None else int(v), _ptr(nd), n,
+                                     int(partition), _ptr(out), len(out), C.byref(cnt)))
+        return out[:cnt.value].copy()
+
+    # reference method names (Graph.cpp:241-253, 272-285, 413-447, 468-493)
+    def cnt_common_neis_merge(self, u, v) -> int:
+        return int(self.cnt_common_pairs([u], [v])[0])
+
+    cnt_common_neis_galloping = cnt_common_neis_merge
+
+    def cnt_common_neis_less_than_merge(self, u, v, threshold) -> int:
+        return int(self.cnt_common_pairs([u], [v], thr=[threshold])[0])
+
+    def cnt_common_neis_merge_nodes(self, u, nodes) -> int:
+        return len(self.common_neis(u, nodes=nodes))
+
+    def get_common_neis(self, u, v=None, nodes=None) -> np.ndarray:
+        return self.common_neis(u, v=v, nodes=nodes)
+
+    def batch_cnt_common_neis_merge(self, queries, num_threads=0) -> np.ndarray:
+        """Pairs (a sequence of (u, v)) or k-way queries (sequences of any length)."""
+        if len(queries) and all(len(q) == 2 for q in queries):
+            q = np.asarray(queries, np.int32).reshape(-1, 2)
+            return self.cnt_common_pairs(q[:, 0], q[:, 1])
+        return self.cnt_common_kway(queries)
+
+    batch_cnt_common_neis_galloping = batch_cnt_common_neis_merge
 
     def friend_triangle(self, root, nodes) -> int:
         nodes = np.ascontiguousarray(nodes, np.int32)
@@ -514,6 +570,7 @@ class PartitionedGraph:
         self.g = g
         v2p = None if vertex2partition is None else np.ascontiguousarray(vertex2partition, np.int32)
         g.prepare(num_partitions, v2p)
+        self.v2p = v2p
 
     def get_partition_num(self) -> int:
         return self.g.info().num_partitions
@@ -535,6 +592,59 @@ class PartitionedGraph:
 
     def solve_lcc_galloping(self) -> np.ndarray:
         return self.g.lcc(dispatch=FORCE_GALLOP)[1]
+
+    # per-partition queries (Graph.cpp:1071-1195, 1475-1498): partition_id restricts the
+    # common neighbours to that partition's vertices
+    def cnt_common_neis_merge(self, u, v, partition_id) -> int:
+        return int(self.g.cnt_common_pairs([u], [v], partition=partition_id)[0])
+
+    cnt_common_neis_galloping = cnt_common_neis_merge
+
+    def cnt_common_neis_less_than_merge(self, u, v, threshold, partition_id) -> int:
+        return int(self.g.cnt_common_pairs([u], [v], thr=[threshold], partition=partition_id)[0])
+
+    def cnt_common_neis_merge_k(self, q, partition_id) -> int:
+        return int(self.g.cnt_common_kway([q], partition=partition_id)[0])
+
+    cnt_common_neis_galloping_k = cnt_common_neis_merge_k
+
+    def cnt_common_neis_merge_nodes(self, u, nodes, partition_id) -> int:
+        return len(self.g.common_neis(u, nodes=nodes, partition=partition_id))
+
+    def get_common_neis(self, u, nodes, partition_id) -> np.ndarray:
+        return self.g.common_neis(u, nodes=nodes, partition=partition_id)
+
+    def get_neis(self, u, partition_id) -> np.ndarray:
+        return self.g.common_neis(u, v=u, partition=partition_id)
+
+    def batch_cnt_common_neis_merge(self, queries) -> np.ndarray:
+        """Sum over partitions of the per-partition counts = the whole-graph counts."""
+        return self.g.batch_cnt_common_neis_merge(queries)
+
+    batch_cnt_common_neis_galloping = batch_cnt_common_neis_merge
+    batch_cnt_common_neis_merge_fine_grained = batch_cnt_common_neis_merge
+    batch_cnt_common_neis_galloping_fine_grained = batch_cnt_common_neis_merge
+
+    def solve_friend_triangle_merge(self, root, nodes) -> int:
+        return self.g.friend_triangle(root, nodes)
+
+    solve_friend_triangle_galloping = solve_friend_triangle_merge
+
+    def solve_square_counting(self) -> int:
+        return self.g.square_count()
+
+    def solve_jsim_merge(self, queries) -> np.ndarray:
+        """The Graph:: formula (Graph.cpp:798-812), not the partitioned variant's
+        mis-indexed union (Graph.cpp:2192)."""
+        return self.g.solve_jsim_merge(queries)
+
+    solve_jsim_galloping = solve_jsim_merge
+
+    def evaluate_partition(self) -> float:
+        """Balance ratio P * max partition degree / max degree (Graph.cpp:1460-1473)."""
+        P = self.get_partition_num()
+        v2p = self.v2p if self.v2p is not None else contiguous_partition(self.get_vertex_num(), P)
+        return self.g.evaluate_partition(v2p, P)
 
 
 def random_partition(V: int, P: int, seed=1) -> np.ndarray:

@@ -11,6 +11,7 @@ pytestmark = pytest.mark.gpu
 
 CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                    "partitionandconquer_amd", "pac_main")
+FACADE_CHECK = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_bin", "facade_check")
 
 
 def run(*args, env=None):
@@ -77,3 +78,87 @@ def test_missing_partition_file_fails_cleanly(tmp_path, orc):
     r = subprocess.run([CLI, "tc", str(tmp_path / "nope"), "merge", "0", "x", "1", "1"],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "error" in r.stderr
+
+
+def write_graph(g, d):
+    (d / "origin").mkdir(parents=True)
+    g.vlist.tofile(d / "origin" / "csr_vlist.bin")
+    g.elist.tofile(d / "origin" / "csr_elist.bin")
+
+
+def test_generate_queries_and_common(orc, tmp_path):
+    """generate_queries (test.cpp:321-330): 2n values of the C library's rand() as raw int32;
+    the common verb runs the batched pair queries of both classes."""
+    import ctypes
+    libc = ctypes.CDLL(None)
+    run("generate_queries", 50, tmp_path / "q.bin")
+    libc.srand(1)                                          # rand() unseeded = seed 1
+    assert np.fromfile(tmp_path / "q.bin", np.int32).tolist() == [libc.rand() for _ in range(100)]
+    g = orc.gen_rmat(10, 16, seed=9)
+    d = tmp_path / "g"
+    write_graph(g, d)
+    run("generate_queries", 400, tmp_path / "qv.bin", g.V)
+    q = np.fromfile(tmp_path / "qv.bin", np.int32).reshape(-1, 2)
+    assert q.min() >= 0 and q.max() < g.V
+    want = [orc.cnt_merge(g, a, b) for a, b in q]
+    out = run("common", d, tmp_path / "qv.bin", "merge", 0, "x", 1, tmp_path / "c.bin")
+    assert re.search(r"Time used for paralle merge queries: ", out)
+    assert f"#Queries: 400, #Common neighbours: {sum(want)}." in out
+    assert np.fromfile(tmp_path / "c.bin", np.uint64).tolist() == want
+    (d / "partition" / "rnd").mkdir(parents=True)
+    orc.random_partition(g.V, 6, 3).tofile(d / "partition" / "rnd" / "P6.bin")
+    out = run("common", d, tmp_path / "qv.bin", "galloping", 1, "rnd", 6, tmp_path / "c6.bin")
+    assert re.search(r"Time used for partitioned galloping queries: ", out)
+    assert np.fromfile(tmp_path / "c6.bin", np.uint64).tolist() == want
+
+
+def test_facade_queries(orc, tmp_path):
+    """The C++ facade's neighbour-list queries (pac_ref::Graph / Partitioned_Graph)
+    against the oracle."""
+    assert os.path.exists(FACADE_CHECK), "build first: python -m partitionandconquer_amd.build"
+    g = orc.gen_rmat(10, 16, seed=21)
+    d = tmp_path / "g"
+    write_graph(g, d)
+    P = 5
+    v2p = orc.random_partition(g.V, P, 4)
+    v2p.tofile(tmp_path / "v2p.bin")
+    rng = np.random.default_rng(21)
+    hubs = np.argsort(-g.deg())[:30]
+    nq = 40
+    recs = np.stack([rng.choice(hubs, nq), rng.choice(hubs, nq), rng.integers(0, g.V, nq),
+                     rng.integers(0, P, nq)], 1).astype(np.int32)
+    recs.tofile(tmp_path / "q.bin")
+    nodes = np.sort(rng.choice(g.V, 500)).astype(np.int32)
+    nodes.tofile(tmp_path / "nodes.bin")
+    r = subprocess.run([FACADE_CHECK, str(d), str(tmp_path / "v2p.bin"), str(tmp_path / "q.bin"),
+                        str(tmp_path / "nodes.bin")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = {}
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if f and f[0] in "SGLNPCEDB" and len(f[0]) == 1:
+            rows[(f[0], int(f[1]) if f[0] != "S" else 0)] = [int(x) for x in f[2:]] \
+                if f[0] != "S" else [int(x) for x in f[1:]]
+    assert rows[("S", 0)] == [(g.V + 1) * 4 + g.nnz * 4, g.nnz, 0]
+    pgo = orc.PGraph(g, v2p)
+    N = lambda x: g.elist[g.vlist[x]:g.vlist[x + 1]]
+    for i, (u, v, thr, k) in enumerate(recs.tolist()):
+        c = orc.cnt_merge(g, u, v)
+        last = int(N(u)[-1]) if len(N(u)) else -1
+        assert rows[("G", i)] == [c, c, orc.cnt_less_than_merge(g, u, v, thr),
+                                  orc.cnt_merge_nodes(g, u, nodes), int(v in set(N(u).tolist())),
+                                  last]
+        assert rows[("L", i)] == [-1] + orc.common_neis(g, u, N(v)).tolist()
+        assert rows[("N", i)] == orc.common_neis(g, u, nodes).tolist()
+        w = int(recs[(i + 1) % nq, 0])
+        kw = pgo.kway_merge([u, v, w], k)
+        assert rows[("P", i)] == [pgo.cnt_merge(u, v, k), pgo.cnt_galloping(u, v, k),
+                                  pgo.cnt_less_than_merge(u, v, thr, k),
+                                  pgo.cnt_merge_nodes(u, nodes, k), kw, kw]
+        assert rows[("C", i)] == pgo.common_neis(u, nodes, k).tolist()
+        assert rows[("E", i)] == pgo.get_neis(u, k).tolist()
+        off = g.nnz // 2 + i % 7                                # Graph::get_edge(offset)
+        src = int(np.searchsorted(g.vlist, off, side="right")) - 1
+        assert rows[("D", i)] == [src, int(g.elist[off])]
+        k3 = orc.kway_merge(g, [u, v, w])
+        assert rows[("B", i)] == [c, k3, c + 1, c, k3]          # partitioned batch adds to res

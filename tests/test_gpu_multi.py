@@ -75,6 +75,71 @@ def test_square_counting(ctx, orc):
     assert gpu_graph(ctx, er).square_count() == want
 
 
+@pytest.mark.parametrize("P,seed", [(1, 1), (5, 2), (16, 3)])
+def test_partition_filtered_queries(ctx, orc, P, seed):
+    """Per-partition pair / k-way counts, the threshold variant and the common-neighbour
+    lists (Graph.cpp:241-285, 468-493, 1071-1195, 1475-1498) against the oracle's
+    partition-major restatement."""
+    ref = orc.gen_rmat(11, 16, seed=seed)
+    g = gpu_graph(ctx, ref)
+    v2p = orc.random_partition(ref.V, P, seed) if P > 1 else np.zeros(ref.V, np.int32)
+    pg = pac.PartitionedGraph(g, v2p)
+    opg = orc.PGraph(ref, v2p)
+    rng = np.random.default_rng(seed)
+    deg = ref.deg()
+    hubs = np.argsort(-deg)[:40]
+    n = 300
+    u = rng.choice(hubs, n).astype(np.int32)
+    v = np.where(rng.random(n) < 0.5, rng.choice(hubs, n), rng.integers(0, ref.V, n)).astype(np.int32)
+    thr = rng.integers(-1, ref.V + 2, n).astype(np.int32)
+    for k in sorted({0, opg.P // 2, opg.P - 1}):
+        got = g.cnt_common_pairs(u, v, partition=k)
+        assert got.tolist() == [opg.cnt_merge(a, b, k) for a, b in zip(u, v)]
+        got = g.cnt_common_pairs(u, v, thr=thr, partition=k)
+        assert got.tolist() == [opg.cnt_less_than_merge(a, b, t, k) for a, b, t in zip(u, v, thr)]
+        qs = [rng.choice(hubs, int(rng.integers(1, 5))).astype(np.int32) for _ in range(100)]
+        assert g.cnt_common_kway(qs, partition=k).tolist() == [opg.kway_merge(q, k) for q in qs]
+        for i in range(12):
+            nodes = np.sort(rng.choice(ref.V, int(rng.integers(0, 400)))).astype(np.int32)
+            assert g.common_neis(u[i], nodes=nodes, partition=k).tolist() == \
+                opg.common_neis(u[i], nodes, k).tolist()
+            assert pg.get_neis(u[i], k).tolist() == opg.get_neis(u[i], k).tolist()
+    got = g.cnt_common_pairs(u, v, thr=thr)                      # Graph::less_than
+    assert got.tolist() == [orc.cnt_less_than_merge(ref, a, b, t) for a, b, t in zip(u, v, thr)]
+    for i in range(20):                                          # Graph::get_common_neis
+        vl = ref.elist[ref.vlist[v[i]]:ref.vlist[v[i] + 1]]
+        assert g.common_neis(u[i], v=v[i]).tolist() == orc.common_neis(ref, u[i], vl).tolist()
+    # the partitions split every count (the batch forms sum them)
+    tot = sum(g.cnt_common_pairs(u, v, partition=k).astype(np.int64) for k in range(opg.P))
+    assert np.array_equal(tot, g.cnt_common_pairs(u, v).astype(np.int64))
+
+
+def test_common_list_edges(ctx, orc):
+    """Long lists (several 256-element chunks), duplicates and out-of-range values in the
+    node list, and empty rows."""
+    n = 1100
+    ref = complete(n, orc)
+    g = gpu_graph(ctx, ref)
+    nodes = np.sort(np.concatenate([np.arange(-5, n + 5), np.arange(0, n, 3)])).astype(np.int32)
+    want = [x for x in range(n) if x != 7]
+    assert g.common_neis(7, nodes=nodes).tolist() == want
+    assert g.common_neis(7, v=8).tolist() == [x for x in range(n) if x not in (7, 8)]
+    g.prepare(4)
+    width = -(-n // 4)
+    assert g.common_neis(7, v=7, partition=3).tolist() == list(range(3 * width, n))
+    iso = orc.edges_to_csr(6, [0], [1])                          # vertices 2..5 isolated
+    gi = gpu_graph(ctx, iso)
+    assert gi.common_neis(3, v=0).tolist() == []
+    assert gi.common_neis(0, nodes=np.array([1, 1, 2], np.int32)).tolist() == [1]
+    with pytest.raises(pac.PacError):
+        gi.common_neis(0, nodes=np.array([2, 1], np.int32))     # unsorted
+    with pytest.raises(pac.PacError):
+        gi.cnt_common_pairs([0], [1], partition=0)               # not prepared
+    gi.prepare(2)
+    with pytest.raises(pac.PacError):
+        gi.cnt_common_pairs([0], [1], partition=2)               # P = 2
+
+
 def test_multiway_argument_errors(ctx, orc):
     g = gpu_graph(ctx, complete(5, orc))
     with pytest.raises(pac.PacError):

@@ -328,6 +328,46 @@ def test_kway_vs_sets(seed, orc):
             assert orc.kway_galloping(g, q) == want
 
 
+@pytest.mark.parametrize("P,seed", [(1, 1), (4, 2), (7, 3)])
+def test_partition_queries_vs_sets(P, seed, orc):
+    """Per-partition intersections (Graph.cpp:1071-1195, 1475-1498) against Python sets
+    restricted to the partition's columns; Graph-level node-list queries (241-253, 468-493)
+    against plain sets."""
+    g = orc.gen_rmat(9, 8, seed=seed)
+    N = neighbour_sets(g)
+    v2p = orc.random_partition(g.V, P, seed) if P > 1 else np.zeros(g.V, np.int32)
+    pg = orc.PGraph(g, v2p)
+    Pn = pg.P
+    part = lambda k: {w for w in range(g.V) if v2p[w] == k}
+    cols = [part(k) for k in range(Pn)]
+    rng = np.random.default_rng(seed)
+    hubs = np.argsort(-g.deg())[:30]
+    for _ in range(60):
+        u, v = (int(x) for x in rng.choice(hubs if rng.random() < 0.6 else np.arange(g.V), 2))
+        k = int(rng.integers(Pn))
+        common = N[u] & N[v] & cols[k]
+        assert pg.cnt_merge(u, v, k) == len(common)
+        assert pg.cnt_galloping(u, v, k) == len(common)
+        thr = int(rng.integers(0, g.V + 1))
+        assert pg.cnt_less_than_merge(u, v, thr, k) == len({w for w in common if w < thr})
+        nodes = np.sort(rng.choice(g.V, size=int(rng.integers(0, 60))))   # duplicates kept
+        want = sorted({int(x) for x in nodes} & N[u] & cols[k])
+        assert pg.cnt_merge_nodes(u, nodes, k) == len(want)
+        assert pg.common_neis(u, nodes, k).tolist() == want
+        assert pg.get_neis(u, k).tolist() == sorted(N[u] & cols[k])
+        assert orc.cnt_merge_nodes(g, u, nodes) == len({int(x) for x in nodes} & N[u])
+        assert orc.common_neis(g, u, nodes).tolist() == sorted({int(x) for x in nodes} & N[u])
+        vl = g.elist[g.vlist[v]:g.vlist[v + 1]]
+        assert orc.common_neis(g, u, vl).tolist() == sorted(N[u] & N[v])
+        q = rng.choice(hubs, size=int(rng.integers(1, 5)))
+        want = len(set.intersection(*[N[int(x)] for x in q]) & cols[k])
+        assert pg.kway_merge(q, k) == want
+        assert pg.kway_galloping(q, k) == want
+    # the partitions split every count
+    u, v = int(hubs[0]), int(hubs[1])
+    assert sum(pg.cnt_merge(u, v, k) for k in range(Pn)) == len(N[u] & N[v])
+
+
 def test_friend_triangle(orc):
     n = 9                                    # K_n: every pair sees n - 3 common neighbours
     g = complete(n, orc)
