@@ -78,6 +78,11 @@ struct GraphDev {
   uint32_t *ehead;            // LCC mode: per oriented edge, triangles credited to its head
   uint32_t *etail;            // LCC mode: per oriented edge, triangles credited to its tail
   uint32_t *sup;              // per-oriented-edge support (SUP mode)
+  // SUP mode: the hit words of the streamed lists.  The pair with rank r among the pairs
+  // streaming N+(y) owns the 64-bit words hitw[wbase[y] + r*nw ..+ nw), nw = ceil(d+(y)/64);
+  // bit l of word c is set when element 64c + l of N+(y) was found in the owner's list.
+  const unsigned long long *pword;  // per pair: its first hit word, wbase[y] + r*nw
+  unsigned long long *hitw;
   uint32_t V;
   uint32_t m;                 // oriented edges; oadj[m] holds the sentinel kListEnd
   uint32_t part_base;         // offset of this graph's partitions in the dispatch table
@@ -88,6 +93,14 @@ struct GraphDev {
 struct WorkItem {
   uint32_t g, part, lo, hi, c_lo, c_hi, cost_hi, cost_lo;
 };
+
+// A row (pairs [c_lo, c_hi)) whose out-list exceeds the LDS capacity of the launch's
+// tile, queued by the persistent kernel for a second pass with the widest tile.
+struct DeferItem {
+  uint32_t g, r, c_lo, c_hi, st, pad;
+};
+// The out-list length above which a row can exceed the smallest tile's capacity.
+constexpr uint32_t kMinTileCap = 1024;
 
 // Tile configurations: waves per row, LDS list capacity (entries).
 struct TileCfg {
@@ -118,6 +131,13 @@ struct Graph {
   std::vector<uint32_t> h_big;   // vertices whose degree exceeds kSmallDeg
   DevBuf hubs, big;
   bool pairs_built = false;
+  // Per-edge supports (SUP mode) from hit words instead of one atomic per triangle:
+  // pstreams[y] = number of pairs streaming N+(y); wbase[y] = first word of y's streams;
+  // pword, hitw as in GraphDev.
+  DevBuf pstreams, pword, wbase, hitw, supbig;
+  std::vector<uint32_t> h_supbig;  // (y, first stream) units of kSupRun streams of the
+                                   // vertices streamed by more than 64 pairs
+  bool hitw_built = false;
   // partitioning and work items
   bool prepared = false;
   uint32_t P = 0;
@@ -129,6 +149,7 @@ struct Graph {
   uint64_t learned_version = 0;
   std::vector<WorkItem> h_items;
   DevBuf rows, items;
+  uint64_t n_long_rows = 0;  // rows owning pairs whose out-list exceeds kMinTileCap
   // Partition map of the last prepare: the caller's vertex2partition, or (empty)
   // contiguous ranges of part_width IDs.  Its device copy is built by partition_map().
   std::vector<int32_t> h_v2p;
@@ -152,7 +173,8 @@ struct Ctx {
   int tile = 1;
   DevBuf scratch;     // generic temporary storage (library-owned)
   DevBuf scratch2;
-  DevBuf queue;       // persistent work-queue head
+  DevBuf queue;       // persistent work-queue heads: [0] items, [1] deferred rows, [2] their queue
+  DevBuf defer;       // DeferItem queue of the wide second pass
   DevBuf dispatch;    // per-partition strategy table
   DevBuf gdesc;       // GraphDev array
   DevBuf batch_items;
@@ -183,6 +205,13 @@ constexpr uint32_t kListEnd = 0xFFFFFFFEu;
 constexpr uint32_t kHubIn = 16384;
 // Degree up to which a vertex's credit sum is done by one thread (else by one wave).
 constexpr uint32_t kSmallDeg = 32;
+// Vertices streamed by at most 64 pairs have their hit words summed by one wave (one
+// load per word column); the others are split into units of kSupRun streams.
+constexpr uint32_t kSupRun = 256;
+// Build the pair ranks, word offsets and (zeroed) hit words of SUP mode (once).
+int build_hit_words(Graph *g);
+// sup[y->w] += number of pair streams of N+(y) that found w (after a SUP-mode launch).
+int sup_from_hit_words(Ctx *ctx, Graph *g);
 int graph_validate(const Graph *g, int level);
 
 // ---- gen.hip ---------------------------------------------------------------------

@@ -54,6 +54,8 @@ struct GDev {
   const PAC_GLOBAL uint32_t *ooff, *oadj, *poff, *ppar, *pedge, *rows;
   PAC_GLOBAL unsigned long long *total, *tvert;
   PAC_GLOBAL uint32_t *ehead, *etail, *sup;
+  const PAC_GLOBAL unsigned long long *pword;
+  PAC_GLOBAL unsigned long long *hitw;
   uint32_t V, m, part_base;
 };
 
@@ -70,6 +72,8 @@ __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
   g.ehead = (PAC_GLOBAL uint32_t *)d.ehead;
   g.etail = (PAC_GLOBAL uint32_t *)d.etail;
   g.sup = (PAC_GLOBAL uint32_t *)d.sup;
+  g.pword = (const PAC_GLOBAL unsigned long long *)d.pword;
+  g.hitw = (PAC_GLOBAL unsigned long long *)d.hitw;
   g.V = d.V;
   g.m = d.m;
   g.part_base = d.part_base;
@@ -133,10 +137,10 @@ struct Smem {
     struct {
       uint32_t tab[kSlots + kStash];
       uint32_t cnt16[MODE == kModeTC ? 1 : (kSlots + kStash) / 2];   // 2 x u16 per word
-      uint32_t pos16[MODE == kModeSUP ? (kSlots + kStash) / 2 : 1];  // 2 x u16 per word
     } h;  // hash
   } u;
   uint32_t item;
+  uint32_t defer_slot;
   uint32_t nstash;
   uint32_t rowtot;  // the row's triangle count (LCC mode)
 };
@@ -152,6 +156,11 @@ __device__ __forceinline__ void credit_edge(PAC_GLOBAL uint32_t *p, uint32_t v, 
 
 __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, uint32_t lane) {
+  return (unsigned long long)readlane((uint32_t)v, lane) |
+         ((unsigned long long)readlane((uint32_t)(v >> 32), lane) << 32);
 }
 
 // 16-bit counters packed two per word.  A hash pass covers at most kPassPairs pairs,
@@ -263,7 +272,8 @@ __device__ __forceinline__ uint32_t probe_merge(const PAC_GLOBAL uint32_t *__res
 // and edge index e of the wave's l-th pair and the bounds of N+(y), so the offsets of
 // 64 lists arrive with one round trip; the wave then walks the batch, broadcasting
 // each pair with readlane.  Wave w takes pair positions k_lo + w, k_lo + w + WAVES, ...
-// body(y, e, b0, nb) runs once per pair (e is only loaded when counts are kept).
+// body(y, e, b0, nb, p) runs once per pair, p being its index in the pair lists (e is only
+// loaded when counts are kept).
 template <int WAVES, int MODE, class Body>
 __device__ __forceinline__ void for_each_pair(const GDev &G, uint32_t p0, uint32_t k_lo,
                                               uint32_t k_hi, uint32_t wave, uint32_t lane,
@@ -279,9 +289,19 @@ __device__ __forceinline__ void for_each_pair(const GDev &G, uint32_t p0, uint32
     }
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < k_hi));
     for (uint32_t jj = 0; jj < nvalid; ++jj)
-      body(readlane(yl, jj), readlane(el, jj), readlane(bl, jj), readlane(nl, jj));
+      body(readlane(yl, jj), readlane(el, jj), readlane(bl, jj), readlane(nl, jj),
+           p0 + k_lo + wave + (uint32_t)WAVES * (t0 + jj));
     if (nvalid < 64) break;
   }
+}
+
+// SUP mode, paths that credit the streamed list's edges directly: the pair's hit words
+// are written as zero, so the column sums after the launch add nothing for it.
+__device__ __forceinline__ void clear_hit_words(const GDev &G, uint32_t nb, uint32_t p,
+                                                uint32_t lane) {
+  const uint32_t nw = (nb + 63u) >> 6;
+  const unsigned long long base = G.pword[p];
+  for (uint32_t c = lane; c < nw; c += 64) G.hitw[base + c] = 0ull;
 }
 
 // ---- one row, sorted-list strategies (merge, gallop): A staged in LDS ---------------
@@ -305,8 +325,9 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
   const bool whole = k_lo == 0 && k_hi == np;
   uint32_t row_hits = 0;
   for_each_pair<WAVES, MODE>(G, p0, k_lo, k_hi, wave, lane,
-                             [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb) {
+                             [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb, uint32_t p) {
     uint32_t h = 0;
+    if (MODE == kModeSUP) clear_hit_words(G, nb, p, lane);
     if (nb != 0) {
       if (STRAT == kGallop)
         h = probe_gallop<MODE>(G.oadj, A, nA, cnt, b0, nb, G.sup, lane);
@@ -318,10 +339,8 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
       if (lane == 0 && ch) {
         if (e - a0 < nA)
           atomicAdd(&cnt[e - a0], ch);  // r->y: y = A[e - a0]
-        else if (MODE == kModeLCC)
-          G.etail[e] = ch;  // y->r: credited to y (each pair is visited once)
         else
-          gadd(&G.sup[e], ch);
+          G.etail[e] = ch;  // y->r: credited to y (each pair is visited once)
       }
     }
     row_hits += h;
@@ -335,10 +354,7 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
   if (MODE != kModeTC) {
     for (uint32_t i = tid; i < nA; i += BS) {
       const uint32_t v = cnt[i];
-      if (v) {
-        if (MODE == kModeLCC) credit_edge(&G.ehead[a0 + i], v, whole);
-        if (MODE == kModeSUP) gadd(&G.sup[a0 + i], v);
-      }
+      if (v) credit_edge(&G.ehead[a0 + i], v, whole);
     }
     if (MODE == kModeLCC && tid == 0 && sm.rowtot) {
       if (whole)
@@ -392,9 +408,11 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // The wave's first batch of pairs: partners and edges are loaded before the table is
   // built, the bounds of their lists after the insertions, so both latencies overlap it.
   uint32_t kl = c_lo + wave + (uint32_t)WAVES * lane, yl = 0, el = 0, bl = 0, nl = 0;
+  unsigned long long wl = 0;  // SUP mode: the pair's first hit word
   if (kl < c_hi) {
     yl = G.ppar[p0 + kl];
     if (MODE != kModeTC) el = G.pedge[p0 + kl];
+    if (MODE == kModeSUP) wl = G.pword[p0 + kl];
   }
   uint32_t lg = 4;
   while ((1u << lg) < nA) ++lg;  // 2^lg >= nA: load factor <= 1/2
@@ -402,8 +420,6 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   for (uint32_t i = tid; i < nslot; i += BS) H.tab[i] = kEmpty;
   if (MODE != kModeTC)
     for (uint32_t i = tid; i < (nslot + kStash) / 2; i += BS) H.cnt16[i] = 0;
-  if (MODE == kModeSUP)
-    for (uint32_t i = tid; i < (nslot + kStash) / 2; i += BS) H.pos16[i] = 0;
   if (tid == 0) {
     sm.nstash = 0;
     sm.rowtot = 0;
@@ -477,14 +493,6 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     }
     return inc;
   };
-  if (MODE == kModeSUP) {
-    for (uint32_t i = tid; i < nA; i += BS) {
-      uint32_t word;
-      const uint32_t inc = find(G.oadj[a0 + i], word);
-      atomicAdd(&H.pos16[word], inc * i);  // i < CAP < 2^16
-    }
-    __syncthreads();
-  }
   uint32_t row_hits = 0;  // wave-uniform
   // Blocks of up to 4 x 64 elements of one pair's list N+(y), in pair order; the loads
   // of the next block are issued before the current one is probed.  The loads are
@@ -502,41 +510,50 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     if (t0 != 0) {
       kl = c_lo + wave + (uint32_t)WAVES * (t0 + lane);
       yl = el = bl = nl = 0;
+      wl = 0;
       if (kl < c_hi) {
         yl = G.ppar[p0 + kl];
         if (MODE != kModeTC) el = G.pedge[p0 + kl];
         bl = G.ooff[yl];
         nl = G.ooff[yl + 1] - bl;
+        if (MODE == kModeSUP) wl = G.pword[p0 + kl];
       }
     }
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
     if (nvalid == 0) break;
     uint32_t jj = 0, j0 = 0, b0 = readlane(bl, 0), nb = readlane(nl, 0), ph = 0;
+    unsigned long long hb = MODE == kModeSUP ? readlane64(wl, 0) : 0ull;
     uint32_t wa[4], wb[4];
     load_blk(b0, nb, 0, wa);
     auto step = [&](const uint32_t *cur, uint32_t *nxt) {
       uint32_t jn = jj, j0n = j0 + 256u, b0n = b0, nbn = nb;
+      unsigned long long hbn = hb;
       if (j0n >= nb) {
         jn = jj + 1;
         j0n = 0;
         if (jn < nvalid) {
           b0n = readlane(bl, jn);
           nbn = readlane(nl, jn);
+          if (MODE == kModeSUP) hbn = readlane64(wl, jn);
         }
       }
       // past the batch's last block this reloads the current one (in bounds, unused)
       load_blk(jn < nvalid ? b0n : b0, jn < nvalid ? nbn : nb, jn < nvalid ? j0n : j0, nxt);
+      unsigned long long hw = 0;  // SUP mode: this lane's hit word (lanes 0..3)
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
         if (j0 + 64u * u >= nb) break;  // wave-uniform
-        const uint32_t j = j0 + 64u * u + lane;
         uint32_t word;
         const uint32_t inc = find(cur[u], word);  // 0 for the sentinel past the list's end
-        ph += (uint32_t)__popcll(__ballot(inc != 0u));
-        if (inc != 0u) {
-          if (MODE != kModeTC) atomicAdd(&H.cnt16[word], inc);  // t(w) / edge r->w
-          if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);      // edge y->w
-        }
+        const unsigned long long bal = __ballot(inc != 0u);
+        ph += (uint32_t)__popcll(bal);
+        if (MODE == kModeSUP && lane == u) hw = bal;
+        if (MODE != kModeTC && inc != 0u) atomicAdd(&H.cnt16[word], inc);  // t(w) / edge r->w
+      }
+      if (MODE == kModeSUP) {
+        // edges y->w: one hit word per 64 streamed elements, summed after the launch
+        const uint32_t nsub = min(4u, (nb - j0 + 63u) >> 6);
+        if (lane < nsub) G.hitw[hb + (j0 >> 6) + lane] = hw;
       }
       if (jn != jj) {
         // pair jj is done.  t(y) / the pair's edge: r->y adds to y's slot in the
@@ -548,10 +565,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
             uint32_t word;
             const uint32_t inc = find(y, word);
             atomicAdd(&H.cnt16[word], inc * ph);  // ph <= nb <= CAP < 2^16
-          } else if (MODE == kModeLCC)
+          } else {
             G.etail[e] = ph;
-          else
-            gadd(&G.sup[e], ph);
+          }
         }
         ph = 0;
       }
@@ -559,6 +575,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       j0 = j0n;
       b0 = b0n;
       nb = nbn;
+      hb = hbn;
     };
     while (true) {  // wave-uniform
       step(wa, wb);
@@ -573,7 +590,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   if (lane == 0) acc += row_hits;
   if (MODE == kModeLCC && lane == 0 && row_hits) atomicAdd(&sm.rowtot, row_hits);
   __syncthreads();
-  if (MODE == kModeLCC) {
+  if (MODE != kModeTC) {
     // per edge r->A[i], in A order (coalesced): the credits of A[i]'s slot
     for (uint32_t i = tid; i < nA; i += BS) {
       uint32_t word;
@@ -581,20 +598,12 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       const uint32_t v = (H.cnt16[word] >> (inc == 1u ? 0u : 16u)) & 0xFFFFu;
       if (v) credit_edge(&G.ehead[a0 + i], v, whole);
     }
-    if (tid == 0 && sm.rowtot) {
-      if (whole)
-        G.tvert[r] = sm.rowtot;
-      else
-        gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
-    }
   }
-  if (MODE == kModeSUP) {
-    for (uint32_t i = tid; i < nslot + nst; i += BS) {
-      const uint32_t key = H.tab[i];
-      if (key == kEmpty) continue;
-      const uint32_t v = get16(H.cnt16, i);
-      if (v) gadd(&G.sup[a0 + get16(H.pos16, i)], v);
-    }
+  if (MODE == kModeLCC && tid == 0 && sm.rowtot) {
+    if (whole)
+      G.tvert[r] = sm.rowtot;
+    else
+      gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
   }
   __syncthreads();
   return true;
@@ -611,30 +620,25 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
   if (c_hi > np) c_hi = np;
   uint32_t row_hits = 0;
   for_each_pair<WAVES, MODE>(G, p0, c_lo, c_hi, wave, lane,
-                             [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb) {
+                             [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb, uint32_t p) {
     uint32_t h = 0;
+    if (MODE == kModeSUP) clear_hit_words(G, nb, p, lane);
     for (uint32_t j = lane; j < nb; j += 64) {
       const uint32_t w = G.oadj[b0 + j];
       const uint32_t i = lower_bound_u32(A, nA, w);
       if (i < nA && A[i] == w) {
         ++h;
-        if (MODE == kModeLCC) gadd(&G.ehead[a0 + i], 1u);
-        if (MODE == kModeSUP) {
-          gadd(&G.sup[a0 + i], 1u);
-          gadd(&G.sup[b0 + j], 1u);
-        }
+        if (MODE != kModeTC) gadd(&G.ehead[a0 + i], 1u);
+        if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);  // edge y->w
       }
     }
     if (MODE != kModeTC) {
       const uint32_t ch = wave_sum(h);
       if (lane == 0 && ch) {
-        if (MODE == kModeLCC) {
-          if (e - a0 < nA)
-            gadd(&G.ehead[e], ch);  // r->y
-          else
-            G.etail[e] = ch;        // y->r, visited once
-        }
-        if (MODE == kModeSUP) gadd(&G.sup[e], ch);
+        if (e - a0 < nA)
+          gadd(&G.ehead[e], ch);  // r->y
+        else
+          G.etail[e] = ch;        // y->r, visited once
       }
     }
     row_hits += h;
@@ -690,12 +694,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev Gd, uint32_t row_
 }
 
 // ---- persistent kernel: every workgroup pulls work items from one atomic queue --------
+// With defer_cap != 0, a row whose out-list exceeds this tile's capacity is not looked up
+// in HBM but queued (queue[1] counts them) for k_deferred, which runs with the widest
+// tile; should the queue be full the row is done here after all.
 template <int MODE, int WAVES, int CAP>
 __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__restrict__ Gs,
                                                            const WorkItem *__restrict__ items,
                                                            uint32_t nitems,
                                                            const uint8_t *__restrict__ strat,
-                                                           uint32_t *queue) {
+                                                           uint32_t *queue,
+                                                           DeferItem *__restrict__ defer,
+                                                           uint32_t defer_cap) {
   __shared__ Smem<CAP, MODE, WAVES> s;
   unsigned long long acc = 0;
   PAC_GLOBAL unsigned long long *cur_total = nullptr;
@@ -715,13 +724,55 @@ __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__res
     const uint8_t st = strat[G.part_base + w.part];
     const bool single = (w.hi == w.lo + 1);
     const uint32_t c_lo = single ? w.c_lo : 0u, c_hi = single ? w.c_hi : kAllC;
-    for (uint32_t q = w.lo; q < w.hi; ++q)
-      do_row<MODE, WAVES, CAP>(G, G.rows[q], c_lo, c_hi, st, s, acc);
+    for (uint32_t q = w.lo; q < w.hi; ++q) {
+      const uint32_t r = G.rows[q];
+      if (defer_cap != 0 && G.ooff[r + 1] - G.ooff[r] > (uint32_t)CAP) {  // uniform
+        __syncthreads();  // every thread has read the previous defer_slot
+        if (threadIdx.x == 0) {
+          const uint32_t k = atomicAdd(&queue[1], 1u);
+          if (k < defer_cap) defer[k] = DeferItem{w.g, r, c_lo, c_hi, (uint32_t)st, 0u};
+          s.defer_slot = k;
+        }
+        __syncthreads();
+        if (s.defer_slot < defer_cap) continue;
+      }
+      do_row<MODE, WAVES, CAP>(G, r, c_lo, c_hi, st, s, acc);
+    }
   }
   if (cur_total) flush_total<WAVES>(cur_total, acc);
 }
 
-__global__ void k_reset_queue(uint32_t *q) { *q = 0; }
+// The rows queued by k_persistent (queue[1] of them, at most defer_cap), pulled through
+// queue[2] by workgroups of the widest tile.
+template <int MODE, int WAVES, int CAP>
+__global__ __launch_bounds__(64 * WAVES) void k_deferred(const GraphDev *__restrict__ Gs,
+                                                         const DeferItem *__restrict__ defer,
+                                                         uint32_t defer_cap, uint32_t *queue) {
+  __shared__ Smem<CAP, MODE, WAVES> s;
+  const uint32_t n = min(queue[1], defer_cap);  // written by the previous launch
+  unsigned long long acc = 0;
+  PAC_GLOBAL unsigned long long *cur_total = nullptr;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) s.item = atomicAdd(&queue[2], 1u);
+    __syncthreads();
+    const uint32_t it = s.item;
+    if (it >= n) break;
+    const DeferItem d = defer[it];
+    const GDev G = to_dev(Gs[d.g]);
+    if (G.total != cur_total) {
+      if (cur_total) flush_total<WAVES>(cur_total, acc);
+      acc = 0;
+      cur_total = G.total;
+    }
+    do_row<MODE, WAVES, CAP>(G, d.r, d.c_lo, d.c_hi, (int)d.st, s, acc);
+  }
+  if (cur_total) flush_total<WAVES>(cur_total, acc);
+}
+
+__global__ void k_reset_queue(uint32_t *q) {
+  if (threadIdx.x < 4) q[threadIdx.x] = 0;
+}
 
 // Plain streaming copy (16-byte vectors, grid-stride): the bandwidth reference.
 __global__ void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
@@ -844,25 +895,45 @@ __global__ void k_lcc(const unsigned long long *__restrict__ tvert,
 }
 
 // Per-edge Jaccard (pair formula of Graph.cpp:808-809 on every oriented edge).
-__global__ void k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
-                          const uint32_t *__restrict__ sup, const uint32_t *__restrict__ off,
-                          uint32_t V, int32_t *src, int32_t *dst, uint32_t *inter,
-                          uint32_t *uni, float *ratio) {
+// The support of edge e = its credits as an edge of the staged list (ehead), as a pair's
+// edge (etail) and as an edge of the streamed lists (sup); the total is kept in sup.
+// Each wave takes 64 consecutive vertices and walks their out-edges (contiguous in the
+// oriented CSR) 64 at a time; an edge's tail is found by a binary search over the
+// vertices' first edges held one per lane.
+__global__ void __launch_bounds__(256)
+k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
+          const uint32_t *__restrict__ ehead, const uint32_t *__restrict__ etail,
+          uint32_t *__restrict__ sup, const uint32_t *__restrict__ deg, uint32_t V, int32_t *src,
+          int32_t *dst, uint32_t *inter, uint32_t *uni, float *ratio) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wpb = blockDim.x >> 6;
-  for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
-       u += (uint64_t)gridDim.x * wpb) {
-    const uint32_t a0 = ooff[u], a1 = ooff[u + 1];
-    const uint32_t du = off[u + 1] - off[u];
-    for (uint32_t e = a0 + lane; e < a1; e += 64) {
-      const uint32_t v = oadj[e];
-      const uint32_t c = sup[e];
-      const uint32_t un = du + (off[v + 1] - off[v]) - c;
-      if (src) src[e] = (int32_t)u;
-      if (dst) dst[e] = (int32_t)v;
-      if (inter) inter[e] = c;
-      if (uni) uni[e] = un;
-      if (ratio) ratio[e] = (float)((double)c / (double)un);
+  for (uint64_t v0 = ((uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6)) * 64u; v0 < V;
+       v0 += (uint64_t)gridDim.x * wpb * 64u) {
+    const uint64_t vi = v0 + lane;
+    const uint32_t a = ooff[vi < V ? vi : V];  // first out-edge (the end past the last vertex)
+    const uint32_t du = vi < V ? deg[vi] : 0u;
+    const uint32_t E0 = readlane(a, 0);
+    const uint32_t E1 = ooff[v0 + 64u < V ? v0 + 64u : V];
+    for (uint32_t eb = E0; eb < E1; eb += 64) {  // wave-uniform
+      const uint32_t e = eb + lane;
+      uint32_t t = 0;  // last lane whose vertex starts at or before e
+#pragma unroll
+      for (uint32_t step = 32; step; step >>= 1) {
+        const uint32_t at = (uint32_t)__shfl((int)a, (int)(t + step), 64);
+        t = at <= e ? t + step : t;
+      }
+      const uint32_t dt = (uint32_t)__shfl((int)du, (int)t, 64);
+      if (e < E1) {
+        const uint32_t v = oadj[e];
+        const uint32_t c = ehead[e] + etail[e] + sup[e];
+        sup[e] = c;
+        const uint32_t un = dt + deg[v] - c;
+        if (src) src[e] = (int32_t)(v0 + t);
+        if (dst) dst[e] = (int32_t)v;
+        if (inter) inter[e] = c;
+        if (uni) uni[e] = un;
+        if (ratio) ratio[e] = (float)((double)c / (double)un);
+      }
     }
   }
 }
@@ -919,41 +990,66 @@ __global__ void k_pairs(const uint32_t *__restrict__ off, const uint32_t *__rest
   }
 }
 
-template <int MODE, int WAVES, int CAP>
-int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint64_t *launches) {
+template <class K>
+uint32_t persistent_grid(Ctx *ctx, K kernel, int threads, uint32_t cap) {
   int per_cu = 0;
-  PAC_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void *>(k_persistent<MODE, WAVES, CAP>),
-      64 * WAVES, 0));
-  if (per_cu < 1) per_cu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kernel),
+                                                   threads, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
   uint32_t grid = (uint32_t)(per_cu * ctx->num_cus);
-  if (grid > nitems) grid = nitems ? nitems : 1;
-  hipLaunchKernelGGL((k_reset_queue), dim3(1), dim3(1), 0, ctx->stream, ctx->queue.as<uint32_t>());
+  if (grid > cap) grid = cap ? cap : 1;
+  return grid;
+}
+
+// The widest tile: the second pass over rows that exceed the launch tile's capacity.
+constexpr int kWideWaves = 8, kWideCap = 4096;
+
+// long_rows: the capacity of the deferred-row queue, 0 when no row exceeds CAP.
+template <int MODE, int WAVES, int CAP>
+int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint32_t long_rows,
+                       uint64_t *launches) {
+  const uint32_t defer_cap = CAP < kWideCap ? long_rows : 0u;
+  if (defer_cap) PAC_TRY(ctx->defer.reserve(sizeof(DeferItem) * defer_cap));
+  const uint32_t grid = persistent_grid(ctx, k_persistent<MODE, WAVES, CAP>, 64 * WAVES, nitems);
+  hipLaunchKernelGGL((k_reset_queue), dim3(1), dim3(64), 0, ctx->stream, ctx->queue.as<uint32_t>());
   if (ctx->time_kernel) PAC_CHECK_HIP(hipEventRecord(ctx->kev[0], ctx->stream));
   hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP>), dim3(grid), dim3(64 * WAVES), 0,
                      ctx->stream, ctx->gdesc.as<const GraphDev>(), items, nitems,
-                     ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>());
+                     ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>(),
+                     ctx->defer.as<DeferItem>(), defer_cap);
   PAC_CHECK_HIP(hipGetLastError());
+  if (defer_cap) {
+    const uint32_t g2 = persistent_grid(ctx, k_deferred<MODE, kWideWaves, kWideCap>,
+                                        64 * kWideWaves, defer_cap);
+    hipLaunchKernelGGL((k_deferred<MODE, kWideWaves, kWideCap>), dim3(g2), dim3(64 * kWideWaves),
+                       0, ctx->stream, ctx->gdesc.as<const GraphDev>(),
+                       ctx->defer.as<const DeferItem>(), defer_cap, ctx->queue.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+    if (launches) *launches += 1;
+  }
   if (ctx->time_kernel) PAC_CHECK_HIP(hipEventRecord(ctx->kev[1], ctx->stream));
   if (launches) *launches += 2;
   return PAC_OK;
 }
 
 template <int MODE>
-int run_persistent_mode(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint64_t *launches) {
+int run_persistent_mode(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint32_t long_rows,
+                        uint64_t *launches) {
   switch (ctx->tile) {
-    case 0: return run_persistent_cfg<MODE, 2, 1024>(ctx, items, nitems, launches);
-    case 1: return run_persistent_cfg<MODE, 4, 2048>(ctx, items, nitems, launches);
-    case 2: return run_persistent_cfg<MODE, 4, 4096>(ctx, items, nitems, launches);
-    default: return run_persistent_cfg<MODE, 8, 4096>(ctx, items, nitems, launches);
+    case 0: return run_persistent_cfg<MODE, 2, 1024>(ctx, items, nitems, long_rows, launches);
+    case 1: return run_persistent_cfg<MODE, 4, 2048>(ctx, items, nitems, long_rows, launches);
+    case 2: return run_persistent_cfg<MODE, 4, 4096>(ctx, items, nitems, long_rows, launches);
+    default: return run_persistent_cfg<MODE, 8, 4096>(ctx, items, nitems, long_rows, launches);
   }
 }
 
 int run_persistent_any(Ctx *ctx, int mode, const WorkItem *items, uint32_t nitems,
-                       uint64_t *launches) {
-  if (mode == kModeLCC) return run_persistent_mode<kModeLCC>(ctx, items, nitems, launches);
-  if (mode == kModeSUP) return run_persistent_mode<kModeSUP>(ctx, items, nitems, launches);
-  return run_persistent_mode<kModeTC>(ctx, items, nitems, launches);
+                       uint32_t long_rows, uint64_t *launches) {
+  if (mode == kModeLCC)
+    return run_persistent_mode<kModeLCC>(ctx, items, nitems, long_rows, launches);
+  if (mode == kModeSUP)
+    return run_persistent_mode<kModeSUP>(ctx, items, nitems, long_rows, launches);
+  return run_persistent_mode<kModeTC>(ctx, items, nitems, long_rows, launches);
 }
 
 template <int STRAT, int MODE, int WAVES, int CAP>
@@ -989,6 +1085,153 @@ int run_plain_any(Ctx *ctx, const GraphDev &G, int mode, int strat, uint32_t row
   return run_plain<kModeTC>(ctx, G, strat, row_begin, nrows);
 }
 
+// ---- SUP mode: per-edge credits of the streamed lists from the hit words ----------
+// 64 x 64 bit transpose across the wave: lane L holds row L as (lo, hi) = bits 0..31,
+// 32..63; afterwards lane j holds column j (its bit L = bit j of row L).  At each scale s
+// (32, 16, ..., 1) the two off-diagonal s x s blocks of every 2s x 2s block are swapped:
+// a lane in the upper half of its 2s-group (lane & s == 0) replaces its right bits
+// (j & s != 0) with its partner's left bits moved up by s, and the partner the reverse --
+// each lane sends the bits it replaces (x & mask), rotates what it receives by s into
+// place, and merges it under the same mask.
+__device__ __forceinline__ void transpose64(uint32_t &lo, uint32_t &hi, uint32_t lane) {
+  {
+    const bool top = (lane & 32u) == 0;
+    const uint32_t recv = (uint32_t)__shfl_xor((int)(top ? hi : lo), 32, 64);
+    if (top)
+      hi = recv;
+    else
+      lo = recv;
+  }
+  constexpr uint32_t kLeft[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u,
+                                 0x55555555u};
+#pragma unroll
+  for (uint32_t k = 0; k < 5; ++k) {
+    const uint32_t s = 16u >> k;
+    const bool top = (lane & s) == 0;
+    const uint32_t mask = top ? ~kLeft[k] : kLeft[k];  // the bits this lane replaces
+    const uint32_t rot = top ? 32u - s : s;            // right-rotate amount into place
+    const uint32_t rl = (uint32_t)__shfl_xor((int)(lo & mask), (int)s, 64);
+    const uint32_t rh = (uint32_t)__shfl_xor((int)(hi & mask), (int)s, 64);
+    lo = (lo & ~mask) | (__builtin_amdgcn_alignbit(rl, rl, rot) & mask);
+    hi = (hi & ~mask) | (__builtin_amdgcn_alignbit(rh, rh, rot) & mask);
+  }
+}
+
+// Column c of vertex y's hit words summed over its streams r0..r1: lane l gets the number
+// of streams with bit l of word c set (element 64c + l of N+(y)).  Lane i loads stream
+// r + i's word, 64 streams at a time (the next 64 loaded before the current ones are
+// counted), and the transpose turns the 64 rows into columns.
+__device__ __forceinline__ uint32_t hit_column(const PAC_GLOBAL unsigned long long *__restrict__ W,
+                                               unsigned long long base, uint32_t nw, uint32_t r0,
+                                               uint32_t r1, uint32_t c, uint32_t lane) {
+  uint32_t acc = 0;
+  unsigned long long w =
+      r0 + lane < r1 ? W[base + (unsigned long long)(r0 + lane) * nw + c] : 0ull;
+  for (uint32_t r = r0; r < r1; r += 64) {  // wave-uniform
+    const uint32_t rn = r + 64u;
+    const unsigned long long wn =
+        rn + lane < r1 ? W[base + (unsigned long long)(rn + lane) * nw + c] : 0ull;
+    uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+    transpose64(lo, hi, lane);
+    acc += (uint32_t)__popc(lo) + (uint32_t)__popc(hi);
+    w = wn;
+  }
+  return acc;
+}
+
+// Vertices streamed by at most 64 pairs (one load per word column): each wave loads the
+// metadata of 64 consecutive vertices (lane i: vertex v0 + i) and walks their (vertex,
+// column) units in turn, the next unit's loads issued before the current transpose.  It
+// is the only writer of those vertices' out-edge supports here, so plain read-modify-writes.
+__global__ void __launch_bounds__(256)
+k_sup_small(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ pstreams,
+            const unsigned long long *__restrict__ wbase,
+            const unsigned long long *__restrict__ hitw, uint32_t V, uint32_t *__restrict__ sup) {
+  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
+  const PAC_GLOBAL unsigned long long *W = (const PAC_GLOBAL unsigned long long *)hitw;
+  for (uint64_t v0 = ((uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6)) * 64u; v0 < V;
+       v0 += (uint64_t)gridDim.x * wpb * 64u) {
+    const uint64_t vi = v0 + lane;
+    uint32_t Rl = 0, a0l = 0, ndl = 0;
+    unsigned long long wbl = 0;
+    if (vi < V) {
+      Rl = pstreams[vi];
+      if (Rl > 64u) Rl = 0;  // summed by k_sup_big
+      if (Rl) {
+        a0l = ooff[vi];
+        ndl = ooff[vi + 1] - a0l;
+        wbl = wbase[vi];
+      }
+    }
+    unsigned long long todo = __ballot(Rl != 0);
+    if (todo) {
+      uint32_t i = (uint32_t)__builtin_ctzll(todo), c = 0;
+      todo &= todo - 1ull;
+      uint32_t R = readlane(Rl, i), a0 = readlane(a0l, i), nd = readlane(ndl, i);
+      uint32_t nw = (nd + 63u) >> 6;
+      unsigned long long base = readlane64(wbl, i);
+      unsigned long long w = lane < R ? W[base + (unsigned long long)lane * nw] : 0ull;
+      uint32_t old = lane < nd ? sup[a0 + lane] : 0u;  // the support being added to
+      while (true) {  // wave-uniform
+        uint32_t R2 = R, a02 = a0, nd2 = nd, nw2 = nw, c2 = c + 1;
+        unsigned long long base2 = base;
+        bool more = true;
+        if (c2 >= nw) {
+          more = todo != 0ull;
+          if (more) {
+            const uint32_t i2 = (uint32_t)__builtin_ctzll(todo);
+            todo &= todo - 1ull;
+            R2 = readlane(Rl, i2);
+            a02 = readlane(a0l, i2);
+            nd2 = readlane(ndl, i2);
+            nw2 = (nd2 + 63u) >> 6;
+            base2 = readlane64(wbl, i2);
+            c2 = 0;
+          }
+        }
+        const unsigned long long w2 =
+            more && lane < R2 ? W[base2 + (unsigned long long)lane * nw2 + c2] : 0ull;
+        const uint32_t e2 = 64u * c2 + lane;
+        const uint32_t old2 = more && e2 < nd2 ? sup[a02 + e2] : 0u;
+        uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+        transpose64(lo, hi, lane);
+        const uint32_t acc = (uint32_t)__popc(lo) + (uint32_t)__popc(hi);
+        const uint32_t e = 64u * c + lane;
+        if (e < nd && acc) sup[a0 + e] = old + acc;
+        if (!more) break;
+        old = old2;
+        R = R2;
+        a0 = a02;
+        nd = nd2;
+        nw = nw2;
+        c = c2;
+        base = base2;
+        w = w2;
+      }
+    }
+  }
+}
+
+// Vertices streamed by more than 64 pairs, one workgroup per unit (y, first stream r0) of
+// kSupRun streams; each wave takes a set of word columns; units add atomically.
+__global__ void __launch_bounds__(256)
+k_sup_big(const uint32_t *__restrict__ units, const uint32_t *__restrict__ ooff,
+          const uint32_t *__restrict__ pstreams, const unsigned long long *__restrict__ wbase,
+          const unsigned long long *__restrict__ hitw, uint32_t *__restrict__ sup) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  const PAC_GLOBAL unsigned long long *W = (const PAC_GLOBAL unsigned long long *)hitw;
+  const uint32_t y = units[2u * blockIdx.x], r0 = units[2u * blockIdx.x + 1u];
+  const uint32_t R = pstreams[y];
+  const uint32_t a0 = ooff[y], nd = ooff[y + 1] - a0, nw = (nd + 63u) >> 6;
+  const unsigned long long base = wbase[y];
+  const uint32_t r1 = min(R, r0 + kSupRun);
+  for (uint32_t c = wave; c < nw; c += wpb) {
+    const uint32_t acc = hit_column(W, base, nw, r0, r1, c, lane);
+    const uint32_t e = 64u * c + lane;
+    if (e < nd && acc) gadd((PAC_GLOBAL uint32_t *)(sup + a0 + e), acc);
+  }
+}
+
 // Device descriptor of a graph; its output pointers must already be allocated.
 GraphDev make_desc(Graph *g, uint32_t part_base) {
   GraphDev d;
@@ -1003,6 +1246,8 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   d.ehead = g->ehead.as<uint32_t>();
   d.etail = g->etail.as<uint32_t>();
   d.sup = g->sup.as<uint32_t>();
+  d.pword = g->pword.as<const unsigned long long>();
+  d.hitw = g->hitw.as<unsigned long long>();
   d.V = g->V;
   d.m = (uint32_t)g->m;
   d.part_base = part_base;
@@ -1057,8 +1302,9 @@ int alloc_outputs(Graph *const *gs, uint32_t G, int mode) {
   for (uint32_t i = 0; i < G; ++i) {
     Graph *g = gs[i];
     PAC_TRY(g->total.reserve(sizeof(unsigned long long)));
-    if (mode == kModeLCC) {
+    if (mode == kModeLCC)
       PAC_TRY(g->tvert.reserve(sizeof(unsigned long long) * (size_t)(g->V ? g->V : 1)));
+    if (mode != kModeTC) {
       PAC_TRY(g->ehead.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
       PAC_TRY(g->etail.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
     }
@@ -1077,7 +1323,7 @@ int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
     if (mode == kModeLCC && g->V)
       hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
                          g->tvert.as<unsigned long long>(), (uint64_t)g->V);
-    if (mode == kModeLCC && g->m) {
+    if (mode != kModeTC && g->m) {
       PAC_TRY(fill_u32(ctx->stream, g->ehead.as<uint32_t>(), g->m, 0u));
       PAC_TRY(fill_u32(ctx->stream, g->etail.as<uint32_t>(), g->m, 0u));
     }
@@ -1134,12 +1380,19 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   // (on the hipGraph path nothing has allocated them yet).
   PAC_TRY(alloc_outputs(gs, G, mode));
   std::vector<GraphDev> descs(G);
-  uint64_t nparts = 0, nitems = 0;
+  uint64_t nparts = 0, nitems = 0, long_rows = 0;
+  bool any_long = false;
+  const uint32_t cap = (uint32_t)kTiles[ctx->tile].cap;
   for (uint32_t i = 0; i < G; ++i) {
     descs[i] = make_desc(gs[i], (uint32_t)nparts);
     nparts += gs[i]->P;
     nitems += gs[i]->h_items.size();
+    // rows over the tile's capacity, each queued at most once per item that holds it
+    long_rows += gs[i]->h_items.size() + gs[i]->n_long_rows;
+    any_long = any_long || gs[i]->max_outdeg > cap;
   }
+  if (!any_long) long_rows = 0;
+  if (long_rows > 0xFFFFFFF0ull) long_rows = 0xFFFFFFF0ull;
   if (nitems > 0xFFFFFFF0ull) {
     set_error("too many work items (%llu)", (unsigned long long)nitems);
     return PAC_ERR_UNSUPPORTED;
@@ -1178,7 +1431,7 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   if (!use_graph) {
     // the caller has zeroed the outputs
     if (nitems == 0) return PAC_OK;
-    return run_persistent_any(ctx, mode, items, (uint32_t)nitems, launches);
+    return run_persistent_any(ctx, mode, items, (uint32_t)nitems, (uint32_t)long_rows, launches);
   }
   // hipGraph: zero-fill kernels + queue reset + persistent kernel, captured once per
   // (graph set, mode, tile, buffer addresses) and replayed.
@@ -1186,6 +1439,8 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   key.push_back(ctx->gdesc.p);
   key.push_back(ctx->dispatch.p);
   key.push_back(ctx->queue.p);
+  if (long_rows) PAC_TRY(ctx->defer.reserve(sizeof(DeferItem) * long_rows));
+  key.push_back(ctx->defer.p);
   for (uint32_t i = 0; i < G; ++i) {
     key.push_back(gs[i]->tvert.p);
     key.push_back(gs[i]->sup.p);
@@ -1201,7 +1456,8 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
     const bool timing = ctx->time_kernel;
     ctx->time_kernel = false;  // no event records inside the captured graph
     int rc = zero_outputs(ctx, gs, G, mode);
-    if (rc == PAC_OK && nitems) rc = run_persistent_any(ctx, mode, items, (uint32_t)nitems, nullptr);
+    if (rc == PAC_OK && nitems)
+      rc = run_persistent_any(ctx, mode, items, (uint32_t)nitems, (uint32_t)long_rows, nullptr);
     ctx->time_kernel = timing;
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
@@ -1251,6 +1507,24 @@ int vertex_counts(Ctx *ctx, Graph *g) {
   return PAC_OK;
 }
 
+int sup_from_hit_words(Ctx *ctx, Graph *g) {
+  if (g->V == 0 || !g->hitw_built) return PAC_OK;
+  hipLaunchKernelGGL(k_sup_small, dim3(grid_for(g->V, 4 * 64)), dim3(256), 0, ctx->stream,
+                     g->ooff.as<const uint32_t>(), g->pstreams.as<const uint32_t>(),
+                     g->wbase.as<const unsigned long long>(),
+                     g->hitw.as<const unsigned long long>(), g->V, g->sup.as<uint32_t>());
+  PAC_CHECK_HIP(hipGetLastError());
+  if (!g->h_supbig.empty()) {
+    hipLaunchKernelGGL(k_sup_big, dim3((uint32_t)(g->h_supbig.size() / 2)), dim3(256), 0,
+                       ctx->stream, g->supbig.as<const uint32_t>(),
+                       g->ooff.as<const uint32_t>(), g->pstreams.as<const uint32_t>(),
+                       g->wbase.as<const unsigned long long>(),
+                       g->hitw.as<const unsigned long long>(), g->sup.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  return PAC_OK;
+}
+
 int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64) {
   if (want32) PAC_TRY(g->lcc32.reserve(sizeof(float) * (size_t)(g->V ? g->V : 1)));
   if (want64) PAC_TRY(g->lcc64.reserve(sizeof(double) * (size_t)(g->V ? g->V : 1)));
@@ -1266,10 +1540,11 @@ int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64) {
 int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *inter,
                      uint32_t *uni, float *ratio) {
   if (g->V == 0) return PAC_OK;
-  hipLaunchKernelGGL(k_jaccard, dim3(grid_for(g->V, 4)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(k_jaccard, dim3(grid_for(g->V, 4 * 64)), dim3(256), 0, ctx->stream,
                      g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
-                     g->sup.as<const uint32_t>(), g->off.as<const uint32_t>(), g->V, src, dst,
-                     inter, uni, ratio);
+                     g->ehead.as<const uint32_t>(), g->etail.as<const uint32_t>(),
+                     g->sup.as<uint32_t>(), g->deg.as<const uint32_t>(), g->V, src, dst, inter,
+                     uni, ratio);
   PAC_CHECK_HIP(hipGetLastError());
   return PAC_OK;
 }

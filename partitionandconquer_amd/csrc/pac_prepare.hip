@@ -415,4 +415,115 @@ int build_pairs(Graph *g) {
   return PAC_OK;
 }
 
+namespace {
+
+// Number of pairs streaming each vertex's out-list.
+__global__ void k_partner_count(const uint32_t *__restrict__ ppar, uint64_t np,
+                                uint32_t *__restrict__ R) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
+       p += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&R[ppar[p]], 1u);
+}
+
+// Hit words per vertex: pairs streaming it x words per stream; entry V is 0, so the
+// exclusive scan ends with the total.
+__global__ void k_word_counts(const uint32_t *__restrict__ R, const uint32_t *__restrict__ odeg,
+                              uint32_t V, unsigned long long *__restrict__ w) {
+  for (uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; y <= V;
+       y += (uint64_t)gridDim.x * blockDim.x)
+    w[y] = y < V ? (unsigned long long)R[y] * ((odeg[y] + 63u) >> 6) : 0ull;
+}
+
+// First hit word of each pair: its partner's word base plus its rank among the pairs
+// streaming that partner times the words per stream.  The ranks come in any order (the
+// words of all of a vertex's streams are summed, so only their distinctness matters).
+__global__ void k_pair_words(const uint32_t *__restrict__ ppar, uint64_t np,
+                             const uint32_t *__restrict__ odeg,
+                             const unsigned long long *__restrict__ wbase,
+                             uint32_t *__restrict__ cursor, unsigned long long *__restrict__ pword) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t y = ppar[p];
+    const uint32_t r = atomicAdd(&cursor[y], 1u);
+    pword[p] = wbase[y] + (unsigned long long)r * ((odeg[y] + 63u) >> 6);
+  }
+}
+
+uint32_t grid_elems(uint64_t n) {
+  uint64_t b = (n + 255) / 256;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 16));
+}
+
+}  // namespace
+
+int build_hit_words(Graph *g) {
+  if (g->hitw_built) return PAC_OK;
+  PAC_TRY(build_pairs(g));
+  Ctx *ctx = g->ctx;
+  const uint32_t V = g->V;
+  uint32_t np = 0;
+  PAC_CHECK_HIP(hipMemcpy(&np, g->poff.as<uint32_t>() + V, 4, hipMemcpyDeviceToHost));
+  PAC_TRY(g->pstreams.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  PAC_TRY(g->pword.reserve(sizeof(unsigned long long) * (np ? np : 1)));
+  PAC_TRY(g->wbase.reserve(sizeof(unsigned long long) * ((uint64_t)V + 1)));
+  DevBuf words, cursor;
+  PAC_TRY(words.reserve(sizeof(unsigned long long) * ((uint64_t)V + 1)));
+  PAC_TRY(cursor.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  PAC_TRY(fill_u32(ctx->stream, g->pstreams.as<uint32_t>(), (uint64_t)V + 1, 0u));
+  PAC_TRY(fill_u32(ctx->stream, cursor.as<uint32_t>(), (uint64_t)V + 1, 0u));
+  if (np) {
+    hipLaunchKernelGGL(k_partner_count, dim3(grid_elems(np)), dim3(256), 0, ctx->stream,
+                       g->ppar.as<const uint32_t>(), (uint64_t)np, g->pstreams.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_word_counts, dim3(grid_elems((uint64_t)V + 1)), dim3(256), 0, ctx->stream,
+                     g->pstreams.as<const uint32_t>(), g->odeg.as<const uint32_t>(), V,
+                     words.as<unsigned long long>());
+  PAC_CHECK_HIP(hipGetLastError());
+  size_t tmp = 0;
+  PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp, words.as<unsigned long long>(),
+                                        g->wbase.as<unsigned long long>(), 0ull, (size_t)V + 1,
+                                        rocprim::plus<unsigned long long>(), ctx->stream));
+  PAC_TRY(ctx->scratch.reserve(tmp));
+  PAC_CHECK_HIP(rocprim::exclusive_scan(ctx->scratch.p, tmp, words.as<unsigned long long>(),
+                                        g->wbase.as<unsigned long long>(), 0ull, (size_t)V + 1,
+                                        rocprim::plus<unsigned long long>(), ctx->stream));
+  if (np) {
+    hipLaunchKernelGGL(k_pair_words, dim3(grid_elems(np)), dim3(256), 0, ctx->stream,
+                       g->ppar.as<const uint32_t>(), (uint64_t)np, g->odeg.as<const uint32_t>(),
+                       g->wbase.as<const unsigned long long>(), cursor.as<uint32_t>(),
+                       g->pword.as<unsigned long long>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  unsigned long long nwords = 0;
+  PAC_CHECK_HIP(hipMemcpyAsync(&nwords, g->wbase.as<unsigned long long>() + V, 8,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<uint32_t> R(V);
+  if (V)
+    PAC_CHECK_HIP(hipMemcpyAsync(R.data(), g->pstreams.p, 4ull * V, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // words and cursor are freed on return
+  // Every pair writes all of its words in each SUP launch; the zero fill only makes the
+  // words of a graph that has never been solved well defined.
+  // 8 words of padding: the column sums load 8 words per stream from any word position
+  PAC_TRY(g->hitw.reserve(sizeof(unsigned long long) * (nwords + 8)));
+  PAC_TRY(fill_u32(ctx->stream, g->hitw.as<uint32_t>(), 2ull * (nwords + 8), 0u));
+  // units (y, first stream) of kSupRun streams of the vertices streamed more than 64 times
+  g->h_supbig.clear();
+  for (uint32_t y = 0; y < V; ++y)
+    if (R[y] > 64u)
+      for (uint32_t r0 = 0; r0 < R[y]; r0 += kSupRun) {
+        g->h_supbig.push_back(y);
+        g->h_supbig.push_back(r0);
+      }
+  if (!g->h_supbig.empty()) {
+    PAC_TRY(g->supbig.reserve(sizeof(uint32_t) * g->h_supbig.size()));
+    PAC_CHECK_HIP(hipMemcpy(g->supbig.p, g->h_supbig.data(),
+                            sizeof(uint32_t) * g->h_supbig.size(), hipMemcpyHostToDevice));
+  }
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  g->hitw_built = true;
+  return PAC_OK;
+}
+
 }  // namespace pac

@@ -131,7 +131,7 @@ def test_rows_above_lds_capacity(ctx, orc, tile):
     try:
         g.prepare(4)
         for d in STRATS:
-            for l in (pac.LAUNCH_PLAIN, pac.LAUNCH_PERSISTENT):
+            for l in LAUNCHES:
                 t, pv = g.tc(d, l, per_vertex=True)
                 assert t == math.comb(n, 3)
                 assert np.all(pv == math.comb(n - 1, 2))
@@ -139,6 +139,37 @@ def test_rows_above_lds_capacity(ctx, orc, tile):
         assert np.all(l64 == 1.0)
         src, dst, inter, uni, ratio = g.jaccard_edges(pac.FORCE_BITMAP)
         assert np.all(inter == n - 2) and np.all(uni == n)
+    finally:
+        ctx.set_tile(1)
+
+
+@pytest.mark.parametrize("launch", [pac.LAUNCH_PERSISTENT, pac.LAUNCH_GRAPH])
+def test_long_rows_deferred_to_wide_pass(ctx, orc, launch):
+    """With the 1024-entry tile, the out-lists of a 1100-clique inside a sparse random
+    graph exceed the LDS table: the persistent kernel queues those rows (whole or split
+    by pair range) for the second pass with the 4096-entry tile.  TC, per-vertex counts,
+    LCC and per-edge supports equal the oracle's."""
+    rng = np.random.default_rng(5)
+    n, V = 1100, 4000
+    cs, cd = zip(*itertools.combinations(range(n), 2))
+    rs = rng.integers(0, V, 60000)
+    rd = rng.integers(0, V, 60000)
+    ref = orc.edges_to_csr(V, np.concatenate([cs, rs]).astype(np.uint32),
+                           np.concatenate([cd, rd]).astype(np.uint32))
+    T = orc.tc_merge(ref, NT)
+    lref = orc.lcc_merge(ref, NT)
+    g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    ctx.set_tile(0)
+    try:
+        g.prepare(8)
+        for d in (pac.FORCE_BITMAP, pac.FORCE_GALLOP, pac.DISPATCH_FIXED):
+            t, pv = g.tc(d, launch, per_vertex=True)
+            assert t == T and int(pv.sum()) == 3 * T
+            l32, l64, t2 = g.lcc(d, launch)
+            assert t2 == T and bits_equal(l64, lref)
+            src, dst, inter, uni, ratio = g.jaccard_edges(d, launch)
+            ri, ru, rr = orc.jaccard_edges(ref, src, dst, NT)
+            assert np.array_equal(inter, ri) and bits_equal(ratio, rr)
     finally:
         ctx.set_tile(1)
 
@@ -175,6 +206,24 @@ def test_jaccard_edges_scale13(ctx, orc):
     # every undirected edge appears once
     key = np.minimum(src, dst).astype(np.int64) * ref.V + np.maximum(src, dst)
     assert len(np.unique(key)) == ref.nnz // 2
+
+
+def test_jaccard_strategy_switches(ctx, orc):
+    """Per-edge supports stay exact when consecutive solves of one graph switch between the
+    hash path (hit words of the streamed lists, summed after the launch) and the sorted
+    paths (direct credits, hit words cleared), including vertices streamed by more than
+    512 pairs (the sliced column sums)."""
+    ref = orc.gen_rmat(14, 16, 0.57, 0.19, 0.19, seed=141)
+    g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    g.prepare(16)
+    want = None
+    for d in (pac.FORCE_BITMAP, pac.FORCE_MERGE, pac.FORCE_BITMAP, pac.FORCE_GALLOP,
+              pac.DISPATCH_FIXED, pac.FORCE_BITMAP):
+        src, dst, inter, uni, ratio = g.jaccard_edges(d, pac.LAUNCH_PERSISTENT)
+        if want is None:
+            want = orc.jaccard_edges(ref, src, dst, NT)
+        assert np.array_equal(inter, want[0]) and np.array_equal(uni, want[1])
+        assert bits_equal(ratio, want[2])
 
 
 def test_query_matrix_and_pairs(ctx, orc):
