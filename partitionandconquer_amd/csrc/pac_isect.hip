@@ -459,12 +459,13 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // a key is placed in its second bucket only when its first is full.  So a key whose
   // first bucket still has a free slot cannot be in the second bucket or the stash: in
   // TC mode the second read is skipped for those lookups, which saves LDS bandwidth (in
-  // LCC mode, at lower occupancy, the dependent read costs more than it saves).
+  // LCC mode, at lower occupancy, the dependent read costs more than it saves, and so it
+  // does with the smallest tile, whose tables are short enough to stay cheap to read).
   auto find = [&](uint32_t w, uint32_t &word) -> uint32_t {
     uint32_t b1, b2;
     buckets(w, sh, b1, b2);
     const uint32_t k10 = H.tab[2u * b1], k11 = H.tab[2u * b1 + 1u];
-    if (MODE == kModeTC) {
+    if (MODE == kModeTC && CAP >= 2048) {
       uint32_t inc = (k10 == w) ? 1u : ((k11 == w) ? 0x10000u : 0u);
       word = b1;
       if (inc == 0u && k11 != kEmpty) {
@@ -694,10 +695,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev Gd, uint32_t row_
 }
 
 // ---- persistent kernel: every workgroup pulls work items from one atomic queue --------
-// With defer_cap != 0, a row whose out-list exceeds this tile's capacity is not looked up
-// in HBM but queued (queue[1] counts them) for k_deferred, which runs with the widest
-// tile; should the queue be full the row is done here after all.
-template <int MODE, int WAVES, int CAP>
+// With DEFER, a row whose out-list exceeds this tile's capacity is not looked up in HBM
+// but queued (queue[1] counts them, at most defer_cap) for k_deferred, which runs with the
+// widest tile; should the queue be full the row is done here after all.  Launches whose
+// rows all fit use the variant without it.
+template <int MODE, int WAVES, int CAP, bool DEFER>
 __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__restrict__ Gs,
                                                            const WorkItem *__restrict__ items,
                                                            uint32_t nitems,
@@ -726,7 +728,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__res
     const uint32_t c_lo = single ? w.c_lo : 0u, c_hi = single ? w.c_hi : kAllC;
     for (uint32_t q = w.lo; q < w.hi; ++q) {
       const uint32_t r = G.rows[q];
-      if (defer_cap != 0 && G.ooff[r + 1] - G.ooff[r] > (uint32_t)CAP) {  // uniform
+      if (DEFER && G.ooff[r + 1] - G.ooff[r] > (uint32_t)CAP) {  // uniform
         __syncthreads();  // every thread has read the previous defer_slot
         if (threadIdx.x == 0) {
           const uint32_t k = atomicAdd(&queue[1], 1u);
@@ -1008,15 +1010,26 @@ constexpr int kWideWaves = 8, kWideCap = 4096;
 template <int MODE, int WAVES, int CAP>
 int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint32_t long_rows,
                        uint64_t *launches) {
-  const uint32_t defer_cap = CAP < kWideCap ? long_rows : 0u;
+  constexpr bool kCanDefer = CAP < kWideCap;
+  const uint32_t defer_cap = kCanDefer ? long_rows : 0u;
   if (defer_cap) PAC_TRY(ctx->defer.reserve(sizeof(DeferItem) * defer_cap));
-  const uint32_t grid = persistent_grid(ctx, k_persistent<MODE, WAVES, CAP>, 64 * WAVES, nitems);
   hipLaunchKernelGGL((k_reset_queue), dim3(1), dim3(64), 0, ctx->stream, ctx->queue.as<uint32_t>());
   if (ctx->time_kernel) PAC_CHECK_HIP(hipEventRecord(ctx->kev[0], ctx->stream));
-  hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP>), dim3(grid), dim3(64 * WAVES), 0,
-                     ctx->stream, ctx->gdesc.as<const GraphDev>(), items, nitems,
-                     ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>(),
-                     ctx->defer.as<DeferItem>(), defer_cap);
+  if (defer_cap) {
+    const uint32_t grid =
+        persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, kCanDefer>, 64 * WAVES, nitems);
+    hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, kCanDefer>), dim3(grid), dim3(64 * WAVES),
+                       0, ctx->stream, ctx->gdesc.as<const GraphDev>(), items, nitems,
+                       ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>(),
+                       ctx->defer.as<DeferItem>(), defer_cap);
+  } else {
+    const uint32_t grid =
+        persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, false>, 64 * WAVES, nitems);
+    hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, false>), dim3(grid), dim3(64 * WAVES), 0,
+                       ctx->stream, ctx->gdesc.as<const GraphDev>(), items, nitems,
+                       ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>(), nullptr,
+                       0u);
+  }
   PAC_CHECK_HIP(hipGetLastError());
   if (defer_cap) {
     const uint32_t g2 = persistent_grid(ctx, k_deferred<MODE, kWideWaves, kWideCap>,
