@@ -139,11 +139,18 @@ def exp_batch(ctx, G=256, check=True):
     return res
 
 
-def exp_jaccard(ctx, scale=25, partitions=4096, sample=200000, cpu_budget=15.0):
-    from oracle import oracle as O
+def exp_jaccard(ctx, scale=25, partitions=4096, sample=200000, cpu_budget=15.0,
+                profile_only=False):
     g = pac.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed=scale)
     g.prepare(partitions)
     info = g.info()
+    if profile_only:  # three batched solves, for a kernel trace
+        ms = []
+        for _ in range(3):
+            g.jaccard_edges_device(pac.DISPATCH_FIXED, pac.LAUNCH_PERSISTENT)
+            ms.append(round(ctx.last_timing().total_ms, 3))
+        return {"config": f"rmat-s{scale}-ef16-P{partitions}-jaccard", "device_ms": ms}
+    from oracle import oracle as O
     res = {"config": f"rmat-s{scale}-ef16-P{partitions}-jaccard", "m": info.num_edges,
            "W": info.work_W, "device_ms": {}}
     for name, d, l in (("plain_fixed", pac.DISPATCH_FIXED, pac.LAUNCH_PLAIN),
@@ -179,6 +186,8 @@ def main(argv=None):
     ap.add_argument("--graphs", type=int, default=256)
     ap.add_argument("--scale", type=int, default=None)
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="jaccard: only three timed solves (for rocprofv3)")
     a = ap.parse_args(argv)
     ctx = pac.Context(0)
     if a.which == "scale22":
@@ -186,7 +195,7 @@ def main(argv=None):
     elif a.which == "batch":
         r = exp_batch(ctx, a.graphs, not a.no_check)
     else:
-        r = exp_jaccard(ctx, a.scale or 25)
+        r = exp_jaccard(ctx, a.scale or 25, profile_only=a.profile_only)
     print(json.dumps(r))
 
 
