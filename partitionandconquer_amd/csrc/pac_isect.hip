@@ -380,9 +380,10 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
 // coalesced loads, software-pipelined one step ahead across the wave's pairs; the
 // pair's hit count is a ballot popcount held in scalar registers.
 constexpr uint32_t kMaxKicks = 48;
-// The two buckets of x among 2^lg (sh = 32 - lg), from full-rate 24-bit multiplies: the
-// top lg bits of (y * 0x9E3779) and of ((h ^ (y >> 5)) * 0xC2B2AE) with y = x ^ (x >> 11),
-// forced distinct so that every key has two choices.
+// The two buckets of x among 2^lg (sh = 32 - lg), from full-rate 24-bit multiplies: b1 is
+// the top lg bits of h = y * 0x9E3779 with y = x ^ (x >> 11), and b2 = b1 ^ d with d the top
+// lg bits of (h ^ (y >> 5)) * 0xC2B2AE made odd -- so the two choices always differ (one
+// bucket of each parity, the two tables of classic cuckoo hashing).
 // (__umul24 is declared returning int: its results are taken as unsigned before any
 // shift, so the bucket indices stay below 2^lg.)
 __device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t &b1, uint32_t &b2) {
@@ -390,8 +391,7 @@ __device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t &b1, u
   const uint32_t h = (uint32_t)__umul24(y, 0x9E3779u);
   const uint32_t g = (uint32_t)__umul24(h ^ (y >> 5), 0xC2B2AEu);
   b1 = h >> sh;
-  b2 = g >> sh;
-  b2 = b2 == b1 ? b1 ^ 1u : b2;
+  b2 = b1 ^ ((g >> sh) | 1u);
 }
 
 template <int MODE, int WAVES, int CAP>
@@ -500,6 +500,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // branch-free (lanes past the list's end read the sentinel oadj[m], which never hits)
   // and the two register sets swap roles every step, so the waits cover only the block
   // being probed.
+  // (Unconditional loads: the wait for a block then counts exactly the loads issued after
+  // it, which keeps the next block's loads in flight while this one is probed.)
   auto load_blk = [&](uint32_t b0, uint32_t nb, uint32_t j0, uint32_t *w) {
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) {
