@@ -461,38 +461,45 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // TC mode the second read is skipped for those lookups, which saves LDS bandwidth (in
   // LCC mode, at lower occupancy, the dependent read costs more than it saves, and so it
   // does with the smallest tile, whose tables are short enough to stay cheap to read).
-  auto find = [&](uint32_t w, uint32_t &word) -> uint32_t {
+  // find(w): whether w is in the table; if so, word is its bucket's counter word and inc
+  // its half of it (1 or 1 << 16).
+  auto find = [&](uint32_t w, uint32_t &word, uint32_t &inc) -> bool {
     uint32_t b1, b2;
     buckets(w, sh, b1, b2);
     const uint32_t k10 = H.tab[2u * b1], k11 = H.tab[2u * b1 + 1u];
     if (MODE == kModeTC && CAP >= 2048) {
-      uint32_t inc = (k10 == w) ? 1u : ((k11 == w) ? 0x10000u : 0u);
+      bool hit = (k10 == w) | (k11 == w);
+      inc = (k11 == w) ? 0x10000u : 1u;
       word = b1;
-      if (inc == 0u && k11 != kEmpty) {
+      if (!hit && k11 != kEmpty) {
         const uint32_t k20 = H.tab[2u * b2], k21 = H.tab[2u * b2 + 1u];
-        inc = (k20 == w) ? 1u : ((k21 == w) ? 0x10000u : 0u);
+        hit = (k20 == w) | (k21 == w);
+        inc = (k21 == w) ? 0x10000u : 1u;
         word = b2;
         for (uint32_t k = 0; k < nst; ++k) {
           if (H.tab[nslot + k] == w) {
+            hit = true;
             word = (nslot + k) >> 1;
             inc = (k & 1u) ? 0x10000u : 1u;  // nslot is even
           }
         }
       }
-      return inc;
+      return hit;
     }
     const uint32_t k20 = H.tab[2u * b2], k21 = H.tab[2u * b2 + 1u];
     // bitwise | (no short-circuit): all four slots are read up front, no branches
     const bool in1 = (k10 == w) | (k11 == w), hi = (k11 == w) | (k21 == w);
-    uint32_t inc = (in1 | (k20 == w) | hi) ? (hi ? 0x10000u : 1u) : 0u;
+    bool hit = in1 | (k20 == w) | hi;
+    inc = hi ? 0x10000u : 1u;
     word = in1 ? b1 : b2;
     for (uint32_t k = 0; k < nst; ++k) {
       if (H.tab[nslot + k] == w) {
+        hit = true;
         word = (nslot + k) >> 1;
         inc = (k & 1u) ? 0x10000u : 1u;  // nslot is even
       }
     }
-    return inc;
+    return hit;
   };
   uint32_t row_hits = 0;  // wave-uniform
   // Blocks of up to 4 x 64 elements of one pair's list N+(y), in pair order; the loads
@@ -546,12 +553,12 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
         if (j0 + 64u * u >= nb) break;  // wave-uniform
-        uint32_t word;
-        const uint32_t inc = find(cur[u], word);  // 0 for the sentinel past the list's end
-        const unsigned long long bal = __ballot(inc != 0u);
+        uint32_t word, inc;
+        const bool hit = find(cur[u], word, inc);  // never for the sentinel past the list's end
+        const unsigned long long bal = __ballot(hit);
         ph += (uint32_t)__popcll(bal);
         if (MODE == kModeSUP && lane == u) hw = bal;
-        if (MODE != kModeTC && inc != 0u) atomicAdd(&H.cnt16[word], inc);  // t(w) / edge r->w
+        if (MODE != kModeTC && hit) atomicAdd(&H.cnt16[word], inc);  // t(w) / edge r->w
       }
       if (MODE == kModeSUP) {
         // edges y->w: one hit word per 64 streamed elements, summed after the launch
@@ -565,8 +572,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         if (MODE != kModeTC && ph != 0 && lane == 0) {
           const uint32_t y = readlane(yl, jj), e = readlane(el, jj);
           if (e - a0 < nA) {
-            uint32_t word;
-            const uint32_t inc = find(y, word);
+            uint32_t word, inc;
+            find(y, word, inc);  // y = A[e - a0] is in the table
             atomicAdd(&H.cnt16[word], inc * ph);  // ph <= nb <= CAP < 2^16
           } else {
             G.etail[e] = ph;
@@ -596,8 +603,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   if (MODE != kModeTC) {
     // per edge r->A[i], in A order (coalesced): the credits of A[i]'s slot
     for (uint32_t i = tid; i < nA; i += BS) {
-      uint32_t word;
-      const uint32_t inc = find(G.oadj[a0 + i], word);
+      uint32_t word, inc;
+      find(G.oadj[a0 + i], word, inc);
       const uint32_t v = (H.cnt16[word] >> (inc == 1u ? 0u : 16u)) & 0xFFFFu;
       if (v) credit_edge(&G.ehead[a0 + i], v, whole);
     }
