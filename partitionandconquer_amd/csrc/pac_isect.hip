@@ -457,17 +457,17 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   //
   // Slots never become empty again (insertion fills them, evictions exchange keys), and
   // a key is placed in its second bucket only when its first is full.  So a key whose
-  // first bucket still has a free slot cannot be in the second bucket or the stash: in
-  // TC mode the second read is skipped for those lookups, which saves LDS bandwidth (in
-  // LCC mode, at lower occupancy, the dependent read costs more than it saves, and so it
-  // does with the smallest tile, whose tables are short enough to stay cheap to read).
+  // first bucket still has a free slot cannot be in the second bucket or the stash: the
+  // second read is skipped for those lookups, which saves LDS bandwidth.  Not with the
+  // smallest tile, whose short tables stay cheap to read, nor in SUP mode, where the
+  // dependent read costs more than it saves.
   // find(w): whether w is in the table; if so, word is its bucket's counter word and inc
   // its half of it (1 or 1 << 16).
   auto find = [&](uint32_t w, uint32_t &word, uint32_t &inc) -> bool {
     uint32_t b1, b2;
     buckets(w, sh, b1, b2);
     const uint32_t k10 = H.tab[2u * b1], k11 = H.tab[2u * b1 + 1u];
-    if (MODE == kModeTC && CAP >= 2048) {
+    if (MODE != kModeSUP && CAP >= 2048) {
       bool hit = (k10 == w) | (k11 == w);
       inc = (k11 == w) ? 0x10000u : 1u;
       word = b1;
