@@ -244,8 +244,9 @@ int pac_square_count(pac_graph *g, uint64_t *res);
 
 /* ---- batching ------------------------------------------------------------------- */
 /* Triangle counts of G prepared graphs in one persistent launch (or one hipGraph
- * replay when launch == PAC_LAUNCH_GRAPH): totals[i] = T of gs[i].  No reference
- * counterpart (the reference handles one graph per process). */
+ * replay when launch == PAC_LAUNCH_GRAPH): totals[i] = T of gs[i]; a graph may appear
+ * only once (PAC_ERR_ARG otherwise).  No reference counterpart (the reference handles
+ * one graph per process). */
 int pac_tc_batch(pac_ctx *ctx, pac_graph *const *gs, uint32_t G, int dispatch, int launch,
                  uint64_t *totals);
 

@@ -1347,6 +1347,16 @@ int pac_tc_batch(pac_ctx *ctx, pac_graph *const *hs, uint32_t G, int dispatch, i
   }
   std::vector<Graph *> gs(G);
   std::vector<uint8_t> strat;
+  {
+    // each graph's count accumulates in its own device total: a graph listed twice would
+    // receive both counts
+    std::vector<const pac_graph *> seen(hs, hs + G);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) {
+      set_error("a graph appears more than once in the batch");
+      return PAC_ERR_ARG;
+    }
+  }
   for (uint32_t i = 0; i < G; ++i) {
     if (!hs[i] || !hs[i]->g.prepared || hs[i]->g.ctx != &ctx->c) {
       set_error("graph %u is null, unprepared or from another context", i);

@@ -170,6 +170,15 @@ def test_long_rows_deferred_to_wide_pass(ctx, orc, launch):
             src, dst, inter, uni, ratio = g.jaccard_edges(d, launch)
             ri, ru, rr = orc.jaccard_edges(ref, src, dst, NT)
             assert np.array_equal(inter, ri) and bits_equal(ratio, rr)
+        # batched with graphs whose rows all fit: the queue holds the long rows of one graph
+        smalls = [orc.gen_rmat(10, 8, 0.57, 0.19, 0.19, seed=s) for s in (3, 4)]
+        g2, g3 = (pac.Graph.from_csr(ctx, x.vlist, x.elist) for x in smalls)
+        g2.prepare(4)
+        g3.prepare(2)
+        got = ctx.tc_batch([g2, g, g3], pac.DISPATCH_FIXED, launch)
+        assert got.tolist() == [orc.tc_merge(smalls[0], NT), T, orc.tc_merge(smalls[1], NT)]
+        with pytest.raises(pac.PacError):
+            ctx.tc_batch([g2, g, g2], pac.DISPATCH_FIXED, launch)  # a graph listed twice
     finally:
         ctx.set_tile(1)
 
