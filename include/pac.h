@@ -115,7 +115,10 @@ int pac_graph_upload(pac_ctx *ctx, const uint64_t *vlist, uint64_t num_vertices,
 int pac_graph_load_csr_dir(pac_ctx *ctx, const char *csrdir, pac_graph **out);
 /* Replaces Graph::Graph(const std::string graphfile), Graph.cpp:27-90 (text edge
  * list: '#'/'%' comments, one-character separator, 1-based IDs shifted when no 0
- * appears, self-loops dropped, symmetrized, rows sorted and deduplicated). */
+ * appears, self-loops dropped, symmetrized, rows sorted and deduplicated).  Parsed on
+ * the device, one thread per line; a file with a line over 511 characters or a line
+ * ending in its first number is parsed on the host with the reference's 512-byte
+ * read loop, whose buffer those lines depend on. */
 int pac_graph_load_text(pac_ctx *ctx, const char *path, pac_graph **out);
 /* Builds the graph from an undirected edge list on the device: both directions,
  * self-loops dropped, duplicates removed, rows sorted (the row building of

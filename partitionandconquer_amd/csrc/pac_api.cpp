@@ -570,11 +570,10 @@ int pac_graph_load_csr_dir(pac_ctx *ctx, const char *csrdir, pac_graph **out) {
 }
 
 // Graph::Graph(const std::string graphfile), Graph.cpp:27-90.
-int pac_graph_load_text(pac_ctx *ctx, const char *path, pac_graph **out) {
-  if (!path) {
-    set_error("null path");
-    return PAC_ERR_ARG;
-  }
+namespace {
+// Graph::Graph(graphfile), Graph.cpp:27-90, on the host: the reference's fgets loop with its
+// 512-byte buffer, for files with a line the device parser leaves to it.
+int load_text_host(pac_ctx *ctx, const char *path, pac_graph **out) {
   FILE *fp = fopen(path, "r");
   if (!fp) {
     set_error("cannot open %s", path);
@@ -612,6 +611,52 @@ int pac_graph_load_text(pac_ctx *ctx, const char *path, pac_graph **out) {
   std::vector<std::pair<uint32_t, uint32_t>>().swap(edges);
   // symmetrize, sort and deduplicate on the device
   return edges_to_graph(ctx, src.data(), dst.data(), src.size(), V, out);
+}
+}  // namespace
+
+// The file is read whole and parsed on the device (line starts by a flag compaction, one
+// thread per line); the edges then go through the device symmetrize / sort / dedup.
+int pac_graph_load_text(pac_ctx *ctx, const char *path, pac_graph **out) {
+  PAC_TRY(ensure_ctx(ctx));
+  if (!path || !out) {
+    set_error("null path or output handle");
+    return PAC_ERR_ARG;
+  }
+  std::vector<char> text;
+  {
+    FILE *fp = fopen(path, "rb");
+    if (!fp) {
+      set_error("cannot open %s", path);
+      return PAC_ERR_IO;
+    }
+    char buf[1 << 16];
+    size_t got;
+    while ((got = fread(buf, 1, sizeof buf, fp)) > 0) text.insert(text.end(), buf, buf + got);
+    fclose(fp);
+  }
+  DevBuf d_src, d_dst;
+  uint64_t ne = 0, V = 0;
+  bool host_needed = false;
+  PAC_TRY(parse_text_device(&ctx->c, text.data(), text.size(), d_src, d_dst, &ne, &V,
+                            &host_needed));
+  std::vector<char>().swap(text);
+  if (host_needed) return load_text_host(ctx, path, out);
+  if (2 * ne >= (1ull << 32) || V > 0xFFFFFFFFull) {
+    set_error("%s: %llu edges / %llu vertices exceed the uint32 CSR limit", path,
+              (unsigned long long)ne, (unsigned long long)V);
+    return PAC_ERR_UNSUPPORTED;
+  }
+  pac_graph *h = nullptr;
+  PAC_TRY(new_graph(ctx, &h));
+  int rc = graph_from_device_edges(&h->g, d_src.as<uint32_t>(), d_dst.as<uint32_t>(), ne,
+                                   (uint32_t)V);
+  if (rc == PAC_OK) rc = orient(&h->g);
+  if (rc != PAC_OK) {
+    delete_graph(h);
+    return rc;
+  }
+  *out = h;
+  return PAC_OK;
 }
 
 int pac_graph_from_edges(pac_ctx *ctx, const int32_t *src, const int32_t *dst, uint64_t ne,

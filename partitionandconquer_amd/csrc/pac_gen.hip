@@ -16,6 +16,8 @@
 // symmetric CSR with sorted, duplicate-free rows.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "pac_internal.hpp"
 
@@ -215,6 +217,162 @@ int generate_rmat(Graph *g, uint32_t scale, uint32_t ef, double a, double b, dou
                      key, keys_a.as<uint64_t>());
   PAC_CHECK_HIP(hipGetLastError());
   return csr_from_keys_device(g, keys_a, n2, scale, 1u << scale);
+}
+
+namespace {
+
+// ---- text edge lists (Graph::Graph(graphfile), Graph.cpp:27-90) on the device --------
+// The reference reads the file with fgets into a 512-byte buffer: a line of up to 511
+// characters is one read.  Its rules per read: '#' or '%' first -> skipped; a number
+// (digits from the start), exactly one skipped character, a second number; a 0 in either
+// marks the file as 0-based; self-loops are dropped after that check; the largest ID of
+// the kept edges sets V.  A read ends at its terminating NUL, so the parse below stops at
+// the end of the line -- except when the first number runs into that end (no separator):
+// the skip then steps over the NUL into bytes left by earlier reads, and a line longer
+// than 511 characters is split into several reads.  Such lines are flagged and the caller
+// parses the file on the host instead, with the reference's own buffer discipline.
+
+__global__ void k_line_starts(const char *__restrict__ t, uint64_t n, uint8_t *__restrict__ f) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    f[i] = (i == 0 || t[i - 1] == '\n') ? 1 : 0;
+}
+
+// flags[0]: a 0 was read; [1]: a line needs the host parser; [2]: an ID >= 2^31.
+// Edge k (line k) is kept as u << 32 | v with keep[k] = 1.
+__global__ void k_parse_lines(const char *__restrict__ t, uint64_t n,
+                              const uint64_t *__restrict__ ls, uint64_t nl,
+                              unsigned long long *__restrict__ edge, uint8_t *__restrict__ keep,
+                              uint32_t *__restrict__ flags, uint32_t *__restrict__ maxid) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nl;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = ls[k], e = k + 1 < nl ? ls[k + 1] : n;  // [s, e), e - s >= 1
+    keep[k] = 0;
+    if (e - s > 511u) {
+      atomicOr(&flags[1], 1u);
+      continue;
+    }
+    if (t[s] == '#' || t[s] == '%') continue;
+    // (the accumulators saturate above 2^31, which is rejected below, so that no digit
+    // string wraps around)
+    uint64_t p = s, u = 0, v = 0;
+    while (p < e && t[p] >= '0' && t[p] <= '9') {
+      u = u * 10 + (uint64_t)(t[p++] - '0');
+      u = u > 0xFFFFFFFFull ? 0xFFFFFFFFull : u;
+    }
+    if (p == e) {  // no separator before the end of the read
+      atomicOr(&flags[1], 1u);
+      continue;
+    }
+    ++p;  // exactly one separator character
+    while (p < e && t[p] >= '0' && t[p] <= '9') {
+      v = v * 10 + (uint64_t)(t[p++] - '0');
+      v = v > 0xFFFFFFFFull ? 0xFFFFFFFFull : v;
+    }
+    if (u > 0x7FFFFFFFull || v > 0x7FFFFFFFull) {
+      atomicOr(&flags[2], 1u);
+      continue;
+    }
+    if (u == 0 || v == 0) atomicOr(&flags[0], 1u);
+    if (u == v) continue;
+    atomicMax(maxid, (uint32_t)(u > v ? u : v));
+    edge[k] = (u << 32) | v;
+    keep[k] = 1;
+  }
+}
+
+__global__ void k_split_edges(const unsigned long long *__restrict__ edge, uint64_t ne,
+                              uint32_t shift, uint32_t *__restrict__ src,
+                              uint32_t *__restrict__ dst) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ne;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    src[i] = (uint32_t)(edge[i] >> 32) - shift;
+    dst[i] = (uint32_t)edge[i] - shift;
+  }
+}
+
+}  // namespace
+
+int parse_text_device(Ctx *ctx, const char *text, uint64_t n, DevBuf &d_src, DevBuf &d_dst,
+                      uint64_t *ne, uint64_t *V, bool *host_needed) {
+  *ne = 0;
+  *V = 0;
+  *host_needed = false;
+  if (n == 0) return PAC_OK;
+  DevBuf dt, fl, ls, cnt, edge, keep, edges, flags;
+  PAC_TRY(dt.reserve(n));
+  PAC_TRY(fl.reserve(n));
+  PAC_TRY(cnt.reserve(16));
+  PAC_CHECK_HIP(hipMemcpy(dt.p, text, n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_line_starts, dim3(grid1d(n)), dim3(256), 0, ctx->stream,
+                     dt.as<const char>(), n, fl.as<uint8_t>());
+  PAC_CHECK_HIP(hipGetLastError());
+  // line starts: the positions whose flag is set (at most n of them)
+  uint64_t nl = 0;
+  {
+    size_t tmp = 0;
+    uint64_t lines = 0;
+    PAC_TRY(ls.reserve(sizeof(uint64_t) * n));
+    rocprim::counting_iterator<uint64_t> pos(0);
+    PAC_CHECK_HIP(rocprim::select(nullptr, tmp, pos, fl.as<const uint8_t>(), ls.as<uint64_t>(),
+                                  cnt.as<uint64_t>(), (size_t)n, ctx->stream));
+    PAC_TRY(ctx->scratch.reserve(tmp));
+    PAC_CHECK_HIP(rocprim::select(ctx->scratch.p, tmp, pos, fl.as<const uint8_t>(),
+                                  ls.as<uint64_t>(), cnt.as<uint64_t>(), (size_t)n, ctx->stream));
+    PAC_CHECK_HIP(hipMemcpyAsync(&lines, cnt.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    nl = lines;
+  }
+  fl.release();
+  PAC_TRY(edge.reserve(sizeof(unsigned long long) * nl));
+  PAC_TRY(keep.reserve(nl));
+  PAC_TRY(flags.reserve(16));
+  PAC_TRY(fill_u32(ctx->stream, flags.as<uint32_t>(), 4, 0u));
+  hipLaunchKernelGGL(k_parse_lines, dim3(grid1d(nl)), dim3(256), 0, ctx->stream,
+                     dt.as<const char>(), n, ls.as<const uint64_t>(), nl,
+                     edge.as<unsigned long long>(), keep.as<uint8_t>(), flags.as<uint32_t>(),
+                     flags.as<uint32_t>() + 3);
+  PAC_CHECK_HIP(hipGetLastError());
+  uint32_t hf[4] = {0, 0, 0, 0};
+  PAC_CHECK_HIP(hipMemcpyAsync(hf, flags.p, 16, hipMemcpyDeviceToHost, ctx->stream));
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (hf[1]) {
+    *host_needed = true;
+    return PAC_OK;
+  }
+  if (hf[2]) {
+    set_error("vertex ID exceeds int32");
+    return PAC_ERR_UNSUPPORTED;
+  }
+  dt.release();
+  ls.release();
+  PAC_TRY(edges.reserve(sizeof(unsigned long long) * (nl ? nl : 1)));
+  {
+    size_t tmp = 0;
+    PAC_CHECK_HIP(rocprim::select(nullptr, tmp, edge.as<const unsigned long long>(),
+                                  keep.as<const uint8_t>(), edges.as<unsigned long long>(),
+                                  cnt.as<uint64_t>(), (size_t)nl, ctx->stream));
+    PAC_TRY(ctx->scratch.reserve(tmp));
+    PAC_CHECK_HIP(rocprim::select(ctx->scratch.p, tmp, edge.as<const unsigned long long>(),
+                                  keep.as<const uint8_t>(), edges.as<unsigned long long>(),
+                                  cnt.as<uint64_t>(), (size_t)nl, ctx->stream));
+  }
+  uint64_t m = 0;
+  PAC_CHECK_HIP(hipMemcpyAsync(&m, cnt.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  const bool zero_based = hf[0] != 0;
+  *V = zero_based ? (uint64_t)hf[3] + 1 : (uint64_t)hf[3];
+  *ne = m;
+  PAC_TRY(d_src.reserve(4 * (m ? m : 1)));
+  PAC_TRY(d_dst.reserve(4 * (m ? m : 1)));
+  if (m) {
+    hipLaunchKernelGGL(k_split_edges, dim3(grid1d(m)), dim3(256), 0, ctx->stream,
+                       edges.as<const unsigned long long>(), m, zero_based ? 0u : 1u,
+                       d_src.as<uint32_t>(), d_dst.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // the temporaries are freed on return
+  return PAC_OK;
 }
 
 int graph_from_device_edges(Graph *g, const uint32_t *d_src, const uint32_t *d_dst, uint64_t ne,

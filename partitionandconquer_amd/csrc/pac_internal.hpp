@@ -218,6 +218,13 @@ int graph_validate(const Graph *g, int level);
 int generate_rmat(Graph *g, uint32_t scale, uint32_t ef, double a, double b, double c,
                   uint64_t seed);
 // Symmetric, deduplicated, self-loop-free CSR from an undirected edge list on the device.
+// Parse a text edge list (n bytes, host memory) on the device with the reference's
+// rules; on success d_src/d_dst hold the *ne kept edges with 0-based IDs and *V the vertex
+// count.  *host_needed is set instead when a line can only be parsed with the reference's
+// buffer discipline on the host (a line over 511 characters, or one ending in its first
+// number).
+int parse_text_device(Ctx *ctx, const char *text, uint64_t n, DevBuf &d_src, DevBuf &d_dst,
+                      uint64_t *ne, uint64_t *V, bool *host_needed);
 int graph_from_device_edges(Graph *g, const uint32_t *d_src, const uint32_t *d_dst, uint64_t ne,
                             uint32_t V);
 

@@ -291,6 +291,46 @@ def test_text_loader_matches_oracle(ctx, orc):
         assert vl.tolist() == exp[f]["vlist"] and el.tolist() == exp[f]["elist"]
 
 
+@pytest.mark.parametrize("one_based", [False, True])
+def test_text_loader_device_parse(ctx, orc, tmp_path, one_based):
+    """The device parser (one thread per line) on a 60 000-line file with the reference's
+    quirks -- comments, tab / double-space / CRLF separators, trailing text, self-loops,
+    a last line without newline -- and the host fallback for a line over 511 characters and
+    for a file ending in a bare number; every CSR equal to the oracle's restatement of
+    Graph.cpp:27-90."""
+    rng = np.random.default_rng(11 + one_based)
+    lo = 1 if one_based else 0
+    lines = []
+    for _ in range(60000):
+        u, v = (int(x) for x in rng.integers(lo, 5000, 2))
+        r = rng.random()
+        if r < 0.03:
+            lines.append("# comment %d %d\n" % (u, v))
+        elif r < 0.05:
+            lines.append("%% meta %d\n" % u)
+        elif r < 0.08:
+            lines.append("%d\t%d\r\n" % (u, v))
+        elif r < 0.09 and not one_based:
+            lines.append("%d  %d\n" % (u, v))       # second number read as 0
+        elif r < 0.11:
+            lines.append("%d %d extra 77\n" % (u, v))
+        elif r < 0.13:
+            lines.append("%d %d\n" % (u, u))        # self-loop
+        else:
+            lines.append("%d %d\n" % (u, v))
+    body = "".join(lines)
+    cases = {"plain.txt": body + "%d %d" % (lo + 3, lo + 4),          # no final newline
+             "longline.txt": body + "1 2 " + "x" * 600 + "\n",       # host fallback
+             "bare.txt": body + "%d" % (lo + 9)}                      # host fallback
+    for name, text in cases.items():
+        path = tmp_path / name
+        path.write_text(text)
+        ref = orc.text_to_csr(str(path))
+        g = pac.Graph.load_text(ctx, str(path))
+        vl, el = g.csr()
+        assert np.array_equal(vl, ref.vlist) and np.array_equal(el, ref.elist), name
+
+
 def test_error_paths(ctx, orc, tmp_path):
     ref = orc.gen_rmat(6, 4, 0.57, 0.19, 0.19, seed=1)
     g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
