@@ -569,7 +569,6 @@ int pac_graph_load_csr_dir(pac_ctx *ctx, const char *csrdir, pac_graph **out) {
   return upload_csr(ctx, vl.data(), vl.size() - 1, el.data(), el.size(), out);
 }
 
-// Graph::Graph(const std::string graphfile), Graph.cpp:27-90.
 namespace {
 // Graph::Graph(graphfile), Graph.cpp:27-90, on the host: the reference's fgets loop with its
 // 512-byte buffer, for files with a line the device parser leaves to it.

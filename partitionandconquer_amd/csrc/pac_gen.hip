@@ -232,10 +232,21 @@ namespace {
 // than 511 characters is split into several reads.  Such lines are flagged and the caller
 // parses the file on the host instead, with the reference's own buffer discipline.
 
-__global__ void k_line_starts(const char *__restrict__ t, uint64_t n, uint8_t *__restrict__ f) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    f[i] = (i == 0 || t[i - 1] == '\n') ? 1 : 0;
+// Position i starts a line when i == 0 or byte i-1 is a newline.
+struct LineStart {
+  const char *t;
+  __device__ bool operator()(uint64_t i) const { return i == 0 || t[i - 1] == '\n'; }
+};
+
+__global__ void k_count_lines(const char *__restrict__ t, uint64_t n,
+                              unsigned long long *__restrict__ count) {
+  const LineStart is_start{t};
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n;
+       i0 += (uint64_t)gridDim.x * blockDim.x) {  // block-uniform trip count
+    const uint64_t i = i0 + threadIdx.x;
+    const int c = __syncthreads_count(i < n && is_start(i));
+    if (threadIdx.x == 0 && c) atomicAdd(count, (unsigned long long)c);
+  }
 }
 
 // flags[0]: a 0 was read; [1]: a line needs the host parser; [2]: an ID >= 2^31.
@@ -299,31 +310,37 @@ int parse_text_device(Ctx *ctx, const char *text, uint64_t n, DevBuf &d_src, Dev
   *V = 0;
   *host_needed = false;
   if (n == 0) return PAC_OK;
-  DevBuf dt, fl, ls, cnt, edge, keep, edges, flags;
+  DevBuf dt, ls, cnt, edge, keep, edges, flags;
   PAC_TRY(dt.reserve(n));
-  PAC_TRY(fl.reserve(n));
   PAC_TRY(cnt.reserve(16));
   PAC_CHECK_HIP(hipMemcpy(dt.p, text, n, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_line_starts, dim3(grid1d(n)), dim3(256), 0, ctx->stream,
-                     dt.as<const char>(), n, fl.as<uint8_t>());
-  PAC_CHECK_HIP(hipGetLastError());
-  // line starts: the positions whose flag is set (at most n of them)
+  // line starts: count them, then select the positions that start a line
   uint64_t nl = 0;
+  PAC_TRY(fill_u32(ctx->stream, cnt.as<uint32_t>(), 4, 0u));
+  hipLaunchKernelGGL(k_count_lines, dim3(grid1d(n)), dim3(256), 0, ctx->stream,
+                     dt.as<const char>(), n, reinterpret_cast<unsigned long long *>(cnt.p));
+  PAC_CHECK_HIP(hipGetLastError());
+  PAC_CHECK_HIP(hipMemcpyAsync(&nl, cnt.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  PAC_TRY(ls.reserve(sizeof(uint64_t) * (nl ? nl : 1)));
   {
     size_t tmp = 0;
-    uint64_t lines = 0;
-    PAC_TRY(ls.reserve(sizeof(uint64_t) * n));
+    uint64_t got = 0;
     rocprim::counting_iterator<uint64_t> pos(0);
-    PAC_CHECK_HIP(rocprim::select(nullptr, tmp, pos, fl.as<const uint8_t>(), ls.as<uint64_t>(),
-                                  cnt.as<uint64_t>(), (size_t)n, ctx->stream));
+    const LineStart pred{dt.as<const char>()};
+    PAC_CHECK_HIP(rocprim::select(nullptr, tmp, pos, ls.as<uint64_t>(), cnt.as<uint64_t>(),
+                                  (size_t)n, pred, ctx->stream));
     PAC_TRY(ctx->scratch.reserve(tmp));
-    PAC_CHECK_HIP(rocprim::select(ctx->scratch.p, tmp, pos, fl.as<const uint8_t>(),
-                                  ls.as<uint64_t>(), cnt.as<uint64_t>(), (size_t)n, ctx->stream));
-    PAC_CHECK_HIP(hipMemcpyAsync(&lines, cnt.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PAC_CHECK_HIP(rocprim::select(ctx->scratch.p, tmp, pos, ls.as<uint64_t>(), cnt.as<uint64_t>(),
+                                  (size_t)n, pred, ctx->stream));
+    PAC_CHECK_HIP(hipMemcpyAsync(&got, cnt.p, 8, hipMemcpyDeviceToHost, ctx->stream));
     PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-    nl = lines;
+    if (got != nl) {
+      set_error("line count mismatch (%llu vs %llu)", (unsigned long long)got,
+                (unsigned long long)nl);
+      return PAC_ERR_HIP;
+    }
   }
-  fl.release();
   PAC_TRY(edge.reserve(sizeof(unsigned long long) * nl));
   PAC_TRY(keep.reserve(nl));
   PAC_TRY(flags.reserve(16));
