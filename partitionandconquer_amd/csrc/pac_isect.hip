@@ -124,10 +124,11 @@ constexpr uint32_t kStash = 32;           // overflow entries of the hash table
 
 // LDS of one workgroup.  The sorted-list strategies (merge, gallop) keep A and one
 // 32-bit counter per position; the hash strategy keeps the cuckoo table of A (plus
-// its stash) with packed 16-bit counters per slot and, in SUP mode, 16-bit positions.
-// Both live in one union: a workgroup runs one strategy per row.
+// its stash) with packed 16-bit counters per slot.  Both live in one union: a workgroup
+// runs one strategy per row.  (8-byte aligned: a bucket's two slots are read with one
+// 64-bit LDS load.)
 template <int CAP, int MODE, int WAVES>
-struct Smem {
+struct alignas(8) Smem {
   static constexpr uint32_t kSlots = 2u * pow2_at_least(CAP);
   union {
     struct {
@@ -466,13 +467,15 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   auto find = [&](uint32_t w, uint32_t &word, uint32_t &inc) -> bool {
     uint32_t b1, b2;
     buckets(w, sh, b1, b2);
-    const uint32_t k10 = H.tab[2u * b1], k11 = H.tab[2u * b1 + 1u];
+    const uint2 q1 = reinterpret_cast<const uint2 *>(H.tab)[b1];
+    const uint32_t k10 = q1.x, k11 = q1.y;
     if (MODE != kModeSUP && CAP >= 2048) {
       bool hit = (k10 == w) | (k11 == w);
       inc = (k11 == w) ? 0x10000u : 1u;
       word = b1;
       if (!hit && k11 != kEmpty) {
-        const uint32_t k20 = H.tab[2u * b2], k21 = H.tab[2u * b2 + 1u];
+        const uint2 q2 = reinterpret_cast<const uint2 *>(H.tab)[b2];
+        const uint32_t k20 = q2.x, k21 = q2.y;
         hit = (k20 == w) | (k21 == w);
         inc = (k21 == w) ? 0x10000u : 1u;
         word = b2;
@@ -486,7 +489,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       }
       return hit;
     }
-    const uint32_t k20 = H.tab[2u * b2], k21 = H.tab[2u * b2 + 1u];
+    const uint2 q2 = reinterpret_cast<const uint2 *>(H.tab)[b2];
+    const uint32_t k20 = q2.x, k21 = q2.y;
     // bitwise | (no short-circuit): all four slots are read up front, no branches
     const bool in1 = (k10 == w) | (k11 == w), hi = (k11 == w) | (k21 == w);
     bool hit = in1 | (k20 == w) | hi;
