@@ -120,6 +120,9 @@ __device__ __forceinline__ uint32_t lower_bound_u32(P a, uint32_t n, uint32_t x)
 }
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // empty hash-table slot (IDs are < 2^31)
+// Word 3 of a raw (stride 0) buffer resource on this target: 32-bit data format, which
+// untyped buffer loads require; out-of-range lanes read 0.
+constexpr int kRawBufferFlags = 0x00020000;
 constexpr uint32_t kStash = 32;           // overflow entries of the hash table
 
 // LDS of one workgroup.  The sorted-list strategies (merge, gallop) keep A and one
@@ -507,18 +510,20 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   };
   uint32_t row_hits = 0;  // wave-uniform
   // Blocks of up to 4 x 64 elements of one pair's list N+(y), in pair order; the loads
-  // of the next block are issued before the current one is probed.  The loads are
-  // branch-free (lanes past the list's end read the sentinel oadj[m], which never hits)
-  // and the two register sets swap roles every step, so the waits cover only the block
-  // being probed.
-  // (Unconditional loads: the wait for a block then counts exactly the loads issued after
-  // it, which keeps the next block's loads in flight while this one is probed.)
+  // of the next block are issued before the current one is probed.  The loads go through
+  // a buffer resource that covers exactly N+(y): one per-lane offset per block (the four
+  // sub-blocks are immediate offsets), no per-element address arithmetic, and lanes past
+  // the list's end read 0 from the hardware's range check -- their lookups are masked.
+  // The two register sets swap roles every step, so the waits cover only the block being
+  // probed.  (Unconditional loads: the wait for a block then counts exactly the loads
+  // issued after it, which keeps the next block's loads in flight.)
   auto load_blk = [&](uint32_t b0, uint32_t nb, uint32_t j0, uint32_t *w) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(const uint32_t *)(G.oadj + b0), (short)0, (int)(nb * 4u), kRawBufferFlags);
+    const uint32_t vo = (j0 + lane) * 4u;
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) {
-      const uint32_t j = j0 + 64u * u + lane;
-      w[u] = G.oadj[j < nb ? b0 + j : G.m];
-    }
+    for (uint32_t u = 0; u < 4; ++u)
+      w[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo + 256u * u, 0, 0);
   };
   for (uint32_t t0 = 0;; t0 += 64) {
     if (t0 != 0) {
@@ -558,7 +563,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       for (uint32_t u = 0; u < 4; ++u) {
         if (j0 + 64u * u >= nb) break;  // wave-uniform
         uint32_t word, inc;
-        const bool hit = find(cur[u], word, inc);  // never for the sentinel past the list's end
+        const bool live = j0 + 64u * u + lane < nb;  // lanes past the end read 0
+        const bool hit = find(cur[u], word, inc) & live;
         const unsigned long long bal = __ballot(hit);
         ph += (uint32_t)__popcll(bal);
         if (MODE == kModeSUP && lane == u) hw = bal;
