@@ -457,41 +457,16 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   if (nst > kStash) return false;
   // Lookup of w.  A bucket's two slots share one word of the packed 16-bit counters, so
   // a hit is reported as that word (the bucket) and the increment for w's half of it
-  // (1 or 1 << 16); inc = 0 when w is absent.  Stash entry k sits at slot nslot + k.
-  //
-  // Slots never become empty again (insertion fills them, evictions exchange keys), and
-  // a key is placed in its second bucket only when its first is full.  So a key whose
-  // first bucket still has a free slot cannot be in the second bucket or the stash: the
-  // second read is skipped for those lookups, which saves LDS bandwidth.  Not with the
-  // smallest tile, whose short tables stay cheap to read, nor in SUP mode, where the
-  // dependent read costs more than it saves.
-  // find(w): whether w is in the table; if so, word is its bucket's counter word and inc
-  // its half of it (1 or 1 << 16).
+  // (1 or 1 << 16).  Stash entry k sits at slot nslot + k.  Both buckets are always read
+  // (two 8-byte LDS loads, no branch): with 64-bit bucket loads LDS has headroom, and a
+  // lookup that skips the second bucket when the first has a free slot -- exact, since a
+  // key only goes to its second bucket when the first is full -- costs more in divergent
+  // scalar work than it saves.
   auto find = [&](uint32_t w, uint32_t &word, uint32_t &inc) -> bool {
     uint32_t b1, b2;
     buckets(w, sh, b1, b2);
     const uint2 q1 = reinterpret_cast<const uint2 *>(H.tab)[b1];
     const uint32_t k10 = q1.x, k11 = q1.y;
-    if (MODE != kModeSUP && CAP >= 2048) {
-      bool hit = (k10 == w) | (k11 == w);
-      inc = (k11 == w) ? 0x10000u : 1u;
-      word = b1;
-      if (!hit && k11 != kEmpty) {
-        const uint2 q2 = reinterpret_cast<const uint2 *>(H.tab)[b2];
-        const uint32_t k20 = q2.x, k21 = q2.y;
-        hit = (k20 == w) | (k21 == w);
-        inc = (k21 == w) ? 0x10000u : 1u;
-        word = b2;
-        for (uint32_t k = 0; k < nst; ++k) {
-          if (H.tab[nslot + k] == w) {
-            hit = true;
-            word = (nslot + k) >> 1;
-            inc = (k & 1u) ? 0x10000u : 1u;  // nslot is even
-          }
-        }
-      }
-      return hit;
-    }
     const uint2 q2 = reinterpret_cast<const uint2 *>(H.tab)[b2];
     const uint32_t k20 = q2.x, k21 = q2.y;
     // bitwise | (no short-circuit): all four slots are read up front, no branches
