@@ -123,13 +123,12 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // empty hash-table slot (IDs are < 2^
 // Word 3 of a raw (stride 0) buffer resource on this target: 32-bit data format, which
 // untyped buffer loads require; out-of-range lanes read 0.
 constexpr int kRawBufferFlags = 0x00020000;
-constexpr uint32_t kStash = 32;           // overflow entries of the hash table
 
 // LDS of one workgroup.  The sorted-list strategies (merge, gallop) keep A and one
 // 32-bit counter per position; the hash strategy keeps the cuckoo table of A (plus
 // its stash) with packed 16-bit counters per slot.  Both live in one union: a workgroup
 // runs one strategy per row.  (8-byte aligned: a bucket's two slots are read with one
-// 64-bit LDS load.)
+// 64-bit LDS load.)  `homeless` counts the keys a hash-table build could not place.
 template <int CAP, int MODE, int WAVES>
 struct alignas(8) Smem {
   static constexpr uint32_t kSlots = 2u * pow2_at_least(CAP);
@@ -139,13 +138,13 @@ struct alignas(8) Smem {
       uint32_t cnt[MODE == kModeTC ? 1 : CAP];
     } s;  // merge / gallop
     struct {
-      uint32_t tab[kSlots + kStash];
-      uint32_t cnt16[MODE == kModeTC ? 1 : (kSlots + kStash) / 2];   // 2 x u16 per word
+      uint32_t tab[kSlots];
+      uint32_t cnt16[MODE == kModeTC ? 1 : kSlots / 2];   // 2 x u16 per word
     } h;  // hash
   } u;
   uint32_t item;
   uint32_t defer_slot;
-  uint32_t nstash;
+  uint32_t homeless;
   uint32_t rowtot;  // the row's triangle count (LCC mode)
 };
 
@@ -372,30 +371,33 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
 
 // ---- one row, hash strategy: two-choice cuckoo table of A in LDS ---------------------
 // A is held in a table of 2^lg buckets of two slots (2^lg >= nA, so the load factor is
-// at most 1/2, far below the ~0.89 limit of this layout: insertions almost never reach
-// the stash, whose scan every lookup of the row would pay).  Key x may sit in bucket
-// h1(x) or h2(x); a lookup reads both buckets (two 8-byte LDS reads) and compares four
-// slots -- a fixed cost with no loop and no divergence.  Insertion is a bounded random
-// walk: a key takes a free slot of
-// its two buckets, else evicts a resident and carries it on; a key still homeless
-// after kMaxKicks evictions goes to a small stash that lookups also scan.  If the
-// stash overflows the row is done by the sorted-list path instead (exact either way).
+// at most 1/2, far below the ~0.89 limit of this layout).  Key x may sit in bucket
+// b1(x) or b2(x); a lookup reads both buckets (two 8-byte LDS reads) and compares four
+// slots -- a fixed cost with no loop and no divergence.  Insertion is a bounded walk: a
+// key takes a free slot of its two buckets, else evicts a resident of the bucket it was
+// not itself evicted from and carries the evicted key on (always evicting from the
+// other bucket is what keeps two keys from trading one slot back and forth: on R-MAT
+// out-lists it leaves about 2 rows in 100 000 with a key still homeless after
+// kMaxKicks evictions, where alternating between the two buckets by kick number left
+// 2 in 1 000, most of them rows of 129-512 keys).  Such a row is done by the
+// sorted-list path instead (exact either way), so lookups never scan an overflow area.
 // Each pair's list N+(y) is streamed by one wave, 4 x 64 elements per step with
 // coalesced loads, software-pipelined one step ahead across the wave's pairs; the
 // pair's hit count is a ballot popcount held in scalar registers.
 constexpr uint32_t kMaxKicks = 48;
-// The two buckets of x among 2^lg (sh = 32 - lg), from full-rate 24-bit multiplies: b1 is
-// the top lg bits of h = y * 0x9E3779 with y = x ^ (x >> 11), and b2 = b1 ^ d with d the top
-// lg bits of (h ^ (y >> 5)) * 0xC2B2AE made odd -- so the two choices always differ (one
-// bucket of each parity, the two tables of classic cuckoo hashing).
-// (__umul24 is declared returning int: its results are taken as unsigned before any
-// shift, so the bucket indices stay below 2^lg.)
-__device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t &b1, uint32_t &b2) {
+// The two buckets of x among 2^lg (sh = 32 - lg), from one full-rate 24-bit multiply:
+// h = y * 0x9E3779 with y = x ^ (x >> 11) (folding bits 24.. into the multiplier's
+// 24-bit input); b1 is the top lg bits of h and b2 = b1 ^ d with d = bits [4, 4 + lg)
+// of h made odd (lg <= 12, so the two fields are disjoint; one bit-field extract with a
+// constant offset) -- so the two choices always differ (one bucket of each parity, the
+// two tables of classic cuckoo hashing).  (__umul24 is declared returning int: its
+// result is taken as unsigned before any shift, so the bucket indices stay below 2^lg.)
+__device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t lg, uint32_t &b1,
+                                        uint32_t &b2) {
   const uint32_t y = x ^ (x >> 11);
   const uint32_t h = (uint32_t)__umul24(y, 0x9E3779u);
-  const uint32_t g = (uint32_t)__umul24(h ^ (y >> 5), 0xC2B2AEu);
   b1 = h >> sh;
-  b2 = b1 ^ ((g >> sh) | 1u);
+  b2 = b1 ^ (__builtin_amdgcn_ubfe(h, 4u, lg) | 1u);
 }
 
 template <int MODE, int WAVES, int CAP>
@@ -423,28 +425,30 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   const uint32_t nslot = 2u << lg, sh = 32u - lg;
   for (uint32_t i = tid; i < nslot; i += BS) H.tab[i] = kEmpty;
   if (MODE != kModeTC)
-    for (uint32_t i = tid; i < (nslot + kStash) / 2; i += BS) H.cnt16[i] = 0;
+    for (uint32_t i = tid; i < nslot / 2; i += BS) H.cnt16[i] = 0;
   if (tid == 0) {
-    sm.nstash = 0;
+    sm.homeless = 0;
     sm.rowtot = 0;
   }
   __syncthreads();
   for (uint32_t i = tid; i < nA; i += BS) {
     uint32_t x = G.oadj[a0 + i];
+    uint32_t from = kEmpty;  // the bucket x was just evicted from (none yet)
     for (uint32_t kick = 0;; ++kick) {
       uint32_t b1, b2;
-      buckets(x, sh, b1, b2);
+      buckets(x, sh, lg, b1, b2);
       if (atomicCAS(&H.tab[2u * b1], kEmpty, x) == kEmpty) break;
       if (atomicCAS(&H.tab[2u * b1 + 1u], kEmpty, x) == kEmpty) break;
       if (atomicCAS(&H.tab[2u * b2], kEmpty, x) == kEmpty) break;
       if (atomicCAS(&H.tab[2u * b2 + 1u], kEmpty, x) == kEmpty) break;
       if (kick == kMaxKicks) {
-        const uint32_t k = atomicAdd(&sm.nstash, 1u);
-        if (k < kStash) H.tab[nslot + k] = x;
+        atomicAdd(&sm.homeless, 1u);
         break;
       }
-      // evict from the two buckets in turn, alternating slots
-      x = atomicExch(&H.tab[((kick & 1u) ? 2u * b2 : 2u * b1) + ((kick >> 1) & 1u)], x);
+      // evict from the bucket x did not just come from, alternating slots every two kicks
+      const uint32_t vb = from == b1 ? b2 : b1;
+      x = atomicExch(&H.tab[2u * vb + ((kick >> 1) & 1u)], x);
+      from = vb;
       if (x == kEmpty) break;
     }
   }
@@ -453,35 +457,44 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     nl = G.ooff[yl + 1] - bl;
   }
   __syncthreads();
-  const uint32_t nst = sm.nstash;  // workgroup-uniform
-  if (nst > kStash) return false;
+  if (sm.homeless != 0) return false;  // workgroup-uniform
   // Lookup of w.  A bucket's two slots share one word of the packed 16-bit counters, so
   // a hit is reported as that word (the bucket) and the increment for w's half of it
-  // (1 or 1 << 16).  Stash entry k sits at slot nslot + k.  Both buckets are always read
-  // (two 8-byte LDS loads, no branch): with 64-bit bucket loads LDS has headroom, and a
-  // lookup that skips the second bucket when the first has a free slot -- exact, since a
-  // key only goes to its second bucket when the first is full -- costs more in divergent
-  // scalar work than it saves.
+  // (1 or 1 << 16); word and increment are only meaningful on a hit.  Both buckets are
+  // always read (two 8-byte LDS loads, no branch): with 64-bit bucket loads LDS has
+  // headroom, and a lookup that skips the second bucket when the first has a free slot --
+  // exact, since a key only goes to its second bucket when the first is full -- costs
+  // more in divergent scalar work than it saves.
   auto find = [&](uint32_t w, uint32_t &word, uint32_t &inc) -> bool {
     uint32_t b1, b2;
-    buckets(w, sh, b1, b2);
+    buckets(w, sh, lg, b1, b2);
     const uint2 q1 = reinterpret_cast<const uint2 *>(H.tab)[b1];
     const uint32_t k10 = q1.x, k11 = q1.y;
     const uint2 q2 = reinterpret_cast<const uint2 *>(H.tab)[b2];
     const uint32_t k20 = q2.x, k21 = q2.y;
     // bitwise | (no short-circuit): all four slots are read up front, no branches
     const bool in1 = (k10 == w) | (k11 == w), hi = (k11 == w) | (k21 == w);
-    bool hit = in1 | (k20 == w) | hi;
     inc = hi ? 0x10000u : 1u;
     word = in1 ? b1 : b2;
-    for (uint32_t k = 0; k < nst; ++k) {
-      if (H.tab[nslot + k] == w) {
-        hit = true;
-        word = (nslot + k) >> 1;
-        inc = (k & 1u) ? 0x10000u : 1u;  // nslot is even
-      }
-    }
-    return hit;
+    return in1 | (k20 == w) | hi;
+  };
+  // The streamed lookups: the four slot compares of w are balloted one by one (each
+  // ballot is the compare's own lane mask), so the wave's hit mask never passes through
+  // a vector register; the hit's counter word and increment are only formed under it.
+  struct Probe {
+    bool e10, e11, e20, e21;
+    uint32_t b1, b2;
+  };
+  auto probe = [&](uint32_t w) -> Probe {
+    Probe q;
+    buckets(w, sh, lg, q.b1, q.b2);
+    const uint2 q1 = reinterpret_cast<const uint2 *>(H.tab)[q.b1];
+    const uint2 q2 = reinterpret_cast<const uint2 *>(H.tab)[q.b2];
+    q.e10 = q1.x == w;
+    q.e11 = q1.y == w;
+    q.e20 = q2.x == w;
+    q.e21 = q2.y == w;
+    return q;
   };
   uint32_t row_hits = 0;  // wave-uniform
   // Blocks of up to 4 x 64 elements of one pair's list N+(y), in pair order; the loads
@@ -493,12 +506,14 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // probed.  (Unconditional loads: the wait for a block then counts exactly the loads
   // issued after it, which keeps the next block's loads in flight.)
   auto load_blk = [&](uint32_t b0, uint32_t nb, uint32_t j0, uint32_t *w) {
+    // the resource starts at the block (j0 < nb), so the per-lane offsets are
+    // loop-invariant: no vector register is rewritten between the pipelined loads
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(const uint32_t *)(G.oadj + b0), (short)0, (int)(nb * 4u), kRawBufferFlags);
-    const uint32_t vo = (j0 + lane) * 4u;
+        (void *)(const uint32_t *)(G.oadj + b0 + j0), (short)0, (int)((nb - j0) * 4u),
+        kRawBufferFlags);
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u)
-      w[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo + 256u * u, 0, 0);
+      w[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4u + 256u * u, 0, 0);
   };
   for (uint32_t t0 = 0;; t0 += 64) {
     if (t0 != 0) {
@@ -520,6 +535,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     uint32_t wa[4], wb[4];
     load_blk(b0, nb, 0, wa);
     auto step = [&](const uint32_t *cur, uint32_t *nxt) {
+      const uint32_t rem = nb - j0;  // > 0: elements left in the pair's list
       uint32_t jn = jj, j0n = j0 + 256u, b0n = b0, nbn = nb;
       unsigned long long hbn = hb;
       if (j0n >= nb) {
@@ -536,18 +552,21 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       unsigned long long hw = 0;  // SUP mode: this lane's hit word (lanes 0..3)
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
-        if (j0 + 64u * u >= nb) break;  // wave-uniform
-        uint32_t word, inc;
-        const bool live = j0 + 64u * u + lane < nb;  // lanes past the end read 0
-        const bool hit = find(cur[u], word, inc) & live;
-        const unsigned long long bal = __ballot(hit);
+        if (64u * u >= rem) break;  // wave-uniform
+        const bool live = lane + 64u * u < rem;  // lanes past the end read 0
+        const Probe q = probe(cur[u]);
+        const unsigned long long bal = (__ballot(q.e10) | __ballot(q.e11) | __ballot(q.e20) |
+                                        __ballot(q.e21)) & __ballot(live);
         ph += (uint32_t)__popcll(bal);
         if (MODE == kModeSUP && lane == u) hw = bal;
-        if (MODE != kModeTC && hit) atomicAdd(&H.cnt16[word], inc);  // t(w) / edge r->w
+        if (MODE != kModeTC && ((q.e10 | q.e11 | q.e20 | q.e21) & live)) {
+          // t(w) / edge r->w
+          atomicAdd(&H.cnt16[(q.e10 | q.e11) ? q.b1 : q.b2], (q.e11 | q.e21) ? 0x10000u : 1u);
+        }
       }
       if (MODE == kModeSUP) {
         // edges y->w: one hit word per 64 streamed elements, summed after the launch
-        const uint32_t nsub = min(4u, (nb - j0 + 63u) >> 6);
+        const uint32_t nsub = min(4u, (rem + 63u) >> 6);
         if (lane < nsub) G.hitw[hb + (j0 >> 6) + lane] = hw;
       }
       if (jn != jj) {

@@ -200,6 +200,51 @@ def test_degenerate_graphs(ctx, orc):
     assert inter.tolist() == [0] and uni.tolist() == [2] and ratio.tolist() == [0.0]
 
 
+def _cuckoo_buckets(x, lg):
+    """The two LDS hash-table buckets of key x among 2^lg (buckets() in pac_isect.hip),
+    used here only to build inputs whose tables cannot be filled."""
+    sh = 32 - lg
+    y = x ^ (x >> 11)
+    h = ((y & 0xFFFFFF) * 0x9E3779) & 0xFFFFFFFF
+    b1 = h >> sh
+    return b1, b1 ^ (((h >> 4) & ((1 << lg) - 1)) | 1)
+
+
+def test_hash_table_build_failure_falls_back(ctx, orc):
+    """Rows whose out-lists hold six keys sharing both buckets (four slots between them):
+    two keys stay homeless, the hash build gives up and the row is counted by the
+    sorted-list path.  Vertex r joins the 16 keys of K, which form a clique, so every row
+    here has 2^lg = 16 buckets; a second, larger row (lg = 6) does the same among 40 keys.
+    Counts, LCC and Jaccard must still equal the oracle's."""
+    rng = np.random.default_rng(7)
+    src, dst = [], []
+    for r, n, lg, lo, hi in ((5, 16, 4, 1000, 1 << 19), (7, 40, 6, 1 << 19, 1 << 20)):
+        by = {}
+        for x in range(lo, hi):
+            by.setdefault(_cuckoo_buckets(x, lg), []).append(x)
+            if len(by[_cuckoo_buckets(x, lg)]) == 6:
+                bad = by[_cuckoo_buckets(x, lg)]
+                break
+        pool = np.setdiff1d(np.arange(lo, hi), bad)
+        K = sorted(bad + rng.choice(pool, n - 6, replace=False).tolist())
+        for i, a in enumerate(K):
+            src.append(r)
+            dst.append(a)
+            for b in K[i + 1:]:
+                src.append(a)
+                dst.append(b)
+    ref = orc.edges_to_csr(1 << 20, src, dst)
+    g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    T = check_all(ctx, g, ref, orc, P=4, strategies=[pac.FORCE_BITMAP, pac.DISPATCH_FIXED])
+    assert T == 560 + 120 + 9880 + 780
+    for l in LAUNCHES:
+        l32, l64, t = g.lcc(pac.FORCE_BITMAP, l)
+        assert t == T and bits_equal(l64, orc.lcc_merge(ref, NT))
+        src2, dst2, inter, uni, ratio = g.jaccard_edges(pac.FORCE_BITMAP, l)
+        ri, ru, rr = orc.jaccard_edges(ref, src2, dst2, NT)
+        assert np.array_equal(inter, ri) and np.array_equal(uni, ru) and bits_equal(ratio, rr)
+
+
 def test_jaccard_edges_scale13(ctx, orc):
     ref = orc.gen_rmat(13, 16, 0.57, 0.19, 0.19, seed=13)
     g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
