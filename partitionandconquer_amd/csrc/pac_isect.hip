@@ -509,7 +509,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     // the resource starts at the block (j0 < nb), so the per-lane offsets are
     // loop-invariant: no vector register is rewritten between the pipelined loads
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(const uint32_t *)(G.oadj + b0 + j0), (short)0, (int)((nb - j0) * 4u),
+        (void *)(const uint32_t *)(G.oadj + (b0 + j0)), (short)0, (int)((nb - j0) * 4u),
         kRawBufferFlags);
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u)
@@ -535,20 +535,17 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     uint32_t wa[4], wb[4];
     load_blk(b0, nb, 0, wa);
     auto step = [&](const uint32_t *cur, uint32_t *nxt) {
-      const uint32_t rem = nb - j0;  // > 0: elements left in the pair's list
-      uint32_t jn = jj, j0n = j0 + 256u, b0n = b0, nbn = nb;
-      unsigned long long hbn = hb;
-      if (j0n >= nb) {
-        jn = jj + 1;
-        j0n = 0;
-        if (jn < nvalid) {
-          b0n = readlane(bl, jn);
-          nbn = readlane(nl, jn);
-          if (MODE == kModeSUP) hbn = readlane64(wl, jn);
-        }
-      }
-      // past the batch's last block this reloads the current one (in bounds, unused)
-      load_blk(jn < nvalid ? b0n : b0, jn < nvalid ? nbn : nb, jn < nvalid ? j0n : j0, nxt);
+      const uint32_t rem = nb - j0;  // elements left in the pair's list (0 if it is empty)
+      // Branch-free advance: the next block is the pair's next 256 elements, or the first
+      // block of the next pair (past the batch's last pair, that pair's first block again:
+      // in bounds, unused).
+      const bool adv = j0 + 256u >= nb;
+      const uint32_t jn = jj + (adv ? 1u : 0u);
+      const uint32_t jl = min(jn, nvalid - 1u);
+      const uint32_t b0r = readlane(bl, jl), nbr = readlane(nl, jl);
+      const uint32_t b0n = adv ? b0r : b0, nbn = adv ? nbr : nb, j0n = adv ? 0u : j0 + 256u;
+      const unsigned long long hbn = MODE == kModeSUP ? (adv ? readlane64(wl, jl) : hb) : 0ull;
+      load_blk(b0n, nbn, j0n, nxt);
       unsigned long long hw = 0;  // SUP mode: this lane's hit word (lanes 0..3)
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
@@ -591,12 +588,14 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       nb = nbn;
       hb = hbn;
     };
-    while (true) {  // wave-uniform
+    // Two steps per trip (the register sets swap roles) and a single exit, at the bottom:
+    // with an exit after each step the compiler materialises undefined values for the
+    // loop's scalar state on every trip.  (The exit test reads jj through readfirstlane so
+    // that it is not folded into the guard of the second step.)
+    do {  // wave-uniform
       step(wa, wb);
-      if (jj >= nvalid) break;
-      step(wb, wa);
-      if (jj >= nvalid) break;
-    }
+      if (jj < nvalid) step(wb, wa);
+    } while ((uint32_t)__builtin_amdgcn_readfirstlane((int)jj) < nvalid);
     if (nvalid < 64) break;
   }
   // row_hits is the wave's total (uniform): counted once into acc, which is summed over
