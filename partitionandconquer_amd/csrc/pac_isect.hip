@@ -499,18 +499,18 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   uint32_t row_hits = 0;  // wave-uniform
   // Blocks of up to 4 x 64 elements of one pair's list N+(y), in pair order; the loads
   // of the next block are issued before the current one is probed.  The loads go through
-  // a buffer resource that covers exactly N+(y): one per-lane offset per block (the four
-  // sub-blocks are immediate offsets), no per-element address arithmetic, and lanes past
-  // the list's end read 0 from the hardware's range check -- their lookups are masked.
-  // The two register sets swap roles every step, so the waits cover only the block being
-  // probed.  (Unconditional loads: the wait for a block then counts exactly the loads
-  // issued after it, which keeps the next block's loads in flight.)
-  auto load_blk = [&](uint32_t b0, uint32_t nb, uint32_t j0, uint32_t *w) {
-    // the resource starts at the block (j0 < nb), so the per-lane offsets are
-    // loop-invariant: no vector register is rewritten between the pipelined loads
+  // a buffer resource that starts at the block and ends with N+(y): the per-lane offsets
+  // are loop-invariant (no vector register is rewritten between the pipelined loads, no
+  // per-element address arithmetic; the four sub-blocks are immediate offsets), and lanes
+  // past the list's end read 0 from the hardware's range check -- their lookups are
+  // masked.  The two register sets swap roles every step, so the waits cover only the
+  // block being probed.  (Unconditional loads: the wait for a block then counts exactly
+  // the loads issued after it, which keeps the next block's loads in flight.)
+  // The stream position is a scalar cursor: the block's address and the bytes left in the
+  // pair's list from there.
+  auto load_blk = [&](unsigned long long ptr, uint32_t nbytes, uint32_t *w) {
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(const uint32_t *)(G.oadj + (b0 + j0)), (short)0, (int)((nb - j0) * 4u),
-        kRawBufferFlags);
+        (void *)(uintptr_t)ptr, (short)0, (int)nbytes, kRawBufferFlags);
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u)
       w[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4u + 256u * u, 0, 0);
@@ -530,52 +530,62 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     }
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
     if (nvalid == 0) break;
-    uint32_t jj = 0, j0 = 0, b0 = readlane(bl, 0), nb = readlane(nl, 0), ph = 0;
+    // per pair (lane): the address of N+(y) and its length in bytes
+    const unsigned long long pa = (unsigned long long)(uintptr_t)(G.oadj + bl);
+    const uint32_t pn = nl * 4u;
+    uint32_t jj = 0, nrem = readlane(pn, 0), ph = 0;
+    unsigned long long ptr = readlane64(pa, 0);
     unsigned long long hb = MODE == kModeSUP ? readlane64(wl, 0) : 0ull;
     uint32_t wa[4], wb[4];
-    load_blk(b0, nb, 0, wa);
+    load_blk(ptr, nrem, wa);
     auto step = [&](const uint32_t *cur, uint32_t *nxt) {
-      const uint32_t rem = nb - j0;  // elements left in the pair's list (0 if it is empty)
-      // Branch-free advance: the next block is the pair's next 256 elements, or the first
-      // block of the next pair (past the batch's last pair, that pair's first block again:
-      // in bounds, unused).
-      const bool adv = j0 + 256u >= nb;
-      const uint32_t jn = jj + (adv ? 1u : 0u);
-      const uint32_t jl = min(jn, nvalid - 1u);
-      const uint32_t b0r = readlane(bl, jl), nbr = readlane(nl, jl);
-      const uint32_t b0n = adv ? b0r : b0, nbn = adv ? nbr : nb, j0n = adv ? 0u : j0 + 256u;
-      const unsigned long long hbn = MODE == kModeSUP ? (adv ? readlane64(wl, jl) : hb) : 0ull;
-      load_blk(b0n, nbn, j0n, nxt);
+      const uint32_t rem = nrem;  // bytes left in the pair's list (0 if it is empty)
+      // The next block: the pair's next 256 elements, or the first block of the next pair
+      // (past the batch's last pair, that pair's first block again: in bounds, unused).
+      uint32_t jn = jj, nremn = rem - 1024u;
+      unsigned long long ptrn = ptr + 1024u, hbn = hb + 4u;
+      if (rem <= 1024u) {  // wave-uniform
+        jn = jj + 1u;
+        const uint32_t jl = min(jn, nvalid - 1u);
+        ptrn = readlane64(pa, jl);
+        nremn = readlane(pn, jl);
+        if (MODE == kModeSUP) hbn = readlane64(wl, jl);
+      }
+      load_blk(ptrn, nremn, nxt);
       unsigned long long hw = 0;  // SUP mode: this lane's hit word (lanes 0..3)
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
-        if (64u * u >= rem) break;  // wave-uniform
-        const bool live = lane + 64u * u < rem;  // lanes past the end read 0
+        if (256u * u >= rem) break;  // wave-uniform
+        const bool live = lane * 4u + 256u * u < rem;  // lanes past the end read 0
         const Probe q = probe(cur[u]);
         const unsigned long long bal = (__ballot(q.e10) | __ballot(q.e11) | __ballot(q.e20) |
                                         __ballot(q.e21)) & __ballot(live);
+        if (MODE == kModeTC) {
+          row_hits += (uint32_t)__popcll(bal);  // only the row's total is kept
+          continue;
+        }
         ph += (uint32_t)__popcll(bal);
         if (MODE == kModeSUP && lane == u) hw = bal;
-        if (MODE != kModeTC && ((q.e10 | q.e11 | q.e20 | q.e21) & live)) {
+        if ((q.e10 | q.e11 | q.e20 | q.e21) & live) {
           // t(w) / edge r->w
           atomicAdd(&H.cnt16[(q.e10 | q.e11) ? q.b1 : q.b2], (q.e11 | q.e21) ? 0x10000u : 1u);
         }
       }
       if (MODE == kModeSUP) {
         // edges y->w: one hit word per 64 streamed elements, summed after the launch
-        const uint32_t nsub = min(4u, (rem + 63u) >> 6);
-        if (lane < nsub) G.hitw[hb + (j0 >> 6) + lane] = hw;
+        const uint32_t nsub = min(4u, (rem + 255u) >> 8);
+        if (lane < nsub) G.hitw[hb + lane] = hw;
       }
-      if (jn != jj) {
+      if (MODE != kModeTC && jn != jj) {
         // pair jj is done.  t(y) / the pair's edge: r->y adds to y's slot in the
         // table, y->r is credited to y (each pair is visited once)
         row_hits += ph;
-        if (MODE != kModeTC && ph != 0 && lane == 0) {
+        if (ph != 0 && lane == 0) {
           const uint32_t y = readlane(yl, jj), e = readlane(el, jj);
           if (e - a0 < nA) {
             uint32_t word, inc;
             find(y, word, inc);  // y = A[e - a0] is in the table
-            atomicAdd(&H.cnt16[word], inc * ph);  // ph <= nb <= CAP < 2^16
+            atomicAdd(&H.cnt16[word], inc * ph);  // ph <= nA <= CAP < 2^16
           } else {
             G.etail[e] = ph;
           }
@@ -583,9 +593,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         ph = 0;
       }
       jj = jn;
-      j0 = j0n;
-      b0 = b0n;
-      nb = nbn;
+      ptr = ptrn;
+      nrem = nremn;
       hb = hbn;
     };
     // Two steps per trip (the register sets swap roles) and a single exit, at the bottom:
