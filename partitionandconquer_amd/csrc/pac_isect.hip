@@ -385,17 +385,17 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
 // coalesced loads, software-pipelined one step ahead across the wave's pairs; the
 // pair's hit count is a ballot popcount held in scalar registers.
 constexpr uint32_t kMaxKicks = 48;
-// The two buckets of x among 2^lg (sh = 32 - lg), from one full-rate 24-bit multiply:
-// h = y * 0x9E3779 with y = x ^ (x >> 11) (folding bits 24.. into the multiplier's
-// 24-bit input); b1 is the top lg bits of h and b2 = b1 ^ d with d = bits [4, 4 + lg)
-// of h made odd (lg <= 12, so the two fields are disjoint; one bit-field extract with a
-// constant offset) -- so the two choices always differ (one bucket of each parity, the
-// two tables of classic cuckoo hashing).  (__umul24 is declared returning int: its
-// result is taken as unsigned before any shift, so the bucket indices stay below 2^lg.)
+// The two buckets of x among 2^lg (sh = 32 - lg), from one full-rate 24-bit multiply-add:
+// h = (x mod 2^24) * 0x9E3779 + (x >> 20) -- the added high bits keep IDs that differ only
+// above bit 23 apart in the second choice; b1 is the top lg bits of h and b2 = b1 ^ d with
+// d = bits [4, 4 + lg) of h made odd (lg <= 12, so the two fields are disjoint; one
+// bit-field extract with a constant offset) -- so the two choices always differ (one
+// bucket of each parity, the two tables of classic cuckoo hashing).  (__umul24 is
+// declared returning int: its result is taken as unsigned before any shift, so the
+// bucket indices stay below 2^lg.)
 __device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t lg, uint32_t &b1,
                                         uint32_t &b2) {
-  const uint32_t y = x ^ (x >> 11);
-  const uint32_t h = (uint32_t)__umul24(y, 0x9E3779u);
+  const uint32_t h = (uint32_t)__umul24(x, 0x9E3779u) + (x >> 20);
   b1 = h >> sh;
   b2 = b1 ^ (__builtin_amdgcn_ubfe(h, 4u, lg) | 1u);
 }

@@ -204,8 +204,7 @@ def _cuckoo_buckets(x, lg):
     """The two LDS hash-table buckets of key x among 2^lg (buckets() in pac_isect.hip),
     used here only to build inputs whose tables cannot be filled."""
     sh = 32 - lg
-    y = x ^ (x >> 11)
-    h = ((y & 0xFFFFFF) * 0x9E3779) & 0xFFFFFFFF
+    h = (((x & 0xFFFFFF) * 0x9E3779) + (x >> 20)) & 0xFFFFFFFF
     b1 = h >> sh
     return b1, b1 ^ (((h >> 4) & ((1 << lg) - 1)) | 1)
 
