@@ -19,8 +19,8 @@ bytes 4*W, W = sum over oriented u->v of (d+(u) + d+(v)), against 8 TB/s; `traff
 the kernel's measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, committed
 under profiles/) and `traffic_GBps` that traffic over the live kernel time.  The CPU
 baseline is the oracle's restatement of Graph::solve_triangle_counting_merge and
-Graph::solve_lcc_merge (Graph.cpp:355-368, 828-840) on a bounded random sample of the
-same graph's rows, extrapolated to all rows.
+Graph::solve_lcc_merge (Graph.cpp:355-368, 828-840) on a bounded sample of the same
+graph's rows, stratified by estimated row cost and extrapolated to all rows.
 """
 from __future__ import annotations
 
@@ -74,32 +74,76 @@ def timed_steps(ctx, g, dispatch, launch, steps, warmup):
     return wall, float(np.median(dev)), float(np.median(ker)) if any(ker) else 0.0
 
 
-def cpu_baseline(vl, el, budget_s=15.0, seed=0):
-    """Oracle TC-merge + LCC-merge on uniformly sampled rows, extrapolated by V/k."""
+def cpu_baseline(vl, el, budget_s=15.0, seed=0, strata=12):
+    """Oracle TC-merge + LCC-merge on a bounded sample of rows, extrapolated to all rows.
+
+    Row costs are heavy-tailed (a few hub rows hold much of the work), so a uniform
+    sample's estimate swings by 2-3x between runs.  Instead the rows are ranked by the
+    merge work the reference does for them, c(u) = d(u)^2 + sum over v in N(u) of d(v)
+    (the LCC merges of Graph.cpp:828-840 dominate), and cut into `strata` groups of equal
+    total c.  Each group gets an equal share of the time budget: random rows of it are
+    timed (all threads) until the share is spent, and the group's time is estimated as
+    its total c times the measured seconds per unit of c (a ratio estimate, exact when
+    time is proportional to c within the group).  A group whose cheapest row alone would
+    overrun its share several times (the few hub rows) is priced at the rate of the
+    group below it.  The samples are timed on all threads over random rows, so the result
+    is the work spread evenly over the threads: a lower bound on the reference's own run,
+    whose `omp parallel for` (static schedule, rows in ID order) leaves the hub rows at
+    the lowest IDs to one thread.  Returns (estimated seconds, rows timed, threads,
+    seconds spent)."""
     from oracle import oracle as O
     nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     g = O.CSR(vl, el)
     rng = np.random.default_rng(seed)
     V = g.V
-    k = 2000
-    spent = 0.0
+    vl64 = np.asarray(vl, dtype=np.int64)
+    d = np.diff(vl64)
+    dn = np.zeros(V, np.float64)
+    nz = d > 0
+    if len(el):
+        pref = np.concatenate([[0.0], np.cumsum(d[np.asarray(el, dtype=np.int64)], dtype=np.float64)])
+        dn = pref[vl64[1:]] - pref[vl64[:-1]]
+    cost = d.astype(np.float64) ** 2 + dn
+    order = np.argsort(cost, kind="stable")
+    order = order[nz[order]]  # isolated vertices cost nothing
+    if len(order) == 0:
+        return 0.0, 0, nt, 0.0
+    cum = np.cumsum(cost[order])
+    total = cum[-1]
+    cuts = np.searchsorted(cum, total * np.arange(1, strata) / strata, side="right")
+    groups = [grp for grp in np.split(order, cuts) if len(grp)]
+    share = budget_s / len(groups)
+    est_s = spent = 0.0
     rows_done = 0
-    t_rows = 0.0
-    while True:
-        rows = rng.choice(V, size=min(k, V), replace=False).astype(np.int32)
-        t0 = time.perf_counter()
-        O.tc_merge_rows(g, rows, nt)
-        O.lcc_merge_rows(g, rows, nt)
-        dt = time.perf_counter() - t0
-        spent += dt
-        rows_done += len(rows)
-        t_rows += dt
-        if spent >= budget_s or rows_done >= V:
-            break
-        per_row = t_rows / rows_done
-        k = int(min(V, max(k * 2, (budget_s - spent) / max(per_row, 1e-9))))
-    est_full_s = t_rows / rows_done * V
-    return est_full_s, rows_done, nt, spent
+    rate = None  # seconds per unit of c, from the last group timed
+    for grp in groups:
+        gc = float(cost[grp].sum())
+        if rate is not None and cost[grp].min() * rate > 4.0 * share:
+            est_s += gc * rate  # hub rows: priced at the rate of the group below
+            continue
+        perm = rng.permutation(grp)
+        t_g = c_g = 0.0
+        k, pos = max(1, min(len(perm), 4 * nt)), 0
+        while pos < len(perm):
+            rows = perm[pos:pos + k].astype(np.int32)
+            t0 = time.perf_counter()
+            O.tc_merge_rows(g, rows, nt)
+            O.lcc_merge_rows(g, rows, nt)
+            dt = time.perf_counter() - t0
+            t_g += dt
+            c_g += float(cost[rows].sum())
+            pos += len(rows)
+            if t_g >= share or pos >= len(perm):
+                break
+            per_c = t_g / max(c_g, 1.0)
+            nxt_c = (share - t_g) / max(per_c, 1e-15)
+            # grow the next chunk towards the rest of the share (keeps the threads busy)
+            k = int(max(k, min(len(perm) - pos, 2 * k, nxt_c / max(cost[perm[pos:]].mean(), 1.0))))
+        spent += t_g
+        rows_done += pos
+        rate = t_g / max(c_g, 1.0)
+        est_s += gc * rate
+    return est_s, rows_done, nt, spent
 
 
 def main():
@@ -219,9 +263,13 @@ def main():
         est_s, rows, nt, spent = cpu_baseline(vl, el, a.cpu_budget)
         res["cpu_baseline"] = {
             "value": round(m / est_s / 1e9, 6), "unit": "GTEPS", "cores": nt, "kind": "port",
-            "sample": f"{rows} uniformly random rows of this graph, TC-merge + LCC-merge "
+            "sample": f"{rows} rows of this graph drawn from 12 groups of equal estimated merge "
+                      f"work (d(u)^2 + sum of neighbour degrees), TC-merge + LCC-merge "
                       f"(oracle restatement of Graph.cpp:355-368, 828-840), {spent:.1f} s, "
-                      f"extrapolated by V/rows to {est_s:.1f} s for the whole graph"}
+                      f"priced per group by measured seconds per unit of work: {est_s:.1f} s "
+                      f"for the whole graph with the work spread evenly over {nt} threads (a "
+                      f"lower bound: the reference's static row schedule gives the hub rows "
+                      f"to one thread)"}
     print(json.dumps(res))
 
 

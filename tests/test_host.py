@@ -60,3 +60,20 @@ def test_bench_traffic_comes_from_the_committed_profile():
     t = bench.measured_traffic(24, 4096)
     assert t is not None and t > 1e11        # hundreds of GB per launch at scale 24
     assert bench.measured_traffic(22, 1024) is None
+
+
+def test_cpu_baseline_sampling():
+    """The CPU baseline's stratified sample: groups of equal estimated work, every group
+    priced, hub-free graphs timed in full when the budget allows; empty graphs cost 0."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from oracle import oracle as O
+    g = O.gen_rmat(12, 16, 0.57, 0.19, 0.19, seed=12)
+    est, rows, nt, spent = bench.cpu_baseline(g.vlist, g.elist, budget_s=0.5, seed=1)
+    assert est > 0 and 0 < rows <= g.V and nt >= 1 and spent > 0
+    # a budget far above the graph's cost times every non-isolated row once
+    nonempty = int((np.diff(g.vlist) > 0).sum())
+    est2, rows2, _, spent2 = bench.cpu_baseline(g.vlist, g.elist, budget_s=1e3, seed=2)
+    assert rows2 == nonempty and abs(est2 - spent2) <= 1e-9 * max(1.0, spent2)
+    empty = O.edges_to_csr(50, [], [])
+    assert bench.cpu_baseline(empty.vlist, empty.elist, budget_s=0.1)[0] == 0.0
