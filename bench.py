@@ -115,16 +115,29 @@ def cpu_baseline(vl, el, budget_s=15.0, seed=0, strata=12):
     share = budget_s / len(groups)
     est_s = spent = 0.0
     rows_done = 0
-    rate = None  # seconds per unit of c, from the last group timed
+    rate = None  # seconds per unit of c on all threads, from the last chunk timed
+
+    def predict(c_sum, c_max):
+        # a chunk runs no faster than its work spread evenly, nor than its largest row alone
+        return max(c_sum, c_max * nt) * rate
+
     for grp in groups:
         gc = float(cost[grp].sum())
-        if rate is not None and cost[grp].min() * rate > 4.0 * share:
+        if rate is not None and predict(cost[grp].min(), cost[grp].min()) > 4.0 * share:
             est_s += gc * rate  # hub rows: priced at the rate of the group below
             continue
         perm = rng.permutation(grp)
         t_g = c_g = 0.0
-        k, pos = max(1, min(len(perm), 4 * nt)), 0
-        while pos < len(perm):
+        pos = 0
+        while pos < len(perm) and t_g < share:
+            if rate is None:
+                k = min(len(perm) - pos, nt)
+            else:
+                # as many rows as the rest of the share is predicted to allow (at least one)
+                cs = np.cumsum(cost[perm[pos:]])
+                cm = np.maximum.accumulate(cost[perm[pos:]])
+                fits = np.nonzero(np.maximum(cs, cm * nt) * rate <= share - t_g)[0]
+                k = int(fits[-1]) + 1 if len(fits) else 1
             rows = perm[pos:pos + k].astype(np.int32)
             t0 = time.perf_counter()
             O.tc_merge_rows(g, rows, nt)
@@ -133,15 +146,9 @@ def cpu_baseline(vl, el, budget_s=15.0, seed=0, strata=12):
             t_g += dt
             c_g += float(cost[rows].sum())
             pos += len(rows)
-            if t_g >= share or pos >= len(perm):
-                break
-            per_c = t_g / max(c_g, 1.0)
-            nxt_c = (share - t_g) / max(per_c, 1e-15)
-            # grow the next chunk towards the rest of the share (keeps the threads busy)
-            k = int(max(k, min(len(perm) - pos, 2 * k, nxt_c / max(cost[perm[pos:]].mean(), 1.0))))
+            rate = t_g / max(c_g, 1.0)
         spent += t_g
         rows_done += pos
-        rate = t_g / max(c_g, 1.0)
         est_s += gc * rate
     return est_s, rows_done, nt, spent
 
