@@ -75,7 +75,9 @@ struct GraphDev {
   const uint32_t *rows;  // rows owning at least one pair, grouped by partition
   unsigned long long *total;  // T accumulator
   unsigned long long *tvert;  // per-vertex triangle counts (LCC mode: the row totals)
-  uint32_t *ehead;            // LCC mode: per oriented edge, triangles credited to its head
+  uint32_t *ehead;            // per oriented edge, triangles credited to its head (LCC mode:
+                              // stored at the edge's in-edge position, see inpos)
+  const uint32_t *inpos;      // per oriented edge, its position in the in-edge lists
   uint32_t *etail;            // LCC mode: per oriented edge, triangles credited to its tail
   uint32_t *sup;              // per-oriented-edge support (SUP mode)
   // SUP mode: the hit words of the streamed lists.  The pair with rank r among the pairs
@@ -127,6 +129,7 @@ struct Graph {
   DevBuf poff, ppar, pedge;
   // In-edges: for each vertex v, the oadj positions of the edges u->v (V+1 offsets).
   DevBuf inoff, inidx;
+  DevBuf inpos;  // m: inverse of inidx, the in-edge position of each oriented edge
   std::vector<uint32_t> h_hubs;  // vertices whose in-degree exceeds kHubIn
   std::vector<uint32_t> h_big;   // vertices whose degree exceeds kSmallDeg
   DevBuf hubs, big;

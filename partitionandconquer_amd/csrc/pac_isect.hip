@@ -54,6 +54,7 @@ struct GDev {
   const PAC_GLOBAL uint32_t *ooff, *oadj, *poff, *ppar, *pedge, *rows;
   PAC_GLOBAL unsigned long long *total, *tvert;
   PAC_GLOBAL uint32_t *ehead, *etail, *sup;
+  const PAC_GLOBAL uint32_t *inpos;
   const PAC_GLOBAL unsigned long long *pword;
   PAC_GLOBAL unsigned long long *hitw;
   uint32_t V, m, part_base;
@@ -72,6 +73,7 @@ __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
   g.ehead = (PAC_GLOBAL uint32_t *)d.ehead;
   g.etail = (PAC_GLOBAL uint32_t *)d.etail;
   g.sup = (PAC_GLOBAL uint32_t *)d.sup;
+  g.inpos = (const PAC_GLOBAL uint32_t *)d.inpos;
   g.pword = (const PAC_GLOBAL unsigned long long *)d.pword;
   g.hitw = (PAC_GLOBAL unsigned long long *)d.hitw;
   g.V = d.V;
@@ -155,6 +157,16 @@ __device__ __forceinline__ void credit_edge(PAC_GLOBAL uint32_t *p, uint32_t v, 
     *p = v;
   else
     gadd(p, v);
+}
+
+// Where the head credit of oriented edge e is kept.  LCC mode stores it at the edge's
+// position among its head's in-edges, so that the per-vertex sums after the launch read
+// every vertex's in-edge credits as one contiguous range (the scattered stores cost the
+// intersection kernel little: they are one per edge with a triangle, and the kernel is
+// not limited by memory); SUP mode keeps edge order for the per-edge epilogue.
+template <int MODE>
+__device__ __forceinline__ uint32_t head_slot(const GDev &G, uint32_t e) {
+  return MODE == kModeLCC ? G.inpos[e] : e;
 }
 
 __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
@@ -357,7 +369,7 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
   if (MODE != kModeTC) {
     for (uint32_t i = tid; i < nA; i += BS) {
       const uint32_t v = cnt[i];
-      if (v) credit_edge(&G.ehead[a0 + i], v, whole);
+      if (v) credit_edge(&G.ehead[head_slot<MODE>(G, a0 + i)], v, whole);
     }
     if (MODE == kModeLCC && tid == 0 && sm.rowtot) {
       if (whole)
@@ -430,9 +442,17 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     sm.homeless = 0;
     sm.rowtot = 0;
   }
+  // The thread's first element of A and (LCC mode) its in-edge position, loaded here and
+  // kept for the row's credits at the end, off its critical path (most rows have at most
+  // one element per thread).
+  uint32_t key0 = 0, pos0 = 0;
+  if (tid < nA) {
+    key0 = G.oadj[a0 + tid];
+    if (MODE == kModeLCC) pos0 = G.inpos[a0 + tid];
+  }
   __syncthreads();
   for (uint32_t i = tid; i < nA; i += BS) {
-    uint32_t x = G.oadj[a0 + i];
+    uint32_t x = i == tid ? key0 : G.oadj[a0 + i];
     uint32_t from = kEmpty;  // the bucket x was just evicted from (none yet)
     for (uint32_t kick = 0;; ++kick) {
       uint32_t b1, b2;
@@ -616,9 +636,13 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     // per edge r->A[i], in A order (coalesced): the credits of A[i]'s slot
     for (uint32_t i = tid; i < nA; i += BS) {
       uint32_t word, inc;
-      find(G.oadj[a0 + i], word, inc);
+      find(i == tid ? key0 : G.oadj[a0 + i], word, inc);
       const uint32_t v = (H.cnt16[word] >> (inc == 1u ? 0u : 16u)) & 0xFFFFu;
-      if (v) credit_edge(&G.ehead[a0 + i], v, whole);
+      if (v) {
+        const uint32_t slot =
+            MODE == kModeLCC && i == tid ? pos0 : head_slot<MODE>(G, a0 + i);
+        credit_edge(&G.ehead[slot], v, whole);
+      }
     }
   }
   if (MODE == kModeLCC && tid == 0 && sm.rowtot) {
@@ -650,7 +674,7 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
       const uint32_t i = lower_bound_u32(A, nA, w);
       if (i < nA && A[i] == w) {
         ++h;
-        if (MODE != kModeTC) gadd(&G.ehead[a0 + i], 1u);
+        if (MODE != kModeTC) gadd(&G.ehead[head_slot<MODE>(G, a0 + i)], 1u);
         if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);  // edge y->w
       }
     }
@@ -658,7 +682,7 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
       const uint32_t ch = wave_sum(h);
       if (lane == 0 && ch) {
         if (e - a0 < nA)
-          gadd(&G.ehead[e], ch);  // r->y
+          gadd(&G.ehead[head_slot<MODE>(G, e)], ch);  // r->y
         else
           G.etail[e] = ch;        // y->r, visited once
       }
@@ -834,14 +858,14 @@ __global__ void k_rowwork(const uint32_t *__restrict__ poff, const uint32_t *__r
 
 // ---- epilogues -----------------------------------------------------------------------
 // LCC per-vertex triangle counts: t(v) = (v's row totals, already in tvert)
-//   + sum over out-edges e of v of etail[e] + sum over in-edges e of v of ehead[e].
+//   + sum over out-edges e of v of etail[e] + sum over in-edges of v of their head
+//   credits, which LCC mode stores in in-edge order: ehead[inoff[v] .. inoff[v+1]).
 // Vertices of degree <= kSmallDeg (most of them) take one thread each; the others are
 // listed in `big` and take one wave each; a hub's in-edges beyond the first kHubIn are
 // left to k_vertex_hub, which splits them over several workgroups.
 __global__ void k_vertex_small(const uint32_t *__restrict__ off, const uint32_t *__restrict__ ooff,
                                const uint32_t *__restrict__ etail,
                                const uint32_t *__restrict__ inoff,
-                               const uint32_t *__restrict__ inidx,
                                const uint32_t *__restrict__ ehead, uint32_t V,
                                unsigned long long *__restrict__ tvert) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
@@ -849,14 +873,14 @@ __global__ void k_vertex_small(const uint32_t *__restrict__ off, const uint32_t 
     if (off[v + 1] - off[v] > kSmallDeg) continue;
     uint32_t s = 0;  // at most kSmallDeg edges, each credited with fewer than d(v) triangles
     for (uint32_t e = ooff[v]; e < ooff[v + 1]; ++e) s += etail[e];
-    for (uint32_t k = inoff[v]; k < inoff[v + 1]; ++k) s += ehead[inidx[k]];
+    for (uint32_t k = inoff[v]; k < inoff[v + 1]; ++k) s += ehead[k];
     if (s) tvert[v] += s;
   }
 }
 
 __global__ void k_vertex_big(const uint32_t *__restrict__ big, uint32_t nbig,
                              const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ etail,
-                             const uint32_t *__restrict__ inoff, const uint32_t *__restrict__ inidx,
+                             const uint32_t *__restrict__ inoff,
                              const uint32_t *__restrict__ ehead,
                              unsigned long long *__restrict__ tvert) {
   const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
@@ -869,7 +893,7 @@ __global__ void k_vertex_big(const uint32_t *__restrict__ big, uint32_t nbig,
     for (uint32_t k = k0 + lane; k < k1; k += 256) {
       uint32_t x[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) x[u] = k + 64u * u < k1 ? ehead[inidx[k + 64u * u]] : 0u;
+      for (int u = 0; u < 4; ++u) x[u] = k + 64u * u < k1 ? ehead[k + 64u * u] : 0u;
 #pragma unroll
       for (int u = 0; u < 4; ++u) s += x[u];
     }
@@ -880,7 +904,6 @@ __global__ void k_vertex_big(const uint32_t *__restrict__ big, uint32_t nbig,
 
 // The in-edges of hub h past its first kHubIn, in slices of kHubIn per workgroup.
 __global__ void k_vertex_hub(const uint32_t *__restrict__ hubs, const uint32_t *__restrict__ inoff,
-                             const uint32_t *__restrict__ inidx,
                              const uint32_t *__restrict__ ehead,
                              unsigned long long *__restrict__ tvert) {
   const uint32_t h = hubs[blockIdx.x];
@@ -889,7 +912,7 @@ __global__ void k_vertex_hub(const uint32_t *__restrict__ hubs, const uint32_t *
   for (uint64_t k0 = kbeg + (uint64_t)kHubIn * (1u + blockIdx.y); k0 < kend;
        k0 += (uint64_t)kHubIn * gridDim.y) {
     const uint64_t k1 = k0 + kHubIn < kend ? k0 + kHubIn : kend;
-    for (uint64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) s += ehead[inidx[k]];
+    for (uint64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) s += ehead[k];
   }
   s = wave_sum64(s);
   __shared__ unsigned long long ws[16];
@@ -1280,6 +1303,7 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   d.ehead = g->ehead.as<uint32_t>();
   d.etail = g->etail.as<uint32_t>();
   d.sup = g->sup.as<uint32_t>();
+  d.inpos = g->inpos.as<const uint32_t>();
   d.pword = g->pword.as<const unsigned long long>();
   d.hitw = g->hitw.as<unsigned long long>();
   d.V = g->V;
@@ -1516,16 +1540,14 @@ int vertex_counts(Ctx *ctx, Graph *g) {
   hipLaunchKernelGGL(k_vertex_small, dim3(grid_for(g->V, 256)), dim3(256), 0, ctx->stream,
                      g->off.as<const uint32_t>(), g->ooff.as<const uint32_t>(),
                      g->etail.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
-                     g->inidx.as<const uint32_t>(), g->ehead.as<const uint32_t>(), g->V,
-                     g->tvert.as<unsigned long long>());
+                     g->ehead.as<const uint32_t>(), g->V, g->tvert.as<unsigned long long>());
   PAC_CHECK_HIP(hipGetLastError());
   const uint32_t nbig = (uint32_t)g->h_big.size();
   if (nbig) {
     hipLaunchKernelGGL(k_vertex_big, dim3(grid_for(nbig, 4)), dim3(256), 0, ctx->stream,
                        g->big.as<const uint32_t>(), nbig, g->ooff.as<const uint32_t>(),
                        g->etail.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
-                       g->inidx.as<const uint32_t>(), g->ehead.as<const uint32_t>(),
-                       g->tvert.as<unsigned long long>());
+                       g->ehead.as<const uint32_t>(), g->tvert.as<unsigned long long>());
     PAC_CHECK_HIP(hipGetLastError());
   }
   if (!g->h_hubs.empty()) {
@@ -1534,8 +1556,7 @@ int vertex_counts(Ctx *ctx, Graph *g) {
     if (slices > 4096) slices = 4096;  // the kernel strides over further slices
     hipLaunchKernelGGL(k_vertex_hub, dim3((uint32_t)g->h_hubs.size(), (uint32_t)slices), dim3(256),
                        0, ctx->stream, g->hubs.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
-                       g->inidx.as<const uint32_t>(), g->ehead.as<const uint32_t>(),
-                       g->tvert.as<unsigned long long>());
+                       g->ehead.as<const uint32_t>(), g->tvert.as<unsigned long long>());
     PAC_CHECK_HIP(hipGetLastError());
   }
   return PAC_OK;
