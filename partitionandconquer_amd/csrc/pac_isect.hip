@@ -412,20 +412,39 @@ __device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t lg, ui
   b2 = b1 ^ (__builtin_amdgcn_ubfe(h, 4u, lg) | 1u);
 }
 
-template <int MODE, int WAVES, int CAP>
+// SOLO: one wave does the whole row on its own share (1/WAVES) of the table space, with
+// wave-level ordering instead of workgroup barriers, so the workgroup's waves work on
+// different small rows at once (the caller keeps the row within that share and to one
+// batch of pairs).  Otherwise the workgroup's waves share the row, taking its pairs in turn.
+template <int MODE, int WAVES, int CAP, bool SOLO = false>
 __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_lo,
                                          uint32_t c_hi, Smem<CAP, MODE, WAVES> &sm,
                                          unsigned long long &acc) {
-  constexpr uint32_t BS = 64u * WAVES;
-  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  using S = Smem<CAP, MODE, WAVES>;
+  constexpr uint32_t BS = SOLO ? 64u : 64u * WAVES;  // threads on the row
+  constexpr uint32_t PS = SOLO ? 1u : (uint32_t)WAVES;  // pair stride between its waves
+  const uint32_t tid = SOLO ? (threadIdx.x & 63u) : threadIdx.x;  // rank on the row
+  const uint32_t lane = threadIdx.x & 63u, wave = SOLO ? 0u : threadIdx.x >> 6;
+  auto sync = [&]() {
+    if (SOLO) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
+  };
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
   const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
   auto &H = sm.u.h;
+  uint32_t *const tab = H.tab + (SOLO ? (threadIdx.x >> 6) * (S::kSlots / WAVES) : 0u);
+  uint32_t *const cnt16 =
+      MODE == kModeTC ? H.cnt16 : H.cnt16 + (SOLO ? (threadIdx.x >> 6) * (S::kSlots / WAVES / 2) : 0u);
   if (c_hi > np) c_hi = np;
   const bool whole = c_lo == 0 && c_hi == np;
   // The wave's first batch of pairs: partners and edges are loaded before the table is
   // built, the bounds of their lists after the insertions, so both latencies overlap it.
-  uint32_t kl = c_lo + wave + (uint32_t)WAVES * lane, yl = 0, el = 0, bl = 0, nl = 0;
+  uint32_t kl = c_lo + wave + PS * lane, yl = 0, el = 0, bl = 0, nl = 0;
   unsigned long long wl = 0;  // SUP mode: the pair's first hit word
   if (kl < c_hi) {
     yl = G.ppar[p0 + kl];
@@ -435,13 +454,14 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   uint32_t lg = 4;
   while ((1u << lg) < nA) ++lg;  // 2^lg >= nA: load factor <= 1/2
   const uint32_t nslot = 2u << lg, sh = 32u - lg;
-  for (uint32_t i = tid; i < nslot; i += BS) H.tab[i] = kEmpty;
+  for (uint32_t i = tid; i < nslot; i += BS) tab[i] = kEmpty;
   if (MODE != kModeTC)
-    for (uint32_t i = tid; i < nslot / 2; i += BS) H.cnt16[i] = 0;
-  if (tid == 0) {
+    for (uint32_t i = tid; i < nslot / 2; i += BS) cnt16[i] = 0;
+  if (!SOLO && tid == 0) {
     sm.homeless = 0;
     sm.rowtot = 0;
   }
+  bool lost = false;  // SOLO: this lane left a key homeless
   // The thread's first element of A and (LCC mode) its in-edge position, loaded here and
   // kept for the row's credits at the end, off its critical path (most rows have at most
   // one element per thread).
@@ -450,24 +470,27 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     key0 = G.oadj[a0 + tid];
     if (MODE == kModeLCC) pos0 = G.inpos[a0 + tid];
   }
-  __syncthreads();
+  sync();
   for (uint32_t i = tid; i < nA; i += BS) {
     uint32_t x = i == tid ? key0 : G.oadj[a0 + i];
     uint32_t from = kEmpty;  // the bucket x was just evicted from (none yet)
     for (uint32_t kick = 0;; ++kick) {
       uint32_t b1, b2;
       buckets(x, sh, lg, b1, b2);
-      if (atomicCAS(&H.tab[2u * b1], kEmpty, x) == kEmpty) break;
-      if (atomicCAS(&H.tab[2u * b1 + 1u], kEmpty, x) == kEmpty) break;
-      if (atomicCAS(&H.tab[2u * b2], kEmpty, x) == kEmpty) break;
-      if (atomicCAS(&H.tab[2u * b2 + 1u], kEmpty, x) == kEmpty) break;
+      if (atomicCAS(&tab[2u * b1], kEmpty, x) == kEmpty) break;
+      if (atomicCAS(&tab[2u * b1 + 1u], kEmpty, x) == kEmpty) break;
+      if (atomicCAS(&tab[2u * b2], kEmpty, x) == kEmpty) break;
+      if (atomicCAS(&tab[2u * b2 + 1u], kEmpty, x) == kEmpty) break;
       if (kick == kMaxKicks) {
-        atomicAdd(&sm.homeless, 1u);
+        if (SOLO)
+          lost = true;
+        else
+          atomicAdd(&sm.homeless, 1u);
         break;
       }
       // evict from the bucket x did not just come from, alternating slots every two kicks
       const uint32_t vb = from == b1 ? b2 : b1;
-      x = atomicExch(&H.tab[2u * vb + ((kick >> 1) & 1u)], x);
+      x = atomicExch(&tab[2u * vb + ((kick >> 1) & 1u)], x);
       from = vb;
       if (x == kEmpty) break;
     }
@@ -476,8 +499,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     bl = G.ooff[yl];
     nl = G.ooff[yl + 1] - bl;
   }
-  __syncthreads();
-  if (sm.homeless != 0) return false;  // workgroup-uniform
+  sync();
+  if (SOLO ? __ballot(lost) != 0ull : sm.homeless != 0) return false;  // uniform
   // Lookup of w.  A bucket's two slots share one word of the packed 16-bit counters, so
   // a hit is reported as that word (the bucket) and the increment for w's half of it
   // (1 or 1 << 16); word and increment are only meaningful on a hit.  Both buckets are
@@ -488,9 +511,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   auto find = [&](uint32_t w, uint32_t &word, uint32_t &inc) -> bool {
     uint32_t b1, b2;
     buckets(w, sh, lg, b1, b2);
-    const uint2 q1 = reinterpret_cast<const uint2 *>(H.tab)[b1];
+    const uint2 q1 = reinterpret_cast<const uint2 *>(tab)[b1];
     const uint32_t k10 = q1.x, k11 = q1.y;
-    const uint2 q2 = reinterpret_cast<const uint2 *>(H.tab)[b2];
+    const uint2 q2 = reinterpret_cast<const uint2 *>(tab)[b2];
     const uint32_t k20 = q2.x, k21 = q2.y;
     // bitwise | (no short-circuit): all four slots are read up front, no branches
     const bool in1 = (k10 == w) | (k11 == w), hi = (k11 == w) | (k21 == w);
@@ -508,8 +531,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   auto probe = [&](uint32_t w) -> Probe {
     Probe q;
     buckets(w, sh, lg, q.b1, q.b2);
-    const uint2 q1 = reinterpret_cast<const uint2 *>(H.tab)[q.b1];
-    const uint2 q2 = reinterpret_cast<const uint2 *>(H.tab)[q.b2];
+    const uint2 q1 = reinterpret_cast<const uint2 *>(tab)[q.b1];
+    const uint2 q2 = reinterpret_cast<const uint2 *>(tab)[q.b2];
     q.e10 = q1.x == w;
     q.e11 = q1.y == w;
     q.e20 = q2.x == w;
@@ -537,7 +560,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   };
   for (uint32_t t0 = 0;; t0 += 64) {
     if (t0 != 0) {
-      kl = c_lo + wave + (uint32_t)WAVES * (t0 + lane);
+      kl = c_lo + wave + PS * (t0 + lane);
       yl = el = bl = nl = 0;
       wl = 0;
       if (kl < c_hi) {
@@ -588,7 +611,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         if (MODE == kModeSUP && lane == u) hw = bal;
         if ((q.e10 | q.e11 | q.e20 | q.e21) & live) {
           // t(w) / edge r->w
-          atomicAdd(&H.cnt16[(q.e10 | q.e11) ? q.b1 : q.b2], (q.e11 | q.e21) ? 0x10000u : 1u);
+          atomicAdd(&cnt16[(q.e10 | q.e11) ? q.b1 : q.b2], (q.e11 | q.e21) ? 0x10000u : 1u);
         }
       }
       if (MODE == kModeSUP) {
@@ -605,7 +628,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
           if (e - a0 < nA) {
             uint32_t word, inc;
             find(y, word, inc);  // y = A[e - a0] is in the table
-            atomicAdd(&H.cnt16[word], inc * ph);  // ph <= nA <= CAP < 2^16
+            atomicAdd(&cnt16[word], inc * ph);  // ph <= nA <= CAP < 2^16
           } else {
             G.etail[e] = ph;
           }
@@ -630,14 +653,14 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // row_hits is the wave's total (uniform): counted once into acc, which is summed over
   // the wave's lanes at the end, so lane 0 carries it
   if (lane == 0) acc += row_hits;
-  if (MODE == kModeLCC && lane == 0 && row_hits) atomicAdd(&sm.rowtot, row_hits);
-  __syncthreads();
+  if (MODE == kModeLCC && !SOLO && lane == 0 && row_hits) atomicAdd(&sm.rowtot, row_hits);
+  sync();
   if (MODE != kModeTC) {
     // per edge r->A[i], in A order (coalesced): the credits of A[i]'s slot
     for (uint32_t i = tid; i < nA; i += BS) {
       uint32_t word, inc;
       find(i == tid ? key0 : G.oadj[a0 + i], word, inc);
-      const uint32_t v = (H.cnt16[word] >> (inc == 1u ? 0u : 16u)) & 0xFFFFu;
+      const uint32_t v = (cnt16[word] >> (inc == 1u ? 0u : 16u)) & 0xFFFFu;
       if (v) {
         const uint32_t slot =
             MODE == kModeLCC && i == tid ? pos0 : head_slot<MODE>(G, a0 + i);
@@ -645,27 +668,31 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       }
     }
   }
-  if (MODE == kModeLCC && tid == 0 && sm.rowtot) {
-    if (whole)
-      G.tvert[r] = sm.rowtot;
-    else
-      gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
+  if (MODE == kModeLCC) {
+    const uint32_t rt = SOLO ? row_hits : sm.rowtot;  // the row's triangles (uniform)
+    if (tid == 0 && rt) {
+      if (whole)
+        G.tvert[r] = rt;
+      else
+        gadd(&G.tvert[r], (unsigned long long)rt);
+    }
   }
-  __syncthreads();
+  sync();
   return true;
 }
 
 // ---- one row whose N+(r) exceeds the LDS capacity: lookups in HBM ------------------
-template <int MODE, int WAVES>
+// (SOLO: one wave does the whole row, as a fallback of the one-wave hash path.)
+template <int MODE, int WAVES, bool SOLO = false>
 __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo,
                                         uint32_t c_hi, unsigned long long &acc) {
-  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = SOLO ? 0u : tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
   const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
   const PAC_GLOBAL uint32_t *A = G.oadj + a0;
   if (c_hi > np) c_hi = np;
   uint32_t row_hits = 0;
-  for_each_pair<WAVES, MODE>(G, p0, c_lo, c_hi, wave, lane,
+  for_each_pair<SOLO ? 1 : WAVES, MODE>(G, p0, c_lo, c_hi, wave, lane,
                              [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb, uint32_t p) {
     uint32_t h = 0;
     if (MODE == kModeSUP) clear_hit_words(G, nb, p, lane);
@@ -694,6 +721,16 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
     const uint32_t wh = wave_sum(row_hits);
     if (lane == 0 && wh) gadd(&G.tvert[r], (unsigned long long)wh);
   }
+}
+
+// A row one wave can do alone: its pairs fit one batch and its hash table one wave's
+// share (1/WAVES) of the table space.
+template <int CAP, int MODE, int WAVES>
+__device__ __forceinline__ bool solo_row(const GDev &G, uint32_t r) {
+  const uint32_t nA = G.ooff[r + 1] - G.ooff[r], np = G.poff[r + 1] - G.poff[r];
+  uint32_t lg = 4;
+  while ((1u << lg) < nA) ++lg;
+  return np <= 64u && (2u << lg) <= Smem<CAP, MODE, WAVES>::kSlots / WAVES;
 }
 
 // One row (or a pair range of one) with the given strategy.
@@ -771,8 +808,23 @@ __global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__res
     const uint8_t st = strat[G.part_base + w.part];
     const bool single = (w.hi == w.lo + 1);
     const uint32_t c_lo = single ? w.c_lo : 0u, c_hi = single ? w.c_hi : kAllC;
+    // Hash strategy: the item's rows that one wave can do alone go first, one per wave,
+    // with no workgroup barrier between them (a small row is a chain of dependent loads
+    // that a whole workgroup would wait out together); a row whose table cannot be built
+    // falls back to the HBM lookups.  Then the workgroup does the others one by one.
+    const bool solo = !single && st == kBitmap;
+    if (solo) {
+      for (uint32_t q = w.lo + (threadIdx.x >> 6); q < w.hi; q += WAVES) {
+        const uint32_t r = G.rows[q];
+        if (solo_row<CAP, MODE, WAVES>(G, r) &&
+            !row_hash<MODE, WAVES, CAP, true>(G, r, 0u, kAllC, s, acc))
+          row_hbm<MODE, WAVES, true>(G, r, 0u, kAllC, acc);
+      }
+      __syncthreads();
+    }
     for (uint32_t q = w.lo; q < w.hi; ++q) {
       const uint32_t r = G.rows[q];
+      if (solo && solo_row<CAP, MODE, WAVES>(G, r)) continue;  // uniform: done above
       if (DEFER && G.ooff[r + 1] - G.ooff[r] > (uint32_t)CAP) {  // uniform
         __syncthreads();  // every thread has read the previous defer_slot
         if (threadIdx.x == 0) {

@@ -211,13 +211,15 @@ def _cuckoo_buckets(x, lg):
 
 def test_hash_table_build_failure_falls_back(ctx, orc):
     """Rows whose out-lists hold six keys sharing both buckets (four slots between them):
-    two keys stay homeless, the hash build gives up and the row is counted by the
-    sorted-list path.  Vertex r joins the 16 keys of K, which form a clique, so every row
-    here has 2^lg = 16 buckets; a second, larger row (lg = 6) does the same among 40 keys.
-    Counts, LCC and Jaccard must still equal the oracle's."""
+    two keys stay homeless, the hash build gives up and the row is counted by a fallback.
+    Vertex r joins the n keys of K, which form a clique, so r owns n pairs: rows of 16 and
+    40 pairs go one per wave (fallback: HBM lookups by that wave), the row of 80 pairs to
+    the whole workgroup (fallback: the sorted-list path).  Counts, LCC and Jaccard must
+    still equal the oracle's."""
     rng = np.random.default_rng(7)
     src, dst = [], []
-    for r, n, lg, lo, hi in ((5, 16, 4, 1000, 1 << 19), (7, 40, 6, 1 << 19, 1 << 20)):
+    for r, n, lg, lo, hi in ((5, 16, 4, 1000, 1 << 19), (7, 40, 6, 1 << 19, 1 << 20),
+                             (9, 80, 7, 1 << 20, 3 << 19)):
         by = {}
         for x in range(lo, hi):
             by.setdefault(_cuckoo_buckets(x, lg), []).append(x)
@@ -232,10 +234,10 @@ def test_hash_table_build_failure_falls_back(ctx, orc):
             for b in K[i + 1:]:
                 src.append(a)
                 dst.append(b)
-    ref = orc.edges_to_csr(1 << 20, src, dst)
+    ref = orc.edges_to_csr(3 << 19, src, dst)
     g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
     T = check_all(ctx, g, ref, orc, P=4, strategies=[pac.FORCE_BITMAP, pac.DISPATCH_FIXED])
-    assert T == 560 + 120 + 9880 + 780
+    assert T == sum(math.comb(n, 3) + math.comb(n, 2) for n in (16, 40, 80))
     for l in LAUNCHES:
         l32, l64, t = g.lcc(pac.FORCE_BITMAP, l)
         assert t == T and bits_equal(l64, orc.lcc_merge(ref, NT))
