@@ -880,6 +880,14 @@ __global__ void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__
     dst[i] = src[i];
 }
 
+// out[i] = table[idx[i]]
+__global__ void k_gather_u32(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ table,
+                             uint64_t n, uint32_t *__restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = table[idx[i]];
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -1001,8 +1009,9 @@ __global__ void k_lcc(const unsigned long long *__restrict__ tvert,
 __global__ void __launch_bounds__(256)
 k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
           const uint32_t *__restrict__ ehead, const uint32_t *__restrict__ etail,
-          uint32_t *__restrict__ sup, const uint32_t *__restrict__ deg, uint32_t V, int32_t *src,
-          int32_t *dst, uint32_t *inter, uint32_t *uni, float *ratio) {
+          uint32_t *__restrict__ sup, const uint32_t *__restrict__ deg,
+          const uint32_t *__restrict__ hdeg, uint32_t V, int32_t *src, int32_t *dst,
+          uint32_t *inter, uint32_t *uni, float *ratio) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wpb = blockDim.x >> 6;
   for (uint64_t v0 = ((uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6)) * 64u; v0 < V;
@@ -1012,25 +1021,44 @@ k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
     const uint32_t du = vi < V ? deg[vi] : 0u;
     const uint32_t E0 = readlane(a, 0);
     const uint32_t E1 = ooff[v0 + 64u < V ? v0 + 64u : V];
-    for (uint32_t eb = E0; eb < E1; eb += 64) {  // wave-uniform
-      const uint32_t e = eb + lane;
-      uint32_t t = 0;  // last lane whose vertex starts at or before e
+    // Four chunks of 64 edges per trip, each step over all four before the next, so that
+    // a wave has four loads of each kind in flight instead of one.
+    for (uint32_t eb = E0; eb < E1; eb += 256) {  // wave-uniform
+      uint32_t e[4], t[4], v[4], c[4], dv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        e[u] = eb + 64u * u + lane;
+        t[u] = 0;  // last lane whose vertex starts at or before e
+      }
 #pragma unroll
       for (uint32_t step = 32; step; step >>= 1) {
-        const uint32_t at = (uint32_t)__shfl((int)a, (int)(t + step), 64);
-        t = at <= e ? t + step : t;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t at = (uint32_t)__shfl((int)a, (int)(t[u] + step), 64);
+          t[u] = at <= e[u] ? t[u] + step : t[u];
+        }
       }
-      const uint32_t dt = (uint32_t)__shfl((int)du, (int)t, 64);
-      if (e < E1) {
-        const uint32_t v = oadj[e];
-        const uint32_t c = ehead[e] + etail[e] + sup[e];
-        sup[e] = c;
-        const uint32_t un = dt + deg[v] - c;
-        if (src) src[e] = (int32_t)(v0 + t);
-        if (dst) dst[e] = (int32_t)v;
-        if (inter) inter[e] = c;
-        if (uni) uni[e] = un;
-        if (ratio) ratio[e] = (float)((double)c / (double)un);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        v[u] = c[u] = dv[u] = 0;
+        if (e[u] < E1) {
+          if (dst) v[u] = oadj[e[u]];
+          c[u] = ehead[e[u]] + etail[e[u]] + sup[e[u]];
+          dv[u] = hdeg[e[u]];  // d(v), in edge order (no gather)
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t dt = (uint32_t)__shfl((int)du, (int)t[u], 64);
+        if (e[u] < E1) {
+          sup[e[u]] = c[u];
+          const uint32_t un = dt + dv[u] - c[u];
+          if (src) src[e[u]] = (int32_t)(v0 + t[u]);
+          if (dst) dst[e[u]] = (int32_t)v[u];
+          if (inter) inter[e[u]] = c[u];
+          if (uni) uni[e[u]] = un;
+          if (ratio) ratio[e[u]] = (float)((double)c[u] / (double)un);
+        }
       }
     }
   }
@@ -1647,11 +1675,20 @@ int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64) {
 int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *inter,
                      uint32_t *uni, float *ratio) {
   if (g->V == 0) return PAC_OK;
+  if (!g->hdeg.p && g->m) {
+    // the head degrees in edge order, once per graph: a gather of deg over oadj that the
+    // per-solve epilogue would otherwise repeat (a cache line per 4-byte read)
+    PAC_TRY(g->hdeg.reserve(sizeof(uint32_t) * g->m));
+    hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(g->m, 256)), dim3(256), 0, ctx->stream,
+                       g->oadj.as<const uint32_t>(), g->deg.as<const uint32_t>(), g->m,
+                       g->hdeg.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_jaccard, dim3(grid_for(g->V, 4 * 64)), dim3(256), 0, ctx->stream,
                      g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
                      g->ehead.as<const uint32_t>(), g->etail.as<const uint32_t>(),
-                     g->sup.as<uint32_t>(), g->deg.as<const uint32_t>(), g->V, src, dst, inter,
-                     uni, ratio);
+                     g->sup.as<uint32_t>(), g->deg.as<const uint32_t>(),
+                     g->hdeg.as<const uint32_t>(), g->V, src, dst, inter, uni, ratio);
   PAC_CHECK_HIP(hipGetLastError());
   return PAC_OK;
 }
