@@ -83,8 +83,9 @@ def cpu_baseline(vl, el, budget_s=15.0, seed=0, strata=12):
     (the LCC merges of Graph.cpp:828-840 dominate), and cut into `strata` groups of equal
     total c.  Each group gets an equal share of the time budget: random rows of it are
     timed (all threads) until the share is spent, and the group's time is estimated as
-    its total c times the measured seconds per unit of c (a ratio estimate, exact when
-    time is proportional to c within the group).  A group whose cheapest row alone would
+    its total c times the measured single-thread seconds per unit of c (a ratio estimate,
+    exact when time is proportional to c within the group; a chunk of k < threads rows
+    counts as k threads' worth of time), divided by the threads.  A group whose cheapest row alone would
     overrun its share several times (the few hub rows) is priced at the rate of the
     group below it.  The samples are timed on all threads over random rows, so the result
     is the work spread evenly over the threads: a lower bound on the reference's own run,
@@ -115,19 +116,19 @@ def cpu_baseline(vl, el, budget_s=15.0, seed=0, strata=12):
     share = budget_s / len(groups)
     est_s = spent = 0.0
     rows_done = 0
-    rate = None  # seconds per unit of c on all threads, from the last chunk timed
+    rate = None  # single-thread seconds per unit of c, from the last group timed
 
     def predict(c_sum, c_max):
         # a chunk runs no faster than its work spread evenly, nor than its largest row alone
-        return max(c_sum, c_max * nt) * rate
+        return max(c_sum / nt, c_max) * rate
 
     for grp in groups:
         gc = float(cost[grp].sum())
         if rate is not None and predict(cost[grp].min(), cost[grp].min()) > 4.0 * share:
-            est_s += gc * rate  # hub rows: priced at the rate of the group below
+            est_s += gc * rate / nt  # hub rows: priced at the rate of the group below
             continue
         perm = rng.permutation(grp)
-        t_g = c_g = 0.0
+        t_g = w_g = c_g = 0.0  # wall seconds, single-thread seconds, work of the group
         pos = 0
         while pos < len(perm) and t_g < share:
             if rate is None:
@@ -136,7 +137,7 @@ def cpu_baseline(vl, el, budget_s=15.0, seed=0, strata=12):
                 # as many rows as the rest of the share is predicted to allow (at least one)
                 cs = np.cumsum(cost[perm[pos:]])
                 cm = np.maximum.accumulate(cost[perm[pos:]])
-                fits = np.nonzero(np.maximum(cs, cm * nt) * rate <= share - t_g)[0]
+                fits = np.nonzero(np.maximum(cs / nt, cm) * rate <= share - t_g)[0]
                 k = int(fits[-1]) + 1 if len(fits) else 1
             rows = perm[pos:pos + k].astype(np.int32)
             t0 = time.perf_counter()
@@ -144,12 +145,13 @@ def cpu_baseline(vl, el, budget_s=15.0, seed=0, strata=12):
             O.lcc_merge_rows(g, rows, nt)
             dt = time.perf_counter() - t0
             t_g += dt
+            w_g += dt * min(len(rows), nt)  # a chunk of fewer rows than threads uses fewer
             c_g += float(cost[rows].sum())
             pos += len(rows)
-            rate = t_g / max(c_g, 1.0)
+            rate = w_g / max(c_g, 1.0)
         spent += t_g
         rows_done += pos
-        est_s += gc * rate
+        est_s += gc * rate / nt
     return est_s, rows_done, nt, spent
 
 

@@ -71,9 +71,10 @@ def test_cpu_baseline_sampling():
     g = O.gen_rmat(12, 16, 0.57, 0.19, 0.19, seed=12)
     est, rows, nt, spent = bench.cpu_baseline(g.vlist, g.elist, budget_s=0.5, seed=1)
     assert est > 0 and 0 < rows <= g.V and nt >= 1 and spent > 0
-    # a budget far above the graph's cost times every non-isolated row once
+    # a budget far above the graph's cost times every non-isolated row once; the estimate
+    # (the work spread evenly over the threads) is then at most the time spent
     nonempty = int((np.diff(g.vlist) > 0).sum())
     est2, rows2, _, spent2 = bench.cpu_baseline(g.vlist, g.elist, budget_s=1e3, seed=2)
-    assert rows2 == nonempty and abs(est2 - spent2) <= 1e-9 * max(1.0, spent2)
+    assert rows2 == nonempty and 0 < est2 <= spent2 * (1 + 1e-9)
     empty = O.edges_to_csr(50, [], [])
     assert bench.cpu_baseline(empty.vlist, empty.elist, budget_s=0.1)[0] == 0.0
