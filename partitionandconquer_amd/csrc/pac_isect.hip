@@ -391,8 +391,9 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
 // other bucket is what keeps two keys from trading one slot back and forth: on R-MAT
 // out-lists it leaves about 2 rows in 100 000 with a key still homeless after
 // kMaxKicks evictions, where alternating between the two buckets by kick number left
-// 2 in 1 000, most of them rows of 129-512 keys).  Such a row is done by the
-// sorted-list path instead (exact either way), so lookups never scan an overflow area.
+// 2 in 1 000, most of them rows of 129-512 keys).  Such a row is done by another path
+// instead (exact either way: the sorted lists for a workgroup's row, HBM lookups for a
+// one-wave row), so lookups never scan an overflow area.
 // Each pair's list N+(y) is streamed by one wave, 4 x 64 elements per step with
 // coalesced loads, software-pipelined one step ahead across the wave's pairs; the
 // pair's hit count is a ballot popcount held in scalar registers.
@@ -437,9 +438,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
   const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
   auto &H = sm.u.h;
-  uint32_t *const tab = H.tab + (SOLO ? (threadIdx.x >> 6) * (S::kSlots / WAVES) : 0u);
-  uint32_t *const cnt16 =
-      MODE == kModeTC ? H.cnt16 : H.cnt16 + (SOLO ? (threadIdx.x >> 6) * (S::kSlots / WAVES / 2) : 0u);
+  const uint32_t share = SOLO ? threadIdx.x >> 6 : 0u;  // the wave's share of the table
+  uint32_t *const tab = H.tab + share * (S::kSlots / WAVES);
+  uint32_t *const cnt16 = MODE == kModeTC ? H.cnt16 : H.cnt16 + share * (S::kSlots / WAVES / 2);
   if (c_hi > np) c_hi = np;
   const bool whole = c_lo == 0 && c_hi == np;
   // The wave's first batch of pairs: partners and edges are loaded before the table is
