@@ -578,6 +578,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     const unsigned long long pa = (unsigned long long)(uintptr_t)(G.oadj + bl);
     const uint32_t pn = nl * 4u;
     uint32_t jj = 0, nrem = readlane(pn, 0), ph = 0;
+    uint32_t phl = 0;  // lane j: the hit count of the batch's pair j, once it is done
     unsigned long long ptr = readlane64(pa, 0);
     unsigned long long hb = MODE == kModeSUP ? readlane64(wl, 0) : 0ull;
     uint32_t wa[4], wb[4];
@@ -621,19 +622,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         if (lane < nsub) G.hitw[hb + lane] = hw;
       }
       if (MODE != kModeTC && jn != jj) {
-        // pair jj is done.  t(y) / the pair's edge: r->y adds to y's slot in the
-        // table, y->r is credited to y (each pair is visited once)
+        // pair jj is done: its count goes to its lane, credited with the batch's others
         row_hits += ph;
-        if (ph != 0 && lane == 0) {
-          const uint32_t y = readlane(yl, jj), e = readlane(el, jj);
-          if (e - a0 < nA) {
-            uint32_t word, inc;
-            find(y, word, inc);  // y = A[e - a0] is in the table
-            atomicAdd(&cnt16[word], inc * ph);  // ph <= nA <= CAP < 2^16
-          } else {
-            G.etail[e] = ph;
-          }
-        }
+        phl = lane == jj ? ph : phl;
         ph = 0;
       }
       jj = jn;
@@ -649,6 +640,18 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       step(wa, wb);
       if (jj < nvalid) step(wb, wa);
     } while ((uint32_t)__builtin_amdgcn_readfirstlane((int)jj) < nvalid);
+    if (MODE != kModeTC && phl != 0) {
+      // The batch's pair credits, one lane per pair, all at once (one table lookup per
+      // lane rather than one wave-wide lookup per pair).  t(y) / the pair's edge: r->y adds
+      // to y's slot in the table, y->r is credited to y (each pair is visited once).
+      if (el - a0 < nA) {
+        uint32_t word, inc;
+        find(yl, word, inc);  // y = A[e - a0] is in the table
+        atomicAdd(&cnt16[word], inc * phl);  // phl <= nA <= CAP < 2^16
+      } else {
+        G.etail[el] = phl;
+      }
+    }
     if (nvalid < 64) break;
   }
   // row_hits is the wave's total (uniform): counted once into acc, which is summed over
