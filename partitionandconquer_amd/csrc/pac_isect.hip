@@ -785,8 +785,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev Gd, uint32_t row_
 // but queued (queue[1] counts them, at most defer_cap) for k_deferred, which runs with the
 // widest tile; should the queue be full the row is done here after all.  Launches whose
 // rows all fit use the variant without it.
+// (Six waves per SIMD is what the LDS of tiles 1 and 2 allows; the attribute keeps the
+// SUP-mode variants' registers within it too.)
 template <int MODE, int WAVES, int CAP, bool DEFER>
-__global__ __launch_bounds__(64 * WAVES) void k_persistent(const GraphDev *__restrict__ Gs,
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(6)))
+void k_persistent(const GraphDev *__restrict__ Gs,
                                                            const WorkItem *__restrict__ items,
                                                            uint32_t nitems,
                                                            const uint8_t *__restrict__ strat,
