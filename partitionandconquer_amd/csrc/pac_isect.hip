@@ -127,10 +127,11 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // empty hash-table slot (IDs are < 2^
 constexpr int kRawBufferFlags = 0x00020000;
 
 // LDS of one workgroup.  The sorted-list strategies (merge, gallop) keep A and one
-// 32-bit counter per position; the hash strategy keeps the cuckoo table of A (plus
-// its stash) with packed 16-bit counters per slot.  Both live in one union: a workgroup
-// runs one strategy per row.  (8-byte aligned: a bucket's two slots are read with one
-// 64-bit LDS load.)  `homeless` counts the keys a hash-table build could not place.
+// 32-bit counter per position; the hash strategy keeps the cuckoo table of A with packed
+// 16-bit counters per slot (or, for rows one wave does alone, one table per wave in a
+// 1/WAVES share of both arrays).  Both live in one union: a workgroup runs one strategy
+// at a time.  (8-byte aligned: a bucket's two slots are read with one 64-bit LDS load.)
+// `homeless` counts the keys a workgroup's hash-table build could not place.
 template <int CAP, int MODE, int WAVES>
 struct alignas(8) Smem {
   static constexpr uint32_t kSlots = 2u * pow2_at_least(CAP);
