@@ -74,6 +74,21 @@ def timed_steps(ctx, g, dispatch, launch, steps, warmup):
     return wall, float(np.median(dev)), float(np.median(ker)) if any(ker) else 0.0
 
 
+def host_cpu():
+    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
+    model, n = "unknown CPU", 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("processor"):
+                    n += 1
+                elif line.startswith("model name") and model == "unknown CPU":
+                    model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return model, n or (os.cpu_count() or 0)
+
+
 def cpu_baseline(vl, el, budget_s=15.0, seed=0, strata=12):
     """Oracle TC-merge + LCC-merge on a bounded sample of rows, extrapolated to all rows.
 
@@ -270,6 +285,7 @@ def main():
         log("phase: cpu baseline")
         vl, el = g.csr()
         est_s, rows, nt, spent = cpu_baseline(vl, el, a.cpu_budget)
+        model, ncpu = host_cpu()
         res["cpu_baseline"] = {
             "value": round(m / est_s / 1e9, 6), "unit": "GTEPS", "cores": nt, "kind": "port",
             "sample": f"{rows} rows of this graph drawn from 12 groups of equal estimated merge "
@@ -278,7 +294,7 @@ def main():
                       f"priced per group by measured seconds per unit of work: {est_s:.1f} s "
                       f"for the whole graph with the work spread evenly over {nt} threads (a "
                       f"lower bound: the reference's static row schedule gives the hub rows "
-                      f"to one thread)"}
+                      f"to one thread); host: {model}, {ncpu} logical CPUs"}
     print(json.dumps(res))
 
 
