@@ -786,10 +786,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev Gd, uint32_t row_
 // but queued (queue[1] counts them, at most defer_cap) for k_deferred, which runs with the
 // widest tile; should the queue be full the row is done here after all.  Launches whose
 // rows all fit use the variant without it.
-// (Six waves per SIMD is what the LDS of tiles 1 and 2 allows; the attribute keeps the
-// SUP-mode variants' registers within it too.)
+// Waves per SIMD that the workgroup's LDS allows (at most 6): the launch bound asks the
+// compiler to keep the registers within it too (the SUP-mode variants would otherwise
+// take 85 VGPRs, one wave fewer).
+template <int CAP, int MODE, int WAVES>
+constexpr int lds_waves() {
+  return std::max(1, std::min(6, (int)(163840 / sizeof(Smem<CAP, MODE, WAVES>)) * WAVES / 4));
+}
+
 template <int MODE, int WAVES, int CAP, bool DEFER>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ __launch_bounds__(64 * WAVES, (lds_waves<CAP, MODE, WAVES>()))
 void k_persistent(const GraphDev *__restrict__ Gs,
                                                            const WorkItem *__restrict__ items,
                                                            uint32_t nitems,
@@ -1262,28 +1268,6 @@ __device__ __forceinline__ void transpose64(uint32_t &lo, uint32_t &hi, uint32_t
   }
 }
 
-// Column c of vertex y's hit words summed over its streams r0..r1: lane l gets the number
-// of streams with bit l of word c set (element 64c + l of N+(y)).  Lane i loads stream
-// r + i's word, 64 streams at a time (the next 64 loaded before the current ones are
-// counted), and the transpose turns the 64 rows into columns.
-__device__ __forceinline__ uint32_t hit_column(const PAC_GLOBAL unsigned long long *__restrict__ W,
-                                               unsigned long long base, uint32_t nw, uint32_t r0,
-                                               uint32_t r1, uint32_t c, uint32_t lane) {
-  uint32_t acc = 0;
-  unsigned long long w =
-      r0 + lane < r1 ? W[base + (unsigned long long)(r0 + lane) * nw + c] : 0ull;
-  for (uint32_t r = r0; r < r1; r += 64) {  // wave-uniform
-    const uint32_t rn = r + 64u;
-    const unsigned long long wn =
-        rn + lane < r1 ? W[base + (unsigned long long)(rn + lane) * nw + c] : 0ull;
-    uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
-    transpose64(lo, hi, lane);
-    acc += (uint32_t)__popc(lo) + (uint32_t)__popc(hi);
-    w = wn;
-  }
-  return acc;
-}
-
 // Vertices streamed by at most 64 pairs (one load per word column): each wave loads the
 // metadata of 64 consecutive vertices (lane i: vertex v0 + i) and walks their (vertex,
 // column) units in turn, the next unit's loads issued before the current transpose.  It
@@ -1358,11 +1342,14 @@ k_sup_small(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ pstr
 }
 
 // Vertices streamed by more than 64 pairs, one workgroup per unit (y, first stream r0) of
-// kSupRun streams; each wave takes a set of word columns; units add atomically.
+// kSupRun streams; each wave takes bands of kBand word columns, its lanes reading kBand
+// consecutive words of one stream each (a row's words share cache lines, so a band costs
+// one line request per row instead of kBand); units add atomically.
 __global__ void __launch_bounds__(256)
 k_sup_big(const uint32_t *__restrict__ units, const uint32_t *__restrict__ ooff,
           const uint32_t *__restrict__ pstreams, const unsigned long long *__restrict__ wbase,
           const unsigned long long *__restrict__ hitw, uint32_t *__restrict__ sup) {
+  constexpr uint32_t kBand = 8;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
   const PAC_GLOBAL unsigned long long *W = (const PAC_GLOBAL unsigned long long *)hitw;
   const uint32_t y = units[2u * blockIdx.x], r0 = units[2u * blockIdx.x + 1u];
@@ -1370,10 +1357,45 @@ k_sup_big(const uint32_t *__restrict__ units, const uint32_t *__restrict__ ooff,
   const uint32_t a0 = ooff[y], nd = ooff[y + 1] - a0, nw = (nd + 63u) >> 6;
   const unsigned long long base = wbase[y];
   const uint32_t r1 = min(R, r0 + kSupRun);
-  for (uint32_t c = wave; c < nw; c += wpb) {
-    const uint32_t acc = hit_column(W, base, nw, r0, r1, c, lane);
-    const uint32_t e = 64u * c + lane;
-    if (e < nd && acc) gadd((PAC_GLOBAL uint32_t *)(sup + a0 + e), acc);
+  for (uint32_t c0 = wave * kBand; c0 < nw; c0 += wpb * kBand) {  // wave-uniform
+    const uint32_t nc = min(kBand, nw - c0);
+    uint32_t acc[kBand];
+#pragma unroll
+    for (uint32_t k = 0; k < kBand; ++k) acc[k] = 0;
+    for (uint32_t r = r0; r < r1; r += 64) {  // wave-uniform
+      const uint32_t row = r + lane;
+      const PAC_GLOBAL unsigned long long *src = W + base + (unsigned long long)row * nw + c0;
+      unsigned long long w[kBand];
+#pragma unroll
+      for (uint32_t k = 0; k < kBand; ++k) w[k] = 0ull;
+      if (nc == kBand) {  // wave-uniform: whole band, read as 16-byte loads
+        if (row < r1) {
+#pragma unroll
+          for (uint32_t k = 0; k < kBand; k += 2) {
+            HIP_vector_type<unsigned long long, 2> t;
+            __builtin_memcpy(&t, (const unsigned long long *)(src + k), sizeof(t));
+            w[k] = t.x;
+            w[k + 1] = t.y;
+          }
+        }
+      } else {
+#pragma unroll
+        for (uint32_t k = 0; k < kBand; ++k) w[k] = row < r1 && k < nc ? src[k] : 0ull;
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kBand; ++k) {
+        if (k < nc) {  // wave-uniform
+          uint32_t lo = (uint32_t)w[k], hi = (uint32_t)(w[k] >> 32);
+          transpose64(lo, hi, lane);
+          acc[k] += (uint32_t)__popc(lo) + (uint32_t)__popc(hi);
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kBand; ++k) {
+      const uint32_t e = 64u * (c0 + k) + lane;
+      if (k < nc && e < nd && acc[k]) gadd((PAC_GLOBAL uint32_t *)(sup + a0 + e), acc[k]);
+    }
   }
 }
 
