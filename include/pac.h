@@ -94,7 +94,9 @@ int pac_ctx_create(int device, pac_ctx **out);
 void pac_ctx_destroy(pac_ctx *ctx);
 /* Thread-local description of the last failure on this thread. */
 const char *pac_last_error(void);
-/* Device timing of the last solve on this context (hipEvents on the ctx stream). */
+/* Device timing of the last solve on this context (hipEvents on the ctx stream): TC, LCC,
+ * Jaccard and batches as described above; the pair, k-way, friend-triangle and square
+ * queries set total_ms = isect_ms to their device work (inputs already uploaded). */
 int pac_ctx_last_timing(const pac_ctx *ctx, pac_timing *out);
 /* Number of compute units of the context's device. */
 int pac_ctx_num_cus(const pac_ctx *ctx);

@@ -1215,9 +1215,14 @@ int pac_cnt_common_pairs_filtered(pac_graph *h, const int32_t *u, const int32_t 
     PAC_TRY(dt.reserve(4 * n));
     PAC_CHECK_HIP(hipMemcpy(dt.p, thr, 4 * n, hipMemcpyHostToDevice));
   }
+  Timed t(g->ctx);  // the device work alone, queries already resident
+  PAC_TRY(t.mark(0));
+  PAC_TRY(t.mark(1));
   PAC_TRY(pair_counts(g->ctx, g, du.as<int32_t>(), dv.as<int32_t>(), n, dr.as<uint64_t>(),
                       thr ? dt.as<int32_t>() : nullptr, d_part, (uint32_t)partition));
-  PAC_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
+  PAC_TRY(t.mark(2));
+  PAC_TRY(t.mark(3));
+  PAC_TRY(t.finish());
   PAC_CHECK_HIP(hipMemcpy(res, dr.p, 8 * n, hipMemcpyDeviceToHost));
   return PAC_OK;
 }
@@ -1319,9 +1324,14 @@ int pac_cnt_common_kway_filtered(pac_graph *h, const uint64_t *qoff, const int32
   PAC_TRY(dr.reserve(8 * nq));
   PAC_CHECK_HIP(hipMemcpy(dq.p, qoff, 8 * (nq + 1), hipMemcpyHostToDevice));
   PAC_CHECK_HIP(hipMemcpy(dv.p, qv, 4 * nv, hipMemcpyHostToDevice));
+  Timed t(g->ctx);
+  PAC_TRY(t.mark(0));
+  PAC_TRY(t.mark(1));
   PAC_TRY(kway_counts(g->ctx, g, dq.as<uint64_t>(), dv.as<int32_t>(), nq, dr.as<uint64_t>(),
                       d_part, partition < 0 ? 0u : (uint32_t)partition));
-  PAC_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
+  PAC_TRY(t.mark(2));
+  PAC_TRY(t.mark(3));
+  PAC_TRY(t.finish());
   PAC_CHECK_HIP(hipMemcpy(res, dr.p, 8 * nq, hipMemcpyDeviceToHost));
   return PAC_OK;
 }
@@ -1351,9 +1361,14 @@ int pac_friend_triangle(pac_graph *h, int32_t root, const int32_t *nodes, uint64
   PAC_TRY(dc.reserve(4 * (uint64_t)(rdeg ? rdeg : 1)));
   PAC_TRY(dr.reserve(8));
   if (n) PAC_CHECK_HIP(hipMemcpy(dn.p, nodes, 4 * n, hipMemcpyHostToDevice));
+  Timed t(g->ctx);
+  PAC_TRY(t.mark(0));
+  PAC_TRY(t.mark(1));
   PAC_TRY(friend_triangles(g->ctx, g, (uint32_t)root, dn.as<int32_t>(), n, rdeg,
                            dc.as<uint32_t>(), dr.as<uint64_t>()));
-  PAC_CHECK_HIP(hipStreamSynchronize(g->ctx->stream));
+  PAC_TRY(t.mark(2));
+  PAC_TRY(t.mark(3));
+  PAC_TRY(t.finish());
   PAC_CHECK_HIP(hipMemcpy(res, dr.p, 8, hipMemcpyDeviceToHost));
   return PAC_OK;
 }
@@ -1365,7 +1380,13 @@ int pac_square_count(pac_graph *h, uint64_t *res) {
   }
   Graph *g = &h->g;
   PAC_CHECK_HIP(hipSetDevice(g->ctx->device));
-  return square_count(g->ctx, g, res);
+  Timed t(g->ctx);  // includes the scratch allocation and the 8-byte result copy
+  PAC_TRY(t.mark(0));
+  PAC_TRY(t.mark(1));
+  PAC_TRY(square_count(g->ctx, g, res));
+  PAC_TRY(t.mark(2));
+  PAC_TRY(t.mark(3));
+  return t.finish();
 }
 
 int pac_tc_batch(pac_ctx *ctx, pac_graph *const *hs, uint32_t G, int dispatch, int launch,

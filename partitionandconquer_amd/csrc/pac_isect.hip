@@ -1078,10 +1078,50 @@ k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
   }
 }
 
-// |N(u) & N(v)| over the full symmetric lists, one wave per pair: the shorter list's
-// elements are binary-searched in the longer one (Graph::cnt_common_neis_galloping,
-// Graph.cpp:186-198).  Optional Jaccard ratio c / ((d_u + d_v) - c) in double
-// (Graph.cpp:808-809).
+// Chunk merge of two sorted lists by one wave (Graph::cnt_common_neis_merge,
+// Graph.cpp:227-239, 64 elements a side at a time): lane k holds A[ia + k] and B[ib + k]
+// (0xFFFFFFFF past a list's end, above every ID); each lane looks its B element up in the
+// A chunk by a 6-step search across the lanes, and the chunk whose last element is smaller
+// advances (both when they are equal), so each common element meets its match exactly
+// once.  Returns the lane's share of the hits that pass the filters of k_pairs.
+__device__ __forceinline__ uint32_t chunk_merge_hits(const uint32_t *A, uint32_t na,
+                                                     const uint32_t *B, uint32_t nb,
+                                                     uint32_t lane, int32_t lim,
+                                                     const uint32_t *part, uint32_t pid) {
+  uint32_t ia = 0, ib = 0, h = 0;
+  uint32_t av = lane < na ? A[lane] : 0xFFFFFFFFu;
+  uint32_t bv = lane < nb ? B[lane] : 0xFFFFFFFFu;
+  while (ia < na && ib < nb) {  // wave-uniform
+    const uint32_t alast = min(64u, na - ia) - 1u, blast = min(64u, nb - ib) - 1u;
+    const uint32_t amax = __shfl(av, (int)alast), bmax = __shfl(bv, (int)blast);
+    uint32_t pos = 0;  // lower bound of bv among the A chunk's lanes (at most 63)
+#pragma unroll
+    for (uint32_t st = 32; st; st >>= 1)
+      if (__shfl(av, (int)(pos + st - 1u)) < bv) pos += st;
+    // every lane takes part in the cross-lane read (under a short-circuit it would run
+    // with the lanes past blast switched off, and a read from those returns no data)
+    const uint32_t at = __shfl(av, (int)pos);
+    bool hit = lane <= blast && at == bv && (int32_t)bv < lim;
+    if (part && hit) hit = part[bv] == pid;
+    h += hit ? 1u : 0u;
+    if (amax <= bmax) {
+      ia += 64u;
+      av = ia + lane < na ? A[ia + lane] : 0xFFFFFFFFu;
+    }
+    if (bmax <= amax) {
+      ib += 64u;
+      bv = ib + lane < nb ? B[ib + lane] : 0xFFFFFFFFu;
+    }
+  }
+  return h;
+}
+
+// |N(u) & N(v)| over the full symmetric lists, one wave per pair, by whichever of two
+// strategies takes fewer dependent load steps: the chunk merge above, about (na + nb) / 64
+// steps, or each of the shorter list's elements binary-searched in the longer one
+// (Graph::cnt_common_neis_galloping, Graph.cpp:186-198), ceil(na / 64) x log2(nb) steps;
+// the merge is taken when 1.5x its steps is below the search's.  Optional Jaccard ratio
+// c / ((d_u + d_v) - c) in double (Graph.cpp:808-809).
 // A common neighbour w counts when w < thr[p] (thr null: no bound) and, when part is
 // not null, part[w] == pid.
 __global__ void k_pairs(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
@@ -1111,12 +1151,18 @@ __global__ void k_pairs(const uint32_t *__restrict__ off, const uint32_t *__rest
     uint32_t h = 0;
     if (nb) {
       const uint32_t *B = adj + b0;
-      for (uint32_t i = lane; i < na; i += 64) {
-        const uint32_t x = adj[a0 + i];
-        const uint32_t j = lower_bound_u32(B, nb, x);
-        bool hit = j < nb && B[j] == x && (int32_t)x < lim;
-        if (part && hit) hit = part[x] == pid;
-        h += hit ? 1u : 0u;
+      const uint64_t merge_steps = ((uint64_t)na + nb + 63u) >> 6;
+      const uint64_t search_steps = (((uint64_t)na + 63u) >> 6) * (32u - __clz(nb));
+      if (3u * merge_steps < 2u * search_steps) {  // wave-uniform
+        h = chunk_merge_hits(adj + a0, na, B, nb, lane, lim, part, pid);
+      } else {
+        for (uint32_t i = lane; i < na; i += 64) {
+          const uint32_t x = adj[a0 + i];
+          const uint32_t j = lower_bound_u32(B, nb, x);
+          bool hit = j < nb && B[j] == x && (int32_t)x < lim;
+          if (part && hit) hit = part[x] == pid;
+          h += hit ? 1u : 0u;
+        }
       }
     }
     h = wave_sum(h);

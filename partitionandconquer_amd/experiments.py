@@ -3,6 +3,7 @@
     python -m partitionandconquer_amd.experiments scale22   # plain vs batched, every strategy
     python -m partitionandconquer_amd.experiments batch     # 256 uneven graphs, autotuned tile
     python -m partitionandconquer_amd.experiments jaccard   # per-edge Jaccard at R-MAT scale 25
+    python -m partitionandconquer_amd.experiments next      # ingestion, queries, k-way, squares
 
 scale22: R-MAT scale 22, ef 16, (0.57,0.19,0.19,0.05), seed 22, 1024 contiguous
   partitions; TC + LCC device time of the plain launch (one launch per partition
@@ -16,6 +17,15 @@ batch: 256 graphs, graph g seeded 3000+g; even g: Erdos-Renyi G(n,p) with n = 2^
 jaccard: R-MAT scale 25, ef 16, seed 25, 4096 partitions; per-edge |N(u)&N(v)|,
   union and FP32 ratio for every edge with all results left in HBM; sampled edges are
   checked against the oracle; the oracle's per-edge merge is timed on a sample of edges.
+next: the components on either side of the path, on R-MAT scale 22 (seed 22) unless noted:
+  edge-list ingestion (the graph's undirected edge list, shuffled, to the symmetric CSR on
+  the device; an R-MAT scale 18 edge list as text through the device parser), pairwise
+  common-neighbour counts (4M uniform random pairs, as test.cpp's generate_queries draws
+  them, and 4M random adjacent pairs), 3-way counts (1M queries u, v, w with v, w drawn
+  from N(u)), one friend-triangle query (a root of degree about 1000 with its first 1024
+  neighbours) and square counting (R-MAT scale 12 against the oracle, scale 16 timed).
+  Device times are HIP events around the device work with the inputs resident; each
+  is checked exactly against the oracle, which is timed on the same (sampled) inputs.
 """
 from __future__ import annotations
 
@@ -180,9 +190,159 @@ def exp_jaccard(ctx, scale=25, partitions=4096, sample=200000, cpu_budget=15.0,
     return res
 
 
+def exp_next(ctx, scale=22, seed=22, npairs=4_000_000, nkway=1_000_000, sample=200_000):
+    from oracle import oracle as O
+    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rng = np.random.default_rng(seed)
+    res = {"config": f"rmat-s{scale}-ef16-next", "cpu_threads": nt}
+    g = pac.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed=seed)
+    vl, el = g.csr()
+    ref = O.CSR(vl, el)
+    V = len(vl) - 1
+    vli = vl.astype(np.int64)
+    deg = np.diff(vli)
+
+    # (1) ingestion: undirected edge list (u < v, shuffled) -> symmetric CSR
+    src = np.repeat(np.arange(V, dtype=np.int32), deg)
+    keep = src < el
+    perm = rng.permutation(int(keep.sum()))
+    su, sv = src[keep][perm], el[keep][perm]
+    del src, keep, perm
+    wall = []
+    for i in range(3):
+        t0 = time.perf_counter()
+        g1 = pac.Graph.from_edges(ctx, su, sv, V)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        if i == 0:
+            vl1, el1 = g1.csr()
+            same = bool(np.array_equal(vl1, vl) and np.array_equal(el1, el))
+            del vl1, el1
+        g1.free()
+    t0 = time.perf_counter()
+    rc = O.edges_to_csr(V, su, sv)
+    cpu_s = time.perf_counter() - t0
+    same = same and bool(np.array_equal(rc.vlist, vl) and np.array_equal(rc.elist, el))
+    del rc
+    res["ingest_edges"] = {"edges": int(len(su)), "wall_ms": round(float(np.median(wall)), 1),
+                           "edges_per_s": round(len(su) / (np.median(wall) * 1e-3), 1),
+                           "includes": "host checks + host-to-device copy + device sort/dedup/CSR",
+                           "csr_identical": same, "cpu_s": round(cpu_s, 2), "cpu_threads": 1}
+    del su, sv
+    gs = O.gen_rmat(18, 16, 0.57, 0.19, 0.19, 18)
+    d18 = np.diff(gs.vlist.astype(np.int64))
+    s18 = np.repeat(np.arange(gs.V, dtype=np.int32), d18)
+    k18 = s18 < gs.elist
+    path = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"pac_rmat18_{os.getpid()}.txt")
+    np.savetxt(path, np.stack([s18[k18], gs.elist[k18]], 1), fmt="%d %d")
+    try:
+        wall = []
+        for i in range(3):
+            t0 = time.perf_counter()
+            g1 = pac.Graph.load_text(ctx, path)
+            wall.append((time.perf_counter() - t0) * 1e3)
+            if i == 0:
+                vl1, el1 = g1.csr()
+            g1.free()
+        t0 = time.perf_counter()
+        rt = O.text_to_csr(path)  # the reference's text loader, restated
+        cpu_s = time.perf_counter() - t0
+        same = bool(np.array_equal(rt.vlist, vl1) and np.array_equal(rt.elist, el1))
+        res["ingest_text"] = {"bytes": os.path.getsize(path), "edges": int(k18.sum()),
+                              "wall_ms": round(float(np.median(wall)), 1),
+                              "includes": "file read + device parse + device sort/dedup/CSR",
+                              "csr_identical": same, "cpu_s": round(cpu_s, 2), "cpu_threads": 1}
+    finally:
+        os.remove(path)
+
+    # (2) pairwise counts; algorithmic bytes 4 (d(u) + d(v)) per pair
+    def pairs(name, u, v):
+        g.cnt_common_pairs(u[:1000], v[:1000])
+        ms = []
+        for _ in range(3):
+            out = g.cnt_common_pairs(u, v)
+            ms.append(ctx.last_timing().total_ms)
+        t = float(np.median(ms))
+        nbytes = 4.0 * float((deg[u] + deg[v]).sum())
+        idx = rng.choice(len(u), size=min(sample, len(u)), replace=False)
+        t0 = time.perf_counter()
+        ri, _, _ = O.jaccard_edges(ref, u[idx], v[idx], nt)
+        dt = time.perf_counter() - t0
+        res[name] = {"pairs": int(len(u)), "device_ms": round(t, 3),
+                     "pairs_per_s": round(len(u) / (t * 1e-3), 1),
+                     "alg_GBps": round(nbytes / (t * 1e-3) / 1e9, 1),
+                     "roofline_frac": round(nbytes / (t * 1e-3) / 8e12, 4),
+                     "sample_exact": bool(np.array_equal(ri.astype(np.uint64), out[idx])),
+                     "cpu_pairs_per_s": round(len(idx) / dt, 1)}
+    u = rng.integers(0, V, npairs).astype(np.int32)
+    v = rng.integers(0, V, npairs).astype(np.int32)
+    pairs("pairs_uniform", u, v)
+    e = rng.integers(0, len(el), npairs)
+    u = (np.searchsorted(vli, e, side="right") - 1).astype(np.int32)
+    v = el[e].astype(np.int32)
+    pairs("pairs_adjacent", u, v)
+
+    # (3) 3-way counts: u, and v, w drawn from N(u)
+    nz = np.flatnonzero(deg >= 2)
+    qu = rng.choice(nz, nkway).astype(np.int64)
+    qv = el[vli[qu] + (rng.random(nkway) * deg[qu]).astype(np.int64)]
+    qw = el[vli[qu] + (rng.random(nkway) * deg[qu]).astype(np.int64)]
+    queries = np.stack([qu.astype(np.int32), qv, qw], 1)
+    ql = [q for q in queries]
+    g.cnt_common_kway(ql[:1000])
+    ms = []
+    for _ in range(3):
+        out = g.cnt_common_kway(ql)
+        ms.append(ctx.last_timing().total_ms)
+    t = float(np.median(ms))
+    ns = min(20000, nkway)
+    t0 = time.perf_counter()
+    rk = np.array([O.kway_merge(ref, q) for q in queries[:ns]], np.uint64)
+    dt = time.perf_counter() - t0
+    res["kway3"] = {"queries": nkway, "device_ms": round(t, 3),
+                    "queries_per_s": round(nkway / (t * 1e-3), 1),
+                    "sample_exact": bool(np.array_equal(rk, out[:ns])),
+                    "cpu_queries_per_s": round(ns / dt, 1), "cpu_threads": 1}
+
+    # (4) friend triangle: a root of degree about 1000, its first 1024 neighbours
+    root = int(np.argmin(np.abs(deg - 1000)))
+    nodes = el[vli[root]:vli[root] + min(1024, deg[root])].astype(np.int32)
+    g.friend_triangle(root, nodes)
+    ms = []
+    for _ in range(3):
+        ft = g.friend_triangle(root, nodes)
+        ms.append(ctx.last_timing().total_ms)
+    t0 = time.perf_counter()
+    rf = O.friend_triangle_merge(ref, root, nodes, nt)
+    dt = time.perf_counter() - t0
+    res["friend_triangle"] = {"root_degree": int(deg[root]), "nodes": int(len(nodes)),
+                              "device_ms": round(float(np.median(ms)), 3), "value": ft,
+                              "exact": ft == rf, "cpu_s": round(dt, 3)}
+
+    # (5) squares
+    sq = {}
+    for sc in (12, 16):
+        gq = O.gen_rmat(sc, 16, 0.57, 0.19, 0.19, sc)
+        gg = pac.Graph.from_csr(ctx, gq.vlist, gq.elist)
+        gg.square_count()
+        ms = []
+        for _ in range(3):
+            c = gg.square_count()
+            ms.append(ctx.last_timing().total_ms)
+        ent = {"squares": c, "device_ms": round(float(np.median(ms)), 3)}
+        if sc == 12:
+            t0 = time.perf_counter()
+            want = O.PGraph(gq, O.contiguous_partition(gq.V, 4)).square_counting()
+            ent["cpu_s"] = round(time.perf_counter() - t0, 3)
+            ent["exact"] = c == want
+        sq[f"rmat{sc}"] = ent
+        gg.free()
+    res["squares"] = sq
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["scale22", "batch", "jaccard"])
+    ap.add_argument("which", choices=["scale22", "batch", "jaccard", "next"])
     ap.add_argument("--graphs", type=int, default=256)
     ap.add_argument("--scale", type=int, default=None)
     ap.add_argument("--no-check", action="store_true")
@@ -194,6 +354,8 @@ def main(argv=None):
         r = exp_scale22(ctx, a.scale or 22)
     elif a.which == "batch":
         r = exp_batch(ctx, a.graphs, not a.no_check)
+    elif a.which == "next":
+        r = exp_next(ctx, a.scale or 22)
     else:
         r = exp_jaccard(ctx, a.scale or 25, profile_only=a.profile_only)
     print(json.dumps(r))

@@ -21,6 +21,31 @@ def complete(n, orc):
     return orc.edges_to_csr(n, *zip(*[(i, j) for i in range(n) for j in range(i + 1, n)]))
 
 
+def test_pair_counts_chunk_boundaries(ctx, orc):
+    """Pair counts where both lists sit on either side of the 64-element chunks the
+    wave merges (and of the merge/search choice): hubs 0..11 with neighbour sets of
+    the listed sizes drawn from one range, so every pair overlaps; counts are the set
+    intersections, also under thresholds, and equal the oracle's merge."""
+    rng = np.random.default_rng(7)
+    sizes = [0, 1, 63, 64, 65, 127, 128, 129, 191, 300, 640, 2000]
+    base, span = 100, 2500
+    sets = [np.sort(rng.choice(np.arange(base, base + span), n, replace=False)) for n in sizes]
+    src = np.concatenate([np.full(len(S), i) for i, S in enumerate(sets)]).astype(np.int32)
+    dst = np.concatenate(sets).astype(np.int32)
+    V = base + span
+    g = pac.Graph.from_edges(ctx, src, dst, V)
+    ref = orc.edges_to_csr(V, src, dst)
+    H = len(sizes)
+    u = np.repeat(np.arange(H), H).astype(np.int32)
+    v = np.tile(np.arange(H), H).astype(np.int32)
+    want = [len(np.intersect1d(sets[a], sets[b])) for a, b in zip(u, v)]
+    assert g.cnt_common_pairs(u, v).tolist() == want
+    assert want == [orc.cnt_merge(ref, int(a), int(b)) for a, b in zip(u, v)]
+    thr = rng.integers(base - 1, base + span + 1, len(u)).astype(np.int32)
+    want_t = [int(np.sum(np.intersect1d(sets[a], sets[b]) < t)) for a, b, t in zip(u, v, thr)]
+    assert g.cnt_common_pairs(u, v, thr=thr).tolist() == want_t
+
+
 @pytest.mark.parametrize("scale,seed", [(8, 1), (12, 2), (14, 3)])
 def test_kway_queries(ctx, orc, scale, seed):
     ref = orc.gen_rmat(scale, 16, seed=seed)
