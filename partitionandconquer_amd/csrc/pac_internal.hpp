@@ -74,11 +74,13 @@ struct GraphDev {
   const uint32_t *pedge; // oriented-edge index of each pair (position in oadj)
   const uint32_t *rows;  // rows owning at least one pair, grouped by partition
   unsigned long long *total;  // T accumulator
-  unsigned long long *tvert;  // per-vertex triangle counts (LCC mode: the row totals)
+  unsigned long long *tvert;  // per-vertex triangle counts (LCC mode: the row totals and
+                              // the tail credits of the pairs' edges, added atomically)
   uint32_t *ehead;            // per oriented edge, triangles credited to its head (LCC mode:
                               // stored at the edge's in-edge position, see inpos)
   const uint32_t *inpos;      // per oriented edge, its position in the in-edge lists
-  uint32_t *etail;            // LCC mode: per oriented edge, triangles credited to its tail
+  uint32_t *etail;            // SUP mode: per oriented edge, triangles credited to it as a
+                              // pair's edge pointing to the row (LCC mode adds them to tvert)
   uint32_t *sup;              // per-oriented-edge support (SUP mode)
   // SUP mode: the hit words of the streamed lists.  The pair with rank r among the pairs
   // streaming N+(y) owns the 64-bit words hitw[wbase[y] + r*nw ..+ nw), nw = ceil(d+(y)/64);

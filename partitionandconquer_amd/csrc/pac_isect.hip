@@ -355,8 +355,10 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
       if (lane == 0 && ch) {
         if (e - a0 < nA)
           atomicAdd(&cnt[e - a0], ch);  // r->y: y = A[e - a0]
+        else if (MODE == kModeLCC)
+          gadd(&G.tvert[y], (unsigned long long)ch);  // y->r: credited to y
         else
-          G.etail[e] = ch;  // y->r: credited to y (each pair is visited once)
+          G.etail[e] = ch;  // y->r: credited to the edge (each pair is visited once)
       }
     }
     row_hits += h;
@@ -372,12 +374,8 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
       const uint32_t v = cnt[i];
       if (v) credit_edge(&G.ehead[head_slot<MODE>(G, a0 + i)], v, whole);
     }
-    if (MODE == kModeLCC && tid == 0 && sm.rowtot) {
-      if (whole)
-        G.tvert[r] = sm.rowtot;
-      else
-        gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
-    }
+    // an atomic even for a whole row: other rows' tail credits to r arrive meanwhile
+    if (MODE == kModeLCC && tid == 0 && sm.rowtot) gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
   }
   __syncthreads();
 }
@@ -649,6 +647,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         uint32_t word, inc;
         find(yl, word, inc);  // y = A[e - a0] is in the table
         atomicAdd(&cnt16[word], inc * phl);  // phl <= nA <= CAP < 2^16
+      } else if (MODE == kModeLCC) {
+        gadd(&G.tvert[yl], (unsigned long long)phl);
       } else {
         G.etail[el] = phl;
       }
@@ -675,12 +675,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   }
   if (MODE == kModeLCC) {
     const uint32_t rt = SOLO ? row_hits : sm.rowtot;  // the row's triangles (uniform)
-    if (tid == 0 && rt) {
-      if (whole)
-        G.tvert[r] = rt;
-      else
-        gadd(&G.tvert[r], (unsigned long long)rt);
-    }
+    if (tid == 0 && rt) gadd(&G.tvert[r], (unsigned long long)rt);  // atomic: see row_sorted
   }
   sync();
   return true;
@@ -715,6 +710,8 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
       if (lane == 0 && ch) {
         if (e - a0 < nA)
           gadd(&G.ehead[head_slot<MODE>(G, e)], ch);  // r->y
+        else if (MODE == kModeLCC)
+          gadd(&G.tvert[y], (unsigned long long)ch);  // y->r: credited to y
         else
           G.etail[e] = ch;        // y->r, visited once
       }
@@ -931,14 +928,14 @@ __global__ void k_rowwork(const uint32_t *__restrict__ poff, const uint32_t *__r
 }
 
 // ---- epilogues -----------------------------------------------------------------------
-// LCC per-vertex triangle counts: t(v) = (v's row totals, already in tvert)
-//   + sum over out-edges e of v of etail[e] + sum over in-edges of v of their head
-//   credits, which LCC mode stores in in-edge order: ehead[inoff[v] .. inoff[v+1]).
+// LCC per-vertex triangle counts: t(v) = (v's row totals and its credits as the partner
+//   of a pair whose edge points to the row, both already added to tvert by the
+//   intersection kernel) + the sum over in-edges of v of their head credits, which LCC
+//   mode stores in in-edge order: ehead[inoff[v] .. inoff[v+1]).
 // Vertices of degree <= kSmallDeg (most of them) take one thread each; the others are
 // listed in `big` and take one wave each; a hub's in-edges beyond the first kHubIn are
 // left to k_vertex_hub, which splits them over several workgroups.
-__global__ void k_vertex_small(const uint32_t *__restrict__ off, const uint32_t *__restrict__ ooff,
-                               const uint32_t *__restrict__ etail,
+__global__ void k_vertex_small(const uint32_t *__restrict__ off,
                                const uint32_t *__restrict__ inoff,
                                const uint32_t *__restrict__ ehead, uint32_t V,
                                unsigned long long *__restrict__ tvert) {
@@ -946,14 +943,12 @@ __global__ void k_vertex_small(const uint32_t *__restrict__ off, const uint32_t 
        v += (uint64_t)gridDim.x * blockDim.x) {
     if (off[v + 1] - off[v] > kSmallDeg) continue;
     uint32_t s = 0;  // at most kSmallDeg edges, each credited with fewer than d(v) triangles
-    for (uint32_t e = ooff[v]; e < ooff[v + 1]; ++e) s += etail[e];
     for (uint32_t k = inoff[v]; k < inoff[v + 1]; ++k) s += ehead[k];
     if (s) tvert[v] += s;
   }
 }
 
 __global__ void k_vertex_big(const uint32_t *__restrict__ big, uint32_t nbig,
-                             const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ etail,
                              const uint32_t *__restrict__ inoff,
                              const uint32_t *__restrict__ ehead,
                              unsigned long long *__restrict__ tvert) {
@@ -962,7 +957,6 @@ __global__ void k_vertex_big(const uint32_t *__restrict__ big, uint32_t nbig,
        b += (uint64_t)gridDim.x * wpb) {
     const uint32_t v = big[b];
     unsigned long long s = 0;
-    for (uint32_t e = ooff[v] + lane; e < ooff[v + 1]; e += 64) s += etail[e];
     const uint32_t k0 = inoff[v], k1 = min(inoff[v + 1], k0 + kHubIn);
     for (uint32_t k = k0 + lane; k < k1; k += 256) {
       uint32_t x[4];
@@ -1518,11 +1512,11 @@ int alloc_outputs(Graph *const *gs, uint32_t G, int mode) {
     PAC_TRY(g->total.reserve(sizeof(unsigned long long)));
     if (mode == kModeLCC)
       PAC_TRY(g->tvert.reserve(sizeof(unsigned long long) * (size_t)(g->V ? g->V : 1)));
-    if (mode != kModeTC) {
-      PAC_TRY(g->ehead.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
+    if (mode != kModeTC) PAC_TRY(g->ehead.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
+    if (mode == kModeSUP) {
       PAC_TRY(g->etail.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
+      PAC_TRY(g->sup.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
     }
-    if (mode == kModeSUP) PAC_TRY(g->sup.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
   }
   return PAC_OK;
 }
@@ -1537,10 +1531,8 @@ int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
     if (mode == kModeLCC && g->V)
       hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
                          g->tvert.as<unsigned long long>(), (uint64_t)g->V);
-    if (mode != kModeTC && g->m) {
-      PAC_TRY(fill_u32(ctx->stream, g->ehead.as<uint32_t>(), g->m, 0u));
-      PAC_TRY(fill_u32(ctx->stream, g->etail.as<uint32_t>(), g->m, 0u));
-    }
+    if (mode != kModeTC && g->m) PAC_TRY(fill_u32(ctx->stream, g->ehead.as<uint32_t>(), g->m, 0u));
+    if (mode == kModeSUP && g->m) PAC_TRY(fill_u32(ctx->stream, g->etail.as<uint32_t>(), g->m, 0u));
     if (mode == kModeSUP) PAC_TRY(fill_u32(ctx->stream, g->sup.as<uint32_t>(), g->m, 0u));
     PAC_CHECK_HIP(hipGetLastError());
   }
@@ -1694,15 +1686,13 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
 int vertex_counts(Ctx *ctx, Graph *g) {
   if (g->V == 0) return PAC_OK;
   hipLaunchKernelGGL(k_vertex_small, dim3(grid_for(g->V, 256)), dim3(256), 0, ctx->stream,
-                     g->off.as<const uint32_t>(), g->ooff.as<const uint32_t>(),
-                     g->etail.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
+                     g->off.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
                      g->ehead.as<const uint32_t>(), g->V, g->tvert.as<unsigned long long>());
   PAC_CHECK_HIP(hipGetLastError());
   const uint32_t nbig = (uint32_t)g->h_big.size();
   if (nbig) {
     hipLaunchKernelGGL(k_vertex_big, dim3(grid_for(nbig, 4)), dim3(256), 0, ctx->stream,
-                       g->big.as<const uint32_t>(), nbig, g->ooff.as<const uint32_t>(),
-                       g->etail.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
+                       g->big.as<const uint32_t>(), nbig, g->inoff.as<const uint32_t>(),
                        g->ehead.as<const uint32_t>(), g->tvert.as<unsigned long long>());
     PAC_CHECK_HIP(hipGetLastError());
   }
