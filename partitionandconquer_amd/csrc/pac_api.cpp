@@ -329,7 +329,6 @@ int run_intersection(Graph *g, int mode, int dispatch, int launch, Timed &t) {
     ctx->time_kernel = false;
     PAC_TRY(rc);
   }
-  if (mode == kModeLCC) PAC_TRY(vertex_counts(ctx, g));
   if (mode == kModeSUP) PAC_TRY(sup_from_hit_words(ctx, g));
   ctx->last.launches = launches;
   PAC_TRY(t.mark(2));
@@ -905,27 +904,9 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
   uint64_t total_cost = 0;
   for (uint32_t r : rowlist) total_cost += row_cost(r);
   const uint64_t target = item_target(total_cost, ctx->num_cus);
-  // vertices whose credit sum needs a wave, and those whose in-degree needs several
-  // workgroups
   g->n_long_rows = 0;
   for (uint32_t r : rowlist)
     if (odeg[r] > kMinTileCap) ++g->n_long_rows;
-  g->h_big.clear();
-  g->h_hubs.clear();
-  for (uint32_t u = 0; u < V; ++u) {
-    if (deg[u] > kSmallDeg) g->h_big.push_back(u);
-    if (deg[u] - odeg[u] > kHubIn) g->h_hubs.push_back(u);
-  }
-  if (!g->h_big.empty()) {
-    PAC_TRY(g->big.reserve(sizeof(uint32_t) * g->h_big.size()));
-    PAC_CHECK_HIP(hipMemcpy(g->big.p, g->h_big.data(), sizeof(uint32_t) * g->h_big.size(),
-                            hipMemcpyHostToDevice));
-  }
-  if (!g->h_hubs.empty()) {
-    PAC_TRY(g->hubs.reserve(sizeof(uint32_t) * g->h_hubs.size()));
-    PAC_CHECK_HIP(hipMemcpy(g->hubs.p, g->h_hubs.data(), sizeof(uint32_t) * g->h_hubs.size(),
-                            hipMemcpyHostToDevice));
-  }
   g->h_items.clear();
   for (uint32_t k = 0; k < P; ++k) {
     uint32_t lo = count[k];

@@ -74,11 +74,10 @@ struct GraphDev {
   const uint32_t *pedge; // oriented-edge index of each pair (position in oadj)
   const uint32_t *rows;  // rows owning at least one pair, grouped by partition
   unsigned long long *total;  // T accumulator
-  unsigned long long *tvert;  // per-vertex triangle counts (LCC mode: the row totals and
-                              // the tail credits of the pairs' edges, added atomically)
-  uint32_t *ehead;            // per oriented edge, triangles credited to its head (LCC mode:
-                              // stored at the edge's in-edge position, see inpos)
-  const uint32_t *inpos;      // per oriented edge, its position in the in-edge lists
+  unsigned long long *tvert;  // LCC mode: per-vertex triangle counts, every credit added
+                              // atomically by the intersection kernel
+  uint32_t *ehead;            // SUP mode: per oriented edge, triangles credited to it as an
+                              // edge of the row's own list (LCC mode adds them to tvert)
   uint32_t *etail;            // SUP mode: per oriented edge, triangles credited to it as a
                               // pair's edge pointing to the row (LCC mode adds them to tvert)
   uint32_t *sup;              // per-oriented-edge support (SUP mode)
@@ -131,10 +130,6 @@ struct Graph {
   DevBuf poff, ppar, pedge;
   // In-edges: for each vertex v, the oadj positions of the edges u->v (V+1 offsets).
   DevBuf inoff, inidx;
-  DevBuf inpos;  // m: inverse of inidx, the in-edge position of each oriented edge
-  std::vector<uint32_t> h_hubs;  // vertices whose in-degree exceeds kHubIn
-  std::vector<uint32_t> h_big;   // vertices whose degree exceeds kSmallDeg
-  DevBuf hubs, big;
   bool pairs_built = false;
   // Per-edge supports (SUP mode) from hit words instead of one atomic per triangle:
   // pstreams[y] = number of pairs streaming N+(y); wbase[y] = first word of y's streams;
@@ -207,10 +202,6 @@ int build_pairs(Graph *g);
 // Value of the entry past the last oriented edge (oadj[m]): neither a vertex ID (IDs
 // are < 2^31) nor the empty hash-table slot, so a lookup of it never hits.
 constexpr uint32_t kListEnd = 0xFFFFFFFEu;
-// In-degree above which a vertex's credit sum is split over several workgroups.
-constexpr uint32_t kHubIn = 16384;
-// Degree up to which a vertex's credit sum is done by one thread (else by one wave).
-constexpr uint32_t kSmallDeg = 32;
 // Vertices streamed by at most 64 pairs have their hit words summed by one wave (one
 // load per word column); the others are split into units of kSupRun streams.
 constexpr uint32_t kSupRun = 256;
@@ -250,7 +241,6 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
 int launch_partition(Ctx *ctx, Graph *g, int mode, uint32_t part, int strat);
 int compute_rowwork(Graph *g);
 // LCC mode: per-vertex triangle counts = row totals + per-edge credits (after a launch).
-int vertex_counts(Ctx *ctx, Graph *g);
 int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64);
 int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *inter,
                      uint32_t *uni, float *ratio);

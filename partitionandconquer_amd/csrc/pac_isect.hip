@@ -54,7 +54,6 @@ struct GDev {
   const PAC_GLOBAL uint32_t *ooff, *oadj, *poff, *ppar, *pedge, *rows;
   PAC_GLOBAL unsigned long long *total, *tvert;
   PAC_GLOBAL uint32_t *ehead, *etail, *sup;
-  const PAC_GLOBAL uint32_t *inpos;
   const PAC_GLOBAL unsigned long long *pword;
   PAC_GLOBAL unsigned long long *hitw;
   uint32_t V, m, part_base;
@@ -73,7 +72,6 @@ __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
   g.ehead = (PAC_GLOBAL uint32_t *)d.ehead;
   g.etail = (PAC_GLOBAL uint32_t *)d.etail;
   g.sup = (PAC_GLOBAL uint32_t *)d.sup;
-  g.inpos = (const PAC_GLOBAL uint32_t *)d.inpos;
   g.pword = (const PAC_GLOBAL unsigned long long *)d.pword;
   g.hitw = (PAC_GLOBAL unsigned long long *)d.hitw;
   g.V = d.V;
@@ -151,23 +149,13 @@ struct alignas(8) Smem {
   uint32_t rowtot;  // the row's triangle count (LCC mode)
 };
 
-// LCC-mode credits.  A row call that covers all of the row's pairs is the only writer of
-// the row's edges and total, so it stores; a partial call (a split row) adds atomically.
+// Per-edge credits (SUP mode).  A row call that covers all of the row's pairs is the only
+// writer of the row's edges, so it stores; a partial call (a split row) adds atomically.
 __device__ __forceinline__ void credit_edge(PAC_GLOBAL uint32_t *p, uint32_t v, bool whole) {
   if (whole)
     *p = v;
   else
     gadd(p, v);
-}
-
-// Where the head credit of oriented edge e is kept.  LCC mode stores it at the edge's
-// position among its head's in-edges, so that the per-vertex sums after the launch read
-// every vertex's in-edge credits as one contiguous range (the scattered stores cost the
-// intersection kernel little: they are one per edge with a triangle, and the kernel is
-// not limited by memory); SUP mode keeps edge order for the per-edge epilogue.
-template <int MODE>
-__device__ __forceinline__ uint32_t head_slot(const GDev &G, uint32_t e) {
-  return MODE == kModeLCC ? G.inpos[e] : e;
 }
 
 __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
@@ -372,7 +360,8 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
   if (MODE != kModeTC) {
     for (uint32_t i = tid; i < nA; i += BS) {
       const uint32_t v = cnt[i];
-      if (v) credit_edge(&G.ehead[head_slot<MODE>(G, a0 + i)], v, whole);
+      if (v && MODE == kModeLCC) gadd(&G.tvert[A[i]], (unsigned long long)v);  // t(A[i])
+      else if (v) credit_edge(&G.ehead[a0 + i], v, whole);
     }
     // an atomic even for a whole row: other rows' tail credits to r arrive meanwhile
     if (MODE == kModeLCC && tid == 0 && sm.rowtot) gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
@@ -462,14 +451,10 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     sm.rowtot = 0;
   }
   bool lost = false;  // SOLO: this lane left a key homeless
-  // The thread's first element of A and (LCC mode) its in-edge position, loaded here and
-  // kept for the row's credits at the end, off its critical path (most rows have at most
-  // one element per thread).
-  uint32_t key0 = 0, pos0 = 0;
-  if (tid < nA) {
-    key0 = G.oadj[a0 + tid];
-    if (MODE == kModeLCC) pos0 = G.inpos[a0 + tid];
-  }
+  // The thread's first element of A, loaded here and kept for the row's credits at the
+  // end (most rows have at most one element per thread).
+  uint32_t key0 = 0;
+  if (tid < nA) key0 = G.oadj[a0 + tid];
   sync();
   for (uint32_t i = tid; i < nA; i += BS) {
     uint32_t x = i == tid ? key0 : G.oadj[a0 + i];
@@ -661,16 +646,15 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   if (MODE == kModeLCC && !SOLO && lane == 0 && row_hits) atomicAdd(&sm.rowtot, row_hits);
   sync();
   if (MODE != kModeTC) {
-    // per edge r->A[i], in A order (coalesced): the credits of A[i]'s slot
+    // per edge r->A[i], in A order: the credits of A[i]'s slot, to t(A[i]) (LCC mode) or
+    // to the edge (coalesced)
     for (uint32_t i = tid; i < nA; i += BS) {
+      const uint32_t x = i == tid ? key0 : G.oadj[a0 + i];
       uint32_t word, inc;
-      find(i == tid ? key0 : G.oadj[a0 + i], word, inc);
+      find(x, word, inc);
       const uint32_t v = (cnt16[word] >> (inc == 1u ? 0u : 16u)) & 0xFFFFu;
-      if (v) {
-        const uint32_t slot =
-            MODE == kModeLCC && i == tid ? pos0 : head_slot<MODE>(G, a0 + i);
-        credit_edge(&G.ehead[slot], v, whole);
-      }
+      if (v && MODE == kModeLCC) gadd(&G.tvert[x], (unsigned long long)v);
+      else if (v) credit_edge(&G.ehead[a0 + i], v, whole);
     }
   }
   if (MODE == kModeLCC) {
@@ -701,17 +685,18 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
       const uint32_t i = lower_bound_u32(A, nA, w);
       if (i < nA && A[i] == w) {
         ++h;
-        if (MODE != kModeTC) gadd(&G.ehead[head_slot<MODE>(G, a0 + i)], 1u);
+        if (MODE == kModeLCC) gadd(&G.tvert[w], 1ull);  // t(w)
+        if (MODE == kModeSUP) gadd(&G.ehead[a0 + i], 1u);
         if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);  // edge y->w
       }
     }
     if (MODE != kModeTC) {
       const uint32_t ch = wave_sum(h);
       if (lane == 0 && ch) {
-        if (e - a0 < nA)
-          gadd(&G.ehead[head_slot<MODE>(G, e)], ch);  // r->y
-        else if (MODE == kModeLCC)
-          gadd(&G.tvert[y], (unsigned long long)ch);  // y->r: credited to y
+        if (MODE == kModeLCC)
+          gadd(&G.tvert[y], (unsigned long long)ch);  // t(y), either direction
+        else if (e - a0 < nA)
+          gadd(&G.ehead[e], ch);  // r->y
         else
           G.etail[e] = ch;        // y->r, visited once
       }
@@ -928,73 +913,6 @@ __global__ void k_rowwork(const uint32_t *__restrict__ poff, const uint32_t *__r
 }
 
 // ---- epilogues -----------------------------------------------------------------------
-// LCC per-vertex triangle counts: t(v) = (v's row totals and its credits as the partner
-//   of a pair whose edge points to the row, both already added to tvert by the
-//   intersection kernel) + the sum over in-edges of v of their head credits, which LCC
-//   mode stores in in-edge order: ehead[inoff[v] .. inoff[v+1]).
-// Vertices of degree <= kSmallDeg (most of them) take one thread each; the others are
-// listed in `big` and take one wave each; a hub's in-edges beyond the first kHubIn are
-// left to k_vertex_hub, which splits them over several workgroups.
-__global__ void k_vertex_small(const uint32_t *__restrict__ off,
-                               const uint32_t *__restrict__ inoff,
-                               const uint32_t *__restrict__ ehead, uint32_t V,
-                               unsigned long long *__restrict__ tvert) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
-       v += (uint64_t)gridDim.x * blockDim.x) {
-    if (off[v + 1] - off[v] > kSmallDeg) continue;
-    uint32_t s = 0;  // at most kSmallDeg edges, each credited with fewer than d(v) triangles
-    for (uint32_t k = inoff[v]; k < inoff[v + 1]; ++k) s += ehead[k];
-    if (s) tvert[v] += s;
-  }
-}
-
-__global__ void k_vertex_big(const uint32_t *__restrict__ big, uint32_t nbig,
-                             const uint32_t *__restrict__ inoff,
-                             const uint32_t *__restrict__ ehead,
-                             unsigned long long *__restrict__ tvert) {
-  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
-  for (uint64_t b = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); b < nbig;
-       b += (uint64_t)gridDim.x * wpb) {
-    const uint32_t v = big[b];
-    unsigned long long s = 0;
-    const uint32_t k0 = inoff[v], k1 = min(inoff[v + 1], k0 + kHubIn);
-    for (uint32_t k = k0 + lane; k < k1; k += 256) {
-      uint32_t x[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) x[u] = k + 64u * u < k1 ? ehead[k + 64u * u] : 0u;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) s += x[u];
-    }
-    s = wave_sum64(s);
-    if (lane == 0 && s) tvert[v] += s;
-  }
-}
-
-// The in-edges of hub h past its first kHubIn, in slices of kHubIn per workgroup.
-__global__ void k_vertex_hub(const uint32_t *__restrict__ hubs, const uint32_t *__restrict__ inoff,
-                             const uint32_t *__restrict__ ehead,
-                             unsigned long long *__restrict__ tvert) {
-  const uint32_t h = hubs[blockIdx.x];
-  const uint64_t kbeg = inoff[h], kend = inoff[h + 1];
-  unsigned long long s = 0;
-  for (uint64_t k0 = kbeg + (uint64_t)kHubIn * (1u + blockIdx.y); k0 < kend;
-       k0 += (uint64_t)kHubIn * gridDim.y) {
-    const uint64_t k1 = k0 + kHubIn < kend ? k0 + kHubIn : kend;
-    for (uint64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) s += ehead[k];
-  }
-  s = wave_sum64(s);
-  __shared__ unsigned long long ws[16];
-  if ((threadIdx.x & 63u) == 0) ws[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) s += ws[w];
-    if (s) atomicAdd(&tvert[h], s);
-  }
-}
-
-// LCC of Graph::solve_lcc_merge (Graph.cpp:828-840): numerator 2t(u) and denominator
-// d*(d-1) are exact integers converted to double, one IEEE division, then one
-// rounding to float.  d = 0 keeps 0.0, d = 1 gives 0/0 = NaN, as the reference.
 __global__ void k_lcc(const unsigned long long *__restrict__ tvert,
                       const uint32_t *__restrict__ off, uint32_t V, float *lcc32,
                       double *lcc64) {
@@ -1453,7 +1371,6 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   d.ehead = g->ehead.as<uint32_t>();
   d.etail = g->etail.as<uint32_t>();
   d.sup = g->sup.as<uint32_t>();
-  d.inpos = g->inpos.as<const uint32_t>();
   d.pword = g->pword.as<const unsigned long long>();
   d.hitw = g->hitw.as<unsigned long long>();
   d.V = g->V;
@@ -1512,8 +1429,8 @@ int alloc_outputs(Graph *const *gs, uint32_t G, int mode) {
     PAC_TRY(g->total.reserve(sizeof(unsigned long long)));
     if (mode == kModeLCC)
       PAC_TRY(g->tvert.reserve(sizeof(unsigned long long) * (size_t)(g->V ? g->V : 1)));
-    if (mode != kModeTC) PAC_TRY(g->ehead.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
     if (mode == kModeSUP) {
+      PAC_TRY(g->ehead.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
       PAC_TRY(g->etail.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
       PAC_TRY(g->sup.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
     }
@@ -1531,8 +1448,10 @@ int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
     if (mode == kModeLCC && g->V)
       hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
                          g->tvert.as<unsigned long long>(), (uint64_t)g->V);
-    if (mode != kModeTC && g->m) PAC_TRY(fill_u32(ctx->stream, g->ehead.as<uint32_t>(), g->m, 0u));
-    if (mode == kModeSUP && g->m) PAC_TRY(fill_u32(ctx->stream, g->etail.as<uint32_t>(), g->m, 0u));
+    if (mode == kModeSUP && g->m) {
+      PAC_TRY(fill_u32(ctx->stream, g->ehead.as<uint32_t>(), g->m, 0u));
+      PAC_TRY(fill_u32(ctx->stream, g->etail.as<uint32_t>(), g->m, 0u));
+    }
     if (mode == kModeSUP) PAC_TRY(fill_u32(ctx->stream, g->sup.as<uint32_t>(), g->m, 0u));
     PAC_CHECK_HIP(hipGetLastError());
   }
@@ -1680,31 +1599,6 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   }
   PAC_CHECK_HIP(hipGraphLaunch(ctx->hexec, ctx->stream));
   if (launches) *launches += 1;
-  return PAC_OK;
-}
-
-int vertex_counts(Ctx *ctx, Graph *g) {
-  if (g->V == 0) return PAC_OK;
-  hipLaunchKernelGGL(k_vertex_small, dim3(grid_for(g->V, 256)), dim3(256), 0, ctx->stream,
-                     g->off.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
-                     g->ehead.as<const uint32_t>(), g->V, g->tvert.as<unsigned long long>());
-  PAC_CHECK_HIP(hipGetLastError());
-  const uint32_t nbig = (uint32_t)g->h_big.size();
-  if (nbig) {
-    hipLaunchKernelGGL(k_vertex_big, dim3(grid_for(nbig, 4)), dim3(256), 0, ctx->stream,
-                       g->big.as<const uint32_t>(), nbig, g->inoff.as<const uint32_t>(),
-                       g->ehead.as<const uint32_t>(), g->tvert.as<unsigned long long>());
-    PAC_CHECK_HIP(hipGetLastError());
-  }
-  if (!g->h_hubs.empty()) {
-    uint64_t slices = (g->max_deg + kHubIn - 1) / kHubIn;  // in-degree <= max degree
-    if (slices < 1) slices = 1;
-    if (slices > 4096) slices = 4096;  // the kernel strides over further slices
-    hipLaunchKernelGGL(k_vertex_hub, dim3((uint32_t)g->h_hubs.size(), (uint32_t)slices), dim3(256),
-                       0, ctx->stream, g->hubs.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
-                       g->ehead.as<const uint32_t>(), g->tvert.as<unsigned long long>());
-    PAC_CHECK_HIP(hipGetLastError());
-  }
   return PAC_OK;
 }
 

@@ -164,13 +164,6 @@ __global__ void k_iota(uint32_t *__restrict__ p, uint64_t n) {
     p[i] = (uint32_t)i;
 }
 
-// inv[perm[k]] = k for a permutation of [0, n)
-__global__ void k_invert(const uint32_t *__restrict__ perm, uint64_t n, uint32_t *__restrict__ inv) {
-  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n;
-       k += (uint64_t)gridDim.x * blockDim.x)
-    inv[perm[k]] = (uint32_t)k;
-}
-
 // pairs owned per vertex (one wave per vertex, over its symmetric row) and in-degree
 __global__ void k_pair_count(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
                              const uint32_t *__restrict__ deg, const uint32_t *__restrict__ odeg,
@@ -384,12 +377,6 @@ int build_pairs(Graph *g) {
                                             keys_out.as<uint32_t>(), vals_in.as<const uint32_t>(),
                                             g->inidx.as<uint32_t>(), (size_t)g->m, 0u, bits,
                                             ctx->stream));
-    // the inverse permutation: where each oriented edge sits among its head's in-edges
-    PAC_TRY(g->inpos.reserve(sizeof(uint32_t) * g->m));
-    hipLaunchKernelGGL(k_invert, dim3(std::min<uint64_t>(1u << 16, (g->m + 255) / 256)), dim3(256),
-                       0, ctx->stream, g->inidx.as<const uint32_t>(), g->m,
-                       g->inpos.as<uint32_t>());
-    PAC_CHECK_HIP(hipGetLastError());
     // keys_out and vals_in are freed on return
     PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   }
