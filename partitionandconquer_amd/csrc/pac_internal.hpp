@@ -76,10 +76,6 @@ struct GraphDev {
   unsigned long long *total;  // T accumulator
   unsigned long long *tvert;  // LCC mode: per-vertex triangle counts, every credit added
                               // atomically by the intersection kernel
-  uint32_t *ehead;            // SUP mode: per oriented edge, triangles credited to it as an
-                              // edge of the row's own list (LCC mode adds them to tvert)
-  uint32_t *etail;            // SUP mode: per oriented edge, triangles credited to it as a
-                              // pair's edge pointing to the row (LCC mode adds them to tvert)
   uint32_t *sup;              // per-oriented-edge support (SUP mode)
   // SUP mode: the hit words of the streamed lists.  The pair with rank r among the pairs
   // streaming N+(y) owns the 64-bit words hitw[wbase[y] + r*nw ..+ nw), nw = ceil(d+(y)/64);
@@ -159,7 +155,7 @@ struct Graph {
   bool part_ready = false;
   DevBuf rowwork;  // V: streamed length of row u = sum of d+(partner) over u's pairs
   // outputs
-  DevBuf total, tvert, ehead, etail, sup, lcc32, lcc64, juni, jratio;
+  DevBuf total, tvert, sup, lcc32, lcc64, juni, jratio;
   double prepare_ms = 0;
 };
 

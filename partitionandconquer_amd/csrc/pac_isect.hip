@@ -53,7 +53,7 @@ namespace {
 struct GDev {
   const PAC_GLOBAL uint32_t *ooff, *oadj, *poff, *ppar, *pedge, *rows;
   PAC_GLOBAL unsigned long long *total, *tvert;
-  PAC_GLOBAL uint32_t *ehead, *etail, *sup;
+  PAC_GLOBAL uint32_t *sup;
   const PAC_GLOBAL unsigned long long *pword;
   PAC_GLOBAL unsigned long long *hitw;
   uint32_t V, m, part_base;
@@ -69,8 +69,6 @@ __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
   g.rows = (const PAC_GLOBAL uint32_t *)d.rows;
   g.total = (PAC_GLOBAL unsigned long long *)d.total;
   g.tvert = (PAC_GLOBAL unsigned long long *)d.tvert;
-  g.ehead = (PAC_GLOBAL uint32_t *)d.ehead;
-  g.etail = (PAC_GLOBAL uint32_t *)d.etail;
   g.sup = (PAC_GLOBAL uint32_t *)d.sup;
   g.pword = (const PAC_GLOBAL unsigned long long *)d.pword;
   g.hitw = (PAC_GLOBAL unsigned long long *)d.hitw;
@@ -148,15 +146,6 @@ struct alignas(8) Smem {
   uint32_t homeless;
   uint32_t rowtot;  // the row's triangle count (LCC mode)
 };
-
-// Per-edge credits (SUP mode).  A row call that covers all of the row's pairs is the only
-// writer of the row's edges, so it stores; a partial call (a split row) adds atomically.
-__device__ __forceinline__ void credit_edge(PAC_GLOBAL uint32_t *p, uint32_t v, bool whole) {
-  if (whole)
-    *p = v;
-  else
-    gadd(p, v);
-}
 
 __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
@@ -326,7 +315,6 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
   if (tid == 0) sm.rowtot = 0;
   __syncthreads();
   if (k_hi > np) k_hi = np;
-  const bool whole = k_lo == 0 && k_hi == np;
   uint32_t row_hits = 0;
   for_each_pair<WAVES, MODE>(G, p0, k_lo, k_hi, wave, lane,
                              [&](uint32_t y, uint32_t e, uint32_t b0, uint32_t nb, uint32_t p) {
@@ -346,7 +334,7 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
         else if (MODE == kModeLCC)
           gadd(&G.tvert[y], (unsigned long long)ch);  // y->r: credited to y
         else
-          G.etail[e] = ch;  // y->r: credited to the edge (each pair is visited once)
+          gadd(&G.sup[e], ch);  // y->r: credited to the edge
       }
     }
     row_hits += h;
@@ -361,7 +349,7 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
     for (uint32_t i = tid; i < nA; i += BS) {
       const uint32_t v = cnt[i];
       if (v && MODE == kModeLCC) gadd(&G.tvert[A[i]], (unsigned long long)v);  // t(A[i])
-      else if (v) credit_edge(&G.ehead[a0 + i], v, whole);
+      else if (v) gadd(&G.sup[a0 + i], v);  // the edge r->A[i]
     }
     // an atomic even for a whole row: other rows' tail credits to r arrive meanwhile
     if (MODE == kModeLCC && tid == 0 && sm.rowtot) gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
@@ -430,7 +418,6 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   uint32_t *const tab = H.tab + share * (S::kSlots / WAVES);
   uint32_t *const cnt16 = MODE == kModeTC ? H.cnt16 : H.cnt16 + share * (S::kSlots / WAVES / 2);
   if (c_hi > np) c_hi = np;
-  const bool whole = c_lo == 0 && c_hi == np;
   // The wave's first batch of pairs: partners and edges are loaded before the table is
   // built, the bounds of their lists after the insertions, so both latencies overlap it.
   uint32_t kl = c_lo + wave + PS * lane, yl = 0, el = 0, bl = 0, nl = 0;
@@ -635,7 +622,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       } else if (MODE == kModeLCC) {
         gadd(&G.tvert[yl], (unsigned long long)phl);
       } else {
-        G.etail[el] = phl;
+        gadd(&G.sup[el], phl);
       }
     }
     if (nvalid < 64) break;
@@ -654,7 +641,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       find(x, word, inc);
       const uint32_t v = (cnt16[word] >> (inc == 1u ? 0u : 16u)) & 0xFFFFu;
       if (v && MODE == kModeLCC) gadd(&G.tvert[x], (unsigned long long)v);
-      else if (v) credit_edge(&G.ehead[a0 + i], v, whole);
+      else if (v) gadd(&G.sup[a0 + i], v);  // the edge r->A[i]
     }
   }
   if (MODE == kModeLCC) {
@@ -686,7 +673,7 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
       if (i < nA && A[i] == w) {
         ++h;
         if (MODE == kModeLCC) gadd(&G.tvert[w], 1ull);  // t(w)
-        if (MODE == kModeSUP) gadd(&G.ehead[a0 + i], 1u);
+        if (MODE == kModeSUP) gadd(&G.sup[a0 + i], 1u);  // edge r->w
         if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);  // edge y->w
       }
     }
@@ -695,10 +682,8 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
       if (lane == 0 && ch) {
         if (MODE == kModeLCC)
           gadd(&G.tvert[y], (unsigned long long)ch);  // t(y), either direction
-        else if (e - a0 < nA)
-          gadd(&G.ehead[e], ch);  // r->y
         else
-          G.etail[e] = ch;        // y->r, visited once
+          gadd(&G.sup[e], ch);  // the pair's edge, r->y or y->r
       }
     }
     row_hits += h;
@@ -927,14 +912,14 @@ __global__ void k_lcc(const unsigned long long *__restrict__ tvert,
 }
 
 // Per-edge Jaccard (pair formula of Graph.cpp:808-809 on every oriented edge).
-// The support of edge e = its credits as an edge of the staged list (ehead), as a pair's
-// edge (etail) and as an edge of the streamed lists (sup); the total is kept in sup.
+// The support of edge e (in sup): its credits as an edge of the staged list and as a
+// pair's edge, added during the launch, and as an edge of the streamed lists, added from
+// the hit words after it.
 // Each wave takes 64 consecutive vertices and walks their out-edges (contiguous in the
 // oriented CSR) 64 at a time; an edge's tail is found by a binary search over the
 // vertices' first edges held one per lane.
 __global__ void __launch_bounds__(256)
 k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
-          const uint32_t *__restrict__ ehead, const uint32_t *__restrict__ etail,
           uint32_t *__restrict__ sup, const uint32_t *__restrict__ deg,
           const uint32_t *__restrict__ hdeg, uint32_t V, int32_t *src, int32_t *dst,
           uint32_t *inter, uint32_t *uni, float *ratio) {
@@ -969,7 +954,7 @@ k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
         v[u] = c[u] = dv[u] = 0;
         if (e[u] < E1) {
           if (dst) v[u] = oadj[e[u]];
-          c[u] = ehead[e[u]] + etail[e[u]] + sup[e[u]];
+          c[u] = sup[e[u]];
           dv[u] = hdeg[e[u]];  // d(v), in edge order (no gather)
         }
       }
@@ -1368,8 +1353,6 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   d.rows = g->rows.as<const uint32_t>();
   d.total = g->total.as<unsigned long long>();
   d.tvert = g->tvert.as<unsigned long long>();
-  d.ehead = g->ehead.as<uint32_t>();
-  d.etail = g->etail.as<uint32_t>();
   d.sup = g->sup.as<uint32_t>();
   d.pword = g->pword.as<const unsigned long long>();
   d.hitw = g->hitw.as<unsigned long long>();
@@ -1429,11 +1412,7 @@ int alloc_outputs(Graph *const *gs, uint32_t G, int mode) {
     PAC_TRY(g->total.reserve(sizeof(unsigned long long)));
     if (mode == kModeLCC)
       PAC_TRY(g->tvert.reserve(sizeof(unsigned long long) * (size_t)(g->V ? g->V : 1)));
-    if (mode == kModeSUP) {
-      PAC_TRY(g->ehead.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
-      PAC_TRY(g->etail.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
-      PAC_TRY(g->sup.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
-    }
+    if (mode == kModeSUP) PAC_TRY(g->sup.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
   }
   return PAC_OK;
 }
@@ -1448,10 +1427,6 @@ int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
     if (mode == kModeLCC && g->V)
       hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
                          g->tvert.as<unsigned long long>(), (uint64_t)g->V);
-    if (mode == kModeSUP && g->m) {
-      PAC_TRY(fill_u32(ctx->stream, g->ehead.as<uint32_t>(), g->m, 0u));
-      PAC_TRY(fill_u32(ctx->stream, g->etail.as<uint32_t>(), g->m, 0u));
-    }
     if (mode == kModeSUP) PAC_TRY(fill_u32(ctx->stream, g->sup.as<uint32_t>(), g->m, 0u));
     PAC_CHECK_HIP(hipGetLastError());
   }
@@ -1646,7 +1621,6 @@ int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *i
   }
   hipLaunchKernelGGL(k_jaccard, dim3(grid_for(g->V, 4 * 64)), dim3(256), 0, ctx->stream,
                      g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
-                     g->ehead.as<const uint32_t>(), g->etail.as<const uint32_t>(),
                      g->sup.as<uint32_t>(), g->deg.as<const uint32_t>(),
                      g->hdeg.as<const uint32_t>(), g->V, src, dst, inter, uni, ratio);
   PAC_CHECK_HIP(hipGetLastError());
