@@ -614,7 +614,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     if (MODE != kModeTC && phl != 0) {
       // The batch's pair credits, one lane per pair, all at once (one table lookup per
       // lane rather than one wave-wide lookup per pair).  t(y) / the pair's edge: r->y adds
-      // to y's slot in the table, y->r is credited to y (each pair is visited once).
+      // to y's slot in the table; y->r goes to t(y) in LCC mode and to the edge's support in
+      // SUP mode, both atomically (each pair is visited once).
       if (el - a0 < nA) {
         uint32_t word, inc;
         find(yl, word, inc);  // y = A[e - a0] is in the table
