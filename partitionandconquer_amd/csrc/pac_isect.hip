@@ -1029,42 +1029,77 @@ __global__ void k_pairs(const uint32_t *__restrict__ off, const uint32_t *__rest
                         const uint32_t *__restrict__ part, uint32_t pid) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wpb = blockDim.x >> 6;
-  for (uint64_t p = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); p < n;
-       p += (uint64_t)gridDim.x * wpb) {
-    uint32_t u, v;
-    if (q) {
-      u = (uint32_t)q[p / nq];
-      v = (uint32_t)q[p % nq];
-    } else {
-      u = (uint32_t)pu[p];
-      v = (uint32_t)pv[p];
+  // Each wave takes 64 consecutive pairs, one per lane.  A lane whose two lists both have
+  // at most kLanePair entries merges them itself (most pairs of random vertices); the
+  // others are then done one at a time by the whole wave.
+  constexpr uint32_t kLanePair = 16;
+  for (uint64_t p0 = ((uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6)) * 64u; p0 < n;
+       p0 += (uint64_t)gridDim.x * wpb * 64u) {
+    const uint64_t p = p0 + lane;
+    const bool valid = p < n;
+    uint32_t a0 = 0, na = 0, b0 = 0, nb = 0;
+    int32_t lim = 0x7FFFFFFF;
+    if (valid) {
+      uint32_t u, v;
+      if (q) {
+        u = (uint32_t)q[p / nq];
+        v = (uint32_t)q[p % nq];
+      } else {
+        u = (uint32_t)pu[p];
+        v = (uint32_t)pv[p];
+      }
+      a0 = off[u];
+      na = off[u + 1] - a0;
+      b0 = off[v];
+      nb = off[v + 1] - b0;
+      if (thr) lim = thr[p];
     }
-    uint32_t a0 = off[u], na = off[u + 1] - a0, b0 = off[v], nb = off[v + 1] - b0;
     const uint32_t du = na, dv = nb;
     if (na > nb) {
       uint32_t t = a0; a0 = b0; b0 = t;
       t = na; na = nb; nb = t;
     }
-    const int32_t lim = thr ? thr[p] : 0x7FFFFFFF;
+    const bool alone = valid && nb <= kLanePair;
     uint32_t h = 0;
-    if (nb) {
-      const uint32_t *B = adj + b0;
-      const uint64_t merge_steps = ((uint64_t)na + nb + 63u) >> 6;
-      const uint64_t search_steps = (((uint64_t)na + 63u) >> 6) * (32u - __clz(nb));
-      if (3u * merge_steps < 2u * search_steps) {  // wave-uniform
-        h = chunk_merge_hits(adj + a0, na, B, nb, lane, lim, part, pid);
-      } else {
-        for (uint32_t i = lane; i < na; i += 64) {
-          const uint32_t x = adj[a0 + i];
-          const uint32_t j = lower_bound_u32(B, nb, x);
-          bool hit = j < nb && B[j] == x && (int32_t)x < lim;
+    if (alone) {  // two-pointer merge of at most 2 x kLanePair entries
+      uint32_t i = 0, j = 0;
+      while (i < na && j < nb) {
+        const uint32_t x = adj[a0 + i], y = adj[b0 + j];
+        if (x == y) {
+          bool hit = (int32_t)x < lim;
           if (part && hit) hit = part[x] == pid;
           h += hit ? 1u : 0u;
         }
+        i += x <= y ? 1u : 0u;
+        j += y <= x ? 1u : 0u;
       }
     }
-    h = wave_sum(h);
-    if (lane == 0) {
+    unsigned long long rest = __ballot(valid && !alone);
+    while (rest) {  // wave-uniform
+      const uint32_t k = (uint32_t)__builtin_ctzll(rest);
+      rest &= rest - 1ull;
+      const uint32_t A0 = readlane(a0, k), NA = readlane(na, k);
+      const uint32_t B0 = readlane(b0, k), NB = readlane(nb, k);
+      const int32_t L = (int32_t)readlane((uint32_t)lim, k);
+      const uint32_t *B = adj + B0;
+      uint32_t hk = 0;
+      const uint64_t merge_steps = ((uint64_t)NA + NB + 63u) >> 6;
+      const uint64_t search_steps = (((uint64_t)NA + 63u) >> 6) * (32u - __clz(NB));
+      if (3u * merge_steps < 2u * search_steps) {  // wave-uniform
+        hk = chunk_merge_hits(adj + A0, NA, B, NB, lane, L, part, pid);
+      } else {
+        for (uint32_t i = lane; i < NA; i += 64) {
+          const uint32_t x = adj[A0 + i];
+          const uint32_t j = lower_bound_u32(B, NB, x);
+          bool hit = j < NB && B[j] == x && (int32_t)x < L;
+          if (part && hit) hit = part[x] == pid;
+          hk += hit ? 1u : 0u;
+        }
+      }
+      hk = wave_sum(hk);
+      h = lane == k ? hk : h;
+    }
+    if (valid) {
       if (res) res[p] = h;
       if (jac) {
         const double c = (double)h;
@@ -1631,7 +1666,7 @@ int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *i
 int pair_counts(Ctx *ctx, Graph *g, const int32_t *d_u, const int32_t *d_v, uint64_t n,
                 uint64_t *d_res, const int32_t *d_thr, const uint32_t *d_part, uint32_t pid) {
   if (n == 0) return PAC_OK;
-  hipLaunchKernelGGL(k_pairs, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(k_pairs, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream,  // 64 pairs a wave
                      g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(), d_u, d_v, n,
                      reinterpret_cast<unsigned long long *>(d_res), nullptr, nullptr, 1u, d_thr,
                      d_part, pid);
@@ -1642,7 +1677,7 @@ int pair_counts(Ctx *ctx, Graph *g, const int32_t *d_u, const int32_t *d_v, uint
 int query_matrix(Ctx *ctx, Graph *g, const int32_t *d_q, uint32_t nq, double *d_res) {
   const uint64_t n = (uint64_t)nq * nq;
   if (n == 0) return PAC_OK;
-  hipLaunchKernelGGL(k_pairs, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(k_pairs, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream,  // 64 pairs a wave
                      g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(), nullptr, nullptr,
                      n, nullptr, d_res, d_q, nq, nullptr, nullptr, 0u);
   PAC_CHECK_HIP(hipGetLastError());
