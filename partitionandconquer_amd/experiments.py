@@ -190,13 +190,23 @@ def exp_jaccard(ctx, scale=25, partitions=4096, sample=200000, cpu_budget=15.0,
     return res
 
 
-def exp_next(ctx, scale=22, seed=22, npairs=4_000_000, nkway=1_000_000, sample=200_000):
-    from oracle import oracle as O
-    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+def exp_next(ctx, scale=22, seed=22, npairs=4_000_000, nkway=1_000_000, sample=200_000,
+             profile_only=False):
     rng = np.random.default_rng(seed)
-    res = {"config": f"rmat-s{scale}-ef16-next", "cpu_threads": nt}
     g = pac.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed=seed)
     vl, el = g.csr()
+    if profile_only:  # three timed counts of 4M random adjacent pairs, for a kernel trace
+        e = rng.integers(0, len(el), npairs)
+        u = (np.searchsorted(vl.astype(np.int64), e, side="right") - 1).astype(np.int32)
+        v = el[e].astype(np.int32)
+        ms = []
+        for _ in range(3):
+            g.cnt_common_pairs(u, v)
+            ms.append(round(ctx.last_timing().total_ms, 3))
+        return {"config": f"rmat-s{scale}-ef16-adjacent-pairs", "pairs": npairs, "device_ms": ms}
+    from oracle import oracle as O
+    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res = {"config": f"rmat-s{scale}-ef16-next", "cpu_threads": nt}
     ref = O.CSR(vl, el)
     V = len(vl) - 1
     vli = vl.astype(np.int64)
@@ -347,7 +357,7 @@ def main(argv=None):
     ap.add_argument("--scale", type=int, default=None)
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
-                    help="jaccard: only three timed solves (for rocprofv3)")
+                    help="jaccard / next: only three timed solves / pair counts (for rocprofv3)")
     a = ap.parse_args(argv)
     ctx = pac.Context(0)
     if a.which == "scale22":
@@ -355,7 +365,7 @@ def main(argv=None):
     elif a.which == "batch":
         r = exp_batch(ctx, a.graphs, not a.no_check)
     elif a.which == "next":
-        r = exp_next(ctx, a.scale or 22)
+        r = exp_next(ctx, a.scale or 22, profile_only=a.profile_only)
     else:
         r = exp_jaccard(ctx, a.scale or 25, profile_only=a.profile_only)
     print(json.dumps(r))
