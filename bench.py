@@ -12,12 +12,20 @@ per-vertex counts), LCC epilogue -- the analogue of the reference's timed region
 
 The headline path is the batched persistent kernel replayed from a hipGraph with
 learned dispatch (the trained MLP in partitionandconquer_amd/dispatch_mlp.bin; the
-fixed heuristic when no model file exists).  The plain path (one launch per partition
-group, one workgroup per row, fixed heuristic) is timed beside it for the speedup.  GTEPS = m / t / 1e9 with m the
+fixed heuristic when no model file exists).  The tile is autotuned for this mode and dispatch first.  The plain path (one launch per
+partition group, one workgroup per row, fixed heuristic) is timed beside it for the
+speedup, and the fixed heuristic on the headline's own launch path for what the learned
+dispatch alone gains (`learned_vs_fixed_batched`).  GTEPS = m / t / 1e9 with m the
 undirected edge count.  The roofline entry prices the dominant kernel's algorithmic
-bytes 4*W, W = sum over oriented u->v of (d+(u) + d+(v)), against 8 TB/s; `traffic` is
-the kernel's measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, committed
-under profiles/) and `traffic_GBps` that traffic over the live kernel time.  The CPU
+bytes -- what the hash path must move per launch: each pair's streamed list (the shorter
+of the two out-lists, 4 B an element) with its partner ID, edge index and list bounds,
+each row's staged list once per work item, the row and item metadata, and the per-vertex
+counts read and written once (GraphInfo.algorithmic_bytes) -- against 8 TB/s, over the
+kernel's live duration from HIP events; `merge_equivalent_bytes` = 4*W, W = sum over
+oriented u->v of (d+(u) + d+(v)), is the input length of every oriented merge (both
+lists of every pair).  `traffic` is the kernel's measured HBM bytes per launch
+(rocprofv3 FETCH_SIZE + WRITE_SIZE, committed under profiles/) and `traffic_GBps` that
+traffic over the live kernel time.  The CPU
 baseline is the oracle's restatement of Graph::solve_triangle_counting_merge and
 Graph::solve_lcc_merge (Graph.cpp:355-368, 828-840) on a bounded sample of the same
 graph's rows, stratified by estimated row cost and extrapolated to all rows.
@@ -220,9 +228,13 @@ def main():
                           "kernel_ms": ctx.last_timing().kernel_ms}))
         return
 
-    # correctness guard on the timed configuration: all strategies agree
-    log("phase: reference count (bitmap, persistent)")
-    T_ref = g.tc(pac.FORCE_BITMAP, pac.LAUNCH_PERSISTENT)
+    # tile for the timed mode and dispatch (LCC mode: per-vertex counts)
+    log("phase: autotune (LCC mode)")
+    tile, tile_ms = ctx.autotune([g], pac.MODE_LCC, dispatch)
+    # correctness guard on the timed configuration: the merge-path kernel (no hash table)
+    # and the headline path agree
+    log("phase: reference count (merge path, persistent)")
+    T_ref = g.tc(pac.FORCE_MERGE, pac.LAUNCH_PERSISTENT)
     log("phase: headline path once (graph)")
     T_head = g.lcc_device(dispatch, pac.LAUNCH_GRAPH)
     assert T_head == T_ref, (T_head, T_ref)
@@ -240,17 +252,21 @@ def main():
         log("phase: plain fixed-heuristic path")
         wall_pf, dev_pf, _ = timed_steps(ctx, g, pac.DISPATCH_FIXED, pac.LAUNCH_PLAIN,
                                          max(3, a.steps // 2), 1)
+        # the fixed heuristic on the headline's own launch path: what the learned
+        # dispatch alone gains
         _, dev_bf, _ = timed_steps(ctx, g, pac.DISPATCH_FIXED, pac.LAUNCH_GRAPH, max(3, a.steps // 2), 1)
         out["plain_fixed_ms"] = round(wall_pf, 3)
         out["batched_fixed_device_ms"] = round(dev_bf, 3)
         out["speedup_vs_plain_fixed"] = round(wall_pf / ms_step, 3)
+        out["learned_vs_fixed_batched"] = round(dev_bf / dev, 4)
     log("phase: stream copy")
     copy_bytes = 1 << 30
     copy_ms = ctx.stream_copy(copy_bytes, 5)
     stream_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
 
     kms = ker if ker > 0 else dev_p
-    achieved = 4.0 * W / (kms * 1e-3)
+    alg_bytes = info.algorithmic_bytes(per_vertex=True)
+    achieved = alg_bytes / (kms * 1e-3)
     traffic = measured_traffic(scale, a.partitions)
     res = {
         "metric": "TC/LCC GTEPS at RMAT scale 24 (exact); learned vs fixed dispatch speedup",
@@ -269,11 +285,13 @@ def main():
                    "abcd": [*abc, round(1 - sum(abc), 2)], "seed": seed,
                    "partitions": a.partitions, "m": m, "W": W,
                    "dispatch": "learned" if learned else "fixed",
-                   "launch": "persistent+hipGraph", "tile": ctx.get_tile()},
+                   "launch": "persistent+hipGraph", "tile": tile,
+                   "tile_ms": [round(x, 3) for x in tile_ms]},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                     "kernel_ms": round(ker, 3), "kernel": "k_persistent (LCC mode)",
-                     "bytes_per_launch": 4 * W,
+                     "kernel_ms": round(kms, 3), "kernel": "k_persistent (LCC mode)",
+                     "bytes_per_launch": alg_bytes,
+                     "merge_equivalent_bytes": 4 * W,
                      "traffic_GBps": round(traffic / (kms * 1e-3) / 1e9, 1) if traffic else None},
         "device_ms_per_step": round(dev, 3),
         "stream_copy_GBps": round(stream_gbs, 1),

@@ -1,9 +1,9 @@
 """Measurements of the non-headline configurations (one JSON line each).
 
-    python -m partitionandconquer_amd.experiments scale22   # plain vs batched, every strategy
-    python -m partitionandconquer_amd.experiments batch     # 256 uneven graphs, autotuned tile
-    python -m partitionandconquer_amd.experiments jaccard   # per-edge Jaccard at R-MAT scale 25
-    python -m partitionandconquer_amd.experiments next      # ingestion, queries, k-way, squares
+    python experiments.py scale22   # plain vs batched, every strategy
+    python experiments.py batch     # 256 uneven graphs, autotuned tile
+    python experiments.py jaccard   # per-edge Jaccard at R-MAT scale 25
+    python experiments.py next      # ingestion, queries, k-way, squares
 
 scale22: R-MAT scale 22, ef 16, (0.57,0.19,0.19,0.05), seed 22, 1024 contiguous
   partitions; TC + LCC device time of the plain launch (one launch per partition
@@ -38,7 +38,7 @@ import time
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import partitionandconquer_amd as pac  # noqa: E402
@@ -125,7 +125,7 @@ def exp_batch(ctx, G=256, check=True):
         gs.append(g)
     gen_s = time.time() - t0
     m_total = sum(g.info().num_edges for g in gs)
-    tile, tile_ms = ctx.autotune(gs)
+    tile, tile_ms = ctx.autotune(gs, pac.MODE_TC, pac.DISPATCH_FIXED)
     tot = ctx.tc_batch(gs, pac.DISPATCH_FIXED, pac.LAUNCH_PERSISTENT)
     res = {"config": f"batch-{G}", "graphs": G, "m_total": m_total, "gen_s": round(gen_s, 1),
            "tile": tile, "tile_ms": [round(x, 3) for x in tile_ms]}

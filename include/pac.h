@@ -73,6 +73,19 @@ typedef struct {
   uint64_t num_partitions; /* 0 before pac_graph_prepare                             */
   uint64_t num_items;      /* persistent work items after pac_graph_prepare          */
   double prepare_ms;       /* GPU time of the last pac_graph_prepare                 */
+  /* The intersection pairs (after pac_graph_prepare; 0 before).  Each oriented edge
+   * t->h is one pair, owned by the endpoint with the longer out-list (ties: the tail),
+   * which stages that list while the partner's shorter list is streamed; pairs whose
+   * shorter list is empty are dropped.  These are the work model of the hash path:
+   * its algorithmic bytes per launch are 4*streamed + 4*staged_item_elems +
+   * 12*num_pairs + 20*num_rows + 32*num_items (TC), plus 4*num_pairs (edge indices) and
+   * 16*num_vertices (per-vertex counts read and written once) with per-vertex counts. */
+  uint64_t num_pairs;
+  uint64_t num_rows;          /* vertices owning at least one pair                     */
+  uint64_t streamed_elems;    /* sum over pairs of d+(partner) = sum of min(d+(t),d+(h)) */
+  uint64_t staged_elems;      /* sum of d+(x) over the owning vertices                 */
+  uint64_t staged_item_elems; /* the same, once per work item holding the row (a row  *
+                               * split over several items is staged by each)          */
 } pac_graph_info;
 
 typedef struct {
@@ -258,12 +271,21 @@ int pac_tc_batch(pac_ctx *ctx, pac_graph *const *gs, uint32_t G, int dispatch, i
 /* ---- autotuning ------------------------------------------------------------------ */
 /* Tile configurations (waves per row, LDS list capacity). */
 #define PAC_NUM_TILES 4
-/* Times every tile configuration of the persistent kernel on the given graphs and
- * keeps the fastest for this context; ms_out (PAC_NUM_TILES, may be NULL) receives
- * the median times.  cache_path (may be NULL) stores/loads the choice keyed by the
- * graphs' shape. Returns the chosen tile id in *tile. */
-int pac_autotune(pac_ctx *ctx, pac_graph *const *gs, uint32_t G, const char *cache_path,
-                 int *tile, double *ms_out);
+/* What a solve accumulates: the total only (pac_tc without per-vertex counts, batches),
+ * per-vertex counts (pac_tc with per_vertex, pac_lcc) or per-edge supports
+ * (pac_jaccard_edges).  The intersection kernel's registers and LDS differ by mode. */
+enum { PAC_MODE_TC = 0, PAC_MODE_LCC = 1, PAC_MODE_JACCARD = 2 };
+/* No reference counterpart (the reference's knobs are the partition count and the
+ * thread count, scripts/run.py:36, test.cpp:371-379).  Times the intersection launch of
+ * the persistent kernel (HIP events around it, median of 3 after a warm-up) for every
+ * tile configuration on the given graphs, in `mode` with `dispatch`, and keeps the
+ * fastest for this context.  ms_out (PAC_NUM_TILES, may be NULL) receives the times.
+ * cache_path (may be NULL) is read first: a line for the same graphs' shape, mode and
+ * dispatch gives the tile and the times without timing anything; otherwise the result
+ * is appended to it.  The context's tile is left unchanged when an error is returned.
+ * Returns the chosen tile id in *tile. */
+int pac_autotune(pac_ctx *ctx, pac_graph *const *gs, uint32_t G, int mode, int dispatch,
+                 const char *cache_path, int *tile, double *ms_out);
 int pac_set_tile(pac_ctx *ctx, int tile);
 int pac_get_tile(const pac_ctx *ctx, int *tile);
 

@@ -24,6 +24,10 @@ f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 
 
+class Pairs(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("pairs", "rows", "streamed", "staged")]
+
+
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in
                 ("V", "nnz", "m", "max_deg", "max_outdeg", "W", "sum_outdeg_sq", "sum_deg_sq")]
@@ -72,6 +76,9 @@ def lib():
         L.orc_pgraph_tc_merge_partial.restype = C.c_uint64
         L.orc_pgraph_lcc_merge.argtypes = [C.c_void_p, f64p]
         L.orc_pgraph_free.argtypes = [C.c_void_p]
+        L.orc_pgraph_evaluate_partition.argtypes = [C.c_void_p, C.POINTER(C.c_uint64),
+                                                    C.POINTER(C.c_uint64)]
+        L.orc_pgraph_evaluate_partition.restype = C.c_double
         L.orc_random_partition.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, i32p]
         L.orc_contiguous_partition.argtypes = [C.c_uint64, C.c_uint64, i32p]
         L.orc_gen_rmat.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
@@ -80,6 +87,9 @@ def lib():
         L.orc_gen_er.argtypes = [C.c_uint64, C.c_double, C.c_uint64, C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
         L.orc_oriented_stats.argtypes = [u64p, i32p, C.c_uint64, C.POINTER(Stats)]
+        L.orc_pair_stats.argtypes = [u64p, i32p, C.c_uint64, C.POINTER(Pairs)]
+        L.orc_tc_oriented.argtypes = [u64p, i32p, C.c_uint64, C.c_int]
+        L.orc_tc_oriented.restype = C.c_uint64
         L.orc_mlp_forward.argtypes = [f32p, f32p, C.c_uint64, f32p]
         L.orc_mlp_backward.argtypes = [f32p, f32p, i32p, C.c_void_p, C.c_uint64, f32p]
         L.orc_mlp_backward.restype = C.c_float
@@ -308,6 +318,13 @@ class PGraph:
     def square_counting(self) -> int:
         return lib().orc_pgraph_square_counting(self._h)
 
+    def evaluate_partition(self):
+        """Partitioned_Graph::evaluate_partition, Graph.cpp:1460-1473: (balance ratio,
+        max_part_deg, max_deg)."""
+        mp, md = C.c_uint64(), C.c_uint64()
+        r = lib().orc_pgraph_evaluate_partition(self._h, C.byref(mp), C.byref(md))
+        return r, mp.value, md.value
+
     # per-partition queries, Graph.cpp:1071-1195, 1475-1498
     def cnt_merge(self, u, v, k) -> int:
         return lib().orc_pgraph_cnt(self._h, int(u), int(v), int(k), 0)
@@ -387,6 +404,21 @@ def oriented_stats(g: CSR) -> dict:
     s = Stats()
     lib().orc_oriented_stats(g.vlist, g.elist, g.V, C.byref(s))
     return {n: getattr(s, n) for n, _ in Stats._fields_}
+
+
+def pair_stats(g: CSR) -> dict:
+    """Intersection pairs of the degree orientation (the hash path's work model):
+    pairs, owning rows, streamed elements (sum of min(d+(t), d+(h)) over oriented edges)
+    and staged elements (sum of d+ over the owning rows)."""
+    s = Pairs()
+    lib().orc_pair_stats(g.vlist, g.elist, g.V, C.byref(s))
+    return {n: getattr(s, n) for n, _ in Pairs._fields_}
+
+
+def tc_oriented(g: CSR, nt=0) -> int:
+    """T by the degree-ordered forward algorithm: a fast exact checker, pinned against
+    tc_merge (the restatement of Graph.cpp:355-368) by the CPU tests."""
+    return int(lib().orc_tc_oriented(g.vlist, g.elist, g.V, _nt(nt)))
 
 
 N_PARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 3 * 64 + 3

@@ -413,6 +413,27 @@ static inline uint64_t pg_merge(const orc_pgraph *g, int32_t u, int32_t v, uint6
   return merge_count(g->el.data() + ub, ue - ub, g->el.data() + vb, ve - vb);
 }
 
+// Partitioned_Graph::evaluate_partition, Graph.cpp:1460-1473: the largest per-partition
+// degree |N_i(u)| and the largest degree (summed over the partitions), balance ratio
+// P * max_part_deg / max_deg in double (0/0 = NaN for a graph without edges).
+double orc_pgraph_evaluate_partition(const orc_pgraph *g, uint64_t *max_part_deg,
+                                     uint64_t *max_deg) {
+  const uint64_t V = g->V, P = g->P;
+  uint64_t mp = 0, md = 0;
+  for (uint64_t u = 0; u < V; ++u) {
+    uint64_t du = 0;
+    for (uint64_t i = 0; i < P; ++i) {
+      const uint64_t pd = g->vl[i * V + u + 1] - g->vl[i * V + u];
+      du += pd;
+      mp = std::max(mp, pd);
+    }
+    md = std::max(md, du);
+  }
+  *max_part_deg = mp;
+  *max_deg = md;
+  return (double)(P * mp) / (double)md;
+}
+
 // Per-partition pair counts, Graph.cpp:1071-1085 (galloping), 1155-1173 (merge).
 uint64_t orc_pgraph_cnt(const orc_pgraph *g, int32_t u, int32_t v, uint64_t k, int galloping) {
   return galloping ? pg_gallop(g, u, v, k) : pg_merge(g, u, v, k);
@@ -702,6 +723,93 @@ void orc_oriented_stats(const uint64_t *vl, const int32_t *el, uint64_t V, orc_s
   s->W = W;
   s->sum_outdeg_sq = sosq;
   s->sum_deg_sq = sdsq;
+}
+
+namespace {
+// d+(u) under the orientation u->v iff (d(u),u) < (d(v),v).
+std::vector<uint64_t> out_degrees(const uint64_t *vl, const int32_t *el, uint64_t V) {
+  std::vector<uint64_t> dplus(V, 0);
+#pragma omp parallel for
+  for (int64_t u = 0; u < (int64_t)V; ++u) {
+    const uint64_t du = deg(vl, u);
+    uint64_t c = 0;
+    for (uint64_t i = vl[u]; i < vl[u + 1]; ++i) {
+      const int64_t v = el[i];
+      const uint64_t dv = deg(vl, v);
+      if (du < dv || (du == dv && u < v)) ++c;
+    }
+    dplus[u] = c;
+  }
+  return dplus;
+}
+
+inline bool oriented(const uint64_t *vl, int64_t u, int64_t v) {
+  const uint64_t du = deg(vl, u), dv = deg(vl, v);
+  return du < dv || (du == dv && u < v);
+}
+}  // namespace
+
+void orc_pair_stats(const uint64_t *vl, const int32_t *el, uint64_t V, orc_pairs *s) {
+  const std::vector<uint64_t> dplus = out_degrees(vl, el, V);
+  std::vector<uint8_t> owner(V, 0);
+  uint64_t pairs = 0, streamed = 0;
+#pragma omp parallel for reduction(+ : pairs, streamed)
+  for (int64_t t = 0; t < (int64_t)V; ++t) {
+    for (uint64_t i = vl[t]; i < vl[t + 1]; ++i) {
+      const int64_t h = el[i];
+      if (!oriented(vl, t, h)) continue;
+      const uint64_t mn = std::min(dplus[t], dplus[h]);
+      streamed += mn;
+      if (mn == 0) continue;
+      ++pairs;
+      // the longer out-list owns the pair, ties going to the tail (a byte store: races
+      // only ever write 1)
+      const int64_t x = dplus[h] > dplus[t] ? h : t;
+      __atomic_store_n(&owner[x], (uint8_t)1, __ATOMIC_RELAXED);
+    }
+  }
+  uint64_t rows = 0, staged = 0;
+#pragma omp parallel for reduction(+ : rows, staged)
+  for (int64_t x = 0; x < (int64_t)V; ++x)
+    if (owner[x]) {
+      ++rows;
+      staged += dplus[x];
+    }
+  s->pairs = pairs;
+  s->rows = rows;
+  s->streamed = streamed;
+  s->staged = staged;
+}
+
+uint64_t orc_tc_oriented(const uint64_t *vl, const int32_t *el, uint64_t V, int nt) {
+  // oriented CSR: each row keeps its out-neighbours in ID order
+  const std::vector<uint64_t> dplus = out_degrees(vl, el, V);
+  std::vector<uint64_t> ooff(V + 1, 0);
+  for (uint64_t u = 0; u < V; ++u) ooff[u + 1] = ooff[u] + dplus[u];
+  std::vector<int32_t> oadj(ooff[V]);
+#pragma omp parallel for
+  for (int64_t u = 0; u < (int64_t)V; ++u) {
+    uint64_t k = ooff[u];
+    for (uint64_t i = vl[u]; i < vl[u + 1]; ++i)
+      if (oriented(vl, u, el[i])) oadj[k++] = el[i];
+  }
+  // Per thread, N+(u) is marked in a byte array over the vertices; each out-neighbour
+  // v's list is then scanned against the marks.
+  uint64_t res = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : res)
+  {
+    std::vector<uint8_t> mark(V, 0);
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t u = 0; u < (int64_t)V; ++u) {
+      for (uint64_t i = ooff[u]; i < ooff[u + 1]; ++i) mark[oadj[i]] = 1;
+      for (uint64_t i = ooff[u]; i < ooff[u + 1]; ++i) {
+        const int32_t v = oadj[i];
+        for (uint64_t j = ooff[v]; j < ooff[v + 1]; ++j) res += mark[oadj[j]];
+      }
+      for (uint64_t i = ooff[u]; i < ooff[u + 1]; ++i) mark[oadj[i]] = 0;
+    }
+  }
+  return res;
 }
 
 // ---- MLP ---------------------------------------------------------------------------

@@ -19,12 +19,17 @@
  *   orc_jsim_*           (A^2)_uv dense, networkx.jaccard_coefficient, closed forms
  *   orc_jaccard_edges    (A^2)_uv dense on the edge set
  *   orc_pgraph_*         partition-sum invariance against orc_tc_merge and
- *                        brute force, for random and contiguous vertex2partition
+ *                        brute force, for random and contiguous vertex2partition;
+ *                        evaluate_partition: K_n closed forms, numpy per-partition
+ *                        neighbour histograms
  *   orc_kway_*           Python set intersections of k neighbourhoods
  *   orc_friend_*         brute-force pairs over Python sets; closed form on K_n
  *   orc_pgraph_square_*  brute-force enumeration of 4-cycles (u,v,x,w) with
  *                        u > v > w > x on tiny graphs; K_n -> C(n,4); K_{a,b}
  *                        labelled side by side -> 0
+ *   orc_pair_stats       closed forms on K_n and stars; a per-vertex brute-force walk
+ *                        of the ownership rule on R-MAT and Erdos-Renyi graphs
+ *   orc_tc_oriented      golden fixtures; equality with orc_tc_merge on R-MAT / ER
  *   orc_gen_*            structural invariants (symmetric, sorted, unique, no
  *                        self-loops), edge-count expectations; the GPU generator
  *                        must reproduce orc_gen_rmat bit-exactly
@@ -109,6 +114,10 @@ uint64_t orc_pgraph_tc_galloping(const orc_pgraph *g);
 void orc_pgraph_lcc_merge(const orc_pgraph *g, double *res);
 /* Per-partition partial of Graph.cpp:1346-1362 for intersection partition i. */
 uint64_t orc_pgraph_tc_merge_partial(const orc_pgraph *g, uint64_t i);
+/* Partitioned_Graph::evaluate_partition, Graph.cpp:1460-1473: returns the balance ratio
+ * P * max_part_deg / max_deg (NaN without edges) and the two maxima. */
+double orc_pgraph_evaluate_partition(const orc_pgraph *g, uint64_t *max_part_deg,
+                                     uint64_t *max_deg);
 /* Per-partition pair counts |N_k(u) & N_k(v)|: Graph.cpp:1155-1173 (merge) and
  * 1071-1085 (galloping = nonzero). */
 uint64_t orc_pgraph_cnt(const orc_pgraph *g, int32_t u, int32_t v, uint64_t k, int galloping);
@@ -153,6 +162,28 @@ typedef struct {
   uint64_t sum_deg_sq;         /* sum_u d(u)^2 */
 } orc_stats;
 void orc_oriented_stats(const uint64_t *vl, const int32_t *el, uint64_t V, orc_stats *s);
+
+/* Intersection pairs under the same orientation.  Each oriented edge t->h is one pair,
+ * owned by the endpoint with the longer out-list (ties: the tail), whose list a GPU row
+ * stages while the partner's (the shorter) is streamed; a pair whose shorter list is
+ * empty holds no triangle and is dropped.  These counts are the work model of the
+ * hash path (its algorithmic bytes), not a reference function. */
+typedef struct {
+  uint64_t pairs;     /* #{oriented t->h : min(d+(t), d+(h)) > 0}                 */
+  uint64_t rows;      /* vertices owning at least one pair                         */
+  uint64_t streamed;  /* sum over oriented t->h of min(d+(t), d+(h))               */
+  uint64_t staged;    /* sum of d+(x) over the vertices x owning at least one pair */
+} orc_pairs;
+void orc_pair_stats(const uint64_t *vl, const int32_t *el, uint64_t V, orc_pairs *s);
+
+/* ---- a fast exact checker (not a restatement) ---------------------------------- */
+/* T by the degree-ordered forward algorithm (Schank and Wagner, 2005): the sum over
+ * oriented u->v of |N+(u) & N+(v)|, N+(u) marked in a per-thread byte array and each
+ * N+(v) scanned against the marks.  A triangle a < b < c in (degree, id) order is counted once, at the edge
+ * a->b, with c in both out-lists.  Used where orc_tc_merge's ID-ordered
+ * prefix merges take too long (hub rows at low IDs); the CPU tests pin it against
+ * orc_tc_merge. */
+uint64_t orc_tc_oriented(const uint64_t *vl, const int32_t *el, uint64_t V, int nt);
 
 /* ---- MLP oracle (16 -> 64 -> 64 -> 3, ReLU, softmax cross-entropy) ---------- */
 /* Weights are given as float (already rounded to bf16 by the caller when the GPU

@@ -456,6 +456,63 @@ int fixed_heuristic(const Graph *g, std::vector<uint8_t> &out) {
   return PAC_OK;
 }
 
+// One persistent launch (or hipGraph replay) over the concatenated work items of G
+// prepared graphs of this context, accumulating `mode`; totals (may be null) receives
+// each graph's T.  The intersection launch is timed with events unless replayed.
+int solve_batch(Ctx *c, pac_graph *const *hs, uint32_t G, int mode, int dispatch, int launch,
+                uint64_t *totals) {
+  if (!hs || G == 0 || launch == PAC_LAUNCH_PLAIN || launch > PAC_LAUNCH_GRAPH ||
+      mode < kModeTC || mode > kModeSUP) {
+    set_error("need graphs, a batched launch mode and a solve mode");
+    return PAC_ERR_ARG;
+  }
+  std::vector<Graph *> gs(G);
+  std::vector<uint8_t> strat;
+  {
+    // each graph's count accumulates in its own device total: a graph listed twice would
+    // receive both counts
+    std::vector<const pac_graph *> seen(hs, hs + G);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) {
+      set_error("a graph appears more than once in the batch");
+      return PAC_ERR_ARG;
+    }
+  }
+  for (uint32_t i = 0; i < G; ++i) {
+    if (!hs[i] || !hs[i]->g.prepared || hs[i]->g.ctx != c) {
+      set_error("graph %u is null, unprepared or from another context", i);
+      return PAC_ERR_STATE;
+    }
+    gs[i] = &hs[i]->g;
+    std::vector<uint8_t> s;
+    PAC_TRY(choose_strategies(gs[i], dispatch, s));
+    strat.insert(strat.end(), s.begin(), s.end());
+    if (mode == kModeSUP) PAC_TRY(build_hit_words(gs[i]));  // once per graph, untimed
+  }
+  PAC_CHECK_HIP(hipSetDevice(c->device));
+  Timed t(c);
+  PAC_TRY(t.mark(0));
+  if (launch != PAC_LAUNCH_GRAPH) PAC_TRY(zero_outputs(c, gs.data(), G, mode));
+  PAC_TRY(t.mark(1));
+  uint64_t launches = 0;
+  c->time_kernel = (launch == PAC_LAUNCH_PERSISTENT);
+  const int rc = launch_persistent(c, gs.data(), G, mode, strat.data(),
+                                   launch == PAC_LAUNCH_GRAPH, &launches);
+  t.kernel_timed = c->time_kernel;
+  c->time_kernel = false;
+  PAC_TRY(rc);
+  if (mode == kModeSUP)
+    for (uint32_t i = 0; i < G; ++i) PAC_TRY(sup_from_hit_words(c, gs[i]));
+  PAC_TRY(t.mark(2));
+  PAC_TRY(t.mark(3));
+  PAC_TRY(t.finish());
+  c->last.launches = launches;
+  if (totals)
+    for (uint32_t i = 0; i < G; ++i)
+      PAC_CHECK_HIP(hipMemcpy(&totals[i], gs[i]->total.p, 8, hipMemcpyDeviceToHost));
+  return PAC_OK;
+}
+
 }  // namespace pac
 
 using namespace pac;
@@ -793,6 +850,11 @@ int pac_graph_get_info(const pac_graph *h, pac_graph_info *out) {
   out->num_partitions = g.prepared ? g.P : 0;
   out->num_items = g.h_items.size();
   out->prepare_ms = g.prepare_ms;
+  out->num_pairs = g.prepared ? g.n_pairs : 0;
+  out->num_rows = g.prepared ? g.n_rows : 0;
+  out->streamed_elems = g.prepared ? g.streamed : 0;
+  out->staged_elems = g.prepared ? g.staged : 0;
+  out->staged_item_elems = g.prepared ? g.staged_items : 0;
   return PAC_OK;
 }
 
@@ -959,6 +1021,17 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
     }
     g->h_items.swap(sorted);
   }
+  // The pair work model: pairs, owning rows, streamed and staged elements (staged once
+  // per row, and once per item that holds the row).
+  g->n_pairs = poff[V];
+  g->n_rows = rowlist.size();
+  g->streamed = g->staged = g->staged_items = 0;
+  for (uint32_t r : rowlist) {
+    g->streamed += work[r];
+    g->staged += odeg[r];
+  }
+  for (const WorkItem &w : g->h_items)
+    for (uint32_t q = w.lo; q < w.hi; ++q) g->staged_items += odeg[rowlist[q]];
   partition_features(g, odeg, npairs, work, deg, rowlist);
   g->h_learned.clear();
   g->learned_version = 0;
@@ -1040,7 +1113,8 @@ int pac_evaluate_partition(const pac_graph *h, const int32_t *v2p, uint32_t P,
     for (uint64_t x : per) max_part_deg = std::max(max_part_deg, x);
     max_deg = std::max<uint64_t>(max_deg, vl[u + 1] - vl[u]);
   }
-  *balance_ratio = max_deg ? (double)(P * max_part_deg) / (double)max_deg : 0.0;
+  // as Graph.cpp:1471: a graph without edges gives 0/0 = NaN
+  *balance_ratio = (double)(P * max_part_deg) / (double)max_deg;
   return PAC_OK;
 }
 
@@ -1391,47 +1465,7 @@ int pac_tc_batch(pac_ctx *ctx, pac_graph *const *hs, uint32_t G, int dispatch, i
     ctx->c.last.launches = launches;
     return PAC_OK;
   }
-  std::vector<Graph *> gs(G);
-  std::vector<uint8_t> strat;
-  {
-    // each graph's count accumulates in its own device total: a graph listed twice would
-    // receive both counts
-    std::vector<const pac_graph *> seen(hs, hs + G);
-    std::sort(seen.begin(), seen.end());
-    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) {
-      set_error("a graph appears more than once in the batch");
-      return PAC_ERR_ARG;
-    }
-  }
-  for (uint32_t i = 0; i < G; ++i) {
-    if (!hs[i] || !hs[i]->g.prepared || hs[i]->g.ctx != &ctx->c) {
-      set_error("graph %u is null, unprepared or from another context", i);
-      return PAC_ERR_STATE;
-    }
-    gs[i] = &hs[i]->g;
-    std::vector<uint8_t> s;
-    PAC_TRY(choose_strategies(gs[i], dispatch, s));
-    strat.insert(strat.end(), s.begin(), s.end());
-  }
-  Ctx *c = &ctx->c;
-  Timed t(c);
-  PAC_TRY(t.mark(0));
-  if (launch != PAC_LAUNCH_GRAPH) PAC_TRY(zero_outputs(c, gs.data(), G, kModeTC));
-  PAC_TRY(t.mark(1));
-  uint64_t launches = 0;
-  c->time_kernel = (launch == PAC_LAUNCH_PERSISTENT);
-  const int rc = launch_persistent(c, gs.data(), G, kModeTC, strat.data(),
-                                   launch == PAC_LAUNCH_GRAPH, &launches);
-  t.kernel_timed = c->time_kernel;
-  c->time_kernel = false;
-  PAC_TRY(rc);
-  PAC_TRY(t.mark(2));
-  PAC_TRY(t.mark(3));
-  PAC_TRY(t.finish());
-  c->last.launches = launches;
-  for (uint32_t i = 0; i < G; ++i)
-    PAC_CHECK_HIP(hipMemcpy(&totals[i], gs[i]->total.p, 8, hipMemcpyDeviceToHost));
-  return PAC_OK;
+  return pac::solve_batch(&ctx->c, hs, G, kModeTC, dispatch, launch, totals);
 }
 
 int pac_set_tile(pac_ctx *ctx, int tile) {

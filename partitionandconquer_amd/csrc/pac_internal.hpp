@@ -33,6 +33,9 @@ void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 enum Strategy : uint8_t { kMerge = 0, kGallop = 1, kBitmap = 2 };
 // What the intersection kernels accumulate.
 enum Mode : int { kModeTC = 0, kModeLCC = 1, kModeSUP = 2 };
+static_assert(kModeTC == PAC_MODE_TC && kModeLCC == PAC_MODE_LCC &&
+                  kModeSUP == PAC_MODE_JACCARD,
+              "the ABI's solve modes are the kernels' modes");
 
 // Owning device buffer (hipMalloc / hipFree).
 struct DevBuf {
@@ -115,6 +118,8 @@ struct Graph {
   uint64_t nnz = 0;  // 2m
   uint64_t m = 0;    // oriented edges
   uint64_t max_deg = 0, max_outdeg = 0, W = 0;
+  // the pair work model (pac_graph_info), set by pac_graph_prepare
+  uint64_t n_pairs = 0, n_rows = 0, streamed = 0, staged = 0, staged_items = 0;
   // symmetric CSR
   DevBuf off, adj, deg;
   // oriented CSR
@@ -277,6 +282,10 @@ int mlp_store_bf16(Ctx *ctx, const float *d_master);
 
 // ---- host helpers ----------------------------------------------------------------
 int fixed_heuristic(const Graph *g, std::vector<uint8_t> &out);
+// One batched persistent solve of `mode` over G prepared graphs (launch PERSISTENT or
+// GRAPH); totals (may be null) receives each graph's T.
+int solve_batch(Ctx *c, pac_graph *const *hs, uint32_t G, int mode, int dispatch, int launch,
+                uint64_t *totals);
 void partition_features(Graph *g, const std::vector<uint32_t> &h_odeg,
                         const std::vector<uint32_t> &h_npairs,
                         const std::vector<uint32_t> &h_work, const std::vector<uint32_t> &h_deg,

@@ -21,8 +21,9 @@
 //             equal share of the merged sequence after a diagonal binary search.
 //   kGallop : the shorter list's elements are binary-searched in the longer one
 //             (B elements in A held in LDS, or A elements in B in HBM).
-//   kBitmap : a hashed bitmap of A in LDS filters B elements with one LDS read; the
-//             survivors are confirmed in an LDS hash table of A (exact).
+//   kBitmap : A is held in a two-choice cuckoo hash table in LDS (two buckets of two
+//             slots per key, load at most 1/2); each B element is looked up with two
+//             8-byte LDS reads and four compares, no loop (exact).
 // Counting is exact by construction in all three: a hit is only recorded for an
 // element equal to a list entry.
 //
@@ -32,8 +33,10 @@
 // global atomic per pair.  After the row each nonzero LDS counter is added to the
 // global count of its vertex -- about one global atomic per pair instead of one per
 // triangle.  SUP mode (per-edge support, for Jaccard) keeps the same counters per
-// edge: x->w and x->y at their places in A, y->x per pair, and y->w with one atomic
-// per triangle at w's position in B.
+// edge: x->w and x->y at their places in A, y->x per pair; the third edge y->w lies
+// in the streamed list, and the hash path records it as one hit bit per streamed
+// element (the pair's hit words), summed per edge after the launch (k_sup_small,
+// k_sup_big), where the sorted and HBM paths add one atomic per triangle instead.
 #include <algorithm>
 
 #include "pac_internal.hpp"

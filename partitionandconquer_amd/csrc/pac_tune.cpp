@@ -127,51 +127,75 @@ int pac_profile_partitions(pac_graph *h, int reps, double *ms) {
   return rc;
 }
 
-// Times the batched persistent TC kernel for every tile configuration and keeps the
-// fastest.  The cache file holds lines "<shape-key> <tile>".
-int pac_autotune(pac_ctx *ctx, pac_graph *const *gs, uint32_t G, const char *cache_path,
-                 int *tile, double *ms_out) {
+// Times the intersection launch of the batched persistent kernel for every tile
+// configuration in the caller's mode and dispatch and keeps the fastest.  The cache file
+// holds lines "<shape-key> <mode> <dispatch> <tile> <ms per tile...>".
+int pac_autotune(pac_ctx *ctx, pac_graph *const *gs, uint32_t G, int mode, int dispatch,
+                 const char *cache_path, int *tile, double *ms_out) {
   if (!ctx || !gs || G == 0 || !tile) {
     set_error("need a context, graphs and a tile output");
     return PAC_ERR_ARG;
   }
+  if (mode < PAC_MODE_TC || mode > PAC_MODE_JACCARD) {
+    set_error("unknown solve mode %d", mode);
+    return PAC_ERR_ARG;
+  }
+  if (dispatch < PAC_DISPATCH_FIXED || dispatch > PAC_FORCE_BITMAP) {
+    set_error("unknown dispatch mode %d", dispatch);
+    return PAC_ERR_ARG;
+  }
   const std::string key = shape_key(gs, G);
-  if (cache_path && !ms_out) {
+  if (cache_path) {
     std::ifstream in(cache_path);
-    std::string k;
-    int t;
-    while (in >> k >> t)
-      if (k == key && t >= 0 && t < PAC_NUM_TILES) {
+    std::string line;
+    while (std::getline(in, line)) {
+      std::istringstream ls(line);
+      std::string k;
+      int md = -1, dp = -1, t = -1;
+      double ms[PAC_NUM_TILES];
+      if (!(ls >> k >> md >> dp >> t)) continue;
+      bool whole = true;
+      for (int i = 0; i < PAC_NUM_TILES; ++i) whole = whole && (bool)(ls >> ms[i]);
+      if (whole && k == key && md == mode && dp == dispatch && t >= 0 && t < PAC_NUM_TILES) {
         ctx->c.tile = t;
         *tile = t;
+        if (ms_out) std::copy(ms, ms + PAC_NUM_TILES, ms_out);
         return PAC_OK;
       }
+    }
   }
-  std::vector<uint64_t> totals(G);
-  double best = 1e300;
-  int best_t = ctx->c.tile;
   const int saved = ctx->c.tile;
+  double ms[PAC_NUM_TILES];
+  int best_t = saved;
   for (int t = 0; t < PAC_NUM_TILES; ++t) {
     ctx->c.tile = t;
-    PAC_TRY(pac_tc_batch(ctx, gs, G, PAC_DISPATCH_FIXED, PAC_LAUNCH_PERSISTENT, totals.data()));
     std::vector<double> v;
-    for (int r = 0; r < 3; ++r) {
-      PAC_TRY(pac_tc_batch(ctx, gs, G, PAC_DISPATCH_FIXED, PAC_LAUNCH_PERSISTENT, totals.data()));
-      v.push_back(ctx->c.last.total_ms);
+    int rc = PAC_OK;
+    for (int r = 0; r < 4 && rc == PAC_OK; ++r) {  // a warm-up, then three timed
+      rc = solve_batch(&ctx->c, gs, G, mode, dispatch, PAC_LAUNCH_PERSISTENT, nullptr);
+      if (r) v.push_back(ctx->c.last.isect_ms);
     }
-    const double m = median(v);
-    if (ms_out) ms_out[t] = m;
-    if (m < best) {
-      best = m;
-      best_t = t;
+    if (rc != PAC_OK) {
+      ctx->c.tile = saved;
+      return rc;
     }
+    ms[t] = median(v);
+    if (t == 0 || ms[t] < ms[best_t]) best_t = t;
   }
-  (void)saved;
   ctx->c.tile = best_t;
   *tile = best_t;
+  if (ms_out) std::copy(ms, ms + PAC_NUM_TILES, ms_out);
   if (cache_path) {
     std::ofstream out(cache_path, std::ios::app);
-    if (out.is_open()) out << key << " " << best_t << "\n";
+    if (out.is_open()) {
+      out << key << " " << mode << " " << dispatch << " " << best_t;
+      char buf[32];
+      for (int t = 0; t < PAC_NUM_TILES; ++t) {
+        snprintf(buf, sizeof buf, " %.17g", ms[t]);
+        out << buf;
+      }
+      out << "\n";
+    }
   }
   return PAC_OK;
 }

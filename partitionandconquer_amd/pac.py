@@ -28,6 +28,7 @@ ERRORS = {-1: "ARG", -2: "HIP", -3: "OOM", -4: "FORMAT", -5: "IO", -6: "STATE",
 
 DISPATCH_FIXED, DISPATCH_LEARNED, FORCE_MERGE, FORCE_GALLOP, FORCE_BITMAP = range(5)
 LAUNCH_PLAIN, LAUNCH_PERSISTENT, LAUNCH_GRAPH = range(3)
+MODE_TC, MODE_LCC, MODE_JACCARD = range(3)
 STRATEGIES = ("merge", "gallop", "bitmap")
 NUM_TILES = 4
 MLP_IN, MLP_HIDDEN, MLP_OUT = 16, 64, 3
@@ -46,7 +47,22 @@ class GraphInfo(C.Structure):
                 ("num_edges", C.c_uint64), ("max_degree", C.c_uint64),
                 ("max_out_degree", C.c_uint64), ("work_W", C.c_uint64),
                 ("num_partitions", C.c_uint64), ("num_items", C.c_uint64),
-                ("prepare_ms", C.c_double)]
+                ("prepare_ms", C.c_double), ("num_pairs", C.c_uint64),
+                ("num_rows", C.c_uint64), ("streamed_elems", C.c_uint64),
+                ("staged_elems", C.c_uint64), ("staged_item_elems", C.c_uint64)]
+
+    def algorithmic_bytes(self, per_vertex: bool) -> int:
+        """Bytes the hash path must move per intersection launch: every pair's streamed
+        list (4 B per element) with its partner ID and list bounds (12 B), every row's
+        staged list once per work item holding it, the row's list and pair bounds and its
+        row-list entry (20 B), the items (32 B); with per-vertex counts also each pair's
+        edge index (4 B) and the per-vertex counts read and written once (16 B per
+        vertex)."""
+        b = (4 * self.streamed_elems + 4 * self.staged_item_elems + 12 * self.num_pairs
+             + 20 * self.num_rows + 32 * self.num_items)
+        if per_vertex:
+            b += 4 * self.num_pairs + 16 * self.num_vertices
+        return int(b)
 
 
 class Timing(C.Structure):
@@ -104,7 +120,8 @@ def lib():
         "pac_friend_triangle": ([vp, C.c_int32, vp, C.c_uint64, vp], C.c_int),
         "pac_square_count": ([vp, vp], C.c_int),
         "pac_tc_batch": ([vp, vp, C.c_uint32, C.c_int, C.c_int, vp], C.c_int),
-        "pac_autotune": ([vp, vp, C.c_uint32, C.c_char_p, C.POINTER(C.c_int), vp], C.c_int),
+        "pac_autotune": ([vp, vp, C.c_uint32, C.c_int, C.c_int, C.c_char_p, C.POINTER(C.c_int),
+                          vp], C.c_int),
         "pac_set_tile": ([vp, C.c_int], C.c_int),
         "pac_get_tile": ([vp, C.POINTER(C.c_int)], C.c_int),
         "pac_partition_features": ([vp, vp], C.c_int),
@@ -203,13 +220,16 @@ class Context:
         _check(lib().pac_get_tile(self._h, C.byref(t)))
         return t.value
 
-    def autotune(self, graphs, cache_path=None, report=True):
+    def autotune(self, graphs, mode=MODE_TC, dispatch=DISPATCH_FIXED, cache_path=None):
+        """Time every tile on the graphs' persistent launch in `mode` with `dispatch`, keep
+        the fastest for this context; returns (tile, intersection ms per tile).  A cache
+        file line for the same shapes, mode and dispatch is used without timing."""
         arr = (vp * len(graphs))(*[g._h for g in graphs])
         t = C.c_int()
         ms = np.zeros(NUM_TILES, np.float64)
-        _check(lib().pac_autotune(self._h, arr, len(graphs),
+        _check(lib().pac_autotune(self._h, arr, len(graphs), mode, dispatch,
                                   cache_path.encode() if cache_path else None, C.byref(t),
-                                  _ptr(ms) if report else None))
+                                  _ptr(ms)))
         return t.value, ms
 
     def tc_batch(self, graphs, dispatch=DISPATCH_FIXED, launch=LAUNCH_PERSISTENT) -> np.ndarray:

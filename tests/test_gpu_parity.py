@@ -489,3 +489,17 @@ def test_edge_list_ingestion_on_device(ctx, orc):
     assert e.value.code == -4
     empty = pac.Graph.from_edges(ctx, np.zeros(0, np.int32), np.zeros(0, np.int32), 7)
     assert empty.get_vertex_num() == 7 and empty.get_edge_num() == 0
+
+
+def test_evaluate_partition_matches_oracle(ctx, orc):
+    """pac_evaluate_partition against the oracle's Partitioned_Graph::evaluate_partition
+    (Graph.cpp:1460-1473) for contiguous, random and greedy maps, and NaN without edges."""
+    ref = orc.gen_rmat(12, 16, 0.57, 0.19, 0.19, seed=77)
+    g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    for P in (1, 7, 64):
+        for v2p in (orc.contiguous_partition(ref.V, P), orc.random_partition(ref.V, P, seed=P),
+                    g.greedy_partition(P, seed=P)):
+            want = orc.PGraph(ref, v2p).evaluate_partition()[0]
+            assert g.evaluate_partition(v2p, int(v2p.max()) + 1) == want, P
+    empty = pac.Graph.from_csr(ctx, np.zeros(5, np.uint64), np.zeros(0, np.int32))
+    assert math.isnan(empty.evaluate_partition(np.zeros(4, np.int32), 1))

@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_batch_specs_follow_the_config():
-    from partitionandconquer_amd.experiments import batch_specs
+    from experiments import batch_specs
     s = batch_specs(256)
     assert s == batch_specs(256)                       # deterministic
     er = [x for x in s if x["kind"] == "er"]

@@ -257,6 +257,96 @@ def test_oriented_stats_brute(orc):
     assert st["m"] == len(ori) and st["W"] == W and st["max_outdeg"] == dplus.max()
 
 
+def test_evaluate_partition(orc):
+    """K_8 in 2 contiguous halves: vertex 0 sees 3 neighbours in its own half and 4 in the
+    other, so max_part_deg = 4, max_deg = 7, ratio 8/7; in general the per-partition
+    neighbour histogram from numpy."""
+    k8 = orc.gen_er(8, 7, seed=1)
+    r, mp, md = orc.PGraph(k8, orc.contiguous_partition(8, 2)).evaluate_partition()
+    assert (mp, md) == (4, 7) and r == 8 / 7
+    r, mp, md = orc.PGraph(k8, np.zeros(8, np.int32)).evaluate_partition()
+    assert (mp, md, r) == (7, 7, 1.0)
+    for P in (3, 16):
+        g = orc.gen_rmat(10, 8, 0.57, 0.19, 0.19, seed=P)
+        v2p = orc.random_partition(g.V, P, seed=P)
+        src = np.repeat(np.arange(g.V), g.deg())
+        hist = np.zeros((g.V, v2p.max() + 1), np.int64)
+        np.add.at(hist, (src, v2p[g.elist]), 1)
+        r, mp, md = orc.PGraph(g, v2p).evaluate_partition()
+        assert mp == hist.max() and md == g.deg().max()
+        assert r == (v2p.max() + 1) * hist.max() / g.deg().max()
+    empty = orc.edges_to_csr(4, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    assert math.isnan(orc.PGraph(empty, np.zeros(4, np.int32)).evaluate_partition()[0])
+
+
+def brute_pairs(g):
+    """Pair ownership walked per vertex over its symmetric row, as the device builds it
+    (x owns {x, u} when x's out-list is the longer, ties to the tail; empty partner lists
+    hold no pair) -- a different formulation from the oracle's per-edge minimum."""
+    d = g.deg()
+    before = lambda a, b: (d[a], a) < (d[b], b)
+    dplus = np.zeros(g.V, np.int64)
+    for x in range(g.V):
+        for u in g.elist[g.vlist[x]:g.vlist[x + 1]].tolist():
+            dplus[x] += before(x, u)
+    pairs = rows = streamed = staged = 0
+    for x in range(g.V):
+        own = 0
+        for u in g.elist[g.vlist[x]:g.vlist[x + 1]].tolist():
+            if before(x, u):
+                mine = dplus[u] <= dplus[x] and dplus[u] > 0
+            else:
+                mine = dplus[x] > dplus[u]
+            if mine:
+                own += 1
+                streamed += int(dplus[u])
+        pairs += own
+        if own:
+            rows += 1
+            staged += int(dplus[x])
+    return {"pairs": pairs, "rows": rows, "streamed": streamed, "staged": staged}
+
+
+@pytest.mark.parametrize("n", [3, 4, 7, 12])
+def test_pair_stats_complete_graph(n, orc):
+    """K_n orients by ID (all degrees equal), d+(i) = n-1-i: every streamed element is a
+    hit, so the streamed total is C(n,3); pairs are the edges i<j<=n-2 (C(n-1,2)), owned
+    by rows 0..n-3 staging sum of (n-1-i)."""
+    z = np.load(os.path.join(GOLDEN, "K12.npz")) if n == 12 else None
+    g = orc.CSR(z["vlist"], z["elist"]) if z is not None else orc.gen_er(n, n - 1, seed=1)
+    assert g.nnz == n * (n - 1)
+    s = orc.pair_stats(g)
+    assert s["streamed"] == math.comb(n, 3)
+    assert s["pairs"] == math.comb(n - 1, 2)
+    assert s["rows"] == max(0, n - 2)
+    assert s["staged"] == sum(n - 1 - i for i in range(n - 2))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_pair_stats_brute(seed, orc):
+    for g in (orc.gen_rmat(8, 8, 0.57, 0.19, 0.19, seed=seed), orc.gen_er(300, 9.0, seed=seed)):
+        assert orc.pair_stats(g) == brute_pairs(g)
+    star = orc.edges_to_csr(9, np.zeros(8, np.uint32), np.arange(1, 9, dtype=np.uint32))
+    assert orc.pair_stats(star) == {"pairs": 0, "rows": 0, "streamed": 0, "staged": 0}
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_tc_oriented_golden(name, orc):
+    g, z = load(name, orc)
+    assert orc.tc_oriented(g, 2) == int(z["T"])
+
+
+def test_tc_oriented_vs_restatement(orc):
+    """The fast checker against the reference's ID-ordered merge count on R-MAT (hubs at
+    low IDs), skewed and flat Erdos-Renyi graphs."""
+    for s in (6, 9, 12, 14):
+        g = orc.gen_rmat(s, 16, 0.57, 0.19, 0.19, seed=s)
+        assert orc.tc_oriented(g, 4) == orc.tc_merge(g, 4)
+    for n, k in ((500, 8.0), (3000, 40.0)):
+        g = orc.gen_er(n, k, seed=n)
+        assert orc.tc_oriented(g, 4) == orc.tc_merge(g, 4)
+
+
 # ---- MLP oracle vs finite differences of an independent float64 forward ---------------
 
 def np_forward(P, x):
