@@ -260,6 +260,27 @@ int pac_friend_triangle(pac_graph *g, int32_t root, const int32_t *nodes, uint64
  * the partitioning; needs no pac_graph_prepare. */
 int pac_square_count(pac_graph *g, uint64_t *res);
 
+/* ---- solve without allocation ---------------------------------------------------- */
+/* Size of the static device array (one per GPU, held until the process ends) that
+ * pac_tc_noalloc uses for its temporaries. */
+#define PAC_NOALLOC_SCRATCH_BYTES (2u << 20)
+/* No counterpart in the reference.  For callers that cannot allocate device memory while
+ * they solve (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): pac_tc on the persistent kernel (the context's tile) with its temporaries
+ * -- work-queue heads, graph descriptor, deferred-row queue -- in a static device array
+ * of PAC_NOALLOC_SCRATCH_BYTES per GPU instead of library allocations; the call makes no
+ * allocation and no host-device copy.  The graph must be prepared (pac_graph_prepare
+ * allocates).  d_total (device memory, 1 uint64) receives T and d_per_vertex (device
+ * memory, V uint64, or NULL) t(u): both are zeroed and written by kernels on the
+ * context's stream, and the call returns without waiting for them.  dispatch: FIXED or a
+ * FORCE_* mode; LEARNED only once a solve with PAC_DISPATCH_LEARNED has computed the
+ * choice for the loaded model (PAC_ERR_STATE before; with no model loaded LEARNED means
+ * FIXED).  PAC_ERR_UNSUPPORTED when the graph's deferred-row queue (work items plus rows
+ * above 1024 out-neighbours, when its longest out-list exceeds the tile) would exceed the
+ * array.  Every call on one GPU uses that GPU's one array, which keeps the last call's
+ * contents: calls that overlap on one GPU (other contexts or threads) race. */
+int pac_tc_noalloc(pac_graph *g, int dispatch, uint64_t *d_total, uint64_t *d_per_vertex);
+
 /* ---- batching ------------------------------------------------------------------- */
 /* Triangle counts of G prepared graphs in one persistent launch (or one hipGraph
  * replay when launch == PAC_LAUNCH_GRAPH): totals[i] = T of gs[i]; a graph may appear

@@ -251,6 +251,9 @@ int choose_strategies(Graph *g, int dispatch, std::vector<uint8_t> &out) {
         g->h_learned.assign(g->P, (uint8_t)kBitmap);
         PAC_TRY(pac_mlp_choice(h, g->h_learned.data()));
         g->learned_version = g->ctx->mlp_version;
+        PAC_TRY(g->dlearned.reserve(g->P));
+        PAC_CHECK_HIP(hipMemcpy(g->dlearned.p, g->h_learned.data(), g->P, hipMemcpyHostToDevice));
+        g->dlearned_version = g->learned_version;
       }
       out = g->h_learned;
       return PAC_OK;
@@ -561,6 +564,12 @@ int pac_ctx_create(int device, pac_ctx **out) {
     }
   const char *t = getenv("PAC_TILE");
   if (t) c->c.tile = std::max(0, std::min(PAC_NUM_TILES - 1, atoi(t)));
+  // the device is current: find its copy of the no-allocation scratch
+  const int rc = noalloc_scratch_init(&c->c);
+  if (rc != PAC_OK) {
+    pac_ctx_destroy(c);
+    return rc;
+  }
   *out = c;
   return PAC_OK;
 }
@@ -1035,6 +1044,19 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
   partition_features(g, odeg, npairs, work, deg, rowlist);
   g->h_learned.clear();
   g->learned_version = 0;
+  g->dlearned_version = 0;
+  {
+    // device strategy tables: merge, gallop and hash everywhere, then the fixed heuristic
+    std::vector<uint8_t> tables((size_t)4 * P);
+    for (uint32_t k = 0; k < P; ++k) {
+      tables[k] = kMerge;
+      tables[(size_t)P + k] = kGallop;
+      tables[(size_t)2 * P + k] = kBitmap;
+      tables[(size_t)3 * P + k] = g->h_fixed[k];
+    }
+    PAC_TRY(g->dstrat.reserve(tables.size()));
+    PAC_CHECK_HIP(hipMemcpy(g->dstrat.p, tables.data(), tables.size(), hipMemcpyHostToDevice));
+  }
   PAC_TRY(g->rows.reserve(sizeof(uint32_t) * (rowlist.empty() ? 1 : rowlist.size())));
   PAC_TRY(g->items.reserve(sizeof(WorkItem) * (g->h_items.empty() ? 1 : g->h_items.size())));
   if (!rowlist.empty())
@@ -1442,6 +1464,37 @@ int pac_square_count(pac_graph *h, uint64_t *res) {
   PAC_TRY(t.mark(2));
   PAC_TRY(t.mark(3));
   return t.finish();
+}
+
+int pac_tc_noalloc(pac_graph *h, int dispatch, uint64_t *d_total, uint64_t *d_per_vertex) {
+  PAC_TRY(check_solve_args(h, dispatch, PAC_LAUNCH_PERSISTENT));
+  if (!d_total) {
+    set_error("need a device pointer for the total");
+    return PAC_ERR_ARG;
+  }
+  Graph *g = &h->g;
+  const uint8_t *tables = g->dstrat.as<const uint8_t>();
+  const uint8_t *strat = nullptr;
+  switch (dispatch) {
+    case PAC_FORCE_MERGE: strat = tables; break;
+    case PAC_FORCE_GALLOP: strat = tables + g->P; break;
+    case PAC_FORCE_BITMAP: strat = tables + 2ull * g->P; break;
+    case PAC_DISPATCH_FIXED: strat = tables + 3ull * g->P; break;
+    default:  // PAC_DISPATCH_LEARNED
+      if (!g->ctx->mlp_loaded) {
+        strat = tables + 3ull * g->P;  // the documented fallback: no model loaded
+      } else if (g->dlearned_version == g->ctx->mlp_version && g->dlearned_version != 0 &&
+                 g->dlearned.p) {
+        strat = g->dlearned.as<const uint8_t>();
+      } else {
+        set_error("the learned choice for the loaded model is not computed yet: run one "
+                  "solve with PAC_DISPATCH_LEARNED first");
+        return PAC_ERR_STATE;
+      }
+  }
+  return launch_persistent_noalloc(g->ctx, g, d_per_vertex ? kModeLCC : kModeTC, strat,
+                                   reinterpret_cast<unsigned long long *>(d_total),
+                                   reinterpret_cast<unsigned long long *>(d_per_vertex));
 }
 
 int pac_tc_batch(pac_ctx *ctx, pac_graph *const *hs, uint32_t G, int dispatch, int launch,

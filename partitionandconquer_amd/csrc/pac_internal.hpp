@@ -159,6 +159,11 @@ struct Graph {
   DevBuf part;
   bool part_ready = false;
   DevBuf rowwork;  // V: streamed length of row u = sum of d+(partner) over u's pairs
+  // Strategy tables on the device, P bytes each: all merge, all gallop, all hash, the fixed
+  // heuristic (uploaded by pac_graph_prepare), and the MLP's choice for dlearned_version
+  // (uploaded when a learned solve computes it).  Read by pac_tc_noalloc.
+  DevBuf dstrat, dlearned;
+  uint64_t dlearned_version = 0;
   // outputs
   DevBuf total, tvert, sup, lcc32, lcc64, juni, jratio;
   double prepare_ms = 0;
@@ -173,6 +178,7 @@ struct Ctx {
   bool time_kernel = false;
   pac_timing last{};
   int tile = 1;
+  void *noalloc_scratch = nullptr;  // this device's static scratch of pac_tc_noalloc
   DevBuf scratch;     // generic temporary storage (library-owned)
   DevBuf scratch2;
   DevBuf queue;       // persistent work-queue heads: [0] items, [1] deferred rows, [2] their queue
@@ -240,6 +246,13 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
                       uint64_t *launches);
 // One partition alone with one strategy (profiling).
 int launch_partition(Ctx *ctx, Graph *g, int mode, uint32_t part, int strat);
+// The no-allocation scratch: its device address for ctx->device (hipGetSymbolAddress once
+// per device; call with the device current).
+int noalloc_scratch_init(Ctx *ctx);
+// Persistent launch of a prepared graph in TC or LCC mode with its temporaries in the
+// static scratch: zeroes and fills d_total (and d_tvert in LCC mode); allocates nothing.
+int launch_persistent_noalloc(Ctx *ctx, Graph *g, int mode, const uint8_t *d_strat,
+                              unsigned long long *d_total, unsigned long long *d_tvert);
 int compute_rowwork(Graph *g);
 // LCC mode: per-vertex triangle counts = row totals + per-edge credits (after a launch).
 int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64);

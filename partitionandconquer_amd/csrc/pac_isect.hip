@@ -38,8 +38,16 @@
 // element (the pair's hit words), summed per edge after the launch (k_sup_small,
 // k_sup_big), where the sorted and HBM paths add one atomic per triangle instead.
 #include <algorithm>
+#include <mutex>
 
 #include "pac_internal.hpp"
+
+// The temporaries of pac_tc_noalloc: one copy per GPU, held until the process ends.  Host
+// code never uses this name as an address (on the host it is a stand-in symbol): it takes
+// the device address from hipGetSymbolAddress, once per device, in noalloc_scratch_init.
+// Layout: [0, 256) work-queue heads, [256, 512) the graph descriptor, then the
+// deferred-row queue.
+static __device__ __align__(256) unsigned char scratch[PAC_NOALLOC_SCRATCH_BYTES];
 
 namespace pac {
 
@@ -858,6 +866,12 @@ __global__ void k_reset_queue(uint32_t *q) {
   if (threadIdx.x < 4) q[threadIdx.x] = 0;
 }
 
+// Writes a graph descriptor to device memory (a kernel rather than a host copy, so that
+// a no-allocation solve makes no host-to-device transfer).
+__global__ void k_put_desc(GraphDev d, GraphDev *out) {
+  if (threadIdx.x == 0) *out = d;
+}
+
 // Plain streaming copy (16-byte vectors, grid-stride): the bandwidth reference.
 __global__ void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -1126,37 +1140,43 @@ uint32_t persistent_grid(Ctx *ctx, K kernel, int threads, uint32_t cap) {
 // The widest tile: the second pass over rows that exceed the launch tile's capacity.
 constexpr int kWideWaves = 8, kWideCap = 4096;
 
-// long_rows: the capacity of the deferred-row queue, 0 when no row exceeds CAP.
+// Device buffers of one persistent launch: graph descriptors, per-partition strategy
+// table, work-queue heads ([0] items, [1] deferred rows, [2] their queue) and the
+// deferred-row queue (defer_cap entries).
+struct PersistBufs {
+  const GraphDev *gdesc;
+  const uint8_t *strat;
+  uint32_t *queue;
+  DeferItem *defer;
+  uint32_t defer_cap;
+};
+
+// The persistent launch (and the wide second pass when rows were deferred).
 template <int MODE, int WAVES, int CAP>
-int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint32_t long_rows,
+int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, const PersistBufs &b,
                        uint64_t *launches) {
   constexpr bool kCanDefer = CAP < kWideCap;
-  const uint32_t defer_cap = kCanDefer ? long_rows : 0u;
-  if (defer_cap) PAC_TRY(ctx->defer.reserve(sizeof(DeferItem) * defer_cap));
-  hipLaunchKernelGGL((k_reset_queue), dim3(1), dim3(64), 0, ctx->stream, ctx->queue.as<uint32_t>());
+  const uint32_t defer_cap = kCanDefer ? b.defer_cap : 0u;
+  hipLaunchKernelGGL((k_reset_queue), dim3(1), dim3(64), 0, ctx->stream, b.queue);
   if (ctx->time_kernel) PAC_CHECK_HIP(hipEventRecord(ctx->kev[0], ctx->stream));
   if (defer_cap) {
     const uint32_t grid =
         persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, kCanDefer>, 64 * WAVES, nitems);
     hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, kCanDefer>), dim3(grid), dim3(64 * WAVES),
-                       0, ctx->stream, ctx->gdesc.as<const GraphDev>(), items, nitems,
-                       ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>(),
-                       ctx->defer.as<DeferItem>(), defer_cap);
+                       0, ctx->stream, b.gdesc, items, nitems, b.strat, b.queue, b.defer,
+                       defer_cap);
   } else {
     const uint32_t grid =
         persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, false>, 64 * WAVES, nitems);
     hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, false>), dim3(grid), dim3(64 * WAVES), 0,
-                       ctx->stream, ctx->gdesc.as<const GraphDev>(), items, nitems,
-                       ctx->dispatch.as<const uint8_t>(), ctx->queue.as<uint32_t>(), nullptr,
-                       0u);
+                       ctx->stream, b.gdesc, items, nitems, b.strat, b.queue, nullptr, 0u);
   }
   PAC_CHECK_HIP(hipGetLastError());
   if (defer_cap) {
     const uint32_t g2 = persistent_grid(ctx, k_deferred<MODE, kWideWaves, kWideCap>,
                                         64 * kWideWaves, defer_cap);
     hipLaunchKernelGGL((k_deferred<MODE, kWideWaves, kWideCap>), dim3(g2), dim3(64 * kWideWaves),
-                       0, ctx->stream, ctx->gdesc.as<const GraphDev>(),
-                       ctx->defer.as<const DeferItem>(), defer_cap, ctx->queue.as<uint32_t>());
+                       0, ctx->stream, b.gdesc, b.defer, defer_cap, b.queue);
     PAC_CHECK_HIP(hipGetLastError());
     if (launches) *launches += 1;
   }
@@ -1166,23 +1186,35 @@ int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint32_
 }
 
 template <int MODE>
-int run_persistent_mode(Ctx *ctx, const WorkItem *items, uint32_t nitems, uint32_t long_rows,
+int run_persistent_mode(Ctx *ctx, const WorkItem *items, uint32_t nitems, const PersistBufs &b,
                         uint64_t *launches) {
   switch (ctx->tile) {
-    case 0: return run_persistent_cfg<MODE, 2, 1024>(ctx, items, nitems, long_rows, launches);
-    case 1: return run_persistent_cfg<MODE, 4, 2048>(ctx, items, nitems, long_rows, launches);
-    case 2: return run_persistent_cfg<MODE, 4, 4096>(ctx, items, nitems, long_rows, launches);
-    default: return run_persistent_cfg<MODE, 8, 4096>(ctx, items, nitems, long_rows, launches);
+    case 0: return run_persistent_cfg<MODE, 2, 1024>(ctx, items, nitems, b, launches);
+    case 1: return run_persistent_cfg<MODE, 4, 2048>(ctx, items, nitems, b, launches);
+    case 2: return run_persistent_cfg<MODE, 4, 4096>(ctx, items, nitems, b, launches);
+    default: return run_persistent_cfg<MODE, 8, 4096>(ctx, items, nitems, b, launches);
   }
 }
 
 int run_persistent_any(Ctx *ctx, int mode, const WorkItem *items, uint32_t nitems,
-                       uint32_t long_rows, uint64_t *launches) {
-  if (mode == kModeLCC)
-    return run_persistent_mode<kModeLCC>(ctx, items, nitems, long_rows, launches);
-  if (mode == kModeSUP)
-    return run_persistent_mode<kModeSUP>(ctx, items, nitems, long_rows, launches);
-  return run_persistent_mode<kModeTC>(ctx, items, nitems, long_rows, launches);
+                       const PersistBufs &b, uint64_t *launches) {
+  if (mode == kModeLCC) return run_persistent_mode<kModeLCC>(ctx, items, nitems, b, launches);
+  if (mode == kModeSUP) return run_persistent_mode<kModeSUP>(ctx, items, nitems, b, launches);
+  return run_persistent_mode<kModeTC>(ctx, items, nitems, b, launches);
+}
+
+// The context's own buffers for a persistent launch (the deferred-row queue is reserved
+// here for long_rows entries when the tile can defer).
+int ctx_bufs(Ctx *ctx, uint32_t long_rows, PersistBufs &b) {
+  const bool can_defer = kTiles[ctx->tile].cap < kWideCap;
+  const uint32_t defer_cap = can_defer ? long_rows : 0u;
+  if (defer_cap) PAC_TRY(ctx->defer.reserve(sizeof(DeferItem) * defer_cap));
+  b.gdesc = ctx->gdesc.as<const GraphDev>();
+  b.strat = ctx->dispatch.as<const uint8_t>();
+  b.queue = ctx->queue.as<uint32_t>();
+  b.defer = ctx->defer.as<DeferItem>();
+  b.defer_cap = defer_cap;
+  return PAC_OK;
 }
 
 template <int STRAT, int MODE, int WAVES, int CAP>
@@ -1570,7 +1602,9 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   if (!use_graph) {
     // the caller has zeroed the outputs
     if (nitems == 0) return PAC_OK;
-    return run_persistent_any(ctx, mode, items, (uint32_t)nitems, (uint32_t)long_rows, launches);
+    PersistBufs b;
+    PAC_TRY(ctx_bufs(ctx, (uint32_t)long_rows, b));
+    return run_persistent_any(ctx, mode, items, (uint32_t)nitems, b, launches);
   }
   // hipGraph: zero-fill kernels + queue reset + persistent kernel, captured once per
   // (graph set, mode, tile, buffer addresses) and replayed.
@@ -1595,8 +1629,10 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
     const bool timing = ctx->time_kernel;
     ctx->time_kernel = false;  // no event records inside the captured graph
     int rc = zero_outputs(ctx, gs, G, mode);
+    PersistBufs b;
+    if (rc == PAC_OK) rc = ctx_bufs(ctx, (uint32_t)long_rows, b);  // reserved above: no allocation
     if (rc == PAC_OK && nitems)
-      rc = run_persistent_any(ctx, mode, items, (uint32_t)nitems, (uint32_t)long_rows, nullptr);
+      rc = run_persistent_any(ctx, mode, items, (uint32_t)nitems, b, nullptr);
     ctx->time_kernel = timing;
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
@@ -1614,6 +1650,84 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   PAC_CHECK_HIP(hipGraphLaunch(ctx->hexec, ctx->stream));
   if (launches) *launches += 1;
   return PAC_OK;
+}
+
+namespace {
+constexpr size_t kScratchQueue = 0, kScratchDesc = 256, kScratchDefer = 512;
+static_assert(4 * sizeof(uint32_t) <= kScratchDesc - kScratchQueue &&
+                  sizeof(GraphDev) <= kScratchDefer - kScratchDesc &&
+                  kScratchDefer % alignof(DeferItem) == 0,
+              "the queue heads and the descriptor fit their slots of the no-allocation scratch");
+constexpr uint64_t kNoAllocDeferCap =
+    (PAC_NOALLOC_SCRATCH_BYTES - kScratchDefer) / sizeof(DeferItem);
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch_init(Ctx *ctx) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) {
+    set_error("device %d: no-allocation scratch supports devices 0..%d", ctx->device,
+              kMaxDevices - 1);
+    return PAC_ERR_UNSUPPORTED;
+  }
+  if (!g_scratch_ptr[ctx->device]) {
+    // the caller has made ctx->device current: each GPU holds its own copy of the array
+    void *p = nullptr;
+    PAC_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("hipGetSymbolAddress returned no address for the no-allocation scratch");
+      return PAC_ERR_HIP;
+    }
+    g_scratch_ptr[ctx->device] = p;
+  }
+  ctx->noalloc_scratch = g_scratch_ptr[ctx->device];
+  return PAC_OK;
+}
+
+int launch_persistent_noalloc(Ctx *ctx, Graph *g, int mode, const uint8_t *d_strat,
+                              unsigned long long *d_total, unsigned long long *d_tvert) {
+  unsigned char *base = static_cast<unsigned char *>(ctx->noalloc_scratch);
+  if (!base) {
+    set_error("no-allocation scratch is not set up for device %d", ctx->device);
+    return PAC_ERR_STATE;
+  }
+  const uint64_t nitems = g->h_items.size();
+  const uint32_t cap = (uint32_t)kTiles[ctx->tile].cap;
+  const bool defers = cap < (uint32_t)kWideCap && g->max_outdeg > cap;
+  const uint64_t long_rows = defers ? nitems + g->n_long_rows : 0;
+  if (long_rows > kNoAllocDeferCap) {
+    set_error("the deferred-row queue needs %llu entries; the no-allocation scratch holds %llu "
+              "(%u bytes): use pac_tc, or the widest tile",
+              (unsigned long long)long_rows, (unsigned long long)kNoAllocDeferCap,
+              (unsigned)PAC_NOALLOC_SCRATCH_BYTES);
+    return PAC_ERR_UNSUPPORTED;
+  }
+  // outputs zeroed by kernels (no memset nodes should a caller capture the sequence)
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, d_total, (uint64_t)1);
+  if (mode == kModeLCC && g->V)
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
+                       d_tvert, (uint64_t)g->V);
+  PAC_CHECK_HIP(hipGetLastError());
+  GraphDev d = make_desc(g, 0);
+  d.total = d_total;
+  d.tvert = mode == kModeLCC ? d_tvert : nullptr;
+  d.sup = nullptr;
+  d.pword = nullptr;
+  d.hitw = nullptr;
+  GraphDev *desc = reinterpret_cast<GraphDev *>(base + kScratchDesc);
+  hipLaunchKernelGGL(k_put_desc, dim3(1), dim3(64), 0, ctx->stream, d, desc);
+  PAC_CHECK_HIP(hipGetLastError());
+  if (nitems == 0) return PAC_OK;
+  PersistBufs b;
+  b.gdesc = desc;
+  b.strat = d_strat;
+  b.queue = reinterpret_cast<uint32_t *>(base + kScratchQueue);
+  b.defer = reinterpret_cast<DeferItem *>(base + kScratchDefer);
+  b.defer_cap = (uint32_t)long_rows;
+  return run_persistent_any(ctx, mode, g->items.as<const WorkItem>(), (uint32_t)nitems, b,
+                            nullptr);
 }
 
 int sup_from_hit_words(Ctx *ctx, Graph *g) {

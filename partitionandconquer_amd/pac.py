@@ -108,6 +108,7 @@ def lib():
         "pac_greedy_partition": ([vp, C.c_uint32, C.c_uint64, vp], C.c_int),
         "pac_evaluate_partition": ([vp, vp, C.c_uint32, C.POINTER(C.c_double)], C.c_int),
         "pac_tc": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
+        "pac_tc_noalloc": ([vp, C.c_int, vp, vp], C.c_int),
         "pac_lcc": ([vp, C.c_int, C.c_int, vp, vp, vp], C.c_int),
         "pac_jaccard_edges": ([vp, C.c_int, C.c_int, vp, vp, vp, vp, vp], C.c_int),
         "pac_jaccard_queries": ([vp, vp, C.c_uint32, vp], C.c_int),
