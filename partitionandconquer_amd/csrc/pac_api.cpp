@@ -391,21 +391,18 @@ void greedy_partition_host(const std::vector<uint64_t> &vl, const std::vector<in
 }  // namespace
 
 // ---- fixed heuristic and features ---------------------------------------------------
-// Feature vector of a partition (16 floats, log-scaled then divided by 16).  Per row x:
-// a = d+(x) (the list staged in LDS), p = x's pairs, w = the total length streamed for
-// them; mean_b = w/p per pair.
-void partition_features(Graph *g, const std::vector<uint32_t> &h_odeg,
-                        const std::vector<uint32_t> &h_npairs,
-                        const std::vector<uint32_t> &h_work, const std::vector<uint32_t> &h_deg,
-                        const std::vector<uint32_t> &rowlist) {
+// Feature vector of a partition (16 floats, log-scaled then divided by 16).  Per row x
+// (stats[q] of the row at position q of the grouped row list): a = d+(x) (the list staged
+// in LDS), p = x's pairs, w = the total length streamed for them, d = d(x); mean_b = w/p
+// per pair.
+void partition_features(Graph *g, const std::vector<uint4> &stats) {
   g->h_features.assign((size_t)g->P * PAC_MLP_IN, 0.f);
   g->h_fixed.assign(g->P, (uint8_t)kBitmap);
   for (uint32_t k = 0; k < g->P; ++k) {
     double n = 0, sa = 0, sa2 = 0, maxa = 0, sd = 0, maxd = 0, sp = 0, sap = 0, sb = 0,
            maxw = 0, big = 0, small = 0, gal = 0;
     for (uint32_t q = g->h_pstart[k]; q < g->h_pstart[k + 1]; ++q) {
-      const uint32_t r = rowlist[q];
-      const double a = h_odeg[r], p = h_npairs[r], w = h_work[r], d = h_deg[r];
+      const double a = stats[q].x, p = stats[q].z, w = stats[q].y, d = stats[q].w;
       n += 1;
       sa += a;
       sa2 += a * a;
@@ -923,34 +920,47 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
     set_error("need at least one partition");
     return PAC_ERR_ARG;
   }
-  PAC_CHECK_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
-  PAC_TRY(compute_rowwork(g));
-  PAC_CHECK_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
-  std::vector<uint32_t> odeg, work, off, poff;
-  PAC_TRY(d2h(ctx, odeg, g->odeg, V));
-  PAC_TRY(d2h(ctx, work, g->rowwork, V));
-  PAC_TRY(d2h(ctx, off, g->off, (size_t)V + 1));
-  PAC_TRY(d2h(ctx, poff, g->poff, (size_t)V + 1));
-  std::vector<uint32_t> deg(V), npairs(V);
-  uint64_t W = 0, maxo = 0;
-  for (uint32_t u = 0; u < V; ++u) {
-    deg[u] = off[u + 1] - off[u];
-    npairs[u] = poff[u + 1] - poff[u];
-    // sum over oriented edges t->h of d+(t) + d+(h) = sum over u of d+(u) * d(u)
-    W += (uint64_t)odeg[u] * deg[u];
-    maxo = std::max<uint64_t>(maxo, odeg[u]);
-  }
-  g->W = W;
-  g->max_outdeg = maxo;
-  // rows (vertices owning pairs) grouped by partition, stable in ID
   const uint64_t width = ((uint64_t)V + P - 1) / P;
+  PAC_CHECK_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+  // pair lists and row work (once per graph), then the rows that own pairs, in ID order
+  RowSummary rs;
+  PAC_TRY(row_summary(g, v2p ? 0 : (width ? width : 1), P, rs));
+  PAC_CHECK_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+  g->W = rs.W;
+  g->max_outdeg = rs.max_outdeg;
+  // rows grouped by partition, stable in ID (contiguous partitions: ID order already);
+  // rowlist[q] and its summary stats[q] = (d+(r), streamed length, pairs, d(r))
   auto part_of = [&](uint32_t u) -> uint32_t {
     return v2p ? (uint32_t)v2p[u] : (uint32_t)(u / (width ? width : 1));
   };
+  const size_t nrows = rs.rows.size();
   std::vector<uint32_t> count(P + 1, 0);
-  for (uint32_t u = 0; u < V; ++u)
-    if (npairs[u]) count[part_of(u) + 1]++;
+  for (uint32_t r : rs.rows) count[part_of(r) + 1]++;
   for (uint32_t k = 0; k < P; ++k) count[k + 1] += count[k];
+  std::vector<uint32_t> rowlist;
+  std::vector<uint4> stats;
+  if (v2p) {
+    rowlist.resize(nrows);
+    stats.resize(nrows);
+    std::vector<uint32_t> cur(count.begin(), count.end() - 1);
+    for (size_t i = 0; i < nrows; ++i) {
+      const uint32_t q = cur[part_of(rs.rows[i])]++;
+      rowlist[q] = rs.rows[i];
+      stats[q] = rs.stats[i];
+    }
+    if (nrows)
+      PAC_CHECK_HIP(hipMemcpy(g->rows.p, rowlist.data(), sizeof(uint32_t) * nrows,
+                              hipMemcpyHostToDevice));
+    // oriented edges per partition, from the out-degrees of every vertex
+    std::vector<uint32_t> odeg;
+    PAC_TRY(d2h(ctx, odeg, g->odeg, V));
+    g->h_pedges.assign(P, 0);
+    for (uint32_t u = 0; u < V; ++u) g->h_pedges[part_of(u)] += odeg[u];
+  } else {
+    rowlist.swap(rs.rows);
+    stats.swap(rs.stats);
+    g->h_pedges = rs.part_edges;
+  }
   g->P = P;
   g->h_pstart = count;
   g->part_width = width ? width : 1;
@@ -959,38 +969,30 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
   else
     g->h_v2p.clear();
   g->part_ready = false;
-  std::vector<uint32_t> rowlist(count[P]);
-  {
-    std::vector<uint32_t> cur(count.begin(), count.end() - 1);
-    for (uint32_t u = 0; u < V; ++u)
-      if (npairs[u]) rowlist[cur[part_of(u)]++] = u;
-  }
-  g->h_pedges.assign(P, 0);
-  for (uint32_t u = 0; u < V; ++u) g->h_pedges[part_of(u)] += odeg[u];
   // Work items: pack small rows, split heavy rows by pair range.  Row cost = staged
   // list + streamed lists + a per-pair and a per-row overhead.
-  auto row_cost = [&](uint32_t r) {
-    return (uint64_t)odeg[r] + work[r] + 4ull * npairs[r] + 64;
+  auto row_cost = [&](uint32_t q) {
+    return (uint64_t)stats[q].x + stats[q].y + 4ull * stats[q].z + 64;
   };
   uint64_t total_cost = 0;
-  for (uint32_t r : rowlist) total_cost += row_cost(r);
+  for (size_t q = 0; q < nrows; ++q) total_cost += row_cost((uint32_t)q);
   const uint64_t target = item_target(total_cost, ctx->num_cus);
   g->n_long_rows = 0;
-  for (uint32_t r : rowlist)
-    if (odeg[r] > kMinTileCap) ++g->n_long_rows;
+  for (size_t q = 0; q < nrows; ++q)
+    if (stats[q].x > kMinTileCap) ++g->n_long_rows;
   g->h_items.clear();
   for (uint32_t k = 0; k < P; ++k) {
     uint32_t lo = count[k];
     uint64_t acc = 0;
     for (uint32_t q = count[k]; q < count[k + 1]; ++q) {
-      const uint32_t r = rowlist[q];
-      const uint64_t cost = row_cost(r);
+      const uint64_t cost = row_cost(q);
+      const uint32_t np = stats[q].z;
       if (cost > target) {
         if (q > lo) g->h_items.push_back({0, k, lo, q, 0, 0xFFFFFFFFu, 0, 0});
-        const uint64_t nchunk = std::min<uint64_t>(npairs[r], (cost + target - 1) / target);
+        const uint64_t nchunk = std::min<uint64_t>(np, (cost + target - 1) / target);
         for (uint64_t c = 0; c < nchunk; ++c) {
-          const uint32_t c_lo = (uint32_t)(npairs[r] * c / nchunk);
-          const uint32_t c_hi = (uint32_t)(npairs[r] * (c + 1) / nchunk);
+          const uint32_t c_lo = (uint32_t)(np * c / nchunk);
+          const uint32_t c_hi = (uint32_t)(np * (c + 1) / nchunk);
           if (c_hi > c_lo) g->h_items.push_back({0, k, q, q + 1, c_lo, c_hi, 0, 0});
         }
         lo = q + 1;
@@ -1013,10 +1015,9 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
       const WorkItem &w = g->h_items[i];
       uint64_t c = 0;
       if (w.hi == w.lo + 1 && w.c_hi != 0xFFFFFFFFu) {
-        const uint32_t r = rowlist[w.lo];
-        c = row_cost(r) * (w.c_hi - w.c_lo) / std::max(1u, npairs[r]);
+        c = row_cost(w.lo) * (w.c_hi - w.c_lo) / std::max(1u, stats[w.lo].z);
       } else {
-        for (uint32_t q = w.lo; q < w.hi; ++q) c += row_cost(rowlist[q]);
+        for (uint32_t q = w.lo; q < w.hi; ++q) c += row_cost(q);
       }
       cost[i] = {c, (uint32_t)i};
     }
@@ -1032,16 +1033,17 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
   }
   // The pair work model: pairs, owning rows, streamed and staged elements (staged once
   // per row, and once per item that holds the row).
-  g->n_pairs = poff[V];
-  g->n_rows = rowlist.size();
+  g->n_pairs = 0;
+  g->n_rows = nrows;
   g->streamed = g->staged = g->staged_items = 0;
-  for (uint32_t r : rowlist) {
-    g->streamed += work[r];
-    g->staged += odeg[r];
+  for (size_t q = 0; q < nrows; ++q) {
+    g->n_pairs += stats[q].z;
+    g->streamed += stats[q].y;
+    g->staged += stats[q].x;
   }
   for (const WorkItem &w : g->h_items)
-    for (uint32_t q = w.lo; q < w.hi; ++q) g->staged_items += odeg[rowlist[q]];
-  partition_features(g, odeg, npairs, work, deg, rowlist);
+    for (uint32_t q = w.lo; q < w.hi; ++q) g->staged_items += stats[q].x;
+  partition_features(g, stats);
   g->h_learned.clear();
   g->learned_version = 0;
   g->dlearned_version = 0;
@@ -1057,11 +1059,7 @@ int pac_graph_prepare(pac_graph *h, uint32_t num_partitions, const int32_t *v2p)
     PAC_TRY(g->dstrat.reserve(tables.size()));
     PAC_CHECK_HIP(hipMemcpy(g->dstrat.p, tables.data(), tables.size(), hipMemcpyHostToDevice));
   }
-  PAC_TRY(g->rows.reserve(sizeof(uint32_t) * (rowlist.empty() ? 1 : rowlist.size())));
   PAC_TRY(g->items.reserve(sizeof(WorkItem) * (g->h_items.empty() ? 1 : g->h_items.size())));
-  if (!rowlist.empty())
-    PAC_CHECK_HIP(hipMemcpy(g->rows.p, rowlist.data(), sizeof(uint32_t) * rowlist.size(),
-                            hipMemcpyHostToDevice));
   if (!g->h_items.empty())
     PAC_CHECK_HIP(hipMemcpy(g->items.p, g->h_items.data(), sizeof(WorkItem) * g->h_items.size(),
                             hipMemcpyHostToDevice));

@@ -129,9 +129,12 @@ struct Graph {
   // list in LDS while the partner's shorter list is streamed.  Pairs whose partner has
   // an empty out-list are dropped (they hold no triangle).
   DevBuf poff, ppar, pedge;
-  // In-edges: for each vertex v, the oadj positions of the edges u->v (V+1 offsets).
-  DevBuf inoff, inidx;
+  // Construction state, kept for the later passes: crow[c] = the row holding entry 256c;
+  // oflags / pflags = one bit per symmetric entry (kept by the orientation / owned as a
+  // pair), owpre / ppre = exclusive scans of their 64-bit words' popcounts.
+  DevBuf crow, oflags, owpre, pflags, ppre;
   bool pairs_built = false;
+  bool back_edges_built = false;  // pedge of the pairs owned by their head (kBackEdge before)
   // Per-edge supports (SUP mode) from hit words instead of one atomic per triangle:
   // pstreams[y] = number of pairs streaming N+(y); wbase[y] = first word of y's streams;
   // pword, hitw as in GraphDev.
@@ -203,12 +206,30 @@ struct Ctx {
 // ---- prepare.hip -----------------------------------------------------------------
 int build_symmetric_from_host(Graph *g, const uint64_t *vlist, const int32_t *elist);
 int orient(Graph *g);
-// Build the pair lists and in-edge lists of an oriented graph (once; later calls
-// return at once).
+// Build the pair lists and each row's streamed length (rowwork) of an oriented graph
+// (once; later calls return at once).
 int build_pairs(Graph *g);
+// What pac_graph_prepare needs of the rows that own pairs, from the device: the rows in ID
+// order (also left in g->rows), per row (d+(r), streamed length, pairs, d(r)), W and the
+// largest out-degree; with part_width > 0 also the oriented edges of each of the P
+// contiguous partitions of part_width IDs.
+struct RowSummary {
+  std::vector<uint32_t> rows;
+  std::vector<uint4> stats;
+  std::vector<uint64_t> part_edges;
+  uint64_t W = 0, max_outdeg = 0;
+};
+int row_summary(Graph *g, uint64_t part_width, uint32_t P, RowSummary &out);
 // Value of the entry past the last oriented edge (oadj[m]): neither a vertex ID (IDs
 // are < 2^31) nor the empty hash-table slot, so a lookup of it never hits.
 constexpr uint32_t kListEnd = 0xFFFFFFFEu;
+// pedge of a pair u->x owned by its head x until build_back_edges: never an edge of the
+// owner's out-list (e - ooff[x] >= d+(x)), so the TC/LCC kernels take it for u->x, the
+// only fact they need of it.
+constexpr uint32_t kBackEdge = 0xFFFFFFFFu;
+// Fill pedge of the pairs owned by their head with their edge indices (once; the per-edge
+// supports need them).
+int build_back_edges(Graph *g);
 // Vertices streamed by at most 64 pairs have their hit words summed by one wave (one
 // load per word column); the others are split into units of kSupRun streams.
 constexpr uint32_t kSupRun = 256;
@@ -253,7 +274,6 @@ int noalloc_scratch_init(Ctx *ctx);
 // static scratch: zeroes and fills d_total (and d_tvert in LCC mode); allocates nothing.
 int launch_persistent_noalloc(Ctx *ctx, Graph *g, int mode, const uint8_t *d_strat,
                               unsigned long long *d_total, unsigned long long *d_tvert);
-int compute_rowwork(Graph *g);
 // LCC mode: per-vertex triangle counts = row totals + per-edge credits (after a launch).
 int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64);
 int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *inter,
@@ -299,10 +319,9 @@ int fixed_heuristic(const Graph *g, std::vector<uint8_t> &out);
 // GRAPH); totals (may be null) receives each graph's T.
 int solve_batch(Ctx *c, pac_graph *const *hs, uint32_t G, int mode, int dispatch, int launch,
                 uint64_t *totals);
-void partition_features(Graph *g, const std::vector<uint32_t> &h_odeg,
-                        const std::vector<uint32_t> &h_npairs,
-                        const std::vector<uint32_t> &h_work, const std::vector<uint32_t> &h_deg,
-                        const std::vector<uint32_t> &rowlist);
+// Per-partition features and the fixed heuristic's choice, from the per-row summaries
+// (d+(r), streamed length, pairs, d(r)) in the order of the grouped row list.
+void partition_features(Graph *g, const std::vector<uint4> &stats);
 
 }  // namespace pac
 

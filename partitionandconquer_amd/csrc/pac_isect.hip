@@ -899,22 +899,6 @@ __global__ void k_fill_u64(unsigned long long *p, uint64_t n) {
     p[i] = 0ull;
 }
 
-// work[u] = sum of d+(partner) over u's pairs: the streamed length of row u (wave per row).
-__global__ void k_rowwork(const uint32_t *__restrict__ poff, const uint32_t *__restrict__ ppar,
-                          const uint32_t *__restrict__ odeg, uint32_t V,
-                          uint32_t *__restrict__ work) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t wpb = blockDim.x >> 6;
-  for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
-       u += (uint64_t)gridDim.x * wpb) {
-    const uint32_t p0 = poff[u], p1 = poff[u + 1];
-    uint64_t s = 0;
-    for (uint32_t i = p0 + lane; i < p1; i += 64) s += odeg[ppar[i]];
-    s = wave_sum64(s);
-    if (lane == 0) work[u] = s > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)s;
-  }
-}
-
 // ---- epilogues -----------------------------------------------------------------------
 __global__ void k_lcc(const unsigned long long *__restrict__ tvert,
                       const uint32_t *__restrict__ off, uint32_t V, float *lcc32,
@@ -1501,17 +1485,6 @@ int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
     if (mode == kModeSUP) PAC_TRY(fill_u32(ctx->stream, g->sup.as<uint32_t>(), g->m, 0u));
     PAC_CHECK_HIP(hipGetLastError());
   }
-  return PAC_OK;
-}
-
-int compute_rowwork(Graph *g) {
-  PAC_TRY(build_pairs(g));
-  PAC_TRY(g->rowwork.reserve(sizeof(uint32_t) * (size_t)(g->V ? g->V : 1)));
-  if (g->V == 0) return PAC_OK;
-  hipLaunchKernelGGL(k_rowwork, dim3(grid_for(g->V, 4)), dim3(256), 0, g->ctx->stream,
-                     g->poff.as<const uint32_t>(), g->ppar.as<const uint32_t>(),
-                     g->odeg.as<const uint32_t>(), g->V, g->rowwork.as<uint32_t>());
-  PAC_CHECK_HIP(hipGetLastError());
   return PAC_OK;
 }
 

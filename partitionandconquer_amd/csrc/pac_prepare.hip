@@ -1,4 +1,5 @@
-// pac_prepare.hip -- symmetric CSR upload/validation and the degree-ordered orientation.
+// pac_prepare.hip -- symmetric CSR upload/validation, the degree-ordered orientation and
+// the intersection pairs.
 //
 // Reference counterparts (Byy-0024/PartitionAndConquer):
 //   Graph::Graph(csrdir, bool) / Graph(adj_list)  Graph.cpp:92-149 (CSR members)
@@ -7,13 +8,18 @@
 //   is by (degree, id), which bounds every out-list by sqrt(2m); triangle counts do
 //   not depend on the orientation.
 //
-// Orientation = histogram / exclusive scan / scatter into a uint32 CSR:
-//   1. d+(u) = |{v in N(u) : (d(u),u) < (d(v),v)}|    (one wave per row, ballot+popcount)
-//   2. ooff = exclusive_scan(d+)                        (rocPRIM device scan)
-//   3. each row's kept neighbours are written in input order (ballot prefix), so
-//      rows stay sorted by ID and no re-sort is needed.
-#include <rocprim/device/device_radix_sort.hpp>
+// Orientation = flags / scan / scatter into a uint32 CSR, edge-parallel throughout:
+//   1. one keep bit per symmetric entry (u, v): (d(u),u) < (d(v),v), written as aligned
+//      64-bit words (word k holds entries 64k .. 64k+63); each lane finds its row by a
+//      short search between the rows of its chunk's first entry and the next chunk's;
+//   2. exclusive scan of the words' popcounts: the kept entry i goes to oadj position
+//      wpre[i/64] + (kept entries below it in its word), so
+//   3. the scatter needs no row at all, keeps every row's input (ID) order, and
+//      ooff[u] is that same ordinal of off[u].
+// The intersection pairs are built the same way from one ownership bit per entry.
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 
@@ -100,49 +106,121 @@ __global__ void k_check_rows(const uint32_t *__restrict__ off, const uint32_t *_
   if (lane == 0 && bits) atomicOr(err, bits);
 }
 
-// 1. out-degree histogram (one wave per row)
-__global__ void k_outdeg(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
-                         const uint32_t *__restrict__ deg, uint32_t V,
-                         uint32_t *__restrict__ odeg) {
-  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
-  for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
-       u += (uint64_t)gridDim.x * wpb) {
-    const uint32_t a0 = off[u], a1 = off[u + 1], du = a1 - a0;
-    uint32_t cnt = 0;
-    for (uint32_t i0 = a0; i0 < a1; i0 += 64) {
-      const uint32_t i = i0 + lane;
-      bool keep = false;
-      if (i < a1) {
-        const uint32_t v = adj[i];
-        keep = before(du, (uint32_t)u, deg[v], v);
-      }
-      cnt += (uint32_t)__popcll(__ballot(keep));
-    }
-    if (lane == 0) odeg[u] = cnt;
+// Entries are handled in aligned words of 64 and chunks of kChunkWords words; crow[c]
+// is the row holding entry c * kChunk (the first entry of chunk c).
+constexpr uint32_t kChunkWords = 4, kChunk = 64 * kChunkWords;
+
+__host__ __device__ __forceinline__ uint64_t num_chunks(uint64_t nnz) { return (nnz + kChunk - 1) / kChunk; }
+
+// crow: each non-empty row writes the chunk starts that fall inside it (thread per row).
+__global__ void k_chunk_rows(const uint32_t *__restrict__ off, uint32_t V,
+                             uint32_t *__restrict__ crow) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < V;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t a = off[u], b = off[u + 1];
+    for (uint64_t c = (a + kChunk - 1) / kChunk; c * kChunk < b; ++c) crow[c] = (uint32_t)u;
   }
 }
 
-// 3. order-preserving scatter of the kept neighbours
-__global__ void k_scatter(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
-                          const uint32_t *__restrict__ deg, const uint32_t *__restrict__ ooff,
-                          uint32_t V, uint32_t *__restrict__ oadj) {
-  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
-  for (uint64_t u = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); u < V;
-       u += (uint64_t)gridDim.x * wpb) {
-    const uint32_t a0 = off[u], a1 = off[u + 1], du = a1 - a0;
-    uint32_t out = ooff[u];
-    for (uint32_t i0 = a0; i0 < a1; i0 += 64) {
-      const uint32_t i = i0 + lane;
-      bool keep = false;
-      uint32_t v = 0;
-      if (i < a1) {
-        v = adj[i];
-        keep = before(du, (uint32_t)u, deg[v], v);
-      }
-      const uint64_t mask = __ballot(keep);
-      if (keep) oadj[out + lanes_below(mask)] = v;
-      out += (uint32_t)__popcll(mask);
+// The row holding entry i: the largest t in [lo, hi] with off[t] <= i, given off[lo] <= i
+// and that i lies before the first entry of row hi + 1.  (Empty rows t have off[t] =
+// off[t + 1], so the largest such t is the non-empty row holding i.)  The trip count
+// depends only on hi - lo, which is uniform across a chunk's lanes.
+__device__ __forceinline__ uint32_t row_of(const uint32_t *__restrict__ off, uint32_t lo,
+                                           uint32_t hi, uint32_t i) {
+  uint32_t n = hi - lo + 1;
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+    if (off[lo + half] <= i) {
+      lo += half;
+      n -= half;
+    } else {
+      n = half;
     }
+  }
+  return lo;
+}
+
+// The rows bounding chunk c: lo holds its first entry, hi its last (or a later row).
+__device__ __forceinline__ void chunk_rows(const uint32_t *__restrict__ crow, uint64_t c,
+                                           uint64_t nchunks, uint32_t V, uint32_t &lo,
+                                           uint32_t &hi) {
+  lo = crow[c];
+  hi = c + 1 < nchunks ? crow[c + 1] : V - 1;
+}
+
+__device__ __forceinline__ bool bit_of(unsigned long long w, uint32_t b) {
+  return (w >> b) & 1ull;
+}
+
+// Position of entry i among the set bits of flags: wpre[i/64] + the set bits below it
+// in its word (wpre = exclusive scan of the words' popcounts).
+__device__ __forceinline__ uint32_t ordinal(const unsigned long long *__restrict__ flags,
+                                            const uint32_t *__restrict__ wpre, uint32_t i) {
+  const uint32_t k = i >> 6, r = i & 63u;
+  return wpre[k] + (uint32_t)__popcll(flags[k] & ((1ull << r) - 1ull));
+}
+
+// 1. keep bits: entry (t, v) of row t is kept when (d(t),t) < (d(v),v).  One wave per
+// chunk; word k is written once, by the wave of its chunk.
+__global__ void __launch_bounds__(256)
+k_orient_flags(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+               const uint32_t *__restrict__ deg, const uint32_t *__restrict__ crow, uint32_t V,
+               uint32_t nnz, unsigned long long *__restrict__ flags) {
+  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
+  const uint64_t nchunks = num_chunks(nnz);
+  for (uint64_t c = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); c < nchunks;
+       c += (uint64_t)gridDim.x * wpb) {
+    uint32_t lo, hi;
+    chunk_rows(crow, c, nchunks, V, lo, hi);
+#pragma unroll
+    for (uint32_t j = 0; j < kChunkWords; ++j) {
+      const uint64_t i = c * kChunk + 64u * j + lane;
+      bool keep = false;
+      if (i < nnz) {
+        const uint32_t t = row_of(off, lo, hi, (uint32_t)i);
+        const uint32_t v = adj[i];
+        keep = before(deg[t], t, deg[v], v);
+      }
+      const unsigned long long mask = __ballot(keep);
+      const uint64_t k = c * kChunkWords + j;
+      if (lane == 0 && 64u * k < nnz) flags[k] = mask;
+    }
+  }
+}
+
+// cnt[k] = set bits of word k (n words).
+__global__ void k_popc_words(const unsigned long long *__restrict__ flags, uint64_t n,
+                             uint32_t *__restrict__ cnt) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    cnt[k] = (uint32_t)__popcll(flags[k]);
+}
+
+// Row offsets over the set bits: out_off[u] = ordinal(off[u]) for u <= V, out_len[u] =
+// the row's set bits (u < V; may be null).
+__global__ void k_ordinal_offsets(const uint32_t *__restrict__ off,
+                                  const unsigned long long *__restrict__ flags,
+                                  const uint32_t *__restrict__ wpre, uint32_t V,
+                                  uint32_t *__restrict__ out_off, uint32_t *__restrict__ out_len) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u <= V;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t a = ordinal(flags, wpre, off[u]);
+    out_off[u] = a;
+    if (out_len && u < V) out_len[u] = ordinal(flags, wpre, off[u + 1]) - a;
+  }
+}
+
+// 3. scatter: thread per entry, each wave on one aligned word (the grid stride is a
+// multiple of 64), the kept entries written in input order.
+__global__ void __launch_bounds__(256)
+k_orient_scatter(const uint32_t *__restrict__ adj, const unsigned long long *__restrict__ flags,
+                 const uint32_t *__restrict__ wpre, uint32_t nnz, uint32_t *__restrict__ oadj) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nnz;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = i >> 6;
+    const unsigned long long bits = flags[k];
+    if (bit_of(bits, (uint32_t)(i & 63u))) oadj[wpre[k] + lanes_below(bits)] = adj[i];
   }
 }
 
@@ -152,85 +230,109 @@ __global__ void k_scatter(const uint32_t *__restrict__ off, const uint32_t *__re
 // tail:  x->u: d+(u) <= d+(x) (and d+(u) > 0, else the pair holds no triangle);
 //        u->x: d+(x) >  d+(u).
 // Exactly one endpoint owns each edge with d+(h) > 0 or d+(t) < d+(h).
-__device__ __forceinline__ bool owns(uint32_t dx, uint32_t x, uint32_t ox, uint32_t du, uint32_t u,
-                                     uint32_t ou, bool *fwd) {
-  *fwd = before(dx, x, du, u);
-  return *fwd ? (ou <= ox && ou > 0u) : (ox > ou);
+__device__ __forceinline__ bool owns(bool fwd, uint32_t ox, uint32_t ou) {
+  return fwd ? (ou <= ox && ou > 0u) : (ox > ou);
 }
 
-__global__ void k_iota(uint32_t *__restrict__ p, uint64_t n) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    p[i] = (uint32_t)i;
-}
-
-// pairs owned per vertex (one wave per vertex, over its symmetric row) and in-degree
-__global__ void k_pair_count(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
-                             const uint32_t *__restrict__ deg, const uint32_t *__restrict__ odeg,
-                             uint32_t V, uint32_t *__restrict__ npairs,
-                             uint32_t *__restrict__ nin) {
+// Ownership bits of the entries (x, u) of row x, and each row's streamed length
+// rowwork[x] = sum of d+(u) over its pairs (rowwork zeroed before): the lanes of one row
+// are contiguous, so a segmented scan across the wave leaves each row's part of the word
+// in its last lane, which adds it.  fwd = the entry's keep bit from the orientation.
+__global__ void __launch_bounds__(256)
+k_pair_flags(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+             const uint32_t *__restrict__ odeg, const unsigned long long *__restrict__ oflags,
+             const uint32_t *__restrict__ crow, uint32_t V, uint32_t nnz,
+             unsigned long long *__restrict__ pflags, uint32_t *__restrict__ rowwork) {
   const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
-  for (uint64_t x = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); x < V;
-       x += (uint64_t)gridDim.x * wpb) {
-    const uint32_t a0 = off[x], a1 = off[x + 1], dx = a1 - a0, ox = odeg[x];
-    uint32_t cnt = 0;
-    for (uint32_t i0 = a0; i0 < a1; i0 += 64) {
-      const uint32_t i = i0 + lane;
+  const uint64_t nchunks = num_chunks(nnz);
+  for (uint64_t c = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); c < nchunks;
+       c += (uint64_t)gridDim.x * wpb) {
+    uint32_t lo, hi;
+    chunk_rows(crow, c, nchunks, V, lo, hi);
+#pragma unroll
+    for (uint32_t j = 0; j < kChunkWords; ++j) {
+      const uint64_t k = c * kChunkWords + j;
+      const uint64_t i = 64u * k + lane;
       bool own = false;
-      if (i < a1) {
-        const uint32_t u = adj[i];
-        bool fwd;
-        own = owns(dx, (uint32_t)x, ox, deg[u], u, odeg[u], &fwd);
+      uint32_t x = 0xFFFFFFFFu, w = 0;  // lanes past the end: no row
+      if (i < nnz) {
+        x = row_of(off, lo, hi, (uint32_t)i);
+        const uint32_t ou = odeg[adj[i]];
+        own = owns(bit_of(oflags[k], lane), odeg[x], ou);
+        w = own ? ou : 0u;
       }
-      cnt += (uint32_t)__popcll(__ballot(own));
-    }
-    if (lane == 0) {
-      npairs[x] = cnt;
-      nin[x] = dx - ox;
+      const unsigned long long mask = __ballot(own);
+      if (lane == 0 && 64u * k < nnz) pflags[k] = mask;
+#pragma unroll
+      for (uint32_t sft = 1; sft < 64; sft <<= 1) {
+        const uint32_t wu = (uint32_t)__shfl_up((int)w, sft, 64);
+        const uint32_t xu = (uint32_t)__shfl_up((int)x, sft, 64);
+        if (lane >= sft && xu == x) w += wu;
+      }
+      const uint32_t xn = (uint32_t)__shfl_down((int)x, 1, 64);
+      if ((lane == 63u || xn != x) && w != 0u) atomicAdd(&rowwork[x], w);
     }
   }
 }
 
-// Order-preserving fill: x's pairs come out sorted by partner ID.  The edge index of a
-// pair x->u is ooff[x] + (rank of u among x's out-neighbours); of u->x it is the next
-// entry of x's in-edge list, which holds the edges u->x in u order.
-__global__ void k_pair_fill(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
-                            const uint32_t *__restrict__ deg, const uint32_t *__restrict__ odeg,
-                            const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
-                            const uint32_t *__restrict__ poff, const uint32_t *__restrict__ inoff,
-                            const uint32_t *__restrict__ inidx, uint32_t V,
-                            uint32_t *__restrict__ ppar, uint32_t *__restrict__ pedge) {
+// Pair lists: thread per entry (each wave on one aligned word); an owned entry (x, u)
+// becomes pair ordinal(i) of the ownership bits, in input order, so each row's pairs
+// come out sorted by partner.  The pair's edge index: for x->u the entry's position in
+// oadj (its orientation ordinal); for u->x kBackEdge until k_back_edges finds it.
+__global__ void __launch_bounds__(256)
+k_pair_fill(const uint32_t *__restrict__ adj, const unsigned long long *__restrict__ oflags,
+            const uint32_t *__restrict__ owpre, const unsigned long long *__restrict__ pflags,
+            const uint32_t *__restrict__ ppre, uint32_t nnz, uint32_t *__restrict__ ppar,
+            uint32_t *__restrict__ pedge) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nnz;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = i >> 6;
+    const uint32_t b = (uint32_t)(i & 63u);
+    const unsigned long long pb = pflags[k];
+    if (bit_of(pb, b)) {
+      const unsigned long long ob = oflags[k];
+      const uint32_t p = ppre[k] + lanes_below(pb);
+      ppar[p] = adj[i];
+      pedge[p] = bit_of(ob, b) ? owpre[k] + lanes_below(ob) : kBackEdge;
+    }
+  }
+}
+
+// Edge indices of the pairs owned by their head (u->x, entry (x, u) of row x): the
+// position of x in N+(u), found by a binary search of the sorted oriented row.
+__global__ void __launch_bounds__(256)
+k_back_edges(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+             const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
+             const unsigned long long *__restrict__ oflags,
+             const unsigned long long *__restrict__ pflags, const uint32_t *__restrict__ ppre,
+             const uint32_t *__restrict__ crow, uint32_t V, uint32_t nnz,
+             uint32_t *__restrict__ pedge) {
   const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
-  for (uint64_t x = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); x < V;
-       x += (uint64_t)gridDim.x * wpb) {
-    const uint32_t a0 = off[x], a1 = off[x + 1], dx = a1 - a0, ox = odeg[x];
-    uint32_t out = poff[x], rank = ooff[x], in = inoff[x];
-    for (uint32_t i0 = a0; i0 < a1; i0 += 64) {
-      const uint32_t i = i0 + lane;
-      bool own = false, fwd = false;
-      uint32_t u = 0;
-      if (i < a1) {
-        u = adj[i];
-        own = owns(dx, (uint32_t)x, ox, deg[u], u, odeg[u], &fwd);
+  const uint64_t nchunks = num_chunks(nnz);
+  for (uint64_t c = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); c < nchunks;
+       c += (uint64_t)gridDim.x * wpb) {
+    uint32_t lo, hi;
+    chunk_rows(crow, c, nchunks, V, lo, hi);
+    for (uint32_t j = 0; j < kChunkWords; ++j) {
+      const uint64_t k = c * kChunkWords + j;
+      const uint64_t i = 64u * k + lane;
+      if (64u * k >= nnz) break;  // uniform
+      const unsigned long long pb = pflags[k], ob = oflags[k];
+      if (i < nnz && bit_of(pb, lane) && !bit_of(ob, lane)) {
+        const uint32_t x = row_of(off, lo, hi, (uint32_t)i);
+        const uint32_t u = adj[i];
+        uint32_t a = ooff[u], n = ooff[u + 1] - a;  // x is in N+(u): lower_bound finds it
+        while (n > 0) {
+          const uint32_t half = n >> 1;
+          if (oadj[a + half] < x) {
+            a += half + 1;
+            n -= half + 1;
+          } else {
+            n = half;
+          }
+        }
+        pedge[ppre[k] + lanes_below(pb)] = a;
       }
-      const bool back = i < a1 && !fwd;
-      const uint64_t fmask = __ballot(fwd);
-      const uint64_t omask = __ballot(own);
-      const uint64_t bmask = __ballot(back);
-      uint32_t e = 0;
-      if (fwd) {
-        e = rank + lanes_below(fmask);
-      } else if (back) {
-        e = inidx[in + lanes_below(bmask)];
-      }
-      if (own) {
-        const uint32_t k = out + lanes_below(omask);
-        ppar[k] = u;
-        pedge[k] = e;
-      }
-      rank += (uint32_t)__popcll(fmask);
-      out += (uint32_t)__popcll(omask);
-      in += (uint32_t)__popcll(bmask);
     }
   }
 }
@@ -275,40 +377,80 @@ int graph_validate(const Graph *g, int level) {
   return PAC_OK;
 }
 
+namespace {
+
+uint32_t grid_for_n(uint64_t n, uint32_t per_block) {
+  uint64_t b = (n + per_block - 1) / per_block;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 20));
+}
+
+// wpre = exclusive scan of the popcounts of flags[0 .. n) (n = words + 1; the last word
+// is zero, so wpre[n - 1] is the total).  cnt: n uint32 of temporary storage.
+int scan_words(Ctx *ctx, const unsigned long long *flags, uint64_t n, uint32_t *cnt,
+               uint32_t *wpre) {
+  hipLaunchKernelGGL(k_popc_words, dim3(grid_for_n(n, 256)), dim3(256), 0, ctx->stream, flags, n,
+                     cnt);
+  PAC_CHECK_HIP(hipGetLastError());
+  size_t tmp = 0;
+  PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp, cnt, wpre, 0u, (size_t)n,
+                                        rocprim::plus<uint32_t>(), ctx->stream));
+  PAC_TRY(ctx->scratch.reserve(tmp));
+  PAC_CHECK_HIP(rocprim::exclusive_scan(ctx->scratch.p, tmp, cnt, wpre, 0u, (size_t)n,
+                                        rocprim::plus<uint32_t>(), ctx->stream));
+  return PAC_OK;
+}
+
+}  // namespace
+
 int orient(Graph *g) {
   Ctx *ctx = g->ctx;
   const uint32_t V = g->V;
+  const uint64_t nnz = g->nnz, nwords = (nnz + 63) / 64, nchunks = num_chunks(nnz);
   PAC_TRY(g->deg.reserve(sizeof(uint32_t) * (V ? V : 1)));
   PAC_TRY(g->odeg.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
   PAC_TRY(g->ooff.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  PAC_TRY(g->crow.reserve(sizeof(uint32_t) * (nchunks ? nchunks : 1)));
+  PAC_TRY(g->oflags.reserve(sizeof(unsigned long long) * (nwords + 1)));
+  PAC_TRY(g->owpre.reserve(sizeof(uint32_t) * (nwords + 1)));
   PAC_TRY(ctx->scratch2.reserve(64));
   uint32_t *maxdeg = ctx->scratch2.as<uint32_t>();
   PAC_TRY(fill_u32(ctx->stream, maxdeg, 1, 0u));
   PAC_TRY(fill_u32(ctx->stream, g->odeg.as<uint32_t>(), (uint64_t)V + 1, 0u));
+  // the word past the last entry stays zero: ordinal(nnz) = the kept total
+  PAC_TRY(fill_u32(ctx->stream, g->oflags.as<uint32_t>() + 2 * nwords, 2, 0u));
   if (V) {
     hipLaunchKernelGGL(k_deg, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(4096u, (V + 255) / 256))),
                        dim3(256), 0, ctx->stream,
                        g->off.as<const uint32_t>(), V, g->deg.as<uint32_t>(), maxdeg);
-    hipLaunchKernelGGL(k_outdeg, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
-                       g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
-                       g->deg.as<const uint32_t>(), V, g->odeg.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
-  size_t tmp_bytes = 0;
-  PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, g->odeg.as<uint32_t>(),
-                                        g->ooff.as<uint32_t>(), 0u, (size_t)V + 1,
-                                        rocprim::plus<uint32_t>(), ctx->stream));
-  PAC_TRY(ctx->scratch.reserve(tmp_bytes));
-  PAC_CHECK_HIP(rocprim::exclusive_scan(ctx->scratch.p, tmp_bytes, g->odeg.as<uint32_t>(),
-                                        g->ooff.as<uint32_t>(), 0u, (size_t)V + 1,
-                                        rocprim::plus<uint32_t>(), ctx->stream));
-  uint32_t hm[2] = {0, 0};
-  PAC_CHECK_HIP(hipMemcpyAsync(&hm[0], g->ooff.as<uint32_t>() + V, 4, hipMemcpyDeviceToHost,
-                               ctx->stream));
-  PAC_CHECK_HIP(hipMemcpyAsync(&hm[1], maxdeg, 4, hipMemcpyDeviceToHost, ctx->stream));
-  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  g->m = hm[0];
-  g->max_deg = hm[1];
+  if (nnz) {
+    hipLaunchKernelGGL(k_chunk_rows, dim3(grid_for_n(V, 256)), dim3(256), 0, ctx->stream,
+                       g->off.as<const uint32_t>(), V, g->crow.as<uint32_t>());
+    hipLaunchKernelGGL(k_orient_flags, dim3(grid_for_n(nchunks, 4)), dim3(256), 0, ctx->stream,
+                       g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
+                       g->deg.as<const uint32_t>(), g->crow.as<const uint32_t>(), V,
+                       (uint32_t)nnz, g->oflags.as<unsigned long long>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  {
+    DevBuf cnt;
+    PAC_TRY(cnt.reserve(sizeof(uint32_t) * (nwords + 1)));
+    PAC_TRY(scan_words(ctx, g->oflags.as<const unsigned long long>(), nwords + 1,
+                       cnt.as<uint32_t>(), g->owpre.as<uint32_t>()));
+    hipLaunchKernelGGL(k_ordinal_offsets, dim3(grid_for_n((uint64_t)V + 1, 256)), dim3(256), 0,
+                       ctx->stream, g->off.as<const uint32_t>(),
+                       g->oflags.as<const unsigned long long>(), g->owpre.as<const uint32_t>(), V,
+                       g->ooff.as<uint32_t>(), g->odeg.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+    uint32_t hm[2] = {0, 0};
+    PAC_CHECK_HIP(hipMemcpyAsync(&hm[0], g->ooff.as<uint32_t>() + V, 4, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    PAC_CHECK_HIP(hipMemcpyAsync(&hm[1], maxdeg, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // cnt is freed on return
+    g->m = hm[0];
+    g->max_deg = hm[1];
+  }
   if (2 * g->m != g->nnz) {
     set_error("orientation kept %llu of %llu entries: the CSR is not symmetric",
               (unsigned long long)g->m, (unsigned long long)g->nnz);
@@ -316,82 +458,54 @@ int orient(Graph *g) {
   }
   // m entries plus one sentinel (kListEnd) that list streams past their end read
   PAC_TRY(g->oadj.reserve(sizeof(uint32_t) * (g->m + 1)));
-  if (V) {
-    hipLaunchKernelGGL(k_scatter, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
-                       g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
-                       g->deg.as<const uint32_t>(), g->ooff.as<const uint32_t>(), V,
-                       g->oadj.as<uint32_t>());
+  if (nnz) {
+    hipLaunchKernelGGL(k_orient_scatter, dim3(grid_for_n(nnz, 256 * 4)), dim3(256), 0,
+                       ctx->stream, g->adj.as<const uint32_t>(),
+                       g->oflags.as<const unsigned long long>(), g->owpre.as<const uint32_t>(),
+                       (uint32_t)nnz, g->oadj.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
   PAC_TRY(fill_u32(ctx->stream, g->oadj.as<uint32_t>() + g->m, 1, kListEnd));
   g->pairs_built = false;
+  g->back_edges_built = false;
   return PAC_OK;
 }
 
-// Pair lists = histogram / exclusive scan / order-preserving fill, like the orientation.
+// Pair lists and row work = ownership bits / scan / fill, like the orientation.
 int build_pairs(Graph *g) {
   if (g->pairs_built) return PAC_OK;
   Ctx *ctx = g->ctx;
   const uint32_t V = g->V;
-  DevBuf npairs, nin;
-  PAC_TRY(npairs.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
-  PAC_TRY(nin.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
+  const uint64_t nnz = g->nnz, nwords = (nnz + 63) / 64, nchunks = num_chunks(nnz);
   PAC_TRY(g->poff.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
-  PAC_TRY(g->inoff.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
-  PAC_TRY(g->inidx.reserve(sizeof(uint32_t) * (g->m ? g->m : 1)));
-  PAC_TRY(fill_u32(ctx->stream, npairs.as<uint32_t>(), (uint64_t)V + 1, 0u));
-  PAC_TRY(fill_u32(ctx->stream, nin.as<uint32_t>(), (uint64_t)V + 1, 0u));
-  if (V) {
-    hipLaunchKernelGGL(k_pair_count, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
+  PAC_TRY(g->pflags.reserve(sizeof(unsigned long long) * (nwords + 1)));
+  PAC_TRY(g->ppre.reserve(sizeof(uint32_t) * (nwords + 1)));
+  PAC_TRY(g->rowwork.reserve(sizeof(uint32_t) * (V ? V : 1)));
+  PAC_TRY(fill_u32(ctx->stream, g->pflags.as<uint32_t>() + 2 * nwords, 2, 0u));
+  PAC_TRY(fill_u32(ctx->stream, g->rowwork.as<uint32_t>(), V, 0u));
+  if (nnz) {
+    hipLaunchKernelGGL(k_pair_flags, dim3(grid_for_n(nchunks, 4)), dim3(256), 0, ctx->stream,
                        g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
-                       g->deg.as<const uint32_t>(), g->odeg.as<const uint32_t>(), V,
-                       npairs.as<uint32_t>(), nin.as<uint32_t>());
+                       g->odeg.as<const uint32_t>(), g->oflags.as<const unsigned long long>(),
+                       g->crow.as<const uint32_t>(), V, (uint32_t)nnz,
+                       g->pflags.as<unsigned long long>(), g->rowwork.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
-  size_t tmp_in = 0;
-  PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp_in, nin.as<uint32_t>(),
-                                        g->inoff.as<uint32_t>(), 0u, (size_t)V + 1,
-                                        rocprim::plus<uint32_t>(), ctx->stream));
-  PAC_TRY(ctx->scratch.reserve(tmp_in));
-  PAC_CHECK_HIP(rocprim::exclusive_scan(ctx->scratch.p, tmp_in, nin.as<uint32_t>(),
-                                        g->inoff.as<uint32_t>(), 0u, (size_t)V + 1,
-                                        rocprim::plus<uint32_t>(), ctx->stream));
-  // In-edge lists = the oriented edges sorted by head (a CSR transpose).  The radix sort
-  // is stable and the edges enter in tail order, so each in-list is in tail order.
-  if (g->m) {
-    DevBuf keys_out, vals_in;
-    PAC_TRY(keys_out.reserve(sizeof(uint32_t) * g->m));
-    PAC_TRY(vals_in.reserve(sizeof(uint32_t) * g->m));
-    hipLaunchKernelGGL(k_iota, dim3(std::min<uint64_t>(1u << 16, (g->m + 255) / 256)), dim3(256), 0,
-                       ctx->stream, vals_in.as<uint32_t>(), g->m);
-    PAC_CHECK_HIP(hipGetLastError());
-    uint32_t bits = 1;
-    while (bits < 32 && (1ull << bits) < V) ++bits;
-    size_t tmp_sort = 0;
-    PAC_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tmp_sort, g->oadj.as<const uint32_t>(),
-                                            keys_out.as<uint32_t>(), vals_in.as<const uint32_t>(),
-                                            g->inidx.as<uint32_t>(), (size_t)g->m, 0u, bits,
-                                            ctx->stream));
-    PAC_TRY(ctx->scratch.reserve(tmp_sort));
-    PAC_CHECK_HIP(rocprim::radix_sort_pairs(ctx->scratch.p, tmp_sort, g->oadj.as<const uint32_t>(),
-                                            keys_out.as<uint32_t>(), vals_in.as<const uint32_t>(),
-                                            g->inidx.as<uint32_t>(), (size_t)g->m, 0u, bits,
-                                            ctx->stream));
-    // keys_out and vals_in are freed on return
-    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  }
-  size_t tmp_bytes = 0;
-  PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, npairs.as<uint32_t>(),
-                                        g->poff.as<uint32_t>(), 0u, (size_t)V + 1,
-                                        rocprim::plus<uint32_t>(), ctx->stream));
-  PAC_TRY(ctx->scratch.reserve(tmp_bytes));
-  PAC_CHECK_HIP(rocprim::exclusive_scan(ctx->scratch.p, tmp_bytes, npairs.as<uint32_t>(),
-                                        g->poff.as<uint32_t>(), 0u, (size_t)V + 1,
-                                        rocprim::plus<uint32_t>(), ctx->stream));
   uint32_t np = 0;
-  PAC_CHECK_HIP(hipMemcpyAsync(&np, g->poff.as<uint32_t>() + V, 4, hipMemcpyDeviceToHost,
-                               ctx->stream));
-  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  {
+    DevBuf cnt;
+    PAC_TRY(cnt.reserve(sizeof(uint32_t) * (nwords + 1)));
+    PAC_TRY(scan_words(ctx, g->pflags.as<const unsigned long long>(), nwords + 1,
+                       cnt.as<uint32_t>(), g->ppre.as<uint32_t>()));
+    hipLaunchKernelGGL(k_ordinal_offsets, dim3(grid_for_n((uint64_t)V + 1, 256)), dim3(256), 0,
+                       ctx->stream, g->off.as<const uint32_t>(),
+                       g->pflags.as<const unsigned long long>(), g->ppre.as<const uint32_t>(), V,
+                       g->poff.as<uint32_t>(), nullptr);
+    PAC_CHECK_HIP(hipGetLastError());
+    PAC_CHECK_HIP(hipMemcpyAsync(&np, g->poff.as<uint32_t>() + V, 4, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // cnt is freed on return
+  }
   if (np > g->m) {
     set_error("pair construction produced %u pairs for %llu oriented edges", np,
               (unsigned long long)g->m);
@@ -399,19 +513,164 @@ int build_pairs(Graph *g) {
   }
   PAC_TRY(g->ppar.reserve(sizeof(uint32_t) * (np ? np : 1)));
   PAC_TRY(g->pedge.reserve(sizeof(uint32_t) * (np ? np : 1)));
-  if (V && g->m) {
-    hipLaunchKernelGGL(k_pair_fill, dim3(grid_waves(V)), dim3(256), 0, ctx->stream,
-                       g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
-                       g->deg.as<const uint32_t>(), g->odeg.as<const uint32_t>(),
-                       g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
-                       g->poff.as<const uint32_t>(), g->inoff.as<const uint32_t>(),
-                       g->inidx.as<const uint32_t>(), V, g->ppar.as<uint32_t>(),
+  if (nnz) {
+    hipLaunchKernelGGL(k_pair_fill, dim3(grid_for_n(nnz, 256 * 4)), dim3(256), 0, ctx->stream,
+                       g->adj.as<const uint32_t>(), g->oflags.as<const unsigned long long>(),
+                       g->owpre.as<const uint32_t>(), g->pflags.as<const unsigned long long>(),
+                       g->ppre.as<const uint32_t>(), (uint32_t)nnz, g->ppar.as<uint32_t>(),
                        g->pedge.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
-  // npairs and nin are freed on return: finish their readers first
-  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   g->pairs_built = true;
+  g->back_edges_built = false;
+  return PAC_OK;
+}
+
+// The edge indices of the pairs owned by their head (needed by the per-edge supports).
+int build_back_edges(Graph *g) {
+  if (g->back_edges_built) return PAC_OK;
+  PAC_TRY(build_pairs(g));
+  const uint64_t nnz = g->nnz, nchunks = num_chunks(nnz);
+  if (nnz) {
+    hipLaunchKernelGGL(k_back_edges, dim3(grid_for_n(nchunks, 4)), dim3(256), 0, g->ctx->stream,
+                       g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
+                       g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
+                       g->oflags.as<const unsigned long long>(),
+                       g->pflags.as<const unsigned long long>(), g->ppre.as<const uint32_t>(),
+                       g->crow.as<const uint32_t>(), g->V, (uint32_t)nnz,
+                       g->pedge.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  g->back_edges_built = true;
+  return PAC_OK;
+}
+
+namespace {
+
+// Per vertex: whether it owns pairs (flag), and over all vertices W = sum of d+(u) d(u)
+// (= sum over oriented t->h of d+(t) + d+(h)) and the largest d+ (one atomic per block).
+__global__ void __launch_bounds__(256)
+k_vertex_stats(const uint32_t *__restrict__ off, const uint32_t *__restrict__ odeg,
+               const uint32_t *__restrict__ poff, uint32_t V, uint8_t *__restrict__ owns_pairs,
+               unsigned long long *W, uint32_t *maxo) {
+  unsigned long long w = 0;
+  uint32_t mx = 0;
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < V;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t o = odeg[u];
+    w += (unsigned long long)o * (off[u + 1] - off[u]);
+    mx = max(mx, o);
+    owns_pairs[u] = poff[u + 1] > poff[u] ? 1 : 0;
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) {
+    w += __shfl_xor(w, sh, 64);
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, sh, 64));
+  }
+  __shared__ unsigned long long ws[4];
+  __shared__ uint32_t ms[4];
+  if ((threadIdx.x & 63u) == 0) {
+    ws[threadIdx.x >> 6] = w;
+    ms[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t i = 1; i < (blockDim.x >> 6); ++i) {
+      w += ws[i];
+      mx = max(mx, ms[i]);
+    }
+    if (w) atomicAdd(W, w);
+    if (mx) atomicMax(maxo, mx);
+  }
+}
+
+// The summary of each row that owns pairs: d+(r), its streamed length, its pairs, d(r).
+__global__ void k_row_gather(const uint32_t *__restrict__ rows, uint32_t n,
+                             const uint32_t *__restrict__ off, const uint32_t *__restrict__ odeg,
+                             const uint32_t *__restrict__ poff,
+                             const uint32_t *__restrict__ rowwork, uint4 *__restrict__ out) {
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = rows[q];
+    out[q] = make_uint4(odeg[r], rowwork[r], poff[r + 1] - poff[r], off[r + 1] - off[r]);
+  }
+}
+
+// out[k] = ooff[min(k * width, V)], k = 0..P: the oriented-edge bounds of contiguous
+// partitions.
+__global__ void k_part_bounds(const uint32_t *__restrict__ ooff, uint32_t V, uint64_t width,
+                              uint32_t P, uint32_t *__restrict__ out) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k <= P;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    out[k] = ooff[std::min<uint64_t>(k * width, V)];
+}
+
+}  // namespace
+
+int row_summary(Graph *g, uint64_t part_width, uint32_t P, RowSummary &out) {
+  Ctx *ctx = g->ctx;
+  const uint32_t V = g->V;
+  PAC_TRY(build_pairs(g));
+  DevBuf flags, acc, nsel;
+  PAC_TRY(flags.reserve(V ? V : 1));
+  PAC_TRY(acc.reserve(16));
+  PAC_TRY(nsel.reserve(8));
+  PAC_TRY(fill_u32(ctx->stream, acc.as<uint32_t>(), 4, 0u));
+  if (V) {
+    hipLaunchKernelGGL(k_vertex_stats, dim3(grid_for_n(V, 256 * 8)), dim3(256), 0, ctx->stream,
+                       g->off.as<const uint32_t>(), g->odeg.as<const uint32_t>(),
+                       g->poff.as<const uint32_t>(), V, flags.as<uint8_t>(),
+                       acc.as<unsigned long long>(), acc.as<uint32_t>() + 2);
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  // the rows, in ID order
+  PAC_TRY(g->rows.reserve(sizeof(uint32_t) * (V ? V : 1)));
+  PAC_TRY(fill_u32(ctx->stream, nsel.as<uint32_t>(), 1, 0u));
+  if (V) {
+    rocprim::counting_iterator<uint32_t> ids(0u);
+    size_t tmp = 0;
+    PAC_CHECK_HIP(rocprim::select(nullptr, tmp, ids, flags.as<const uint8_t>(),
+                                  g->rows.as<uint32_t>(), nsel.as<uint32_t>(), (size_t)V,
+                                  ctx->stream));
+    PAC_TRY(ctx->scratch.reserve(tmp));
+    PAC_CHECK_HIP(rocprim::select(ctx->scratch.p, tmp, ids, flags.as<const uint8_t>(),
+                                  g->rows.as<uint32_t>(), nsel.as<uint32_t>(), (size_t)V,
+                                  ctx->stream));
+  }
+  uint32_t h_acc[4] = {0, 0, 0, 0}, n = 0;
+  PAC_CHECK_HIP(hipMemcpyAsync(h_acc, acc.p, 16, hipMemcpyDeviceToHost, ctx->stream));
+  PAC_CHECK_HIP(hipMemcpyAsync(&n, nsel.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  out.W = (uint64_t)h_acc[0] | ((uint64_t)h_acc[1] << 32);
+  out.max_outdeg = h_acc[2];
+  out.rows.resize(n);
+  out.stats.resize(n);
+  if (n) {
+    DevBuf st;
+    PAC_TRY(st.reserve(sizeof(uint4) * n));
+    hipLaunchKernelGGL(k_row_gather, dim3(grid_for_n(n, 256)), dim3(256), 0, ctx->stream,
+                       g->rows.as<const uint32_t>(), n, g->off.as<const uint32_t>(),
+                       g->odeg.as<const uint32_t>(), g->poff.as<const uint32_t>(),
+                       g->rowwork.as<const uint32_t>(), st.as<uint4>());
+    PAC_CHECK_HIP(hipGetLastError());
+    PAC_CHECK_HIP(hipMemcpyAsync(out.rows.data(), g->rows.p, 4ull * n, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    PAC_CHECK_HIP(hipMemcpyAsync(out.stats.data(), st.p, sizeof(uint4) * n,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // st is freed on return
+  }
+  if (part_width) {  // contiguous partitions: their oriented-edge counts from ooff
+    DevBuf b;
+    PAC_TRY(b.reserve(4ull * (P + 1)));
+    hipLaunchKernelGGL(k_part_bounds, dim3(grid_for_n((uint64_t)P + 1, 256)), dim3(256), 0,
+                       ctx->stream, g->ooff.as<const uint32_t>(), V, part_width, P,
+                       b.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+    std::vector<uint32_t> h(P + 1);
+    PAC_CHECK_HIP(hipMemcpyAsync(h.data(), b.p, 4ull * (P + 1), hipMemcpyDeviceToHost, ctx->stream));
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    out.part_edges.resize(P);
+    for (uint32_t k = 0; k < P; ++k) out.part_edges[k] = h[k + 1] - h[k];
+  }
   return PAC_OK;
 }
 
@@ -458,7 +717,7 @@ uint32_t grid_elems(uint64_t n) {
 
 int build_hit_words(Graph *g) {
   if (g->hitw_built) return PAC_OK;
-  PAC_TRY(build_pairs(g));
+  PAC_TRY(build_back_edges(g));
   Ctx *ctx = g->ctx;
   const uint32_t V = g->V;
   uint32_t np = 0;
