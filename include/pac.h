@@ -319,8 +319,11 @@ int pac_get_tile(const pac_ctx *ctx, int *tile);
                          PAC_MLP_OUT * PAC_MLP_HIDDEN + PAC_MLP_OUT)
 /* Per-partition features (P x 16 floats) of a prepared graph. */
 int pac_partition_features(pac_graph *g, float *features);
-/* Times each strategy on each partition in isolation (one launch per partition and
- * strategy, median of `reps`); ms has P x 3 entries. */
+/* Times each strategy on each partition inside the persistent launch (LCC mode): in
+ * launch r of a round partition k runs strategy (k + r) mod 3, and the kernel's per-item
+ * clock stamps give each partition's time under each strategy; ms (P x 3) holds the
+ * median over `reps` rounds, 0 for partitions owning no pairs.  These are the learned
+ * dispatch's training labels. */
 int pac_profile_partitions(pac_graph *g, int reps, double *ms);
 /* The strategy the fixed heuristic picks per partition (P bytes). */
 int pac_fixed_choice(pac_graph *g, uint8_t *choice);

@@ -179,6 +179,9 @@ struct Ctx {
   hipEvent_t ev[6] = {};
   hipEvent_t kev[2] = {};     // around the dominant kernel when time_kernel is set
   bool time_kernel = false;
+  // when set (pac_profile_partitions, never inside a captured graph), the persistent
+  // kernel writes each item's start and end clock here: 2 x items entries
+  unsigned long long *item_stamps = nullptr;
   pac_timing last{};
   int tile = 1;
   void *noalloc_scratch = nullptr;  // this device's static scratch of pac_tc_noalloc
@@ -265,8 +268,6 @@ int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
 int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
                       const uint8_t *strategy_per_part_concat, bool use_graph,
                       uint64_t *launches);
-// One partition alone with one strategy (profiling).
-int launch_partition(Ctx *ctx, Graph *g, int mode, uint32_t part, int strat);
 // The no-allocation scratch: its device address for ctx->device (hipGetSymbolAddress once
 // per device; call with the device current).
 int noalloc_scratch_init(Ctx *ctx);

@@ -773,24 +773,36 @@ constexpr int lds_waves() {
   return std::max(1, std::min(6, (int)(163840 / sizeof(Smem<CAP, MODE, WAVES>)) * WAVES / 4));
 }
 
+// With stamps (profiling only; null otherwise) the workgroup records, per item it, the
+// real-time clock when it took the item and when it asked for the next one:
+// stamps[2 it], stamps[2 it + 1] (rows it deferred are not included).
 template <int MODE, int WAVES, int CAP, bool DEFER>
 __global__ __launch_bounds__(64 * WAVES, (lds_waves<CAP, MODE, WAVES>()))
-void k_persistent(const GraphDev *__restrict__ Gs,
-                                                           const WorkItem *__restrict__ items,
-                                                           uint32_t nitems,
-                                                           const uint8_t *__restrict__ strat,
-                                                           uint32_t *queue,
-                                                           DeferItem *__restrict__ defer,
-                                                           uint32_t defer_cap) {
+void k_persistent(const GraphDev *__restrict__ Gs, const WorkItem *__restrict__ items,
+                  uint32_t nitems, const uint8_t *__restrict__ strat, uint32_t *queue,
+                  DeferItem *__restrict__ defer, uint32_t defer_cap,
+                  unsigned long long *__restrict__ stamps) {
   __shared__ Smem<CAP, MODE, WAVES> s;
   unsigned long long acc = 0;
   PAC_GLOBAL unsigned long long *cur_total = nullptr;
+  uint32_t prev = 0xFFFFFFFFu;       // thread 0, with stamps: the item being timed
+  unsigned long long t_start = 0;
   while (true) {
     __syncthreads();
-    if (threadIdx.x == 0) s.item = atomicAdd(queue, 1u);
+    if (threadIdx.x == 0) {
+      if (stamps && prev != 0xFFFFFFFFu) {
+        stamps[2ull * prev] = t_start;
+        stamps[2ull * prev + 1] = wall_clock64();
+      }
+      s.item = atomicAdd(queue, 1u);
+    }
     __syncthreads();
     const uint32_t it = s.item;
     if (it >= nitems) break;
+    if (stamps && threadIdx.x == 0) {
+      prev = it;
+      t_start = wall_clock64();
+    }
     const WorkItem w = items[it];
     const GDev G = to_dev(Gs[w.g]);
     if (G.total != cur_total) {
@@ -1133,6 +1145,7 @@ struct PersistBufs {
   uint32_t *queue;
   DeferItem *defer;
   uint32_t defer_cap;
+  unsigned long long *stamps = nullptr;  // per-item clock stamps (profiling)
 };
 
 // The persistent launch (and the wide second pass when rows were deferred).
@@ -1148,12 +1161,13 @@ int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, const P
         persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, kCanDefer>, 64 * WAVES, nitems);
     hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, kCanDefer>), dim3(grid), dim3(64 * WAVES),
                        0, ctx->stream, b.gdesc, items, nitems, b.strat, b.queue, b.defer,
-                       defer_cap);
+                       defer_cap, b.stamps);
   } else {
     const uint32_t grid =
         persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, false>, 64 * WAVES, nitems);
     hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, false>), dim3(grid), dim3(64 * WAVES), 0,
-                       ctx->stream, b.gdesc, items, nitems, b.strat, b.queue, nullptr, 0u);
+                       ctx->stream, b.gdesc, items, nitems, b.strat, b.queue, nullptr, 0u,
+                       b.stamps);
   }
   PAC_CHECK_HIP(hipGetLastError());
   if (defer_cap) {
@@ -1198,6 +1212,7 @@ int ctx_bufs(Ctx *ctx, uint32_t long_rows, PersistBufs &b) {
   b.queue = ctx->queue.as<uint32_t>();
   b.defer = ctx->defer.as<DeferItem>();
   b.defer_cap = defer_cap;
+  b.stamps = ctx->item_stamps;
   return PAC_OK;
 }
 
@@ -1507,14 +1522,6 @@ int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
     k = e;
   }
   return PAC_OK;
-}
-
-int launch_partition(Ctx *ctx, Graph *g, int mode, uint32_t part, int strat) {
-  Graph *gs[1] = {g};
-  PAC_TRY(alloc_outputs(gs, 1, mode));
-  const GraphDev d = make_desc(g, 0);
-  const uint32_t b = g->h_pstart[part], e = g->h_pstart[part + 1];
-  return run_plain_any(ctx, d, mode, strat, b, e - b);
 }
 
 int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,

@@ -74,8 +74,12 @@ int pac_fixed_choice(pac_graph *h, uint8_t *choice) {
   return PAC_OK;
 }
 
-// Every strategy on every partition, one plain launch each (LCC mode, the mode of the
-// headline benchmark), timed with events on the context stream; median of `reps`.
+// Every strategy on every partition, timed inside the persistent launch the solves use
+// (LCC mode, the headline's).  In launch r of a round, partition k runs strategy
+// (k + r) mod 3, so every launch mixes the three strategies and each partition meets each
+// of them once per round; a partition's time under a strategy is the sum of its items'
+// durations from the kernel's per-item clock stamps.  A warm-up launch precedes the
+// rounds; ms[k][s] is the median over `reps` rounds (0 for a partition owning no pairs).
 int pac_profile_partitions(pac_graph *h, int reps, double *ms) {
   if (!h || !ms || reps < 1) {
     set_error("need a graph, reps >= 1 and an output buffer");
@@ -88,43 +92,54 @@ int pac_profile_partitions(pac_graph *h, int reps, double *ms) {
   }
   Ctx *ctx = g->ctx;
   PAC_CHECK_HIP(hipSetDevice(ctx->device));
-  Graph *gs[1] = {g};
-  PAC_TRY(zero_outputs(ctx, gs, 1, kModeLCC));
-  const int nev = 2 * reps;
-  std::vector<hipEvent_t> ev(nev);
-  for (auto &e : ev) PAC_CHECK_HIP(hipEventCreate(&e));
-  int rc = PAC_OK;
-  for (uint32_t k = 0; k < g->P && rc == PAC_OK; ++k) {
-    for (int s = 0; s < PAC_NUM_STRATEGIES && rc == PAC_OK; ++s) {
-      if (g->h_pstart[k + 1] == g->h_pstart[k]) {
-        ms[(size_t)k * PAC_NUM_STRATEGIES + s] = 0.0;
-        continue;
-      }
-      // warm-up launch, then reps timed launches
-      rc = launch_partition(ctx, g, kModeLCC, k, s);
-      for (int r = 0; r < reps && rc == PAC_OK; ++r) {
-        if (hipEventRecord(ev[2 * r], ctx->stream) != hipSuccess) rc = PAC_ERR_HIP;
-        if (rc == PAC_OK) rc = launch_partition(ctx, g, kModeLCC, k, s);
-        if (rc == PAC_OK && hipEventRecord(ev[2 * r + 1], ctx->stream) != hipSuccess)
-          rc = PAC_ERR_HIP;
-      }
-      if (rc != PAC_OK) break;
-      if (hipEventSynchronize(ev[nev - 1]) != hipSuccess) {
-        set_error("event synchronize failed");
-        rc = PAC_ERR_HIP;
-        break;
-      }
-      std::vector<double> t(reps);
-      for (int r = 0; r < reps; ++r) {
-        float x = 0.f;
-        (void)hipEventElapsedTime(&x, ev[2 * r], ev[2 * r + 1]);
-        t[r] = x;
-      }
-      ms[(size_t)k * PAC_NUM_STRATEGIES + s] = median(t);
-    }
+  const uint32_t P = g->P;
+  const size_t nit = g->h_items.size();
+  std::fill(ms, ms + (size_t)P * PAC_NUM_STRATEGIES, 0.0);
+  if (nit == 0) return PAC_OK;
+  int khz = 0;
+  PAC_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+  if (khz <= 0) {
+    set_error("device reports no wall-clock rate");
+    return PAC_ERR_HIP;
   }
-  for (auto &e : ev) (void)hipEventDestroy(e);
-  return rc;
+  DevBuf st;
+  PAC_TRY(st.reserve(2 * sizeof(unsigned long long) * nit));
+  std::vector<unsigned long long> hs(2 * nit);
+  std::vector<uint8_t> strat(P);
+  std::vector<double> acc((size_t)P * PAC_NUM_STRATEGIES);
+  std::vector<std::vector<double>> t((size_t)P * PAC_NUM_STRATEGIES);
+  Graph *gs[1] = {g};
+  struct StampsOff {  // the context never keeps the stamps pointer past this call
+    Ctx *c;
+    ~StampsOff() { c->item_stamps = nullptr; }
+  } off{ctx};
+  for (int round = -1; round < reps; ++round) {  // round -1: warm-up, not recorded
+    std::fill(acc.begin(), acc.end(), 0.0);
+    for (uint32_t r = 0; r < PAC_NUM_STRATEGIES; ++r) {
+      for (uint32_t k = 0; k < P; ++k) strat[k] = (uint8_t)((k + r) % PAC_NUM_STRATEGIES);
+      PAC_TRY(zero_outputs(ctx, gs, 1, kModeLCC));
+      ctx->item_stamps = round < 0 ? nullptr : st.as<unsigned long long>();
+      PAC_TRY(launch_persistent(ctx, gs, 1, kModeLCC, strat.data(), false, nullptr));
+      ctx->item_stamps = nullptr;
+      if (round < 0) break;
+      PAC_CHECK_HIP(hipMemcpyAsync(hs.data(), st.p, 2 * sizeof(unsigned long long) * nit,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+      PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+      for (size_t i = 0; i < nit; ++i) {
+        const uint32_t k = g->h_items[i].part;
+        acc[(size_t)k * PAC_NUM_STRATEGIES + strat[k]] += (double)(hs[2 * i + 1] - hs[2 * i]);
+      }
+    }
+    if (round < 0) continue;
+    for (uint32_t k = 0; k < P; ++k)
+      if (g->h_pstart[k + 1] > g->h_pstart[k])
+        for (int s = 0; s < PAC_NUM_STRATEGIES; ++s)
+          t[(size_t)k * PAC_NUM_STRATEGIES + s].push_back(
+              acc[(size_t)k * PAC_NUM_STRATEGIES + s] / khz);  // ticks / kHz = ms
+  }
+  for (size_t i = 0; i < t.size(); ++i)
+    if (!t[i].empty()) ms[i] = median(t[i]);
+  return PAC_OK;
 }
 
 // Times the intersection launch of the batched persistent kernel for every tile

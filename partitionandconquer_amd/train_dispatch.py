@@ -7,16 +7,19 @@ Training data: 64 seeded R-MAT graphs, graph i has seed 4000+i, scale 18 + (i mo
 a drawn uniformly from [0.45, 0.65] (SplitMix64 stream seeded 4), d = 0.05 and
 b = c = (1-a-d)/2; edge factor 16; contiguous partitions of 4096 vertex IDs (the
 partition width of the scale-24 / 4096-partition evaluation).  Every strategy is timed
-on every partition in isolation (pac_profile_partitions: one plain launch per
-partition and strategy, median of 3); the label is the fastest strategy and the sample
+on every partition inside the persistent launch the solves use (pac_profile_partitions:
+per-item clock stamps from the kernel, strategies rotated over the partitions so every
+launch mixes them, median of 3 rounds); the label is the fastest strategy and the sample
 weight is the time spread (max - min) normalised to mean 1, so partitions where the
-choice matters weigh more.  The MLP is trained on the GPU by the library's own
-backward and Adam kernels (pac_mlp_train).
+choice matters weigh more.  The MLP is trained on the GPU by the library's own backward
+and Adam kernels (pac_mlp_train).
 
-Evaluation (default R-MAT scale 24, 4096 partitions): misprediction rate (choice !=
-argmin of the measured times) and regret sum(t_chosen - t_best)/sum(t_best) for the
-MLP and for the fixed heuristic, plus end-to-end device times of the batched
-persistent kernel under both dispatches and of the plain per-partition launches.
+Evaluation: on held-out R-MAT graphs (scale 20, a in {0.45, 0.5, 0.6, 0.65}, seeds
+9000+, not in the training set) and on the headline graph (default R-MAT scale 24, 4096
+partitions): misprediction rate (choice != argmin of the measured times) and regret
+sum(t_chosen - t_best)/sum(t_best) for the MLP and for the fixed heuristic, plus
+end-to-end device times of the batched persistent kernel (hipGraph) under both
+dispatches and of the plain per-partition launches.
 """
 from __future__ import annotations
 
@@ -87,8 +90,12 @@ def score(choice, ms):
                 chosen_ms=float(chosen.sum()), best_ms=float(best.sum()))
 
 
-def evaluate(ctx, scale, partitions, steps=5):
-    g = pac.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed=scale)
+HELD_OUT = [dict(seed=9000 + i, scale=20, a=a, b=(0.95 - a) / 2, c=(0.95 - a) / 2)
+            for i, a in enumerate((0.45, 0.5, 0.6, 0.65))]
+
+
+def evaluate(ctx, scale, partitions, steps=5, abc=(0.57, 0.19, 0.19), seed=None):
+    g = pac.Graph.rmat(ctx, scale, 16, *abc, seed=scale if seed is None else seed)
     g.prepare(partitions)
     ms = g.profile_partitions(3)
     keep = ms.max(1) > 0
@@ -115,6 +122,7 @@ def evaluate(ctx, scale, partitions, steps=5):
     res["speedup_learned_batched_vs_plain_fixed"] = times["plain_fixed"] / times["batched_learned"]
     res["speedup_learned_vs_fixed_batched"] = times["batched_fixed"] / times["batched_learned"]
     res["triangles"] = T
+    g.free()
     return res
 
 
@@ -163,6 +171,14 @@ def main(argv=None):
     else:
         ctx.mlp_load(a.out)
     if not a.no_eval:
+        report["held_out"] = []
+        for sp in HELD_OUT:
+            r = evaluate(ctx, sp["scale"], max(16, (1 << sp["scale"]) // 4096),
+                         abc=(sp["a"], sp["b"], sp["c"]), seed=sp["seed"])
+            r["a"] = sp["a"]
+            report["held_out"].append(r)
+            log(json.dumps({k: r[k] for k in ("a", "fixed", "learned",
+                                               "speedup_learned_vs_fixed_batched")}))
         report["evaluation"] = evaluate(ctx, a.eval_scale, a.eval_partitions)
         log(json.dumps(report["evaluation"], indent=1))
     if a.report:

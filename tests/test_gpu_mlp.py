@@ -109,20 +109,40 @@ def test_profile_and_features(ctx, orc):
     assert fx.shape == (32,) and fx.max() < 3
 
 
-def test_trained_model_beats_fixed_heuristic_when_present(ctx):
-    """With the committed model: on a held-out R-MAT graph the MLP's total chosen time
-    (measured per partition) is no worse than the fixed heuristic's."""
+def test_learned_dispatch_gain_on_held_out_graph(ctx):
+    """What the trained network gains over the fixed heuristic (the method without it),
+    on a held-out R-MAT graph (scale 20, a = 0.45, seed 9000; the training graphs have
+    seeds 4000-4063).  Per-partition times of all three strategies come from the persistent
+    launch (pac_profile_partitions); the learned choice must mispredict no more partitions
+    than the fixed one and lose no more time, up to 1e-4 of the total for timing noise in
+    the labels.  Both dispatches are also timed end to end on the batched hipGraph path, and
+    the figures are printed (the strategy matters little there: the hash table is fastest
+    on nearly every partition, and the ones the heuristic gets wrong are cheap)."""
     path = os.path.join(os.path.dirname(pac.pac.__file__), "dispatch_mlp.bin")
-    if not os.path.exists(path):
-        pytest.skip("no trained model committed yet")
+    assert os.path.exists(path)
     ctx.mlp_load(path)
-    g = pac.Graph.rmat(ctx, 19, 16, 0.6, 0.175, 0.175, seed=9999)
-    g.prepare(128)
+    g = pac.Graph.rmat(ctx, 20, 16, 0.45, 0.25, 0.25, seed=9000)
+    g.prepare(256)
     ms = g.profile_partitions(3)
     keep = ms.max(1) > 0
     ms = ms[keep]
-    learned = g.mlp_choice()[keep]
-    fixed = g.fixed_choice()[keep]
-    t_l = ms[np.arange(len(ms)), learned].sum()
-    t_f = ms[np.arange(len(ms)), fixed].sum()
-    assert t_l <= t_f * 1.02
+    best = ms.min(1)
+    rows = np.arange(len(ms))
+    score = {}
+    for name, ch in (("learned", g.mlp_choice()[keep]), ("fixed", g.fixed_choice()[keep])):
+        score[name] = (float(np.mean(ch != ms.argmin(1))),
+                       float((ms[rows, ch] - best).sum() / best.sum()))
+    T = g.tc(pac.FORCE_BITMAP)
+    dev = {}
+    for name, d in (("learned", pac.DISPATCH_LEARNED), ("fixed", pac.DISPATCH_FIXED)):
+        assert g.lcc_device(d, pac.LAUNCH_GRAPH) == T
+        v = []
+        for _ in range(7):
+            g.lcc_device(d, pac.LAUNCH_GRAPH)
+            v.append(ctx.last_timing().total_ms)
+        dev[name] = float(np.median(v))
+    print(f"held-out scale 20 a=0.45: mispredict/regret learned {score['learned']}, fixed "
+          f"{score['fixed']}; batched device ms learned {dev['learned']:.3f}, fixed "
+          f"{dev['fixed']:.3f} (fixed/learned {dev['fixed'] / dev['learned']:.4f})")
+    assert score["learned"][0] <= score["fixed"][0]
+    assert score["learned"][1] <= score["fixed"][1] + 1e-4
