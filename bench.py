@@ -259,6 +259,22 @@ def main():
         out["batched_fixed_device_ms"] = round(dev_bf, 3)
         out["speedup_vs_plain_fixed"] = round(wall_pf / ms_step, 3)
         out["learned_vs_fixed_batched"] = round(dev_bf / dev, 4)
+    if learned:
+        # the dispatch decision itself, per partition: each strategy timed inside the
+        # persistent launch (clock stamps per work item), against each dispatch's choice
+        log("phase: per-partition strategy times")
+        pms = g.profile_partitions(2)
+        keep = pms.max(1) > 0
+        pms = pms[keep]
+        best = pms.min(1)
+        idx = np.arange(len(pms))
+        ev = {"partitions_timed": int(keep.sum()),
+              "best_counts": np.bincount(pms.argmin(1), minlength=3).tolist()}
+        for name, ch in (("learned", g.mlp_choice()[keep]), ("fixed", g.fixed_choice()[keep])):
+            ev[f"{name}_mispredict"] = round(float(np.mean(ch != pms.argmin(1))), 5)
+            ev[f"{name}_regret"] = float(f"{(pms[idx, ch] - best).sum() / best.sum():.3e}")
+            ev[f"{name}_counts"] = np.bincount(ch, minlength=3).tolist()
+        out["dispatch_eval"] = ev
     log("phase: stream copy")
     copy_bytes = 1 << 30
     copy_ms = ctx.stream_copy(copy_bytes, 5)

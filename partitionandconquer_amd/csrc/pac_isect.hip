@@ -884,11 +884,31 @@ __global__ void k_put_desc(GraphDev d, GraphDev *out) {
   if (threadIdx.x == 0) *out = d;
 }
 
-// Plain streaming copy (16-byte vectors, grid-stride): the bandwidth reference.
-__global__ void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+// Streaming copy, the bandwidth reference: each workgroup copies one contiguous chunk of
+// kCopyChunk 16-byte vectors, four vectors per thread in flight, with non-temporal loads
+// and stores (the data is touched once).  Of the variants measured on MI355X over 1 GiB
+// (grid-stride with 1/4/8 vectors per thread, chunked with 4096/16384/65536 vectors per
+// workgroup, with and without the non-temporal hint, and hipMemcpy device to device:
+// 4.2-5.7 TB/s), this one was the fastest.
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
+constexpr uint64_t kCopyChunk = 4096;
+__global__ void __launch_bounds__(256)
+k_stream_copy(const v4u32 *__restrict__ src, v4u32 *__restrict__ dst, uint64_t n) {
+  const uint64_t b0 = blockIdx.x * kCopyChunk;
+  const uint64_t b1 = b0 + kCopyChunk < n ? b0 + kCopyChunk : n;
+  for (uint64_t i = b0 + threadIdx.x; i < b1; i += 4u * blockDim.x) {
+    v4u32 v[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint64_t j = i + u * blockDim.x;
+      if (j < b1) v[u] = __builtin_nontemporal_load(&src[j]);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint64_t j = i + u * blockDim.x;
+      if (j < b1) __builtin_nontemporal_store(v[u], &dst[j]);
+    }
+  }
 }
 
 // out[i] = table[idx[i]]
@@ -1451,12 +1471,17 @@ int stream_copy(Ctx *ctx, uint64_t bytes, int reps, double *ms) {
   PAC_TRY(b.reserve(bytes));
   PAC_TRY(fill_u32(ctx->stream, a.as<uint32_t>(), bytes / 4, 1u));
   const uint64_t n = bytes / 16;
-  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 64);
+  const uint64_t nblk = (n + kCopyChunk - 1) / kCopyChunk;
+  if (nblk > 0x7FFFFFFFull) {
+    set_error("stream copy of %llu bytes exceeds one launch", (unsigned long long)bytes);
+    return PAC_ERR_ARG;
+  }
+  const uint32_t grid = (uint32_t)nblk;
   std::vector<double> t;
   for (int r = 0; r <= reps; ++r) {
     PAC_CHECK_HIP(hipEventRecord(ctx->kev[0], ctx->stream));
-    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, a.as<const uint4>(),
-                       b.as<uint4>(), n);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream,
+                       a.as<const v4u32>(), b.as<v4u32>(), n);
     PAC_CHECK_HIP(hipGetLastError());
     PAC_CHECK_HIP(hipEventRecord(ctx->kev[1], ctx->stream));
     PAC_CHECK_HIP(hipEventSynchronize(ctx->kev[1]));
