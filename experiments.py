@@ -4,6 +4,7 @@
     python experiments.py batch     # 256 uneven graphs, autotuned tile
     python experiments.py jaccard   # per-edge Jaccard at R-MAT scale 25
     python experiments.py next      # ingestion, queries, k-way, squares
+    python experiments.py cpu       # the oracle's whole solves timed on the host
 
 scale22: R-MAT scale 22, ef 16, (0.57,0.19,0.19,0.05), seed 22, 1024 contiguous
   partitions; TC + LCC device time of the plain launch (one launch per partition
@@ -17,6 +18,11 @@ batch: 256 graphs, graph g seeded 3000+g; even g: Erdos-Renyi G(n,p) with n = 2^
 jaccard: R-MAT scale 25, ef 16, seed 25, 4096 partitions; per-edge |N(u)&N(v)|,
   union and FP32 ratio for every edge with all results left in HBM; sampled edges are
   checked against the oracle; the oracle's per-edge merge is timed on a sample of edges.
+cpu: the oracle's restatement of the reference's timed solves, run whole on the host
+  (OMP_NUM_THREADS threads, each leg in a child process under a time limit, on the CSR the
+  GPU generated): R-MAT scale 16 TC-merge + LCC-merge (Graph.cpp:355-368, 828-840) and the
+  64-partition Partitioned_Graph TC (Graph.cpp:1346-1362), median of 3 after a warm-up;
+  R-MAT scale 22 TC-merge, median of 3 after a warm-up; R-MAT scale 24 TC-merge, one run.
 next: the components on either side of the path, on R-MAT scale 22 (seed 22) unless noted:
   edge-list ingestion (the graph's undirected edge list, shuffled, to the symmetric CSR on
   the device; an R-MAT scale 18 edge list as text through the device parser), pairwise
@@ -190,6 +196,65 @@ def exp_jaccard(ctx, scale=25, partitions=4096, sample=200000, cpu_budget=15.0,
     return res
 
 
+CPU_WORKER = r"""
+import json, sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle import oracle as O
+vl, el = np.load(sys.argv[2]), np.load(sys.argv[3])
+what, warm, reps, nt = sys.argv[4], int(sys.argv[5]), int(sys.argv[6]), int(sys.argv[7])
+g = O.CSR(vl, el)
+def run():
+    if what == "tc":
+        return O.tc_merge(g, nt)
+    if what == "tc+lcc":
+        return (O.tc_merge(g, nt), float(np.nansum(O.lcc_merge(g, nt))))
+    if what == "ptc":
+        return O.PGraph(g, O.contiguous_partition(g.V, 64), nt).tc_merge()
+for _ in range(warm):
+    run()
+t = []
+for _ in range(reps):
+    t0 = time.perf_counter()
+    out = run()
+    t.append(time.perf_counter() - t0)
+    print(json.dumps({"s": t[-1]}), flush=True)
+print(json.dumps({"times_s": t, "median_s": sorted(t)[len(t) // 2], "result": str(out)}), flush=True)
+"""
+
+
+def exp_cpu(ctx, limit_s=900.0):
+    import subprocess
+    import tempfile
+    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res = {"threads": nt, "legs": []}
+    tmp = tempfile.mkdtemp(prefix="pac_cpu_")
+    for scale, what, warm, reps in ((16, "tc+lcc", 1, 3), (16, "ptc", 1, 3), (22, "tc", 1, 3),
+                                    (24, "tc", 0, 1)):
+        g = pac.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed=scale)
+        vl, el = g.csr()
+        m = g.info().num_edges
+        g.free()
+        fv, fe = os.path.join(tmp, "vl.npy"), os.path.join(tmp, "el.npy")
+        np.save(fv, vl)
+        np.save(fe, el)
+        del vl, el
+        log(f"cpu leg: scale {scale} {what}, warm-up {warm}, runs {reps}, limit {limit_s:.0f} s")
+        leg = {"scale": scale, "solve": what, "m": m, "warmup": warm, "runs": reps}
+        try:
+            r = subprocess.run([sys.executable, "-c", CPU_WORKER, ROOT, fv, fe, what, str(warm),
+                                str(reps), str(nt)], capture_output=True, text=True,
+                               timeout=limit_s)
+            last = json.loads(r.stdout.strip().splitlines()[-1])
+            leg.update(last)
+            leg["gteps"] = round(m / last["median_s"] / 1e9, 6)
+        except subprocess.TimeoutExpired:
+            leg["timed_out_after_s"] = limit_s
+        log(json.dumps(leg))
+        res["legs"].append(leg)
+    return res
+
+
 def exp_next(ctx, scale=22, seed=22, npairs=4_000_000, nkway=1_000_000, sample=200_000,
              profile_only=False):
     rng = np.random.default_rng(seed)
@@ -352,7 +417,7 @@ def exp_next(ctx, scale=22, seed=22, npairs=4_000_000, nkway=1_000_000, sample=2
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["scale22", "batch", "jaccard", "next"])
+    ap.add_argument("which", choices=["scale22", "batch", "jaccard", "next", "cpu"])
     ap.add_argument("--graphs", type=int, default=256)
     ap.add_argument("--scale", type=int, default=None)
     ap.add_argument("--no-check", action="store_true")
@@ -366,6 +431,8 @@ def main(argv=None):
         r = exp_batch(ctx, a.graphs, not a.no_check)
     elif a.which == "next":
         r = exp_next(ctx, a.scale or 22, profile_only=a.profile_only)
+    elif a.which == "cpu":
+        r = exp_cpu(ctx)
     else:
         r = exp_jaccard(ctx, a.scale or 25, profile_only=a.profile_only)
     print(json.dumps(r))

@@ -39,7 +39,8 @@ extern "C" {
 #define PAC_ERR_IO (-5)          /* file missing, unreadable or truncated            */
 #define PAC_ERR_STATE (-6)       /* call order (e.g. solve before prepare)           */
 #define PAC_ERR_UNSUPPORTED (-7) /* graph exceeds a documented limit                 */
-#define PAC_ERR_TIMEOUT (-8)     /* persistent queue made no progress in time        */
+#define PAC_ERR_TIMEOUT (-8)     /* a solve did not complete within the context's    *
+                                  * timeout (pac_ctx_set_timeout)                     */
 
 typedef struct pac_ctx pac_ctx;
 typedef struct pac_graph pac_graph;
@@ -113,6 +114,12 @@ const char *pac_last_error(void);
 int pac_ctx_last_timing(const pac_ctx *ctx, pac_timing *out);
 /* Number of compute units of the context's device. */
 int pac_ctx_num_cus(const pac_ctx *ctx);
+/* Failure detection (no reference counterpart: the reference waits without bound).  With
+ * ms > 0 every solve that waits for its results (TC, LCC, Jaccard, batches, the queries)
+ * polls its completion for at most ms of wall time and then returns PAC_ERR_TIMEOUT; the
+ * kernels keep running on the context's stream (a launch cannot be withdrawn), so later
+ * calls on the context wait behind them.  ms = 0 (the default) waits without bound. */
+int pac_ctx_set_timeout(pac_ctx *ctx, double ms);
 /* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming kernel,
  * `reps` times; *ms receives the median kernel time.  Used to measure the achievable
  * HBM bandwidth and to calibrate hardware byte counters.  No reference counterpart. */
@@ -177,6 +184,10 @@ int pac_random_partition(uint64_t V, uint32_t P, uint64_t seed, int32_t *v2p);
 /* Graph::greedy_partition, Graph.cpp:898-967 (degree-descending,
  * deficit-proportional; seeded instead of random_device) (host). */
 int pac_greedy_partition(const pac_graph *g, uint32_t P, uint64_t seed, int32_t *v2p);
+/* Graph::greedy_partition_2, Graph.cpp:969-1015 (minimax of the per-partition neighbour
+ * counts, vertices by degree descending, ties by ID; isolated vertices take the seeded
+ * draws of pac_random_partition instead of rand()) (host, P x V counters). */
+int pac_greedy_partition_2(const pac_graph *g, uint32_t P, uint64_t seed, int32_t *v2p);
 /* Partitioned_Graph::evaluate_partition, Graph.cpp:1460-1473: returns the balance
  * ratio P*max_part_deg/max_deg for a vertex2partition. */
 int pac_evaluate_partition(const pac_graph *g, const int32_t *v2p, uint32_t P,

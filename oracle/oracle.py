@@ -81,6 +81,7 @@ def lib():
         L.orc_pgraph_evaluate_partition.restype = C.c_double
         L.orc_random_partition.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, i32p]
         L.orc_contiguous_partition.argtypes = [C.c_uint64, C.c_uint64, i32p]
+        L.orc_greedy_partition_2.argtypes = [u64p, i32p, C.c_uint64, C.c_uint64, C.c_uint64, i32p]
         L.orc_gen_rmat.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                    C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
@@ -391,6 +392,13 @@ def friend_triangle_merge(g: CSR, root, nodes, nt=0) -> int:
 def random_partition(V, P, seed=1) -> np.ndarray:
     out = np.zeros(V, np.int32)
     lib().orc_random_partition(V, P, seed, out)
+    return out
+
+
+def greedy_partition_2(g: CSR, P, seed=1) -> np.ndarray:
+    """Graph::greedy_partition_2 (Graph.cpp:969-1015), seeded."""
+    out = np.zeros(g.V, np.int32)
+    lib().orc_greedy_partition_2(g.vlist, g.elist, g.V, P, seed, out)
     return out
 
 

@@ -596,6 +596,44 @@ void orc_random_partition(uint64_t V, uint64_t P, uint64_t seed, int32_t *v2p) {
   for (uint64_t u = 0; u < V; ++u) v2p[u] = (int32_t)(mix64(key + u * kGamma) % P);
 }
 
+// Graph::greedy_partition_2, Graph.cpp:969-1015, following its loops: the degree list
+// sorted descending (stable, so equal degrees keep ID order -- the reference's std::sort
+// leaves that order open), per vertex with neighbours the first k minimising the largest
+// partition_neis_counts[k][v] over v in N(u) (starting from vertex_num), then the counts of
+// its neighbours in that partition raised; an isolated vertex u takes
+// mix64(key + u * gamma) % P (the random_partition stream) in place of rand() % P.
+void orc_greedy_partition_2(const uint64_t *vl, const int32_t *el, uint64_t V, uint64_t P,
+                            uint64_t seed, int32_t *v2p) {
+  std::vector<std::vector<uint64_t>> counts(P, std::vector<uint64_t>(V, 0));
+  std::vector<std::pair<int64_t, uint64_t>> degree_list(V);
+  for (uint64_t i = 0; i < V; ++i) {
+    v2p[i] = -1;
+    degree_list[i] = {(int64_t)i, deg(vl, (int64_t)i)};
+  }
+  std::stable_sort(degree_list.begin(), degree_list.end(),
+                   [](const auto &a, const auto &b) { return a.second > b.second; });
+  const uint64_t key = mix64(seed + kGamma);
+  for (const auto &item : degree_list) {
+    const int64_t u = item.first;
+    int64_t u_part = -1;
+    if (deg(vl, u) > 0) {
+      uint64_t minimax = V;
+      for (uint64_t k = 0; k < P; ++k) {
+        uint64_t max_curr = 0;
+        for (uint64_t j = vl[u]; j < vl[u + 1]; ++j)
+          if (counts[k][el[j]] > max_curr) max_curr = counts[k][el[j]];
+        if (max_curr < minimax) {
+          minimax = max_curr;
+          u_part = (int64_t)k;
+        }
+      }
+    }
+    if (u_part == -1) u_part = (int64_t)(mix64(key + (uint64_t)u * kGamma) % P);
+    v2p[u] = (int32_t)u_part;
+    for (uint64_t j = vl[u]; j < vl[u + 1]; ++j) counts[u_part][el[j]]++;
+  }
+}
+
 void orc_contiguous_partition(uint64_t V, uint64_t P, int32_t *v2p) {
   const uint64_t w = (V + P - 1) / P;
   for (uint64_t u = 0; u < V; ++u) v2p[u] = (int32_t)(u / (w ? w : 1));

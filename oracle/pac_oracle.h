@@ -27,6 +27,8 @@
  *   orc_pgraph_square_*  brute-force enumeration of 4-cycles (u,v,x,w) with
  *                        u > v > w > x on tiny graphs; K_n -> C(n,4); K_{a,b}
  *                        labelled side by side -> 0
+ *   orc_greedy_part*_2   hand-derived placements (path, star), the minimax property of
+ *                        every placement replayed in Python, seeded isolated vertices
  *   orc_pair_stats       closed forms on K_n and stars; a per-vertex brute-force walk
  *                        of the ownership rule on R-MAT and Erdos-Renyi graphs
  *   orc_tc_oriented      golden fixtures; equality with orc_tc_merge on R-MAT / ER
@@ -139,6 +141,10 @@ uint64_t orc_pgraph_kway(const orc_pgraph *g, const int32_t *q, uint64_t n, uint
 void orc_pgraph_free(orc_pgraph *g);
 
 /* ---- random_partition (Graph.cpp:564-570) with a seeded stream instead of rand() */
+/* Graph::greedy_partition_2, Graph.cpp:969-1015 (ties in degree by ID, seeded draws for
+ * isolated vertices: see the .cpp). */
+void orc_greedy_partition_2(const uint64_t *vl, const int32_t *el, uint64_t V, uint64_t P,
+                            uint64_t seed, int32_t *v2p);
 void orc_random_partition(uint64_t V, uint64_t P, uint64_t seed, int32_t *v2p);
 /* Contiguous-ID partition: part(v) = v / ceil(V/P). */
 void orc_contiguous_partition(uint64_t V, uint64_t P, int32_t *v2p);

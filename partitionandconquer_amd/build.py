@@ -28,7 +28,7 @@ ARCH = os.environ.get("PAC_OFFLOAD_ARCH", "gfx950")
 LIB_SOURCES = ["pac_api.cpp", "pac_tune.cpp", "pac_prepare.hip", "pac_gen.hip",
                "pac_isect.hip", "pac_mlp.hip", "pac_multi.hip"]
 CLI_SOURCES = ["pac_graph.cpp", "pac_main.cpp"]
-HEADERS = ["pac_internal.hpp", "pac_graph.hpp"]
+HEADERS = ["pac_internal.hpp", "pac_graph.hpp", "pac_wait.hpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           # packed-FP32 instructions are disabled for all device code

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -13,9 +14,11 @@
 #include <fstream>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pac_internal.hpp"
+#include "pac_wait.hpp"
 
 namespace pac {
 
@@ -301,6 +304,23 @@ struct Timed {
     return PAC_OK;
   }
   int finish(bool kernel_timed = false) {
+    if (ctx->timeout_ms > 0) {
+      // bounded wait: the solve's last event, polled until the context's deadline
+      const int rc = wait_bounded(
+          [&]() -> int {
+            const hipError_t e = hipEventQuery(ctx->ev[3]);
+            if (e == hipSuccess) return 1;
+            if (e == hipErrorNotReady) return 0;
+            set_error("solve failed: %s", hipGetErrorString(e));
+            return PAC_ERR_HIP;
+          },
+          ctx->timeout_ms, [](unsigned us) { std::this_thread::sleep_for(std::chrono::microseconds(us)); });
+      if (rc == PAC_ERR_TIMEOUT)
+        set_error("the solve did not complete within %.1f ms; its kernels keep running on the "
+                  "context's stream, which later calls wait behind",
+                  ctx->timeout_ms);
+      if (rc != PAC_OK) return rc;
+    }
     PAC_CHECK_HIP(hipEventSynchronize(ctx->ev[3]));
     ctx->last.total_ms = elapsed(ctx->ev[0], ctx->ev[3]);
     ctx->last.isect_ms = elapsed(ctx->ev[1], ctx->ev[2]);
@@ -386,6 +406,51 @@ void greedy_partition_host(const std::vector<uint64_t> &vl, const std::vector<in
   }
   for (uint64_t u = 0; u < V; ++u)
     if (v2p[u] == -1) v2p[u] = (int32_t)(next() % P);
+}
+
+// Graph::greedy_partition_2, Graph.cpp:969-1015: vertices by degree, descending (ties by
+// ID: the reference's std::sort leaves their order to the library); a vertex with
+// neighbours joins the first partition k whose largest count[k][v] over its neighbours v is
+// the smallest (count[k][v] = how many of v's neighbours k already holds), then adds 1 to
+// count[k][v] for each neighbour; an isolated vertex u takes a seeded draw
+// (mix64(key + u * gamma) mod P, the stream of pac_random_partition) instead of rand().
+// The counts are P x V, as in the reference.
+int greedy_partition_2_host(const std::vector<uint64_t> &vl, const std::vector<int32_t> &el,
+                            uint64_t V, uint32_t P, uint64_t seed, int32_t *v2p) {
+  std::vector<uint32_t> order(V);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return vl[a + 1] - vl[a] > vl[b + 1] - vl[b];
+  });
+  std::vector<uint32_t> cnt;
+  try {
+    cnt.assign((size_t)P * V, 0u);
+  } catch (const std::bad_alloc &) {
+    set_error("greedy_partition_2 needs %llu x %llu counters", (unsigned long long)P,
+              (unsigned long long)V);
+    return PAC_ERR_OOM;
+  }
+  const uint64_t key = mix64(seed + kGamma);
+  for (uint32_t u : order) {
+    int64_t part = -1;
+    if (vl[u + 1] > vl[u]) {
+      uint64_t best = V;  // the reference starts the minimum at vertex_num
+      for (uint32_t k = 0; k < P; ++k) {
+        const uint32_t *ck = &cnt[(size_t)k * V];
+        uint64_t mx = 0;
+        for (uint64_t i = vl[u]; i < vl[u + 1]; ++i) mx = std::max<uint64_t>(mx, ck[el[i]]);
+        if (mx < best) {
+          best = mx;
+          part = k;
+        }
+      }
+    }
+    if (part < 0) part = (int64_t)(mix64(key + (uint64_t)u * kGamma) % P);
+    v2p[u] = (int32_t)part;
+    uint32_t *cp = &cnt[(size_t)part * V];
+    for (uint64_t i = vl[u]; i < vl[u + 1]; ++i) cp[el[i]]++;
+  }
+  return PAC_OK;
 }
 
 }  // namespace
@@ -599,6 +664,15 @@ int pac_ctx_last_timing(const pac_ctx *ctx, pac_timing *out) {
 }
 
 int pac_ctx_num_cus(const pac_ctx *ctx) { return ctx ? ctx->c.num_cus : 0; }
+
+int pac_ctx_set_timeout(pac_ctx *ctx, double ms) {
+  if (!ctx || !(ms >= 0)) {
+    set_error("need a context and a timeout >= 0 ms");
+    return PAC_ERR_ARG;
+  }
+  ctx->c.timeout_ms = ms;
+  return PAC_OK;
+}
 
 int pac_stream_copy(pac_ctx *ctx, uint64_t bytes, int reps, double *ms) {
   PAC_TRY(ensure_ctx(ctx));
@@ -1105,6 +1179,18 @@ int pac_greedy_partition(const pac_graph *h, uint32_t P, uint64_t seed, int32_t 
   PAC_TRY(download_csr(&h->g, vl, el));
   greedy_partition_host(vl, el, h->g.V, P, seed, v2p);
   return PAC_OK;
+}
+
+int pac_greedy_partition_2(const pac_graph *h, uint32_t P, uint64_t seed, int32_t *v2p) {
+  if (!h || !v2p || P == 0) {
+    set_error("need a graph, v2p and P > 0");
+    return PAC_ERR_ARG;
+  }
+  std::vector<uint64_t> vl;
+  std::vector<int32_t> el;
+  PAC_CHECK_HIP(hipSetDevice(h->g.ctx->device));
+  PAC_TRY(download_csr(&h->g, vl, el));
+  return greedy_partition_2_host(vl, el, h->g.V, P, seed, v2p);
 }
 
 // Partitioned_Graph::evaluate_partition, Graph.cpp:1460-1473.

@@ -291,6 +291,12 @@ void Graph::greedy_partition(const size_t P, std::vector<int> &v2p) {
   check(pac_greedy_partition(g_, (uint32_t)P, 1, v2p.data()));
 }
 
+// Graph::greedy_partition_2, Graph.cpp:969-1015 (seeded; see pac_greedy_partition_2).
+void Graph::greedy_partition_2(const size_t P, std::vector<int> &v2p) {
+  v2p.assign(get_vertex_num(), -1);
+  check(pac_greedy_partition_2(g_, (uint32_t)P, 1, v2p.data()));
+}
+
 void Graph::random_partition(const size_t P, std::vector<int> &v2p) {
   v2p.assign(get_vertex_num(), 0);
   check(pac_random_partition(get_vertex_num(), (uint32_t)P, 1, v2p.data()));

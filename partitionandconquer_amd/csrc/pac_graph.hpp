@@ -85,6 +85,7 @@ class Graph {
   size_t solve_friend_triangle_galloping(const VID root, const std::vector<VID> &nodes,
                                          const int num_threads) const;
   void greedy_partition(const size_t partition_number, std::vector<int> &vertex2partition);
+  void greedy_partition_2(const size_t partition_num, std::vector<int> &vertex2partition);
   void random_partition(const size_t partition_num, std::vector<int> &vertex2partition);
   void save_as_csr_binfile(const std::string csr_vlist_binfile,
                            const std::string csr_elist_binfile);

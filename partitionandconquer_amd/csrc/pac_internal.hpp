@@ -183,6 +183,7 @@ struct Ctx {
   // kernel writes each item's start and end clock here: 2 x items entries
   unsigned long long *item_stamps = nullptr;
   pac_timing last{};
+  double timeout_ms = 0;  // bound on waiting for a solve's completion (0: none)
   int tile = 1;
   void *noalloc_scratch = nullptr;  // this device's static scratch of pac_tc_noalloc
   DevBuf scratch;     // generic temporary storage (library-owned)

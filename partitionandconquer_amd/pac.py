@@ -106,9 +106,11 @@ def lib():
         "pac_graph_partition_sizes": ([vp, vp], C.c_int),
         "pac_random_partition": ([C.c_uint64, C.c_uint32, C.c_uint64, vp], C.c_int),
         "pac_greedy_partition": ([vp, C.c_uint32, C.c_uint64, vp], C.c_int),
+        "pac_greedy_partition_2": ([vp, C.c_uint32, C.c_uint64, vp], C.c_int),
         "pac_evaluate_partition": ([vp, vp, C.c_uint32, C.POINTER(C.c_double)], C.c_int),
         "pac_tc": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
         "pac_tc_noalloc": ([vp, C.c_int, vp, vp], C.c_int),
+        "pac_ctx_set_timeout": ([vp, C.c_double], C.c_int),
         "pac_lcc": ([vp, C.c_int, C.c_int, vp, vp, vp], C.c_int),
         "pac_jaccard_edges": ([vp, C.c_int, C.c_int, vp, vp, vp, vp, vp], C.c_int),
         "pac_jaccard_queries": ([vp, vp, C.c_uint32, vp], C.c_int),
@@ -205,6 +207,10 @@ class Context:
         _check(lib().pac_ctx_last_timing(self._h, C.byref(t)))
         return LastTiming(t.total_ms, t.isect_ms, t.epilogue_ms, t.kernel_ms, t.launches,
                           tuple(t.strategy_partitions))
+
+    def set_timeout(self, ms: float):
+        """Bound (ms, 0 = none) on waiting for a solve; PacError -8 when it passes."""
+        _check(lib().pac_ctx_set_timeout(self._h, ms))
 
     def stream_copy(self, nbytes: int, reps: int = 5) -> float:
         """Median time (ms) of a plain device-to-device streaming copy of nbytes."""
@@ -394,6 +400,12 @@ class Graph:
     def greedy_partition(self, P: int, seed=1) -> np.ndarray:
         out = np.zeros(self.get_vertex_num(), np.int32)
         _check(lib().pac_greedy_partition(self._h, P, seed, _ptr(out)))
+        return out
+
+    def greedy_partition_2(self, P: int, seed=1) -> np.ndarray:
+        """Graph::greedy_partition_2 (Graph.cpp:969-1015), seeded."""
+        out = np.zeros(self.get_vertex_num(), np.int32)
+        _check(lib().pac_greedy_partition_2(self._h, P, seed, _ptr(out)))
         return out
 
     def evaluate_partition(self, v2p, P: int) -> float:

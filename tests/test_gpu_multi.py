@@ -116,6 +116,22 @@ def test_partition_filtered_queries(ctx, orc, P, seed):
     n = 300
     u = rng.choice(hubs, n).astype(np.int32)
     v = np.where(rng.random(n) < 0.5, rng.choice(hubs, n), rng.integers(0, ref.V, n)).astype(np.int32)
+    # pairs of two vertices with at most 16 neighbours each, through a common neighbour where
+    # one exists: the kernel merges such a pair in one lane, filters included
+    low = np.nonzero((deg >= 1) & (deg <= 16))[0]
+    lu, lv = [], []
+    for a in rng.choice(low, 600):
+        w = ref.elist[ref.vlist[a] + rng.integers(0, deg[a])]
+        nb = ref.elist[ref.vlist[w]:ref.vlist[w + 1]]
+        nb = nb[(deg[nb] <= 16) & (nb != a)]
+        if len(nb):
+            lu.append(a)
+            lv.append(rng.choice(nb))
+    assert len(lu) > 100
+    u = np.concatenate([u, np.array(lu, np.int32)])
+    v = np.concatenate([v, np.array(lv, np.int32)])
+    assert np.any(g.cnt_common_pairs(u[n:], v[n:]) > 0)
+    n = len(u)
     thr = rng.integers(-1, ref.V + 2, n).astype(np.int32)
     for k in sorted({0, opg.P // 2, opg.P - 1}):
         got = g.cnt_common_pairs(u, v, partition=k)

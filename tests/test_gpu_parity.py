@@ -503,3 +503,41 @@ def test_evaluate_partition_matches_oracle(ctx, orc):
             assert g.evaluate_partition(v2p, int(v2p.max()) + 1) == want, P
     empty = pac.Graph.from_csr(ctx, np.zeros(5, np.uint64), np.zeros(0, np.int32))
     assert math.isnan(empty.evaluate_partition(np.zeros(4, np.int32), 1))
+
+
+def test_solve_timeout_forced_from_the_host(ctx, orc):
+    """PAC_ERR_TIMEOUT: with a deadline far shorter than the solve, the host stops waiting
+    while the kernels run to their normal end; the context stays usable (the next solve
+    waits behind them) and a generous deadline changes nothing."""
+    ref = orc.gen_rmat(16, 16, 0.57, 0.19, 0.19, seed=61)
+    T = orc.tc_oriented(ref, NT)
+    g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    g.prepare(64)
+    try:
+        ctx.set_timeout(1e-6)
+        with pytest.raises(pac.PacError) as e:
+            g.lcc(pac.FORCE_MERGE, pac.LAUNCH_PLAIN)
+        assert e.value.code == -8
+        ctx.set_timeout(60000.0)
+        assert g.tc(pac.FORCE_BITMAP, pac.LAUNCH_PERSISTENT) == T
+        with pytest.raises(pac.PacError) as e:
+            ctx.set_timeout(-1.0)
+        assert e.value.code == -1
+    finally:
+        ctx.set_timeout(0.0)
+    assert g.lcc_device(pac.DISPATCH_FIXED, pac.LAUNCH_GRAPH) == T
+
+
+def test_greedy_partition_2_matches_oracle(ctx, orc):
+    """pac_greedy_partition_2 against the oracle's restatement of Graph.cpp:969-1015 (same
+    tie order and seeded draws), with isolated vertices present, and the partitioned count
+    under its map."""
+    ref = orc.gen_rmat(10, 8, 0.57, 0.19, 0.19, seed=31)
+    g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    assert (ref.deg() == 0).any()
+    T = orc.tc_merge(ref, NT)
+    for P in (1, 2, 7, 32):
+        got = g.greedy_partition_2(P, seed=P)
+        assert np.array_equal(got, orc.greedy_partition_2(ref, P, seed=P)), P
+        g.prepare(P, got)
+        assert g.tc(pac.DISPATCH_FIXED) == T

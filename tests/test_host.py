@@ -78,3 +78,27 @@ def test_cpu_baseline_sampling():
     assert rows2 == nonempty and 0 < est2 <= spent2 * (1 + 1e-9)
     empty = O.edges_to_csr(50, [], [])
     assert bench.cpu_baseline(empty.vlist, empty.elist, budget_s=0.1)[0] == 0.0
+
+
+def test_bounded_wait_on_the_cpu(tmp_path):
+    """The solve's bounded wait (pac_wait.hpp) with stand-in completion queries: completion,
+    PAC_ERR_TIMEOUT after the deadline, failures passed through, no bound at 0."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "wait_check")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "partitionandconquer_amd", "csrc"),
+                    os.path.join(root, "tests", "wait_check.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("ok ") == 6
+
+
+def test_oracle_under_address_sanitizer():
+    """`make -C oracle asan`: the oracle built with ASan + UBSan, its functions run on small
+    R-MAT graphs (including the partitioned threshold merge, where the reference reads past a
+    sub-list), and the formulations that must agree do."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(root, "oracle"), "asan"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "asan check ok" in r.stdout

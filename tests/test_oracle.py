@@ -279,6 +279,39 @@ def test_evaluate_partition(orc):
     assert math.isnan(orc.PGraph(empty, np.zeros(4, np.int32)).evaluate_partition()[0])
 
 
+def test_greedy_partition_2_known_answers(orc):
+    """Placements derived by hand from Graph.cpp:969-1015.  Path 0-1-2, P=2: vertex 1
+    (degree 2) takes partition 0; vertex 0 sees count 0 at vertex 1 in both partitions and
+    takes the first, 0; vertex 2 sees count 1 in partition 0 and 0 in partition 1: 1.  Star
+    with centre 0 and leaves 1-4, P=2: the centre takes 0; the leaves see the centre's
+    counts (0,0) -> 0, then (1,0) -> 1, (1,1) -> 0, (2,1) -> 1."""
+    path = orc.edges_to_csr(3, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32))
+    assert orc.greedy_partition_2(path, 2).tolist() == [0, 0, 1]
+    star = orc.edges_to_csr(5, np.zeros(4, np.uint32), np.arange(1, 5, dtype=np.uint32))
+    assert orc.greedy_partition_2(star, 2).tolist() == [0, 0, 1, 0, 1]
+    # isolated vertices take the random_partition stream
+    iso = orc.edges_to_csr(6, np.array([0], np.uint32), np.array([1], np.uint32))
+    got = orc.greedy_partition_2(iso, 4, seed=9)
+    assert got[2:].tolist() == orc.random_partition(6, 4, seed=9)[2:].tolist()
+
+
+@pytest.mark.parametrize("P", [2, 5])
+def test_greedy_partition_2_minimax(P, orc):
+    """Replaying the placements: every vertex with neighbours went to the first partition
+    whose largest neighbour count (at the moment it was placed) is smallest."""
+    g = orc.gen_rmat(8, 8, 0.57, 0.19, 0.19, seed=P)
+    v2p = orc.greedy_partition_2(g, P)
+    d = g.deg()
+    order = sorted(range(g.V), key=lambda u: (-d[u], u))
+    cnt = np.zeros((P, g.V), np.int64)
+    for u in order:
+        nb = g.elist[g.vlist[u]:g.vlist[u + 1]]
+        if len(nb):
+            mx = cnt[:, nb].max(axis=1)
+            assert v2p[u] == int(np.argmin(mx)), u
+        cnt[v2p[u], nb] += 1
+
+
 def brute_pairs(g):
     """Pair ownership walked per vertex over its symmetric row, as the device builds it
     (x owns {x, u} when x's out-list is the longer, ties to the tail; empty partner lists
