@@ -120,9 +120,9 @@ def test_partition_filtered_queries(ctx, orc, P, seed):
     # one exists: the kernel merges such a pair in one lane, filters included
     low = np.nonzero((deg >= 1) & (deg <= 16))[0]
     lu, lv = [], []
-    for a in rng.choice(low, 600):
-        w = ref.elist[ref.vlist[a] + rng.integers(0, deg[a])]
-        nb = ref.elist[ref.vlist[w]:ref.vlist[w + 1]]
+    for a in rng.choice(low, 600).tolist():
+        w = int(ref.elist[int(ref.vlist[a]) + int(rng.integers(0, deg[a]))])
+        nb = ref.elist[int(ref.vlist[w]):int(ref.vlist[w + 1])]
         nb = nb[(deg[nb] <= 16) & (nb != a)]
         if len(nb):
             lu.append(a)
