@@ -281,7 +281,7 @@ def main():
     stream_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
 
     kms = ker if ker > 0 else dev_p
-    alg_bytes = info.algorithmic_bytes(per_vertex=True)
+    alg_bytes = info.algorithmic_bytes(pac.MODE_LCC)
     achieved = alg_bytes / (kms * 1e-3)
     traffic = measured_traffic(scale, a.partitions)
     res = {

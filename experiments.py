@@ -22,7 +22,10 @@ cpu: the oracle's restatement of the reference's timed solves, run whole on the 
   (OMP_NUM_THREADS threads, each leg in a child process under a time limit, on the CSR the
   GPU generated): R-MAT scale 16 TC-merge + LCC-merge (Graph.cpp:355-368, 828-840) and the
   64-partition Partitioned_Graph TC (Graph.cpp:1346-1362), median of 3 after a warm-up;
-  R-MAT scale 22 TC-merge, median of 3 after a warm-up; R-MAT scale 24 TC-merge, one run.
+  R-MAT scale 22 TC-merge, median of 3 after a warm-up; R-MAT scale 24 TC-merge, one run,
+  its rows in 64 consecutive ranges each run as an OpenMP loop (a line printed per range;
+  this spreads the low-ID hub rows over the threads better than the reference's one static
+  loop, so it errs on the CPU's side).  --scale S runs only the legs at scale S.
 next: the components on either side of the path, on R-MAT scale 22 (seed 22) unless noted:
   edge-list ingestion (the graph's undirected edge list, shuffled, to the symmetric CSR on
   the device; an R-MAT scale 18 edge list as text through the device parser), pairwise
@@ -98,7 +101,17 @@ def exp_scale22(ctx, scale=22, partitions=1024):
             res["device_ms"][f"{dname}/{lname}"] = round(ms, 3)
     best = min(res["device_ms"]["fixed/graph"], res["device_ms"]["bitmap/graph"])
     res["gteps_batched"] = round(info.num_edges / (best * 1e-3) / 1e9, 4)
-    res["roofline_frac"] = round(4 * info.work_W / (best * 1e-3) / 8e12, 4)
+    # the intersection kernel alone (HIP events around its launch), against the bytes the
+    # hash path must move in LCC mode
+    ker = []
+    for _ in range(5):
+        g.lcc_device(pac.DISPATCH_FIXED, pac.LAUNCH_PERSISTENT)
+        ker.append(ctx.last_timing().kernel_ms)
+    kms = float(np.median(ker))
+    alg = info.algorithmic_bytes(pac.MODE_LCC)
+    res["kernel_ms"] = round(kms, 3)
+    res["algorithmic_bytes"] = alg
+    res["roofline_frac"] = round(alg / (kms * 1e-3) / 8e12, 4)
     res["speedup_batched_vs_plain_fixed"] = round(res["device_ms"]["fixed/plain"] / best, 3)
     return res
 
@@ -175,6 +188,15 @@ def exp_jaccard(ctx, scale=25, partitions=4096, sample=200000, cpu_budget=15.0,
         res["device_ms"][name] = round(median_ms(ctx, lambda: g.jaccard_edges_device(d, l), 3), 3)
     best = min(res["device_ms"]["batched_fixed"], res["device_ms"]["batched_bitmap"])
     res["gteps"] = round(info.num_edges / (best * 1e-3) / 1e9, 4)
+    ker = []
+    for _ in range(3):
+        g.jaccard_edges_device(pac.DISPATCH_FIXED, pac.LAUNCH_PERSISTENT)
+        ker.append(ctx.last_timing().kernel_ms)
+    kms = float(np.median(ker))
+    alg = info.algorithmic_bytes(pac.MODE_JACCARD)
+    res["kernel_ms"] = round(kms, 3)
+    res["algorithmic_bytes"] = alg
+    res["roofline_frac"] = round(alg / (kms * 1e-3) / 8e12, 4)
     res["speedup_batched_vs_plain_fixed"] = round(res["device_ms"]["plain_fixed"] / best, 3)
     # parity on a sample of edges, and the CPU baseline on the same sample
     src, dst, inter, uni, ratio = g.jaccard_edges(pac.DISPATCH_FIXED, pac.LAUNCH_PERSISTENT)
@@ -211,6 +233,14 @@ def run():
         return (O.tc_merge(g, nt), float(np.nansum(O.lcc_merge(g, nt))))
     if what == "ptc":
         return O.PGraph(g, O.contiguous_partition(g.V, 64), nt).tc_merge()
+    if what == "tc-ranges":
+        # the rows in 64 consecutive ranges, each an OpenMP loop over its rows, one line
+        # printed per range
+        tot = 0
+        for k, rows in enumerate(np.array_split(np.arange(g.V, dtype=np.int32), 64)):
+            tot += O.tc_merge_rows(g, rows, nt)
+            print(json.dumps({"range": k}), flush=True)
+        return tot
 for _ in range(warm):
     run()
 t = []
@@ -223,14 +253,16 @@ print(json.dumps({"times_s": t, "median_s": sorted(t)[len(t) // 2], "result": st
 """
 
 
-def exp_cpu(ctx, limit_s=900.0):
+def exp_cpu(ctx, limit_s=900.0, only=None):
     import subprocess
     import tempfile
     nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     res = {"threads": nt, "legs": []}
     tmp = tempfile.mkdtemp(prefix="pac_cpu_")
-    for scale, what, warm, reps in ((16, "tc+lcc", 1, 3), (16, "ptc", 1, 3), (22, "tc", 1, 3),
-                                    (24, "tc", 0, 1)):
+    legs = ((16, "tc+lcc", 1, 3), (16, "ptc", 1, 3), (22, "tc", 1, 3), (24, "tc-ranges", 0, 1))
+    for scale, what, warm, reps in legs:
+        if only is not None and scale != only:
+            continue
         g = pac.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed=scale)
         vl, el = g.csr()
         m = g.info().num_edges
@@ -242,10 +274,19 @@ def exp_cpu(ctx, limit_s=900.0):
         log(f"cpu leg: scale {scale} {what}, warm-up {warm}, runs {reps}, limit {limit_s:.0f} s")
         leg = {"scale": scale, "solve": what, "m": m, "warmup": warm, "runs": reps}
         try:
-            r = subprocess.run([sys.executable, "-c", CPU_WORKER, ROOT, fv, fe, what, str(warm),
-                                str(reps), str(nt)], capture_output=True, text=True,
-                               timeout=limit_s)
-            last = json.loads(r.stdout.strip().splitlines()[-1])
+            # the child's lines are passed through as they come (a leg can run for minutes)
+            p = subprocess.Popen([sys.executable, "-c", CPU_WORKER, ROOT, fv, fe, what, str(warm),
+                                  str(reps), str(nt)], stdout=subprocess.PIPE, text=True)
+            t_end = time.time() + limit_s
+            lines = []
+            for line in p.stdout:
+                lines.append(line)
+                log(line.rstrip())
+                if time.time() > t_end:
+                    p.kill()
+                    raise subprocess.TimeoutExpired(what, limit_s)
+            p.wait()
+            last = json.loads(lines[-1])
             leg.update(last)
             leg["gteps"] = round(m / last["median_s"] / 1e9, 6)
         except subprocess.TimeoutExpired:
@@ -432,7 +473,7 @@ def main(argv=None):
     elif a.which == "next":
         r = exp_next(ctx, a.scale or 22, profile_only=a.profile_only)
     elif a.which == "cpu":
-        r = exp_cpu(ctx)
+        r = exp_cpu(ctx, only=a.scale)
     else:
         r = exp_jaccard(ctx, a.scale or 25, profile_only=a.profile_only)
     print(json.dumps(r))
