@@ -385,19 +385,47 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
 // coalesced loads, software-pipelined one step ahead across the wave's pairs; the
 // pair's hit count is a ballot popcount held in scalar registers.
 constexpr uint32_t kMaxKicks = 48;
-// The two buckets of x among 2^lg (sh = 32 - lg), from one full-rate 24-bit multiply-add:
-// h = (x mod 2^24) * 0x9E3779 + (x >> 20) -- the added high bits keep IDs that differ only
-// above bit 23 apart in the second choice; b1 is the top lg bits of h and b2 = b1 ^ d with
-// d = bits [4, 4 + lg) of h made odd (lg <= 12, so the two fields are disjoint; one
-// bit-field extract with a constant offset) -- so the two choices always differ (one
-// bucket of each parity, the two tables of classic cuckoo hashing).  (__umul24 is
-// declared returning int: its result is taken as unsigned before any shift, so the
-// bucket indices stay below 2^lg.)
+// The two buckets of x among 2^lg (sh = 32 - lg), from one full-rate 24-bit multiply-add
+// and no shift: h = (x mod 2^24) * 0x9E3778 + x, which for IDs below 2^24 is x times the odd
+// constant 0x9E3779 (the + x also carries ID bits above 23 into the top of h); b1 is the top
+// lg bits of h and b2 = b1 ^ d with d = bits [3, 3 + lg) of h made odd (lg <= 12, so the
+// two fields are disjoint) -- so the two choices always differ (one bucket of each parity,
+// the two tables of classic cuckoo hashing).  On the out-lists of R-MAT scales 20 and 22 a
+// sequential replay of the table build leaves 0.006 % of the rows with a homeless key (the
+// earlier (x mod 2^24) * 0x9E3779 + (x >> 20) with d from bits [4, 4 + lg): 0.0035 %), and
+// the streamed lookups need two vector instructions fewer per element: the bucket field of
+// d sits where a bucket's byte address has it (see probe in row_hash).  (__umul24 is
+// declared returning int: its result is taken as unsigned before any shift, so the bucket
+// indices stay below 2^lg.)
+constexpr uint32_t kHashMul = 0x9E3778u;
+__device__ __forceinline__ uint32_t hash_key(uint32_t x) {
+  return (uint32_t)__umul24(x, kHashMul) + x;
+}
 __device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t lg, uint32_t &b1,
                                         uint32_t &b2) {
-  const uint32_t h = (uint32_t)__umul24(x, 0x9E3779u) + (x >> 20);
+  const uint32_t h = hash_key(x);
   b1 = h >> sh;
-  b2 = b1 ^ (__builtin_amdgcn_ubfe(h, 4u, lg) | 1u);
+  b2 = b1 ^ (__builtin_amdgcn_ubfe(h, 3u, lg) | 1u);
+}
+
+// LDS byte addresses: a shared-memory pointer's offset in the workgroup's LDS, and 8-byte
+// loads / 32-bit atomic adds at such an offset.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+typedef __attribute__((address_space(3))) const u32x2 LdsU2;
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+__device__ __forceinline__ uint2 lds_load_u2(uint32_t a) {
+  const u32x2 v = *(LdsU2 *)(uintptr_t)a;
+  return make_uint2(v.x, v.y);
+}
+// base + off, base a constant where possible: the compiler then folds it into the
+// instruction's offset field.
+__device__ __forceinline__ void lds_add_u32(uint32_t base, uint32_t off, uint32_t v) {
+  __attribute__((address_space(3))) char *p =
+      (__attribute__((address_space(3))) char *)(uintptr_t)base;
+  __hip_atomic_fetch_add((LdsU32 *)(p + off), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // SOLO: one wave does the whole row on its own share (1/WAVES) of the table space, with
@@ -484,6 +512,17 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   }
   sync();
   if (SOLO ? __ballot(lost) != 0ull : sm.homeless != 0) return false;  // uniform
+  // The streamed lookups address the table by byte: bucket b is at tb + 8b, and the table
+  // starts at a multiple of its size (the workgroup's LDS begins with it, and a wave's share
+  // of it is a multiple of the largest table one wave builds alone), so bucket b1 ^ f lies at
+  // (tb + 8 b1) ^ 8f -- checked here, uniformly, and never false in practice: a table placed
+  // otherwise is done by the exact fallback paths.  Counter word b is at
+  // cb + (bucket address >> 1), cb = (counters' address) - tb / 2: a wave's share of the
+  // counters starts half as far into them as its share of the table does into the table,
+  // so cb is the same for every share, taken from the workgroup's arrays (a constant).
+  const uint32_t tb = lds_addr(tab), m3 = ((1u << lg) - 1u) << 3;
+  if ((tb & (m3 | 7u)) != 0) return false;  // uniform
+  const uint32_t cb = MODE == kModeTC ? 0u : lds_addr(H.cnt16) - (lds_addr(H.tab) >> 1);
   // Lookup of w.  A bucket's two slots share one word of the packed 16-bit counters, so
   // a hit is reported as that word (the bucket) and the increment for w's half of it
   // (1 or 1 << 16); word and increment are only meaningful on a hit.  Both buckets are
@@ -507,15 +546,20 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // The streamed lookups: the four slot compares of w are balloted one by one (each
   // ballot is the compare's own lane mask), so the wave's hit mask never passes through
   // a vector register; the hit's counter word and increment are only formed under it.
+  // The buckets of buckets() by byte address: a1 = tb + 8 b1, a2 = a1 ^ 8 (bits [3, 3 + lg)
+  // of h, made odd) = a1 ^ ((h | 8) & m3): a multiply-add, a shift, a shift-add, a bitop3 and
+  // an xor.
   struct Probe {
     bool e10, e11, e20, e21;
-    uint32_t b1, b2;
+    uint32_t a1, a2;
   };
   auto probe = [&](uint32_t w) -> Probe {
     Probe q;
-    buckets(w, sh, lg, q.b1, q.b2);
-    const uint2 q1 = reinterpret_cast<const uint2 *>(tab)[q.b1];
-    const uint2 q2 = reinterpret_cast<const uint2 *>(tab)[q.b2];
+    const uint32_t h = hash_key(w);
+    q.a1 = tb + ((h >> sh) << 3);
+    q.a2 = q.a1 ^ ((h | 8u) & m3);
+    const uint2 q1 = lds_load_u2(q.a1);
+    const uint2 q2 = lds_load_u2(q.a2);
     q.e10 = q1.x == w;
     q.e11 = q1.y == w;
     q.e20 = q2.x == w;
@@ -595,7 +639,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         if (MODE == kModeSUP && lane == u) hw = bal;
         if ((q.e10 | q.e11 | q.e20 | q.e21) & live) {
           // t(w) / edge r->w
-          atomicAdd(&cnt16[(q.e10 | q.e11) ? q.b1 : q.b2], (q.e11 | q.e21) ? 0x10000u : 1u);
+          lds_add_u32(cb, ((q.e10 | q.e11) ? q.a1 : q.a2) >> 1, (q.e11 | q.e21) ? 0x10000u : 1u);
         }
       }
       if (MODE == kModeSUP) {
