@@ -26,6 +26,10 @@ NT = min(16, os.cpu_count() or 1)
 MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                      "partitionandconquer_amd", "dispatch_mlp.bin")
 LAUNCHES = [pac.LAUNCH_PLAIN, pac.LAUNCH_PERSISTENT, pac.LAUNCH_GRAPH]
+# T of R-MAT scale 24, ef 16, (0.57, 0.19, 0.19, 0.05), seed 24, counted by the oracle's
+# tc_merge on the CSR the device generator produced (the generator equals the oracle's
+# gen_rmat bit for bit at the scales the parity tests compare)
+RMAT24_TRIANGLES = 10283762775
 
 
 def log(*a):
@@ -142,6 +146,9 @@ def test_rmat24_learned_dispatch_headline(ctx, orc):
     ctx.mlp_load(MODEL)
     try:
         T = g.lcc_device(pac.DISPATCH_LEARNED, pac.LAUNCH_GRAPH)
+        # the oracle's restated Graph::solve_triangle_counting_merge (Graph.cpp:355-368) over
+        # the whole graph, run on the GPU box's host in 376 s (experiments.py cpu --scale 24)
+        assert T == RMAT24_TRIANGLES
         assert g.lcc_device(pac.DISPATCH_LEARNED, pac.LAUNCH_GRAPH) == T   # a replay
         assert g.tc(pac.FORCE_MERGE, pac.LAUNCH_PERSISTENT) == T
         assert g.tc(pac.FORCE_GALLOP, pac.LAUNCH_PERSISTENT) == T
