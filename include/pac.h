@@ -79,10 +79,11 @@ typedef struct {
    * which stages that list while the partner's shorter list is streamed; pairs whose
    * shorter list is empty are dropped.  These are the work model of the hash path:
    * its algorithmic bytes per launch are 4*streamed + 4*staged_item_elems +
-   * 12*num_pairs + 20*num_rows + 32*num_items (TC), plus 4*num_pairs (edge indices) and
-   * 16*num_vertices (per-vertex counts read and written once) with per-vertex counts, or
-   * 12*num_pairs (edge and hit-word indices) + streamed/8 (one hit bit per streamed
-   * element) + 8*num_edges (supports read and written once) for per-edge supports. */
+   * 8*num_pairs (each pair's list bounds) + 20*num_rows + 32*num_items (TC), plus
+   * 4*num_pairs (partner IDs) and 16*num_vertices (per-vertex counts read and written
+   * once) with per-vertex counts, or 16*num_pairs (partner, edge and hit-word indices) +
+   * streamed/8 (one hit bit per streamed element) + 8*num_edges (supports read and written
+   * once) for per-edge supports. */
   uint64_t num_pairs;
   uint64_t num_rows;          /* vertices owning at least one pair                     */
   uint64_t streamed_elems;    /* sum over pairs of d+(partner) = sum of min(d+(t),d+(h)) */

@@ -75,6 +75,9 @@ struct GraphDev {
   const uint32_t *poff;  // V+1 offsets of the intersection pairs owned by each vertex
   const uint32_t *ppar;  // partner of each pair (its out-list is the one streamed)
   const uint32_t *pedge; // oriented-edge index of each pair (position in oadj)
+  // per pair: the partner's out-list as (first position in oadj, length | kPairFwd when the
+  // pair's edge leaves the owner) -- read with the pair, so its bounds need no gather
+  const uint2 *pinfo;
   const uint32_t *rows;  // rows owning at least one pair, grouped by partition
   unsigned long long *total;  // T accumulator
   unsigned long long *tvert;  // LCC mode: per-vertex triangle counts, every credit added
@@ -128,7 +131,7 @@ struct Graph {
   // owned by the endpoint with the longer out-list (ties: the tail), which stages its
   // list in LDS while the partner's shorter list is streamed.  Pairs whose partner has
   // an empty out-list are dropped (they hold no triangle).
-  DevBuf poff, ppar, pedge;
+  DevBuf poff, ppar, pedge, pinfo;
   // Construction state, kept for the later passes: crow[c] = the row holding entry 256c;
   // oflags / pflags = one bit per symmetric entry (kept by the orientation / owned as a
   // pair), owpre / ppre = exclusive scans of their 64-bit words' popcounts.
@@ -231,6 +234,9 @@ constexpr uint32_t kListEnd = 0xFFFFFFFEu;
 // owner's out-list (e - ooff[x] >= d+(x)), so the TC/LCC kernels take it for u->x, the
 // only fact they need of it.
 constexpr uint32_t kBackEdge = 0xFFFFFFFFu;
+// pinfo[p].y: the partner's out-degree, with this bit set when the pair's edge is owner ->
+// partner (out-degrees stay below 2^31).
+constexpr uint32_t kPairFwd = 0x80000000u;
 // Fill pedge of the pairs owned by their head with their edge indices (once; the per-edge
 // supports need them).
 int build_back_edges(Graph *g);

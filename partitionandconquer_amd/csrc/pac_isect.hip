@@ -61,8 +61,10 @@ namespace {
 // the pointers point and emits flat loads, which also count against the LDS wait
 // counter: every wait for an LDS result would then wait for the outstanding HBM loads.
 #define PAC_GLOBAL __attribute__((address_space(1)))
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 struct GDev {
   const PAC_GLOBAL uint32_t *ooff, *oadj, *poff, *ppar, *pedge, *rows;
+  const PAC_GLOBAL u32x2 *pinfo;
   PAC_GLOBAL unsigned long long *total, *tvert;
   PAC_GLOBAL uint32_t *sup;
   const PAC_GLOBAL unsigned long long *pword;
@@ -77,6 +79,7 @@ __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
   g.poff = (const PAC_GLOBAL uint32_t *)d.poff;
   g.ppar = (const PAC_GLOBAL uint32_t *)d.ppar;
   g.pedge = (const PAC_GLOBAL uint32_t *)d.pedge;
+  g.pinfo = (const PAC_GLOBAL u32x2 *)d.pinfo;
   g.rows = (const PAC_GLOBAL uint32_t *)d.rows;
   g.total = (PAC_GLOBAL unsigned long long *)d.total;
   g.tvert = (PAC_GLOBAL unsigned long long *)d.tvert;
@@ -272,10 +275,10 @@ __device__ __forceinline__ uint32_t probe_merge(const PAC_GLOBAL uint32_t *__res
   return hits;
 }
 
-// The pairs of a row are visited in batches of 64 per wave: lane l loads the partner y
-// and edge index e of the wave's l-th pair and the bounds of N+(y), so the offsets of
-// 64 lists arrive with one round trip; the wave then walks the batch, broadcasting
-// each pair with readlane.  Wave w takes pair positions k_lo + w, k_lo + w + WAVES, ...
+// The pairs of a row are visited in batches of 64 per wave: lane l loads the bounds of
+// N+(y) stored with the wave's l-th pair (and, when counts are kept, its partner y and edge
+// index e), so the offsets of 64 lists arrive with one round trip; the wave then walks the
+// batch, broadcasting each pair with readlane.  Wave w takes pair positions k_lo + w, k_lo + w + WAVES, ...
 // body(y, e, b0, nb, p) runs once per pair, p being its index in the pair lists (e is only
 // loaded when counts are kept).
 template <int WAVES, int MODE, class Body>
@@ -286,10 +289,13 @@ __device__ __forceinline__ void for_each_pair(const GDev &G, uint32_t p0, uint32
     const uint32_t kl = k_lo + wave + (uint32_t)WAVES * (t0 + lane);
     uint32_t yl = 0, el = 0, bl = 0, nl = 0;
     if (kl < k_hi) {
-      yl = G.ppar[p0 + kl];
-      if (MODE != kModeTC) el = G.pedge[p0 + kl];
-      bl = G.ooff[yl];
-      nl = G.ooff[yl + 1] - bl;
+      const u32x2 pi = G.pinfo[p0 + kl];
+      bl = pi.x;
+      nl = pi.y & ~kPairFwd;
+      if (MODE != kModeTC) {
+        yl = G.ppar[p0 + kl];
+        el = G.pedge[p0 + kl];
+      }
     }
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < k_hi));
     for (uint32_t jj = 0; jj < nvalid; ++jj)
@@ -410,7 +416,6 @@ __device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t lg, ui
 
 // LDS byte addresses: a shared-memory pointer's offset in the workgroup's LDS, and 8-byte
 // loads / 32-bit atomic adds at such an offset.
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) uint32_t LdsU32;
 typedef __attribute__((address_space(3))) const u32x2 LdsU2;
 __device__ __forceinline__ uint32_t lds_addr(const void *p) {
@@ -457,15 +462,27 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   uint32_t *const tab = H.tab + share * (S::kSlots / WAVES);
   uint32_t *const cnt16 = MODE == kModeTC ? H.cnt16 : H.cnt16 + share * (S::kSlots / WAVES / 2);
   if (c_hi > np) c_hi = np;
-  // The wave's first batch of pairs: partners and edges are loaded before the table is
-  // built, the bounds of their lists after the insertions, so both latencies overlap it.
+  // The wave's first batch of pairs is loaded before the table is built, so the latency
+  // overlaps it: per pair the bounds of the partner's list and the direction of its edge
+  // (stored with the pair: no gather through ooff), and when counts are kept the partner;
+  // SUP mode also needs the edge index and the first hit word.
   uint32_t kl = c_lo + wave + PS * lane, yl = 0, el = 0, bl = 0, nl = 0;
+  bool fl = false;            // the pair's edge is r -> y (y is in the table)
   unsigned long long wl = 0;  // SUP mode: the pair's first hit word
-  if (kl < c_hi) {
-    yl = G.ppar[p0 + kl];
-    if (MODE != kModeTC) el = G.pedge[p0 + kl];
-    if (MODE == kModeSUP) wl = G.pword[p0 + kl];
-  }
+  auto load_pair = [&]() {
+    if (kl < c_hi) {
+      const u32x2 pi = G.pinfo[p0 + kl];
+      bl = pi.x;
+      nl = pi.y & ~kPairFwd;
+      fl = (pi.y & kPairFwd) != 0;
+      if (MODE != kModeTC) yl = G.ppar[p0 + kl];
+      if (MODE == kModeSUP) {
+        el = G.pedge[p0 + kl];
+        wl = G.pword[p0 + kl];
+      }
+    }
+  };
+  load_pair();
   uint32_t lg = 4;
   while ((1u << lg) < nA) ++lg;  // 2^lg >= nA: load factor <= 1/2
   const uint32_t nslot = 2u << lg, sh = 32u - lg;
@@ -505,10 +522,6 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       from = vb;
       if (x == kEmpty) break;
     }
-  }
-  if (kl < c_hi) {
-    bl = G.ooff[yl];
-    nl = G.ooff[yl + 1] - bl;
   }
   sync();
   if (SOLO ? __ballot(lost) != 0ull : sm.homeless != 0) return false;  // uniform
@@ -589,14 +602,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     if (t0 != 0) {
       kl = c_lo + wave + PS * (t0 + lane);
       yl = el = bl = nl = 0;
+      fl = false;
       wl = 0;
-      if (kl < c_hi) {
-        yl = G.ppar[p0 + kl];
-        if (MODE != kModeTC) el = G.pedge[p0 + kl];
-        bl = G.ooff[yl];
-        nl = G.ooff[yl + 1] - bl;
-        if (MODE == kModeSUP) wl = G.pword[p0 + kl];
-      }
+      load_pair();
     }
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
     if (nvalid == 0) break;
@@ -671,9 +679,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       // lane rather than one wave-wide lookup per pair).  t(y) / the pair's edge: r->y adds
       // to y's slot in the table; y->r goes to t(y) in LCC mode and to the edge's support in
       // SUP mode, both atomically (each pair is visited once).
-      if (el - a0 < nA) {
+      if (fl) {
         uint32_t word, inc;
-        find(yl, word, inc);  // y = A[e - a0] is in the table
+        find(yl, word, inc);  // r -> y: y is a key of the table
         atomicAdd(&cnt16[word], inc * phl);  // phl <= nA <= CAP < 2^16
       } else if (MODE == kModeLCC) {
         gadd(&G.tvert[yl], (unsigned long long)phl);
@@ -1484,6 +1492,7 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   d.poff = g->poff.as<const uint32_t>();
   d.ppar = g->ppar.as<const uint32_t>();
   d.pedge = g->pedge.as<const uint32_t>();
+  d.pinfo = g->pinfo.as<const uint2>();
   d.rows = g->rows.as<const uint32_t>();
   d.total = g->total.as<unsigned long long>();
   d.tvert = g->tvert.as<unsigned long long>();

@@ -278,12 +278,14 @@ k_pair_flags(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
 // Pair lists: thread per entry (each wave on one aligned word); an owned entry (x, u)
 // becomes pair ordinal(i) of the ownership bits, in input order, so each row's pairs
 // come out sorted by partner.  The pair's edge index: for x->u the entry's position in
-// oadj (its orientation ordinal); for u->x kBackEdge until k_back_edges finds it.
+// oadj (its orientation ordinal); for u->x kBackEdge until k_back_edges finds it.  pinfo:
+// u's out-list (first position, length | kPairFwd for x->u).
 __global__ void __launch_bounds__(256)
 k_pair_fill(const uint32_t *__restrict__ adj, const unsigned long long *__restrict__ oflags,
             const uint32_t *__restrict__ owpre, const unsigned long long *__restrict__ pflags,
-            const uint32_t *__restrict__ ppre, uint32_t nnz, uint32_t *__restrict__ ppar,
-            uint32_t *__restrict__ pedge) {
+            const uint32_t *__restrict__ ppre, const uint32_t *__restrict__ ooff,
+            const uint32_t *__restrict__ odeg, uint32_t nnz, uint32_t *__restrict__ ppar,
+            uint32_t *__restrict__ pedge, uint2 *__restrict__ pinfo) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nnz;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t k = i >> 6;
@@ -292,8 +294,11 @@ k_pair_fill(const uint32_t *__restrict__ adj, const unsigned long long *__restri
     if (bit_of(pb, b)) {
       const unsigned long long ob = oflags[k];
       const uint32_t p = ppre[k] + lanes_below(pb);
-      ppar[p] = adj[i];
-      pedge[p] = bit_of(ob, b) ? owpre[k] + lanes_below(ob) : kBackEdge;
+      const uint32_t u = adj[i];
+      const bool fwd = bit_of(ob, b);
+      ppar[p] = u;
+      pedge[p] = fwd ? owpre[k] + lanes_below(ob) : kBackEdge;
+      pinfo[p] = make_uint2(ooff[u], odeg[u] | (fwd ? kPairFwd : 0u));
     }
   }
 }
@@ -513,12 +518,14 @@ int build_pairs(Graph *g) {
   }
   PAC_TRY(g->ppar.reserve(sizeof(uint32_t) * (np ? np : 1)));
   PAC_TRY(g->pedge.reserve(sizeof(uint32_t) * (np ? np : 1)));
+  PAC_TRY(g->pinfo.reserve(sizeof(uint2) * (np ? np : 1)));
   if (nnz) {
     hipLaunchKernelGGL(k_pair_fill, dim3(grid_for_n(nnz, 256 * 4)), dim3(256), 0, ctx->stream,
                        g->adj.as<const uint32_t>(), g->oflags.as<const unsigned long long>(),
                        g->owpre.as<const uint32_t>(), g->pflags.as<const unsigned long long>(),
-                       g->ppre.as<const uint32_t>(), (uint32_t)nnz, g->ppar.as<uint32_t>(),
-                       g->pedge.as<uint32_t>());
+                       g->ppre.as<const uint32_t>(), g->ooff.as<const uint32_t>(),
+                       g->odeg.as<const uint32_t>(), (uint32_t)nnz, g->ppar.as<uint32_t>(),
+                       g->pedge.as<uint32_t>(), g->pinfo.as<uint2>());
     PAC_CHECK_HIP(hipGetLastError());
   }
   g->pairs_built = true;

@@ -53,19 +53,19 @@ class GraphInfo(C.Structure):
 
     def algorithmic_bytes(self, mode: int = 1) -> int:
         """Bytes the hash path's intersection launch must move in `mode` (0 TC, 1 LCC,
-        2 Jaccard): every pair's streamed list (4 B per element) with its partner ID and list
-        bounds (12 B), every row's staged list once per work item holding it, the row's list
-        and pair bounds and its row-list entry (20 B), the items (32 B).  LCC adds each pair's
-        edge index (4 B) and the per-vertex counts read and written once (16 B per vertex);
-        Jaccard adds the edge index and the pair's first hit-word index (12 B per pair), one
-        hit bit per streamed element written, and the per-edge supports read and written
-        once (8 B per edge)."""
-        b = (4 * self.streamed_elems + 4 * self.staged_item_elems + 12 * self.num_pairs
+        2 Jaccard): every pair's streamed list (4 B per element) with the list's bounds stored
+        beside the pair (8 B), every row's staged list once per work item holding it, the
+        row's list and pair bounds and its row-list entry (20 B), the items (32 B).  LCC adds
+        each pair's partner ID (4 B) and the per-vertex counts read and written once (16 B
+        per vertex); Jaccard adds the partner ID, edge index and the pair's first hit-word
+        index (16 B per pair), one hit bit per streamed element written, and the per-edge
+        supports read and written once (8 B per edge)."""
+        b = (4 * self.streamed_elems + 4 * self.staged_item_elems + 8 * self.num_pairs
              + 20 * self.num_rows + 32 * self.num_items)
         if mode == 1:
             b += 4 * self.num_pairs + 16 * self.num_vertices
         elif mode == 2:
-            b += 12 * self.num_pairs + self.streamed_elems // 8 + 8 * self.num_edges
+            b += 16 * self.num_pairs + self.streamed_elems // 8 + 8 * self.num_edges
         return int(b)
 
 
