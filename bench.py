@@ -12,15 +12,18 @@ per-vertex counts), LCC epilogue -- the analogue of the reference's timed region
 
 The headline path is the batched persistent kernel replayed from a hipGraph with
 learned dispatch (the trained MLP in partitionandconquer_amd/dispatch_mlp.bin; the
-fixed heuristic when no model file exists).  The tile is autotuned for this mode and dispatch first.  The plain path (one launch per
+fixed heuristic when no model file exists).  The tile is autotuned for this mode and
+dispatch first.  `dispatch_eval` scores both dispatchers against per-partition timings of
+every strategy on this graph.  The plain path (one launch per
 partition group, one workgroup per row, fixed heuristic) is timed beside it for the
 speedup, and the fixed heuristic on the headline's own launch path for what the learned
 dispatch alone gains (`learned_vs_fixed_batched`).  GTEPS = m / t / 1e9 with m the
 undirected edge count.  The roofline entry prices the dominant kernel's algorithmic
 bytes -- what the hash path must move per launch: each pair's streamed list (the shorter
-of the two out-lists, 4 B an element) with its partner ID, edge index and list bounds,
-each row's staged list once per work item, the row and item metadata, and the per-vertex
-counts read and written once (GraphInfo.algorithmic_bytes) -- against 8 TB/s, over the
+of the two out-lists, 3 B an element in the packed 3-byte lists this graph's 2^24 IDs
+allow) with its partner ID and list bounds, each row's staged list once per work item,
+the row and item metadata, and the per-vertex counts read and written once
+(GraphInfo.algorithmic_bytes) -- against 8 TB/s, over the
 kernel's live duration from HIP events; `merge_equivalent_bytes` = 4*W, W = sum over
 oriented u->v of (d+(u) + d+(v)), is the input length of every oriented merge (both
 lists of every pair).  `traffic` is the kernel's measured HBM bytes per launch

@@ -674,6 +674,15 @@ int pac_ctx_set_timeout(pac_ctx *ctx, double ms) {
   return PAC_OK;
 }
 
+int pac_ctx_set_packed_lists(pac_ctx *ctx, int on) {
+  if (!ctx) {
+    set_error("null context");
+    return PAC_ERR_ARG;
+  }
+  ctx->c.packed_lists = on != 0;
+  return PAC_OK;
+}
+
 int pac_stream_copy(pac_ctx *ctx, uint64_t bytes, int reps, double *ms) {
   PAC_TRY(ensure_ctx(ctx));
   if (!ms) {
@@ -935,6 +944,7 @@ int pac_graph_get_info(const pac_graph *h, pac_graph_info *out) {
   out->streamed_elems = g.prepared ? g.streamed : 0;
   out->staged_elems = g.prepared ? g.staged : 0;
   out->staged_item_elems = g.prepared ? g.staged_items : 0;
+  out->stream_elem_bytes = g.ctx->packed_lists && g.V <= kPackMaxV ? 3 : 4;
   return PAC_OK;
 }
 

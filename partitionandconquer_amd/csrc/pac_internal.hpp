@@ -78,6 +78,13 @@ struct GraphDev {
   // per pair: the partner's out-list as (first position in oadj, length | kPairFwd when the
   // pair's edge leaves the owner) -- read with the pair, so its bounds need no gather
   const uint2 *pinfo;
+  // The lists the hash path streams and their bounds per pair: oadj with pinfo
+  // (selem = 4), or, for a graph whose IDs are all below 2^24, its 3-byte lists opk with
+  // pkinfo (selem = 3).  Pair p's list starts at byte 4 * sinfo[p].x of sadj (element i at
+  // + selem * i); sinfo[p].y = pinfo[p].y.
+  const uint8_t *sadj;
+  const uint2 *sinfo;
+  uint32_t selem;
   const uint32_t *rows;  // rows owning at least one pair, grouped by partition
   unsigned long long *total;  // T accumulator
   unsigned long long *tvert;  // LCC mode: per-vertex triangle counts, every credit added
@@ -127,6 +134,12 @@ struct Graph {
   DevBuf off, adj, deg;
   // oriented CSR
   DevBuf odeg, ooff, oadj;
+  // When every ID is below 2^24, built with the pairs: the oriented lists with 3-byte
+  // elements (little-endian), the list of vertex y from byte 4 ceil(3 (ooff[y] + y) / 4) --
+  // every list starts on a 4-byte boundary and ends before the next one starts, since each
+  // is given 3 bytes more than its length; and per pair, pkinfo = (4-byte word where the
+  // partner's 3-byte list starts, pinfo.y).  Empty otherwise.
+  DevBuf opk, pkinfo;
   // Intersection pairs.  Each oriented edge t->h stands for N+(t) & N+(h); the pair is
   // owned by the endpoint with the longer out-list (ties: the tail), which stages its
   // list in LDS while the partner's shorter list is streamed.  Pairs whose partner has
@@ -187,6 +200,7 @@ struct Ctx {
   unsigned long long *item_stamps = nullptr;
   pac_timing last{};
   double timeout_ms = 0;  // bound on waiting for a solve's completion (0: none)
+  bool packed_lists = true;  // stream 3-byte lists where a graph has them
   int tile = 1;
   void *noalloc_scratch = nullptr;  // this device's static scratch of pac_tc_noalloc
   DevBuf scratch;     // generic temporary storage (library-owned)
@@ -237,6 +251,7 @@ constexpr uint32_t kBackEdge = 0xFFFFFFFFu;
 // pinfo[p].y: the partner's out-degree, with this bit set when the pair's edge is owner ->
 // partner (out-degrees stay below 2^31).
 constexpr uint32_t kPairFwd = 0x80000000u;
+constexpr uint32_t kPackMaxV = 1u << 24;  // 3-byte lists: every ID below this
 // Fill pedge of the pairs owned by their head with their edge indices (once; the per-edge
 // supports need them).
 int build_back_edges(Graph *g);

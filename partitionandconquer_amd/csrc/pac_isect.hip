@@ -39,6 +39,7 @@
 // k_sup_big), where the sorted and HBM paths add one atomic per triangle instead.
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "pac_internal.hpp"
 
@@ -65,11 +66,13 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 struct GDev {
   const PAC_GLOBAL uint32_t *ooff, *oadj, *poff, *ppar, *pedge, *rows;
   const PAC_GLOBAL u32x2 *pinfo;
+  const PAC_GLOBAL uint8_t *sadj;  // the streamed lists, selem bytes per element
+  const PAC_GLOBAL u32x2 *sinfo;   // per pair: 4-byte word of sadj where its list starts
   PAC_GLOBAL unsigned long long *total, *tvert;
   PAC_GLOBAL uint32_t *sup;
   const PAC_GLOBAL unsigned long long *pword;
   PAC_GLOBAL unsigned long long *hitw;
-  uint32_t V, m, part_base;
+  uint32_t V, m, part_base, selem;
 };
 
 __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
@@ -80,6 +83,9 @@ __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
   g.ppar = (const PAC_GLOBAL uint32_t *)d.ppar;
   g.pedge = (const PAC_GLOBAL uint32_t *)d.pedge;
   g.pinfo = (const PAC_GLOBAL u32x2 *)d.pinfo;
+  g.sadj = (const PAC_GLOBAL uint8_t *)d.sadj;
+  g.sinfo = (const PAC_GLOBAL u32x2 *)d.sinfo;
+  g.selem = d.selem;
   g.rows = (const PAC_GLOBAL uint32_t *)d.rows;
   g.total = (PAC_GLOBAL unsigned long long *)d.total;
   g.tvert = (PAC_GLOBAL unsigned long long *)d.tvert;
@@ -471,7 +477,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   unsigned long long wl = 0;  // SUP mode: the pair's first hit word
   auto load_pair = [&]() {
     if (kl < c_hi) {
-      const u32x2 pi = G.pinfo[p0 + kl];
+      const u32x2 pi = G.sinfo[p0 + kl];
       bl = pi.x;
       nl = pi.y & ~kPairFwd;
       fl = (pi.y & kPairFwd) != 0;
@@ -583,47 +589,62 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // Blocks of up to 4 x 64 elements of one pair's list N+(y), in pair order; the loads
   // of the next block are issued before the current one is probed.  The loads go through
   // a buffer resource that starts at the block and ends with N+(y): the per-lane offsets
-  // are loop-invariant (no vector register is rewritten between the pipelined loads, no
-  // per-element address arithmetic; the four sub-blocks are immediate offsets), and lanes
-  // past the list's end read 0 from the hardware's range check -- their lookups are
-  // masked.  The two register sets swap roles every step, so the waits cover only the
-  // block being probed.  (Unconditional loads: the wait for a block then counts exactly
-  // the loads issued after it, which keeps the next block's loads in flight.)
-  // The stream position is a scalar cursor: the block's address and the bytes left in the
-  // pair's list from there.
-  auto load_blk = [&](unsigned long long ptr, uint32_t nbytes, uint32_t *w) {
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(uintptr_t)ptr, (short)0, (int)nbytes, kRawBufferFlags);
+  // are fixed for the pair (no vector register is rewritten between the pipelined loads,
+  // no per-element address arithmetic; the four sub-blocks are immediate offsets), and
+  // lanes past the list's end read 0 from the hardware's range check or, with 3-byte
+  // elements, bytes that follow the list -- either way their lookups are masked.  The
+  // two register sets swap roles every step, so the waits cover only the block being
+  // probed.  (Unconditional loads: the wait for a block then counts exactly the loads
+  // issued after it, which keeps the next block's loads in flight.)  The stream position
+  // is a scalar cursor: the block's byte address and the bytes left in the pair's list
+  // from there.
+  //   ES = 4: lane l reads element 64u + l of sub-block u with one 4-byte load.
+  //   ES = 3: each list starts on a 4-byte boundary; element 64u + l lies in bytes
+  // 192u + 3l .. +2, within the aligned word 192u + 4 floor(3l / 4) and, for l = 1, 2
+  // (mod 4), the next word, which is the word lane l + 1 reads: lane l loads its word and
+  // takes lane l + 1's with a lane shift inside the 16-lane row (lanes 15 (mod 16) never
+  // need it), then picks the element's three bytes with one byte permute.  Aligned loads
+  // throughout: unaligned 4-byte loads of the elements themselves measured far slower.
+  // The range covers the list rounded up to whole words.
+  auto stream_batch = [&](auto es_c, uint32_t nvalid, uint32_t &phl) {
+    constexpr uint32_t ES = decltype(es_c)::value;
+    constexpr uint32_t SSTEP = 64u * ES, BLK = 4u * SSTEP;  // sub-block, block (bytes)
+    // ES = 3: the lane's word in a sub-block and the permute that picks its element
+    // (bytes s, s+1, s+2 of the word pair, s = 3l mod 4, and a zero top byte)
+    const uint32_t lw = ES == 3u ? 4u * (3u * lane / 4u) : 4u * lane;
+    const uint32_t sel = 0x0C020100u + ((3u * lane) & 3u) * 0x00010101u;
+    auto load_blk = [&](unsigned long long p, uint32_t nbytes, uint32_t *w) {
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(uintptr_t)p, (short)0, (int)(ES == 3u ? (nbytes + 3u) & ~3u : nbytes),
+          kRawBufferFlags);
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u)
-      w[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4u + 256u * u, 0, 0);
-  };
-  for (uint32_t t0 = 0;; t0 += 64) {
-    if (t0 != 0) {
-      kl = c_lo + wave + PS * (t0 + lane);
-      yl = el = bl = nl = 0;
-      fl = false;
-      wl = 0;
-      load_pair();
-    }
-    const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
-    if (nvalid == 0) break;
+      for (uint32_t u = 0; u < 4; ++u)
+        w[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, lw + SSTEP * u, 0, 0);
+    };
+    auto elem = [&](const uint32_t *w, uint32_t u) -> uint32_t {
+      if constexpr (ES == 4u) {
+        return w[u];
+      } else {
+        // row_shl:1 -- lane l reads lane l + 1 of its row (lane 15 of a row reads 0)
+        const uint32_t nx = (uint32_t)__builtin_amdgcn_mov_dpp((int)w[u], 0x101, 0xF, 0xF, true);
+        return __builtin_amdgcn_perm(nx, w[u], sel);
+      }
+    };
     // per pair (lane): the address of N+(y) and its length in bytes
-    const unsigned long long pa = (unsigned long long)(uintptr_t)(G.oadj + bl);
-    const uint32_t pn = nl * 4u;
+    const unsigned long long pa = (unsigned long long)(uintptr_t)G.sadj + 4ull * bl;
+    const uint32_t pn = nl * ES;
     uint32_t jj = 0, nrem = readlane(pn, 0), ph = 0;
-    uint32_t phl = 0;  // lane j: the hit count of the batch's pair j, once it is done
     unsigned long long ptr = readlane64(pa, 0);
     unsigned long long hb = MODE == kModeSUP ? readlane64(wl, 0) : 0ull;
-    uint32_t wa[4], wb[4];
-    load_blk(ptr, nrem, wa);
+    uint32_t sa[4], sb[4];
+    load_blk(ptr, nrem, sa);
     auto step = [&](const uint32_t *cur, uint32_t *nxt) {
       const uint32_t rem = nrem;  // bytes left in the pair's list (0 if it is empty)
       // The next block: the pair's next 256 elements, or the first block of the next pair
       // (past the batch's last pair, that pair's first block again: in bounds, unused).
-      uint32_t jn = jj, nremn = rem - 1024u;
-      unsigned long long ptrn = ptr + 1024u, hbn = hb + 4u;
-      if (rem <= 1024u) {  // wave-uniform
+      uint32_t jn = jj, nremn = rem - BLK;
+      unsigned long long ptrn = ptr + BLK, hbn = hb + 4u;
+      if (rem <= BLK) {  // wave-uniform
         jn = jj + 1u;
         const uint32_t jl = min(jn, nvalid - 1u);
         ptrn = readlane64(pa, jl);
@@ -634,9 +655,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       unsigned long long hw = 0;  // SUP mode: this lane's hit word (lanes 0..3)
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
-        if (256u * u >= rem) break;  // wave-uniform
-        const bool live = lane * 4u + 256u * u < rem;  // lanes past the end read 0
-        const Probe q = probe(cur[u]);
+        if (SSTEP * u >= rem) break;  // wave-uniform
+        const bool live = lane * ES + SSTEP * u < rem;  // lanes past the end are masked
+        const Probe q = probe(elem(cur, u));
         const unsigned long long bal = (__ballot(q.e10) | __ballot(q.e11) | __ballot(q.e20) |
                                         __ballot(q.e21)) & __ballot(live);
         if (MODE == kModeTC) {
@@ -652,7 +673,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       }
       if (MODE == kModeSUP) {
         // edges y->w: one hit word per 64 streamed elements, summed after the launch
-        const uint32_t nsub = min(4u, (rem + 255u) >> 8);
+        const uint32_t nsub = min(4u, (rem + SSTEP - 1u) / SSTEP);
         if (lane < nsub) G.hitw[hb + lane] = hw;
       }
       if (MODE != kModeTC && jn != jj) {
@@ -671,9 +692,25 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     // loop's scalar state on every trip.  (The exit test reads jj through readfirstlane so
     // that it is not folded into the guard of the second step.)
     do {  // wave-uniform
-      step(wa, wb);
-      if (jj < nvalid) step(wb, wa);
+      step(sa, sb);
+      if (jj < nvalid) step(sb, sa);
     } while ((uint32_t)__builtin_amdgcn_readfirstlane((int)jj) < nvalid);
+  };
+  for (uint32_t t0 = 0;; t0 += 64) {
+    if (t0 != 0) {
+      kl = c_lo + wave + PS * (t0 + lane);
+      yl = el = bl = nl = 0;
+      fl = false;
+      wl = 0;
+      load_pair();
+    }
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
+    if (nvalid == 0) break;
+    uint32_t phl = 0;  // lane j: the hit count of the batch's pair j, once it is done
+    if (G.selem == 3u)  // uniform
+      stream_batch(std::integral_constant<uint32_t, 3u>(), nvalid, phl);
+    else
+      stream_batch(std::integral_constant<uint32_t, 4u>(), nvalid, phl);
     if (MODE != kModeTC && phl != 0) {
       // The batch's pair credits, one lane per pair, all at once (one table lookup per
       // lane rather than one wave-wide lookup per pair).  t(y) / the pair's edge: r->y adds
@@ -1493,6 +1530,11 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   d.ppar = g->ppar.as<const uint32_t>();
   d.pedge = g->pedge.as<const uint32_t>();
   d.pinfo = g->pinfo.as<const uint2>();
+  // the 3-byte lists where the graph has them and the context streams them
+  const bool packed = g->ctx->packed_lists && g->V <= kPackMaxV && g->opk.p != nullptr;
+  d.sadj = packed ? g->opk.as<const uint8_t>() : g->oadj.as<const uint8_t>();
+  d.sinfo = packed ? g->pkinfo.as<const uint2>() : g->pinfo.as<const uint2>();
+  d.selem = packed ? 3u : 4u;
   d.rows = g->rows.as<const uint32_t>();
   d.total = g->total.as<unsigned long long>();
   d.tvert = g->tvert.as<unsigned long long>();

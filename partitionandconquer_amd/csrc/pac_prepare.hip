@@ -284,8 +284,8 @@ __global__ void __launch_bounds__(256)
 k_pair_fill(const uint32_t *__restrict__ adj, const unsigned long long *__restrict__ oflags,
             const uint32_t *__restrict__ owpre, const unsigned long long *__restrict__ pflags,
             const uint32_t *__restrict__ ppre, const uint32_t *__restrict__ ooff,
-            const uint32_t *__restrict__ odeg, uint32_t nnz, uint32_t *__restrict__ ppar,
-            uint32_t *__restrict__ pedge, uint2 *__restrict__ pinfo) {
+            uint32_t nnz, uint32_t *__restrict__ ppar, uint32_t *__restrict__ pedge,
+            uint2 *__restrict__ pinfo) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nnz;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t k = i >> 6;
@@ -298,8 +298,47 @@ k_pair_fill(const uint32_t *__restrict__ adj, const unsigned long long *__restri
       const bool fwd = bit_of(ob, b);
       ppar[p] = u;
       pedge[p] = fwd ? owpre[k] + lanes_below(ob) : kBackEdge;
-      pinfo[p] = make_uint2(ooff[u], odeg[u] | (fwd ? kPairFwd : 0u));
+      const uint32_t a = ooff[u];  // both bounds from one array: mostly one line per pair
+      pinfo[p] = make_uint2(a, (ooff[u + 1] - a) | (fwd ? kPairFwd : 0u));
     }
+  }
+}
+
+// The 3-byte lists: the list of vertex y starts at 4-byte word pack_word(ooff[y], y).
+__device__ __forceinline__ uint64_t pack_word(uint32_t o, uint32_t y) {
+  return (3 * ((uint64_t)o + y) + 3) / 4;
+}
+
+// One wave per row y: lane k writes word k of y's 3-byte list (its bytes 4k .. 4k+3 hold
+// parts of elements floor(4k/3) and the next), the list's last word zero-padded.
+__global__ void __launch_bounds__(256)
+k_pack24(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj, uint32_t V,
+         uint32_t *__restrict__ opk) {
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint64_t y = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; y < V;
+       y += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
+    const uint32_t o = ooff[y], n = ooff[y + 1] - o;
+    uint32_t *const out = opk + pack_word(o, (uint32_t)y);
+    for (uint32_t k = lane; k < (3 * n + 3) / 4; k += 64) {
+      uint32_t w = 0;
+#pragma unroll
+      for (uint32_t b = 0; b < 4; ++b) {
+        const uint32_t e = (4 * k + b) / 3, sh = (4 * k + b) % 3;  // element, its byte
+        if (e < n) w |= ((oadj[o + e] >> (8 * sh)) & 0xFFu) << (8 * b);
+      }
+      out[k] = w;
+    }
+  }
+}
+
+// Per pair: where the partner's 3-byte list starts (a 4-byte word of opk), with pinfo.y.
+__global__ void __launch_bounds__(256)
+k_pack_info(const uint32_t *__restrict__ ppar, const uint2 *__restrict__ pinfo, uint32_t np,
+            uint2 *__restrict__ pkinfo) {
+  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < np;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 pi = pinfo[p];
+    pkinfo[p] = make_uint2((uint32_t)pack_word(pi.x, ppar[p]), pi.y);
   }
 }
 
@@ -524,8 +563,24 @@ int build_pairs(Graph *g) {
                        g->adj.as<const uint32_t>(), g->oflags.as<const unsigned long long>(),
                        g->owpre.as<const uint32_t>(), g->pflags.as<const unsigned long long>(),
                        g->ppre.as<const uint32_t>(), g->ooff.as<const uint32_t>(),
-                       g->odeg.as<const uint32_t>(), (uint32_t)nnz, g->ppar.as<uint32_t>(),
-                       g->pedge.as<uint32_t>(), g->pinfo.as<uint2>());
+                       (uint32_t)nnz, g->ppar.as<uint32_t>(), g->pedge.as<uint32_t>(),
+                       g->pinfo.as<uint2>());
+    PAC_CHECK_HIP(hipGetLastError());
+  }
+  if (V <= kPackMaxV) {
+    // the lists' words, + 4 words past the end (the last list's word rounding)
+    const uint64_t nw = (3 * (g->m + V) + 3) / 4 + 4;
+    PAC_TRY(g->opk.reserve(4 * nw));
+    PAC_TRY(g->pkinfo.reserve(sizeof(uint2) * (np ? np : 1)));
+    PAC_TRY(fill_u32(ctx->stream, g->opk.as<uint32_t>(), nw, 0u));
+    if (V)
+      hipLaunchKernelGGL(k_pack24, dim3(grid_for_n(V, 4)), dim3(256), 0, ctx->stream,
+                         g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(), V,
+                         g->opk.as<uint32_t>());
+    if (np)
+      hipLaunchKernelGGL(k_pack_info, dim3(grid_for_n(np, 256 * 4)), dim3(256), 0,
+                         ctx->stream, g->ppar.as<const uint32_t>(), g->pinfo.as<const uint2>(),
+                         np, g->pkinfo.as<uint2>());
     PAC_CHECK_HIP(hipGetLastError());
   }
   g->pairs_built = true;

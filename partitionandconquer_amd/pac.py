@@ -49,18 +49,21 @@ class GraphInfo(C.Structure):
                 ("num_partitions", C.c_uint64), ("num_items", C.c_uint64),
                 ("prepare_ms", C.c_double), ("num_pairs", C.c_uint64),
                 ("num_rows", C.c_uint64), ("streamed_elems", C.c_uint64),
-                ("staged_elems", C.c_uint64), ("staged_item_elems", C.c_uint64)]
+                ("staged_elems", C.c_uint64), ("staged_item_elems", C.c_uint64),
+                ("stream_elem_bytes", C.c_uint64)]
 
     def algorithmic_bytes(self, mode: int = 1) -> int:
         """Bytes the hash path's intersection launch must move in `mode` (0 TC, 1 LCC,
-        2 Jaccard): every pair's streamed list (4 B per element) with the list's bounds stored
+        2 Jaccard): every pair's streamed list (stream_elem_bytes per element: 3 with packed
+        lists, else 4) with the list's bounds stored
         beside the pair (8 B), every row's staged list once per work item holding it, the
         row's list and pair bounds and its row-list entry (20 B), the items (32 B).  LCC adds
         each pair's partner ID (4 B) and the per-vertex counts read and written once (16 B
         per vertex); Jaccard adds the partner ID, edge index and the pair's first hit-word
         index (16 B per pair), one hit bit per streamed element written, and the per-edge
         supports read and written once (8 B per edge)."""
-        b = (4 * self.streamed_elems + 4 * self.staged_item_elems + 8 * self.num_pairs
+        b = (self.stream_elem_bytes * self.streamed_elems + 4 * self.staged_item_elems
+             + 8 * self.num_pairs
              + 20 * self.num_rows + 32 * self.num_items)
         if mode == 1:
             b += 4 * self.num_pairs + 16 * self.num_vertices
@@ -115,6 +118,7 @@ def lib():
         "pac_tc": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
         "pac_tc_noalloc": ([vp, C.c_int, vp, vp], C.c_int),
         "pac_ctx_set_timeout": ([vp, C.c_double], C.c_int),
+        "pac_ctx_set_packed_lists": ([vp, C.c_int], C.c_int),
         "pac_lcc": ([vp, C.c_int, C.c_int, vp, vp, vp], C.c_int),
         "pac_jaccard_edges": ([vp, C.c_int, C.c_int, vp, vp, vp, vp, vp], C.c_int),
         "pac_jaccard_queries": ([vp, vp, C.c_uint32, vp], C.c_int),
@@ -215,6 +219,11 @@ class Context:
     def set_timeout(self, ms: float):
         """Bound (ms, 0 = none) on waiting for a solve; PacError -8 when it passes."""
         _check(lib().pac_ctx_set_timeout(self._h, ms))
+
+    def set_packed_lists(self, on: bool):
+        """Stream 3-byte lists on graphs with every ID below 2^24 (the default) or the
+        4-byte lists; applies from the next launch."""
+        _check(lib().pac_ctx_set_packed_lists(self._h, 1 if on else 0))
 
     def stream_copy(self, nbytes: int, reps: int = 5) -> float:
         """Median time (ms) of a plain device-to-device streaming copy of nbytes."""

@@ -541,3 +541,82 @@ def test_greedy_partition_2_matches_oracle(ctx, orc):
         assert np.array_equal(got, orc.greedy_partition_2(ref, P, seed=P)), P
         g.prepare(P, got)
         assert g.tc(pac.DISPATCH_FIXED) == T
+
+
+# 3-byte IDs with every byte pattern that decoding could get wrong (top byte 0xFF, zero
+# middle bytes, bytes that straddle a 4-byte word in the packed copy)
+WIDE_IDS = [0, 1, 0xFF, 0x100, 0xFFFF, 0x10000, 0x00FFFF, 0xFF00FF, 0x7FFFFF, 0x800000,
+            0xFFFF00, 0xFFFFFE, 0xFFFFFF]
+
+
+def _wide_id_edges(V, seed):
+    """Edges among 900 IDs spread over [0, V), the largest three and WIDE_IDS among them:
+    G(900, 0.05), a 600-vertex clique (out-lists of up to 599 elements, so streamed lists
+    span several 256-element blocks) and three hubs adjacent to all."""
+    rng = np.random.default_rng(seed)
+    ids = np.unique(np.concatenate([rng.choice(V - 3, 900 - len(WIDE_IDS) - 3, replace=False),
+                                    [x for x in WIDE_IDS if x < V], [V - 3, V - 2, V - 1]]))
+    n = len(ids)
+    a = rng.random((n, n)) < 0.05
+    cl = rng.choice(n, 600, replace=False)
+    a[np.ix_(cl, cl)] = True
+    a[rng.choice(n, 3, replace=False), :] = True
+    iu = np.triu_indices(n, 1)
+    sel = a[iu] | a.T[iu]
+    return ids[iu[0][sel]].astype(np.int32), ids[iu[1][sel]].astype(np.int32)
+
+
+@pytest.mark.parametrize("V", [1 << 24, (1 << 24) + 1])
+def test_packed_lists_wide_ids(ctx, orc, V):
+    """The hash path streams 3-byte lists when every ID is below 2^24 (V = 2^24: IDs up to
+    0xFFFFFF) and 4-byte lists otherwise (V = 2^24 + 1) or when packing is off: per-vertex
+    counts on every launch path equal the oracle's, and per-edge Jaccard equals the
+    oracle's on every edge, with packing on and off."""
+    src, dst = _wide_id_edges(V, seed=V & 0xFF)
+    g = pac.Graph.from_edges(ctx, src, dst, V)
+    ref = orc.CSR(*g.csr())
+    T = orc.tc_merge(ref, NT)
+    tv = orc.per_vertex_triangles(ref, NT)
+    g.prepare(8)
+    res = {}
+    try:
+        for packed in (True, False):
+            ctx.set_packed_lists(packed)
+            assert g.info().stream_elem_bytes == (3 if packed and V <= 1 << 24 else 4)
+            for d in (pac.FORCE_BITMAP, pac.DISPATCH_FIXED):
+                for l in LAUNCHES:
+                    t, pv = g.tc(d, l, per_vertex=True)
+                    assert t == T and np.array_equal(pv, tv), (packed, d, l)
+            res[packed] = g.jaccard_edges(pac.FORCE_BITMAP, pac.LAUNCH_GRAPH)
+    finally:
+        ctx.set_packed_lists(True)
+    for a, b in zip(res[True], res[False]):
+        assert bits_equal(a, b)
+    es, ed, inter, uni, ratio = res[True]
+    ri, ru, rr = orc.jaccard_edges(ref, es, ed, NT)
+    assert np.array_equal(ri, inter) and np.array_equal(ru, uni) and bits_equal(rr, ratio)
+    assert int(inter.sum(dtype=np.uint64)) == 3 * T
+    g.free()
+
+
+def test_packed_lists_rmat_all_modes(ctx, orc):
+    """R-MAT scale 15: TC, LCC and per-edge supports from the 3-byte and the 4-byte
+    streamed lists are identical, under fixed and forced-hash dispatch."""
+    g = pac.Graph.rmat(ctx, 15, 16, 0.57, 0.19, 0.19, seed=15)
+    g.prepare(32)
+    out = {}
+    try:
+        for packed in (True, False):
+            ctx.set_packed_lists(packed)
+            r = []
+            for d in (pac.FORCE_BITMAP, pac.DISPATCH_FIXED):
+                r.append(g.tc(d, pac.LAUNCH_PERSISTENT, per_vertex=True)[1])
+                r.extend(g.lcc(d, pac.LAUNCH_GRAPH)[:2])
+                r.extend(g.jaccard_edges(d, pac.LAUNCH_PERSISTENT)[2:])
+            out[packed] = r
+    finally:
+        ctx.set_packed_lists(True)
+    for a, b in zip(out[True], out[False]):
+        assert bits_equal(a, b)
+    T = orc.tc_merge(orc.CSR(*g.csr()), NT)
+    assert int(out[True][0].sum()) == 3 * T
