@@ -78,21 +78,23 @@ typedef struct {
    * t->h is one pair, owned by the endpoint with the longer out-list (ties: the tail),
    * which stages that list while the partner's shorter list is streamed; pairs whose
    * shorter list is empty are dropped.  These are the work model of the hash path:
-   * its algorithmic bytes per launch are stream_elem_bytes*streamed + 4*staged_item_elems +
+   * its algorithmic bytes per launch are S*streamed + 4*staged_item_elems +
    * 8*num_pairs (each pair's list bounds) + 20*num_rows + 32*num_items (TC), plus
    * 4*num_pairs (partner IDs) and 16*num_vertices (per-vertex counts read and written
    * once) with per-vertex counts, or 16*num_pairs (partner, edge and hit-word indices) +
    * streamed/8 (one hit bit per streamed element) + 8*num_edges (supports read and written
-   * once) for per-edge supports. */
+   * once) for per-edge supports; S = stream_elem_bytes, 4 for per-edge supports. */
   uint64_t num_pairs;
   uint64_t num_rows;          /* vertices owning at least one pair                     */
   uint64_t streamed_elems;    /* sum over pairs of d+(partner) = sum of min(d+(t),d+(h)) */
   uint64_t staged_elems;      /* sum of d+(x) over the owning vertices                 */
   uint64_t staged_item_elems; /* the same, once per work item holding the row (a row  *
                                * split over several items is staged by each)          */
-  uint64_t stream_elem_bytes; /* bytes per streamed element on the hash path: 3 when   *
-                               * the context streams packed lists and every ID is     *
-                               * below 2^24 (pac_ctx_set_packed_lists), else 4        */
+  uint64_t stream_elem_bytes; /* bytes per streamed element on the hash path of TC and *
+                               * per-vertex counts: 3 when the context streams packed *
+                               * lists and every ID is below 2^24                     *
+                               * (pac_ctx_set_packed_lists), else 4; per-edge         *
+                               * supports always stream 4-byte lists                  */
 } pac_graph_info;
 
 typedef struct {
@@ -127,11 +129,12 @@ int pac_ctx_num_cus(const pac_ctx *ctx);
  * calls on the context wait behind them.  ms = 0 (the default) waits without bound. */
 int pac_ctx_set_timeout(pac_ctx *ctx, double ms);
 /* No reference counterpart (the reference's lists are int arrays).  With on != 0 (the
- * default) the hash path streams the partner lists of graphs whose IDs are all below 2^24
- * from a copy of the oriented lists with 3-byte elements, built by pac_graph_prepare
- * (0.75 bytes per undirected edge more device memory): a quarter fewer streamed bytes,
- * the same results.  With on == 0 it streams the 4-byte lists.  Applies from the next
- * launch, hipGraph replays included. */
+ * default) the hash path of TC and per-vertex counts (LCC) streams the partner lists of
+ * graphs whose IDs are all below 2^24 from a copy of the oriented lists with 3-byte
+ * elements, built by pac_graph_prepare (about 0.75 bytes per undirected edge and 8 bytes
+ * per intersection pair more device memory): a quarter fewer streamed bytes, the same
+ * results.  With on == 0 it streams the 4-byte lists, as per-edge supports (Jaccard)
+ * always do.  Applies from the next launch, hipGraph replays included. */
 int pac_ctx_set_packed_lists(pac_ctx *ctx, int on);
 /* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming kernel,
  * `reps` times; *ms receives the median kernel time.  Used to measure the achievable

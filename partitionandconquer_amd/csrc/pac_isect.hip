@@ -707,7 +707,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
     if (nvalid == 0) break;
     uint32_t phl = 0;  // lane j: the hit count of the batch's pair j, once it is done
-    if (G.selem == 3u)  // uniform
+    if (MODE != kModeSUP && G.selem == 3u)  // uniform
       stream_batch(std::integral_constant<uint32_t, 3u>(), nvalid, phl);
     else
       stream_batch(std::integral_constant<uint32_t, 4u>(), nvalid, phl);
@@ -1522,7 +1522,7 @@ k_sup_big(const uint32_t *__restrict__ units, const uint32_t *__restrict__ ooff,
 }
 
 // Device descriptor of a graph; its output pointers must already be allocated.
-GraphDev make_desc(Graph *g, uint32_t part_base) {
+GraphDev make_desc(Graph *g, uint32_t part_base, int mode) {
   GraphDev d;
   d.ooff = g->ooff.as<const uint32_t>();
   d.oadj = g->oadj.as<const uint32_t>();
@@ -1531,7 +1531,8 @@ GraphDev make_desc(Graph *g, uint32_t part_base) {
   d.pedge = g->pedge.as<const uint32_t>();
   d.pinfo = g->pinfo.as<const uint2>();
   // the 3-byte lists where the graph has them and the context streams them
-  const bool packed = g->ctx->packed_lists && g->V <= kPackMaxV && g->opk.p != nullptr;
+  const bool packed = mode != kModeSUP && g->ctx->packed_lists && g->V <= kPackMaxV &&
+                      g->opk.p != nullptr;
   d.sadj = packed ? g->opk.as<const uint8_t>() : g->oadj.as<const uint8_t>();
   d.sinfo = packed ? g->pkinfo.as<const uint2>() : g->pinfo.as<const uint2>();
   d.selem = packed ? 3u : 4u;
@@ -1629,7 +1630,7 @@ int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
                  uint64_t *launches) {
   Graph *gs[1] = {g};
   PAC_TRY(alloc_outputs(gs, 1, mode));
-  const GraphDev d = make_desc(g, 0);
+  const GraphDev d = make_desc(g, 0, mode);
   uint32_t k = 0;
   while (k < g->P) {
     uint32_t e = k + 1;
@@ -1655,7 +1656,7 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   bool any_long = false;
   const uint32_t cap = (uint32_t)kTiles[ctx->tile].cap;
   for (uint32_t i = 0; i < G; ++i) {
-    descs[i] = make_desc(gs[i], (uint32_t)nparts);
+    descs[i] = make_desc(gs[i], (uint32_t)nparts, mode);
     nparts += gs[i]->P;
     nitems += gs[i]->h_items.size();
     // rows over the tile's capacity, each queued at most once per item that holds it
@@ -1810,7 +1811,7 @@ int launch_persistent_noalloc(Ctx *ctx, Graph *g, int mode, const uint8_t *d_str
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
                        d_tvert, (uint64_t)g->V);
   PAC_CHECK_HIP(hipGetLastError());
-  GraphDev d = make_desc(g, 0);
+  GraphDev d = make_desc(g, 0, mode);
   d.total = d_total;
   d.tvert = mode == kModeLCC ? d_tvert : nullptr;
   d.sup = nullptr;

@@ -54,15 +54,16 @@ class GraphInfo(C.Structure):
 
     def algorithmic_bytes(self, mode: int = 1) -> int:
         """Bytes the hash path's intersection launch must move in `mode` (0 TC, 1 LCC,
-        2 Jaccard): every pair's streamed list (stream_elem_bytes per element: 3 with packed
-        lists, else 4) with the list's bounds stored
+        2 Jaccard): every pair's streamed list (stream_elem_bytes per element in TC and LCC:
+        3 with packed lists, else 4; 4 in Jaccard) with the list's bounds stored
         beside the pair (8 B), every row's staged list once per work item holding it, the
         row's list and pair bounds and its row-list entry (20 B), the items (32 B).  LCC adds
         each pair's partner ID (4 B) and the per-vertex counts read and written once (16 B
         per vertex); Jaccard adds the partner ID, edge index and the pair's first hit-word
         index (16 B per pair), one hit bit per streamed element written, and the per-edge
         supports read and written once (8 B per edge)."""
-        b = (self.stream_elem_bytes * self.streamed_elems + 4 * self.staged_item_elems
+        es = 4 if mode == 2 else self.stream_elem_bytes  # per-edge supports: 4-byte lists
+        b = (es * self.streamed_elems + 4 * self.staged_item_elems
              + 8 * self.num_pairs
              + 20 * self.num_rows + 32 * self.num_items)
         if mode == 1:
@@ -221,8 +222,8 @@ class Context:
         _check(lib().pac_ctx_set_timeout(self._h, ms))
 
     def set_packed_lists(self, on: bool):
-        """Stream 3-byte lists on graphs with every ID below 2^24 (the default) or the
-        4-byte lists; applies from the next launch."""
+        """TC and LCC stream 3-byte lists on graphs with every ID below 2^24 (the default)
+        or the 4-byte lists; applies from the next launch."""
         _check(lib().pac_ctx_set_packed_lists(self._h, 1 if on else 0))
 
     def stream_copy(self, nbytes: int, reps: int = 5) -> float:

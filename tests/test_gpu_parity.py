@@ -568,10 +568,10 @@ def _wide_id_edges(V, seed):
 
 @pytest.mark.parametrize("V", [1 << 24, (1 << 24) + 1])
 def test_packed_lists_wide_ids(ctx, orc, V):
-    """The hash path streams 3-byte lists when every ID is below 2^24 (V = 2^24: IDs up to
+    """TC and LCC stream 3-byte lists when every ID is below 2^24 (V = 2^24: IDs up to
     0xFFFFFF) and 4-byte lists otherwise (V = 2^24 + 1) or when packing is off: per-vertex
-    counts on every launch path equal the oracle's, and per-edge Jaccard equals the
-    oracle's on every edge, with packing on and off."""
+    counts on every launch path equal the oracle's with packing on and off; per-edge
+    Jaccard (always 4-byte lists) equals the oracle's on every edge either way."""
     src, dst = _wide_id_edges(V, seed=V & 0xFF)
     g = pac.Graph.from_edges(ctx, src, dst, V)
     ref = orc.CSR(*g.csr())
@@ -600,8 +600,9 @@ def test_packed_lists_wide_ids(ctx, orc, V):
 
 
 def test_packed_lists_rmat_all_modes(ctx, orc):
-    """R-MAT scale 15: TC, LCC and per-edge supports from the 3-byte and the 4-byte
-    streamed lists are identical, under fixed and forced-hash dispatch."""
+    """R-MAT scale 15: TC and LCC from the 3-byte and the 4-byte streamed lists are
+    identical, under fixed and forced-hash dispatch, and per-edge supports do not change
+    with the setting."""
     g = pac.Graph.rmat(ctx, 15, 16, 0.57, 0.19, 0.19, seed=15)
     g.prepare(32)
     out = {}
