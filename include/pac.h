@@ -91,9 +91,8 @@ typedef struct {
   uint64_t staged_item_elems; /* the same, once per work item holding the row (a row  *
                                * split over several items is staged by each)          */
   uint64_t stream_elem_bytes; /* bytes per streamed element on the hash path of TC and *
-                               * per-vertex counts: 3 when the context streams packed *
-                               * lists and every ID is below 2^24                     *
-                               * (pac_ctx_set_packed_lists), else 4; per-edge         *
+                               * per-vertex counts when this graph is solved alone:   *
+                               * 3 or 4 as pac_ctx_set_packed_lists decides; per-edge *
                                * supports always stream 4-byte lists                  */
 } pac_graph_info;
 
@@ -128,14 +127,18 @@ int pac_ctx_num_cus(const pac_ctx *ctx);
  * kernels keep running on the context's stream (a launch cannot be withdrawn), so later
  * calls on the context wait behind them.  ms = 0 (the default) waits without bound. */
 int pac_ctx_set_timeout(pac_ctx *ctx, double ms);
-/* No reference counterpart (the reference's lists are int arrays).  With on != 0 (the
- * default) the hash path of TC and per-vertex counts (LCC) streams the partner lists of
- * graphs whose IDs are all below 2^24 from a copy of the oriented lists with 3-byte
- * elements, built by pac_graph_prepare (about 0.75 bytes per undirected edge and 8 bytes
- * per intersection pair more device memory): a quarter fewer streamed bytes, the same
- * results.  With on == 0 it streams the 4-byte lists, as per-edge supports (Jaccard)
- * always do.  Applies from the next launch, hipGraph replays included. */
-int pac_ctx_set_packed_lists(pac_ctx *ctx, int on);
+/* No reference counterpart (the reference's lists are int arrays).  pac_graph_prepare gives
+ * every graph whose IDs are all below 2^24 a copy of its oriented lists with 3-byte
+ * elements (about 0.75 bytes per undirected edge and 8 bytes per intersection pair more
+ * device memory).  The hash path of TC and per-vertex counts (LCC) streams the partner
+ * lists from them -- a quarter fewer streamed bytes, the same results -- when every graph
+ * of the launch has them and the mode allows: PAC_PACKED_AUTO (the default) when the
+ * launch's largest graph has at least 2^24 undirected edges (smaller graphs' lists stay
+ * in the caches, where unpacking costs more than the bytes save), PAC_PACKED_ALWAYS
+ * whatever the size, PAC_PACKED_OFF never.  Per-edge supports (Jaccard) always stream the
+ * 4-byte lists.  Applies from the next launch, hipGraph replays included. */
+enum { PAC_PACKED_OFF = 0, PAC_PACKED_AUTO = 1, PAC_PACKED_ALWAYS = 2 };
+int pac_ctx_set_packed_lists(pac_ctx *ctx, int mode);
 /* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming kernel,
  * `reps` times; *ms receives the median kernel time.  Used to measure the achievable
  * HBM bandwidth and to calibrate hardware byte counters.  No reference counterpart. */

@@ -674,12 +674,12 @@ int pac_ctx_set_timeout(pac_ctx *ctx, double ms) {
   return PAC_OK;
 }
 
-int pac_ctx_set_packed_lists(pac_ctx *ctx, int on) {
-  if (!ctx) {
-    set_error("null context");
+int pac_ctx_set_packed_lists(pac_ctx *ctx, int mode) {
+  if (!ctx || mode < PAC_PACKED_OFF || mode > PAC_PACKED_ALWAYS) {
+    set_error("need a context and a mode of PAC_PACKED_OFF, _AUTO or _ALWAYS");
     return PAC_ERR_ARG;
   }
-  ctx->c.packed_lists = on != 0;
+  ctx->c.packed_lists = mode;
   return PAC_OK;
 }
 
@@ -944,7 +944,8 @@ int pac_graph_get_info(const pac_graph *h, pac_graph_info *out) {
   out->streamed_elems = g.prepared ? g.streamed : 0;
   out->staged_elems = g.prepared ? g.staged : 0;
   out->staged_item_elems = g.prepared ? g.staged_items : 0;
-  out->stream_elem_bytes = g.ctx->packed_lists && g.V <= kPackMaxV ? 3 : 4;
+  Graph *gs[1] = {const_cast<Graph *>(&g)};
+  out->stream_elem_bytes = launch_elem_bytes(g.ctx, gs, 1, kModeTC);
   return PAC_OK;
 }
 

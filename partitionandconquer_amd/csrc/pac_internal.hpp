@@ -200,7 +200,7 @@ struct Ctx {
   unsigned long long *item_stamps = nullptr;
   pac_timing last{};
   double timeout_ms = 0;  // bound on waiting for a solve's completion (0: none)
-  bool packed_lists = true;  // stream 3-byte lists where a graph has them
+  int packed_lists = PAC_PACKED_AUTO;  // when launches stream the 3-byte lists
   int tile = 1;
   void *noalloc_scratch = nullptr;  // this device's static scratch of pac_tc_noalloc
   DevBuf scratch;     // generic temporary storage (library-owned)
@@ -252,6 +252,25 @@ constexpr uint32_t kBackEdge = 0xFFFFFFFFu;
 // partner (out-degrees stay below 2^31).
 constexpr uint32_t kPairFwd = 0x80000000u;
 constexpr uint32_t kPackMaxV = 1u << 24;  // 3-byte lists: every ID below this
+// PAC_PACKED_AUTO streams them when the launch's largest graph has this many edges: at
+// R-MAT scale 20 (m = 1.6e7, 62 MB of 4-byte lists) packed and 4-byte launches took the
+// same time, at scale 22 (m = 6.4e7) the packed TC was 9 % faster, and a batch of 256
+// graphs of at most 4e6 edges ran 2.4 % slower packed.
+constexpr uint64_t kPackMinEdges = 1ull << 24;
+// The element width of the lists one launch streams, the same for all its graphs (its
+// kernels are compiled for one): 3 for TC or LCC when every graph has its 3-byte lists and
+// the context's mode allows (PAC_PACKED_AUTO: the largest graph has kPackMinEdges edges),
+// else 4.
+inline uint32_t launch_elem_bytes(const Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
+  if (mode == kModeSUP || ctx->packed_lists == PAC_PACKED_OFF || G == 0) return 4u;
+  uint64_t max_m = 0;
+  for (uint32_t i = 0; i < G; ++i) {
+    if (gs[i]->V > kPackMaxV || gs[i]->opk.p == nullptr || gs[i]->pkinfo.p == nullptr)
+      return 4u;
+    max_m = gs[i]->m > max_m ? gs[i]->m : max_m;
+  }
+  return ctx->packed_lists == PAC_PACKED_ALWAYS || max_m >= kPackMinEdges ? 3u : 4u;
+}
 // Fill pedge of the pairs owned by their head with their edge indices (once; the per-edge
 // supports need them).
 int build_back_edges(Graph *g);

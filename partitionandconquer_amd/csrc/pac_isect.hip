@@ -443,7 +443,9 @@ __device__ __forceinline__ void lds_add_u32(uint32_t base, uint32_t off, uint32_
 // wave-level ordering instead of workgroup barriers, so the workgroup's waves work on
 // different small rows at once (the caller keeps the row within that share and to one
 // batch of pairs).  Otherwise the workgroup's waves share the row, taking its pairs in turn.
-template <int MODE, int WAVES, int CAP, bool SOLO = false>
+// ESF: the streamed lists' element width, fixed at compile time (3 or 4) or, with 0, read
+// from the descriptor (then both stream loops are compiled in).
+template <int MODE, int WAVES, int CAP, bool SOLO = false, int ESF = 0>
 __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_lo,
                                          uint32_t c_hi, Smem<CAP, MODE, WAVES> &sm,
                                          unsigned long long &acc) {
@@ -707,10 +709,17 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
     if (nvalid == 0) break;
     uint32_t phl = 0;  // lane j: the hit count of the batch's pair j, once it is done
-    if (MODE != kModeSUP && G.selem == 3u)  // uniform
+    static_assert(!(MODE == kModeSUP && ESF == 3), "per-edge supports stream 4-byte lists");
+    if constexpr (ESF == 3) {
       stream_batch(std::integral_constant<uint32_t, 3u>(), nvalid, phl);
-    else
+    } else if constexpr (ESF == 4 || MODE == kModeSUP) {
       stream_batch(std::integral_constant<uint32_t, 4u>(), nvalid, phl);
+    } else {
+      if (G.selem == 3u)  // uniform
+        stream_batch(std::integral_constant<uint32_t, 3u>(), nvalid, phl);
+      else
+        stream_batch(std::integral_constant<uint32_t, 4u>(), nvalid, phl);
+    }
     if (MODE != kModeTC && phl != 0) {
       // The batch's pair credits, one lane per pair, all at once (one table lookup per
       // lane rather than one wave-wide lookup per pair).  t(y) / the pair's edge: r->y adds
@@ -807,7 +816,7 @@ __device__ __forceinline__ bool solo_row(const GDev &G, uint32_t r) {
 }
 
 // One row (or a pair range of one) with the given strategy.
-template <int MODE, int WAVES, int CAP>
+template <int MODE, int WAVES, int CAP, int ESF = 0>
 __device__ __forceinline__ void do_row(const GDev &G, uint32_t r, uint32_t c_lo,
                                        uint32_t c_hi, int st, Smem<CAP, MODE, WAVES> &sm,
                                        unsigned long long &acc) {
@@ -820,7 +829,7 @@ __device__ __forceinline__ void do_row(const GDev &G, uint32_t r, uint32_t c_lo,
     const uint32_t hi = c_hi < np ? c_hi : np;
     for (uint32_t lo = c_lo; lo < hi;) {
       const uint32_t e = (MODE == kModeTC || hi - lo <= kPassPairs) ? hi : lo + kPassPairs;
-      if (!row_hash<MODE, WAVES, CAP>(G, r, lo, e, sm, acc))
+      if (!row_hash<MODE, WAVES, CAP, false, ESF>(G, r, lo, e, sm, acc))
         row_sorted<kGallop, MODE, WAVES, CAP>(G, r, lo, e, sm, acc);
       lo = e;
     }
@@ -865,7 +874,7 @@ constexpr int lds_waves() {
 // With stamps (profiling only; null otherwise) the workgroup records, per item it, the
 // real-time clock when it took the item and when it asked for the next one:
 // stamps[2 it], stamps[2 it + 1] (rows it deferred are not included).
-template <int MODE, int WAVES, int CAP, bool DEFER>
+template <int MODE, int WAVES, int CAP, bool DEFER, int ESF>
 __global__ __launch_bounds__(64 * WAVES, (lds_waves<CAP, MODE, WAVES>()))
 void k_persistent(const GraphDev *__restrict__ Gs, const WorkItem *__restrict__ items,
                   uint32_t nitems, const uint8_t *__restrict__ strat, uint32_t *queue,
@@ -911,7 +920,7 @@ void k_persistent(const GraphDev *__restrict__ Gs, const WorkItem *__restrict__ 
       for (uint32_t q = w.lo + (threadIdx.x >> 6); q < w.hi; q += WAVES) {
         const uint32_t r = G.rows[q];
         if (solo_row<CAP, MODE, WAVES>(G, r) &&
-            !row_hash<MODE, WAVES, CAP, true>(G, r, 0u, kAllC, s, acc))
+            !row_hash<MODE, WAVES, CAP, true, ESF>(G, r, 0u, kAllC, s, acc))
           row_hbm<MODE, WAVES, true>(G, r, 0u, kAllC, acc);
       }
       __syncthreads();
@@ -929,7 +938,7 @@ void k_persistent(const GraphDev *__restrict__ Gs, const WorkItem *__restrict__ 
         __syncthreads();
         if (s.defer_slot < defer_cap) continue;
       }
-      do_row<MODE, WAVES, CAP>(G, r, c_lo, c_hi, st, s, acc);
+      do_row<MODE, WAVES, CAP, ESF>(G, r, c_lo, c_hi, st, s, acc);
     }
   }
   if (cur_total) flush_total<WAVES>(cur_total, acc);
@@ -937,7 +946,7 @@ void k_persistent(const GraphDev *__restrict__ Gs, const WorkItem *__restrict__ 
 
 // The rows queued by k_persistent (queue[1] of them, at most defer_cap), pulled through
 // queue[2] by workgroups of the widest tile.
-template <int MODE, int WAVES, int CAP>
+template <int MODE, int WAVES, int CAP, int ESF>
 __global__ __launch_bounds__(64 * WAVES) void k_deferred(const GraphDev *__restrict__ Gs,
                                                          const DeferItem *__restrict__ defer,
                                                          uint32_t defer_cap, uint32_t *queue) {
@@ -958,7 +967,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_deferred(const GraphDev *__restr
       acc = 0;
       cur_total = G.total;
     }
-    do_row<MODE, WAVES, CAP>(G, d.r, d.c_lo, d.c_hi, (int)d.st, s, acc);
+    do_row<MODE, WAVES, CAP, ESF>(G, d.r, d.c_lo, d.c_hi, (int)d.st, s, acc);
   }
   if (cur_total) flush_total<WAVES>(cur_total, acc);
 }
@@ -1258,7 +1267,7 @@ struct PersistBufs {
 };
 
 // The persistent launch (and the wide second pass when rows were deferred).
-template <int MODE, int WAVES, int CAP>
+template <int MODE, int WAVES, int CAP, int ES>
 int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, const PersistBufs &b,
                        uint64_t *launches) {
   constexpr bool kCanDefer = CAP < kWideCap;
@@ -1267,22 +1276,22 @@ int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, const P
   if (ctx->time_kernel) PAC_CHECK_HIP(hipEventRecord(ctx->kev[0], ctx->stream));
   if (defer_cap) {
     const uint32_t grid =
-        persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, kCanDefer>, 64 * WAVES, nitems);
-    hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, kCanDefer>), dim3(grid), dim3(64 * WAVES),
+        persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, kCanDefer, ES>, 64 * WAVES, nitems);
+    hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, kCanDefer, ES>), dim3(grid), dim3(64 * WAVES),
                        0, ctx->stream, b.gdesc, items, nitems, b.strat, b.queue, b.defer,
                        defer_cap, b.stamps);
   } else {
     const uint32_t grid =
-        persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, false>, 64 * WAVES, nitems);
-    hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, false>), dim3(grid), dim3(64 * WAVES), 0,
+        persistent_grid(ctx, k_persistent<MODE, WAVES, CAP, false, ES>, 64 * WAVES, nitems);
+    hipLaunchKernelGGL((k_persistent<MODE, WAVES, CAP, false, ES>), dim3(grid), dim3(64 * WAVES), 0,
                        ctx->stream, b.gdesc, items, nitems, b.strat, b.queue, nullptr, 0u,
                        b.stamps);
   }
   PAC_CHECK_HIP(hipGetLastError());
   if (defer_cap) {
-    const uint32_t g2 = persistent_grid(ctx, k_deferred<MODE, kWideWaves, kWideCap>,
+    const uint32_t g2 = persistent_grid(ctx, k_deferred<MODE, kWideWaves, kWideCap, ES>,
                                         64 * kWideWaves, defer_cap);
-    hipLaunchKernelGGL((k_deferred<MODE, kWideWaves, kWideCap>), dim3(g2), dim3(64 * kWideWaves),
+    hipLaunchKernelGGL((k_deferred<MODE, kWideWaves, kWideCap, ES>), dim3(g2), dim3(64 * kWideWaves),
                        0, ctx->stream, b.gdesc, b.defer, defer_cap, b.queue);
     PAC_CHECK_HIP(hipGetLastError());
     if (launches) *launches += 1;
@@ -1292,22 +1301,26 @@ int run_persistent_cfg(Ctx *ctx, const WorkItem *items, uint32_t nitems, const P
   return PAC_OK;
 }
 
-template <int MODE>
+template <int MODE, int ES>
 int run_persistent_mode(Ctx *ctx, const WorkItem *items, uint32_t nitems, const PersistBufs &b,
                         uint64_t *launches) {
   switch (ctx->tile) {
-    case 0: return run_persistent_cfg<MODE, 2, 1024>(ctx, items, nitems, b, launches);
-    case 1: return run_persistent_cfg<MODE, 4, 2048>(ctx, items, nitems, b, launches);
-    case 2: return run_persistent_cfg<MODE, 4, 4096>(ctx, items, nitems, b, launches);
-    default: return run_persistent_cfg<MODE, 8, 4096>(ctx, items, nitems, b, launches);
+    case 0: return run_persistent_cfg<MODE, 2, 1024, ES>(ctx, items, nitems, b, launches);
+    case 1: return run_persistent_cfg<MODE, 4, 2048, ES>(ctx, items, nitems, b, launches);
+    case 2: return run_persistent_cfg<MODE, 4, 4096, ES>(ctx, items, nitems, b, launches);
+    default: return run_persistent_cfg<MODE, 8, 4096, ES>(ctx, items, nitems, b, launches);
   }
 }
 
-int run_persistent_any(Ctx *ctx, int mode, const WorkItem *items, uint32_t nitems,
+// es: the element width of the lists the launch's descriptors stream (launch_elem_bytes).
+int run_persistent_any(Ctx *ctx, int mode, uint32_t es, const WorkItem *items, uint32_t nitems,
                        const PersistBufs &b, uint64_t *launches) {
-  if (mode == kModeLCC) return run_persistent_mode<kModeLCC>(ctx, items, nitems, b, launches);
-  if (mode == kModeSUP) return run_persistent_mode<kModeSUP>(ctx, items, nitems, b, launches);
-  return run_persistent_mode<kModeTC>(ctx, items, nitems, b, launches);
+  if (mode == kModeSUP) return run_persistent_mode<kModeSUP, 4>(ctx, items, nitems, b, launches);
+  if (mode == kModeLCC)
+    return es == 3u ? run_persistent_mode<kModeLCC, 3>(ctx, items, nitems, b, launches)
+                    : run_persistent_mode<kModeLCC, 4>(ctx, items, nitems, b, launches);
+  return es == 3u ? run_persistent_mode<kModeTC, 3>(ctx, items, nitems, b, launches)
+                  : run_persistent_mode<kModeTC, 4>(ctx, items, nitems, b, launches);
 }
 
 // The context's own buffers for a persistent launch (the deferred-row queue is reserved
@@ -1521,8 +1534,9 @@ k_sup_big(const uint32_t *__restrict__ units, const uint32_t *__restrict__ ooff,
   }
 }
 
+
 // Device descriptor of a graph; its output pointers must already be allocated.
-GraphDev make_desc(Graph *g, uint32_t part_base, int mode) {
+GraphDev make_desc(Graph *g, uint32_t part_base, uint32_t es) {
   GraphDev d;
   d.ooff = g->ooff.as<const uint32_t>();
   d.oadj = g->oadj.as<const uint32_t>();
@@ -1531,8 +1545,7 @@ GraphDev make_desc(Graph *g, uint32_t part_base, int mode) {
   d.pedge = g->pedge.as<const uint32_t>();
   d.pinfo = g->pinfo.as<const uint2>();
   // the 3-byte lists where the graph has them and the context streams them
-  const bool packed = mode != kModeSUP && g->ctx->packed_lists && g->V <= kPackMaxV &&
-                      g->opk.p != nullptr;
+  const bool packed = es == 3u;  // launch_elem_bytes checked that the lists exist
   d.sadj = packed ? g->opk.as<const uint8_t>() : g->oadj.as<const uint8_t>();
   d.sinfo = packed ? g->pkinfo.as<const uint2>() : g->pinfo.as<const uint2>();
   d.selem = packed ? 3u : 4u;
@@ -1630,7 +1643,7 @@ int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
                  uint64_t *launches) {
   Graph *gs[1] = {g};
   PAC_TRY(alloc_outputs(gs, 1, mode));
-  const GraphDev d = make_desc(g, 0, mode);
+  const GraphDev d = make_desc(g, 0, launch_elem_bytes(ctx, gs, 1, mode));
   uint32_t k = 0;
   while (k < g->P) {
     uint32_t e = k + 1;
@@ -1652,11 +1665,12 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   // (on the hipGraph path nothing has allocated them yet).
   PAC_TRY(alloc_outputs(gs, G, mode));
   std::vector<GraphDev> descs(G);
+  const uint32_t es = launch_elem_bytes(ctx, gs, G, mode);
   uint64_t nparts = 0, nitems = 0, long_rows = 0;
   bool any_long = false;
   const uint32_t cap = (uint32_t)kTiles[ctx->tile].cap;
   for (uint32_t i = 0; i < G; ++i) {
-    descs[i] = make_desc(gs[i], (uint32_t)nparts, mode);
+    descs[i] = make_desc(gs[i], (uint32_t)nparts, es);
     nparts += gs[i]->P;
     nitems += gs[i]->h_items.size();
     // rows over the tile's capacity, each queued at most once per item that holds it
@@ -1705,7 +1719,7 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
     if (nitems == 0) return PAC_OK;
     PersistBufs b;
     PAC_TRY(ctx_bufs(ctx, (uint32_t)long_rows, b));
-    return run_persistent_any(ctx, mode, items, (uint32_t)nitems, b, launches);
+    return run_persistent_any(ctx, mode, es, items, (uint32_t)nitems, b, launches);
   }
   // hipGraph: zero-fill kernels + queue reset + persistent kernel, captured once per
   // (graph set, mode, tile, buffer addresses) and replayed.
@@ -1715,6 +1729,7 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   key.push_back(ctx->queue.p);
   if (long_rows) PAC_TRY(ctx->defer.reserve(sizeof(DeferItem) * long_rows));
   key.push_back(ctx->defer.p);
+  key.push_back(reinterpret_cast<const void *>((uintptr_t)es));  // the kernels' list width
   for (uint32_t i = 0; i < G; ++i) {
     key.push_back(gs[i]->tvert.p);
     key.push_back(gs[i]->sup.p);
@@ -1733,7 +1748,7 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
     PersistBufs b;
     if (rc == PAC_OK) rc = ctx_bufs(ctx, (uint32_t)long_rows, b);  // reserved above: no allocation
     if (rc == PAC_OK && nitems)
-      rc = run_persistent_any(ctx, mode, items, (uint32_t)nitems, b, nullptr);
+      rc = run_persistent_any(ctx, mode, es, items, (uint32_t)nitems, b, nullptr);
     ctx->time_kernel = timing;
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
@@ -1811,7 +1826,9 @@ int launch_persistent_noalloc(Ctx *ctx, Graph *g, int mode, const uint8_t *d_str
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
                        d_tvert, (uint64_t)g->V);
   PAC_CHECK_HIP(hipGetLastError());
-  GraphDev d = make_desc(g, 0, mode);
+  Graph *gs1[1] = {g};
+  const uint32_t es = launch_elem_bytes(ctx, gs1, 1, mode);
+  GraphDev d = make_desc(g, 0, es);
   d.total = d_total;
   d.tvert = mode == kModeLCC ? d_tvert : nullptr;
   d.sup = nullptr;
@@ -1827,7 +1844,7 @@ int launch_persistent_noalloc(Ctx *ctx, Graph *g, int mode, const uint8_t *d_str
   b.queue = reinterpret_cast<uint32_t *>(base + kScratchQueue);
   b.defer = reinterpret_cast<DeferItem *>(base + kScratchDefer);
   b.defer_cap = (uint32_t)long_rows;
-  return run_persistent_any(ctx, mode, g->items.as<const WorkItem>(), (uint32_t)nitems, b,
+  return run_persistent_any(ctx, mode, es, g->items.as<const WorkItem>(), (uint32_t)nitems, b,
                             nullptr);
 }
 

@@ -29,6 +29,7 @@ ERRORS = {-1: "ARG", -2: "HIP", -3: "OOM", -4: "FORMAT", -5: "IO", -6: "STATE",
 DISPATCH_FIXED, DISPATCH_LEARNED, FORCE_MERGE, FORCE_GALLOP, FORCE_BITMAP = range(5)
 LAUNCH_PLAIN, LAUNCH_PERSISTENT, LAUNCH_GRAPH = range(3)
 MODE_TC, MODE_LCC, MODE_JACCARD = range(3)
+PACKED_OFF, PACKED_AUTO, PACKED_ALWAYS = range(3)
 STRATEGIES = ("merge", "gallop", "bitmap")
 NUM_TILES = 4
 MLP_IN, MLP_HIDDEN, MLP_OUT = 16, 64, 3
@@ -221,10 +222,11 @@ class Context:
         """Bound (ms, 0 = none) on waiting for a solve; PacError -8 when it passes."""
         _check(lib().pac_ctx_set_timeout(self._h, ms))
 
-    def set_packed_lists(self, on: bool):
-        """TC and LCC stream 3-byte lists on graphs with every ID below 2^24 (the default)
-        or the 4-byte lists; applies from the next launch."""
-        _check(lib().pac_ctx_set_packed_lists(self._h, 1 if on else 0))
+    def set_packed_lists(self, mode: int = 1):
+        """When TC and LCC launches stream the 3-byte lists of graphs with every ID below
+        2^24: PACKED_AUTO (the default; the launch's largest graph has at least 2^24 edges),
+        PACKED_ALWAYS or PACKED_OFF.  Applies from the next launch."""
+        _check(lib().pac_ctx_set_packed_lists(self._h, int(mode)))
 
     def stream_copy(self, nbytes: int, reps: int = 5) -> float:
         """Median time (ms) of a plain device-to-device streaming copy of nbytes."""

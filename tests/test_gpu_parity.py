@@ -569,9 +569,10 @@ def _wide_id_edges(V, seed):
 @pytest.mark.parametrize("V", [1 << 24, (1 << 24) + 1])
 def test_packed_lists_wide_ids(ctx, orc, V):
     """TC and LCC stream 3-byte lists when every ID is below 2^24 (V = 2^24: IDs up to
-    0xFFFFFF) and 4-byte lists otherwise (V = 2^24 + 1) or when packing is off: per-vertex
-    counts on every launch path equal the oracle's with packing on and off; per-edge
-    Jaccard (always 4-byte lists) equals the oracle's on every edge either way."""
+    0xFFFFFF) and packing is forced on, and 4-byte lists otherwise (V = 2^24 + 1, packing
+    off, or, in the default mode, a graph of fewer than 2^24 edges): per-vertex counts on
+    every launch path equal the oracle's in every mode; per-edge Jaccard (always 4-byte
+    lists) equals the oracle's on every edge either way."""
     src, dst = _wide_id_edges(V, seed=V & 0xFF)
     g = pac.Graph.from_edges(ctx, src, dst, V)
     ref = orc.CSR(*g.csr())
@@ -580,44 +581,70 @@ def test_packed_lists_wide_ids(ctx, orc, V):
     g.prepare(8)
     res = {}
     try:
-        for packed in (True, False):
-            ctx.set_packed_lists(packed)
-            assert g.info().stream_elem_bytes == (3 if packed and V <= 1 << 24 else 4)
+        for mode in (pac.PACKED_ALWAYS, pac.PACKED_OFF, pac.PACKED_AUTO):
+            ctx.set_packed_lists(mode)
+            want = 3 if mode == pac.PACKED_ALWAYS and V <= 1 << 24 else 4
+            assert g.info().stream_elem_bytes == want, mode
             for d in (pac.FORCE_BITMAP, pac.DISPATCH_FIXED):
                 for l in LAUNCHES:
                     t, pv = g.tc(d, l, per_vertex=True)
-                    assert t == T and np.array_equal(pv, tv), (packed, d, l)
-            res[packed] = g.jaccard_edges(pac.FORCE_BITMAP, pac.LAUNCH_GRAPH)
+                    assert t == T and np.array_equal(pv, tv), (mode, d, l)
+            res[mode] = g.jaccard_edges(pac.FORCE_BITMAP, pac.LAUNCH_GRAPH)
     finally:
-        ctx.set_packed_lists(True)
-    for a, b in zip(res[True], res[False]):
+        ctx.set_packed_lists(pac.PACKED_AUTO)
+    for a, b in zip(res[pac.PACKED_ALWAYS], res[pac.PACKED_OFF]):
         assert bits_equal(a, b)
-    es, ed, inter, uni, ratio = res[True]
+    es, ed, inter, uni, ratio = res[pac.PACKED_ALWAYS]
     ri, ru, rr = orc.jaccard_edges(ref, es, ed, NT)
     assert np.array_equal(ri, inter) and np.array_equal(ru, uni) and bits_equal(rr, ratio)
     assert int(inter.sum(dtype=np.uint64)) == 3 * T
+    with pytest.raises(pac.PacError):
+        ctx.set_packed_lists(3)
     g.free()
 
 
 def test_packed_lists_rmat_all_modes(ctx, orc):
     """R-MAT scale 15: TC and LCC from the 3-byte and the 4-byte streamed lists are
-    identical, under fixed and forced-hash dispatch, and per-edge supports do not change
-    with the setting."""
+    identical, under fixed and forced-hash dispatch, on the persistent (one kernel per list
+    width) and plain (both widths in one kernel) launch paths, and per-edge supports do not
+    change with the setting."""
     g = pac.Graph.rmat(ctx, 15, 16, 0.57, 0.19, 0.19, seed=15)
     g.prepare(32)
     out = {}
     try:
-        for packed in (True, False):
-            ctx.set_packed_lists(packed)
+        for mode in (pac.PACKED_ALWAYS, pac.PACKED_OFF):
+            ctx.set_packed_lists(mode)
             r = []
             for d in (pac.FORCE_BITMAP, pac.DISPATCH_FIXED):
                 r.append(g.tc(d, pac.LAUNCH_PERSISTENT, per_vertex=True)[1])
+                r.append(g.tc(d, pac.LAUNCH_PLAIN, per_vertex=True)[1])
                 r.extend(g.lcc(d, pac.LAUNCH_GRAPH)[:2])
                 r.extend(g.jaccard_edges(d, pac.LAUNCH_PERSISTENT)[2:])
-            out[packed] = r
+            out[mode] = r
     finally:
-        ctx.set_packed_lists(True)
-    for a, b in zip(out[True], out[False]):
+        ctx.set_packed_lists(pac.PACKED_AUTO)
+    for a, b in zip(out[pac.PACKED_ALWAYS], out[pac.PACKED_OFF]):
         assert bits_equal(a, b)
     T = orc.tc_merge(orc.CSR(*g.csr()), NT)
-    assert int(out[True][0].sum()) == 3 * T
+    assert int(out[pac.PACKED_ALWAYS][0].sum()) == 3 * T
+
+
+def test_packed_lists_batch_of_mixed_graphs(ctx, orc):
+    """A batch whose graphs all have 3-byte lists streams them (forced on); one graph with
+    IDs beyond 2^24 makes the whole launch stream 4-byte lists: both give every graph's
+    oracle count."""
+    gs = [pac.Graph.rmat(ctx, s, 16, 0.57, 0.19, 0.19, seed=100 + s) for s in (10, 12, 14)]
+    src, dst = _wide_id_edges((1 << 24) + 1, seed=7)
+    gs.append(pac.Graph.from_edges(ctx, src, dst, (1 << 24) + 1))
+    for g in gs:
+        g.prepare(4)
+    want = np.array([orc.tc_oriented(orc.CSR(*g.csr()), NT) for g in gs], np.uint64)
+    try:
+        ctx.set_packed_lists(pac.PACKED_ALWAYS)
+        for l in (pac.LAUNCH_PERSISTENT, pac.LAUNCH_GRAPH):
+            assert np.array_equal(ctx.tc_batch(gs[:3], pac.FORCE_BITMAP, l), want[:3])
+            assert np.array_equal(ctx.tc_batch(gs, pac.FORCE_BITMAP, l), want)
+    finally:
+        ctx.set_packed_lists(pac.PACKED_AUTO)
+    for g in gs:
+        g.free()
