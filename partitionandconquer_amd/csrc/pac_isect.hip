@@ -544,6 +544,10 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   const uint32_t tb = lds_addr(tab), m3 = ((1u << lg) - 1u) << 3;
   if ((tb & (m3 | 7u)) != 0) return false;  // uniform
   const uint32_t cb = MODE == kModeTC ? 0u : lds_addr(H.cnt16) - (lds_addr(H.tab) >> 1);
+  // The counters' base as an instruction offset: a constant when the workgroup's LDS (one
+  // Smem) starts at address 0, checked here (uniform; never false in practice).
+  constexpr uint32_t kCntBase = 4u * S::kSlots;
+  if (MODE != kModeTC && cb != kCntBase) return false;
   // Lookup of w.  A bucket's two slots share one word of the packed 16-bit counters, so
   // a hit is reported as that word (the bucket) and the increment for w's half of it
   // (1 or 1 << 16); word and increment are only meaningful on a hit.  Both buckets are
@@ -670,7 +674,13 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         if (MODE == kModeSUP && lane == u) hw = bal;
         if ((q.e10 | q.e11 | q.e20 | q.e21) & live) {
           // t(w) / edge r->w
-          lds_add_u32(cb, ((q.e10 | q.e11) ? q.a1 : q.a2) >> 1, (q.e11 | q.e21) ? 0x10000u : 1u);
+          // (written out: the compiler adds the base with a vector instruction instead of
+          // folding it into the atomic's offset; the atomic returns nothing, and LDS
+          // operations complete in order, so later waits on other LDS accesses cover it)
+          const uint32_t ca = ((q.e10 | q.e11) ? q.a1 : q.a2) >> 1;
+          const uint32_t inc = (q.e11 | q.e21) ? 0x10000u : 1u;
+          asm volatile("ds_add_u32 %0, %1 offset:%2" : : "v"(ca), "v"(inc), "i"(kCntBase)
+                       : "memory");
         }
       }
       if (MODE == kModeSUP) {
