@@ -791,19 +791,25 @@ void orc_pair_stats(const uint64_t *vl, const int32_t *el, uint64_t V, orc_pairs
   const std::vector<uint64_t> dplus = out_degrees(vl, el, V);
   std::vector<uint8_t> owner(V, 0);
   uint64_t pairs = 0, streamed = 0;
-#pragma omp parallel for reduction(+ : pairs, streamed)
-  for (int64_t t = 0; t < (int64_t)V; ++t) {
-    for (uint64_t i = vl[t]; i < vl[t + 1]; ++i) {
-      const int64_t h = el[i];
-      if (!oriented(vl, t, h)) continue;
-      const uint64_t mn = std::min(dplus[t], dplus[h]);
-      streamed += mn;
-      if (mn == 0) continue;
-      ++pairs;
-      // the longer out-list owns the pair, ties going to the tail (a byte store: races
-      // only ever write 1)
-      const int64_t x = dplus[h] > dplus[t] ? h : t;
-      __atomic_store_n(&owner[x], (uint8_t)1, __ATOMIC_RELAXED);
+#pragma omp parallel reduction(+ : pairs, streamed)
+  {
+    std::vector<std::pair<uint64_t, int64_t>> out;  // N+(t) as (degree, id), sorted
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t t = 0; t < (int64_t)V; ++t) {
+      out.clear();
+      for (uint64_t i = vl[t]; i < vl[t + 1]; ++i)
+        if (oriented(vl, t, el[i])) out.emplace_back(deg(vl, el[i]), el[i]);
+      std::sort(out.begin(), out.end());
+      const uint64_t n = out.size();
+      for (uint64_t i = 0; i < n; ++i) {
+        const int64_t h = out[i].second;
+        const uint64_t suffix = n - 1 - i;  // the out-neighbours of t ranked above h
+        if (suffix == 0 || dplus[h] == 0) continue;
+        ++pairs;
+        streamed += suffix;
+        // the head owns the pair (a byte store: races only ever write 1)
+        __atomic_store_n(&owner[h], (uint8_t)1, __ATOMIC_RELAXED);
+      }
     }
   }
   uint64_t rows = 0, staged = 0;

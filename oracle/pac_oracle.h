@@ -170,14 +170,15 @@ typedef struct {
 void orc_oriented_stats(const uint64_t *vl, const int32_t *el, uint64_t V, orc_stats *s);
 
 /* Intersection pairs under the same orientation.  Each oriented edge t->h is one pair,
- * owned by the endpoint with the longer out-list (ties: the tail), whose list a GPU row
- * stages while the partner's (the shorter) is streamed; a pair whose shorter list is
- * empty holds no triangle and is dropped.  These counts are the work model of the
- * hash path (its algorithmic bytes), not a reference function. */
+ * owned by the head h, whose out-list a GPU row stages, while the suffix of N+(t) above
+ * h -- S(t,h) = #{w in N+(t) : (d(h),h) < (d(w),w)}, the only part of N+(t) that can meet
+ * N+(h) -- is streamed; a pair with S(t,h) = 0 or d+(h) = 0 holds no triangle and is
+ * dropped.  These counts are the work model of the hash path (its algorithmic bytes),
+ * not a reference function. */
 typedef struct {
-  uint64_t pairs;     /* #{oriented t->h : min(d+(t), d+(h)) > 0}                 */
+  uint64_t pairs;     /* #{oriented t->h : S(t,h) > 0 and d+(h) > 0}              */
   uint64_t rows;      /* vertices owning at least one pair                         */
-  uint64_t streamed;  /* sum over oriented t->h of min(d+(t), d+(h))               */
+  uint64_t streamed;  /* sum of S(t,h) over the pairs                              */
   uint64_t staged;    /* sum of d+(x) over the vertices x owning at least one pair */
 } orc_pairs;
 void orc_pair_stats(const uint64_t *vl, const int32_t *el, uint64_t V, orc_pairs *s);

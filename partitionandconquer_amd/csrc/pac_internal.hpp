@@ -71,17 +71,19 @@ struct DevBuf {
 // One graph as the persistent kernel sees it (device-side descriptor).
 struct GraphDev {
   const uint32_t *ooff;  // V+1 oriented row offsets
-  const uint32_t *oadj;  // m oriented neighbours, rows sorted by ID
+  const uint32_t *oadj;  // m oriented neighbours, each row in (degree, id) order
+  const uint32_t *okey;  // per oriented edge, its head's (degree, id) rank: each row's keys
+                         // ascend, so the sorted strategies compare these, not the IDs
   const uint32_t *poff;  // V+1 offsets of the intersection pairs owned by each vertex
   const uint32_t *ppar;  // partner of each pair (its out-list is the one streamed)
   const uint32_t *pedge; // oriented-edge index of each pair (position in oadj)
-  // per pair: the partner's out-list as (first position in oadj, length | kPairFwd when the
-  // pair's edge leaves the owner) -- read with the pair, so its bounds need no gather
+  // per pair: the part of the partner's out-list it streams, as (first position in oadj,
+  // length) -- read with the pair, so its bounds need no gather
   const uint2 *pinfo;
   // The lists the hash path streams and their bounds per pair: oadj with pinfo
   // (selem = 4), or, for a graph whose IDs are all below 2^24, its 3-byte lists opk with
-  // pkinfo (selem = 3).  Pair p's list starts at byte 4 * sinfo[p].x of sadj (element i at
-  // + selem * i); sinfo[p].y = pinfo[p].y.
+  // pkinfo (selem = 3).  Pair p's stream starts at byte 4 * sinfo[p].x of sadj (element i
+  // at + selem * i) and has sinfo[p].y elements.
   const uint8_t *sadj;
   const uint2 *sinfo;
   uint32_t selem;
@@ -132,25 +134,33 @@ struct Graph {
   uint64_t n_pairs = 0, n_rows = 0, streamed = 0, staged = 0, staged_items = 0;
   // symmetric CSR
   DevBuf off, adj, deg;
-  // oriented CSR
-  DevBuf odeg, ooff, oadj;
+  // Oriented CSR.  Rows in ID order; each row's out-neighbours in (degree, id) order, with
+  // okey[e] = the rank of oadj[e] in that order (vertex rank[v] = its position among all
+  // vertices sorted by (degree, id)).
+  DevBuf odeg, ooff, oadj, okey, rank;
   // When every ID is below 2^24, built with the pairs: the oriented lists with 3-byte
   // elements (little-endian), the list of vertex y from byte 4 ceil(3 (ooff[y] + y) / 4) --
   // every list starts on a 4-byte boundary and ends before the next one starts, since each
-  // is given 3 bytes more than its length; and per pair, pkinfo = (4-byte word where the
-  // partner's 3-byte list starts, pinfo.y).  Empty otherwise.
+  // is given 3 bytes more than its length; and per pair, pkinfo = (4-byte word where its
+  // stream starts, length): the suffix start rounded down to a multiple of 4 elements (a
+  // word boundary of the list), the length grown by the up to 3 elements before it, which
+  // rank at most h and so are never found in N+(h).  Empty otherwise.
   DevBuf opk, pkinfo;
-  // Intersection pairs.  Each oriented edge t->h stands for N+(t) & N+(h); the pair is
-  // owned by the endpoint with the longer out-list (ties: the tail), which stages its
-  // list in LDS while the partner's shorter list is streamed.  Pairs whose partner has
-  // an empty out-list are dropped (they hold no triangle).
+  // Intersection pairs.  Each oriented edge t->h stands for N+(t) & N+(h), whose elements
+  // all rank above h: the pair is owned by the head h, which stages N+(h) in LDS, and
+  // streams only the suffix of N+(t) after h (N+(t) is in rank order, and h is in it).
+  // Pairs with an empty suffix or an empty N+(h) are dropped (they hold no triangle).
+  // pedge = the position of t->h in oadj; pinfo = (pedge + 1, suffix length).
   DevBuf poff, ppar, pedge, pinfo;
+  // SUP mode streams from a 64-element boundary of N+(t) (so a block's hit word is a
+  // column of the list): per pair (first position, length) with the start rounded down
+  // to a multiple of 64 within the list.  Built with the hit words.
+  DevBuf pinfo64;
   // Construction state, kept for the later passes: crow[c] = the row holding entry 256c;
   // oflags / pflags = one bit per symmetric entry (kept by the orientation / owned as a
   // pair), owpre / ppre = exclusive scans of their 64-bit words' popcounts.
   DevBuf crow, oflags, owpre, pflags, ppre;
   bool pairs_built = false;
-  bool back_edges_built = false;  // pedge of the pairs owned by their head (kBackEdge before)
   // Per-edge supports (SUP mode) from hit words instead of one atomic per triangle:
   // pstreams[y] = number of pairs streaming N+(y); wbase[y] = first word of y's streams;
   // pword, hitw as in GraphDev.
@@ -244,13 +254,6 @@ int row_summary(Graph *g, uint64_t part_width, uint32_t P, RowSummary &out);
 // Value of the entry past the last oriented edge (oadj[m]): neither a vertex ID (IDs
 // are < 2^31) nor the empty hash-table slot, so a lookup of it never hits.
 constexpr uint32_t kListEnd = 0xFFFFFFFEu;
-// pedge of a pair u->x owned by its head x until build_back_edges: never an edge of the
-// owner's out-list (e - ooff[x] >= d+(x)), so the TC/LCC kernels take it for u->x, the
-// only fact they need of it.
-constexpr uint32_t kBackEdge = 0xFFFFFFFFu;
-// pinfo[p].y: the partner's out-degree, with this bit set when the pair's edge is owner ->
-// partner (out-degrees stay below 2^31).
-constexpr uint32_t kPairFwd = 0x80000000u;
 constexpr uint32_t kPackMaxV = 1u << 24;  // 3-byte lists: every ID below this
 // PAC_PACKED_AUTO streams them when the launch's largest graph has this many edges: at
 // R-MAT scale 20 (m = 1.6e7, 62 MB of 4-byte lists) packed and 4-byte launches took the
@@ -271,9 +274,6 @@ inline uint32_t launch_elem_bytes(const Ctx *ctx, Graph *const *gs, uint32_t G, 
   }
   return ctx->packed_lists == PAC_PACKED_ALWAYS || max_m >= kPackMinEdges ? 3u : 4u;
 }
-// Fill pedge of the pairs owned by their head with their edge indices (once; the per-edge
-// supports need them).
-int build_back_edges(Graph *g);
 // Vertices streamed by at most 64 pairs have their hit words summed by one wave (one
 // load per word column); the others are split into units of kSupRun streams.
 constexpr uint32_t kSupRun = 256;

@@ -64,7 +64,7 @@ namespace {
 #define PAC_GLOBAL __attribute__((address_space(1)))
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 struct GDev {
-  const PAC_GLOBAL uint32_t *ooff, *oadj, *poff, *ppar, *pedge, *rows;
+  const PAC_GLOBAL uint32_t *ooff, *oadj, *okey, *poff, *ppar, *pedge, *rows;
   const PAC_GLOBAL u32x2 *pinfo;
   const PAC_GLOBAL uint8_t *sadj;  // the streamed lists, selem bytes per element
   const PAC_GLOBAL u32x2 *sinfo;   // per pair: 4-byte word of sadj where its list starts
@@ -79,6 +79,7 @@ __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
   GDev g;
   g.ooff = (const PAC_GLOBAL uint32_t *)d.ooff;
   g.oadj = (const PAC_GLOBAL uint32_t *)d.oadj;
+  g.okey = (const PAC_GLOBAL uint32_t *)d.okey;
   g.poff = (const PAC_GLOBAL uint32_t *)d.poff;
   g.ppar = (const PAC_GLOBAL uint32_t *)d.ppar;
   g.pedge = (const PAC_GLOBAL uint32_t *)d.pedge;
@@ -192,11 +193,12 @@ __device__ __forceinline__ void on_hit(uint32_t *cnt, uint32_t i, PAC_GLOBAL uin
 }
 
 // ---- the three lookup engines: one wave, one pair (x, y) ---------------------------
-// B = oadj[b0 .. b0+nb): sorted N+(y) in HBM.  Each returns this lane's number of
-// confirmed common neighbours.
+// The sorted strategies compare ranks: A = okey of N+(x) (staged, ascending) and
+// B = okey[b0 .. b0+nb), the ranks of the pair's stream (ascending; rank order is each
+// list's order).  Each returns this lane's number of confirmed common neighbours.
 
 template <int MODE>
-__device__ __forceinline__ uint32_t probe_gallop(const PAC_GLOBAL uint32_t *__restrict__ oadj,
+__device__ __forceinline__ uint32_t probe_gallop(const PAC_GLOBAL uint32_t *__restrict__ okey,
                                                  const uint32_t *A, uint32_t nA, uint32_t *cnt,
                                                  uint32_t b0, uint32_t nb,
                                                  PAC_GLOBAL uint32_t *sup, uint32_t lane) {
@@ -208,7 +210,7 @@ __device__ __forceinline__ uint32_t probe_gallop(const PAC_GLOBAL uint32_t *__re
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const uint32_t j = j0 + 64u * u;
-        w[u] = j < nb ? oadj[b0 + j] : 0u;
+        w[u] = j < nb ? okey[b0 + j] : 0u;
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
@@ -225,7 +227,7 @@ __device__ __forceinline__ uint32_t probe_gallop(const PAC_GLOBAL uint32_t *__re
   } else {
     // A is the shorter list: each A element is binary-searched in B (HBM), skipping
     // A elements outside B's value range.
-    const PAC_GLOBAL uint32_t *B = oadj + b0;
+    const PAC_GLOBAL uint32_t *B = okey + b0;
     const uint32_t lo = B[0], hi = B[nb - 1];
     for (uint32_t i = lane; i < nA; i += 64) {
       const uint32_t x = A[i];
@@ -245,11 +247,11 @@ __device__ __forceinline__ uint32_t probe_gallop(const PAC_GLOBAL uint32_t *__re
 // on its diagonal; then it walks.  When B[j] is taken, every A element <= B[j] has
 // already been taken, so B[j] is a common element iff A[i-1] == B[j].
 template <int MODE>
-__device__ __forceinline__ uint32_t probe_merge(const PAC_GLOBAL uint32_t *__restrict__ oadj,
+__device__ __forceinline__ uint32_t probe_merge(const PAC_GLOBAL uint32_t *__restrict__ okey,
                                                 const uint32_t *A, uint32_t nA, uint32_t *cnt,
                                                 uint32_t b0, uint32_t nb,
                                                 PAC_GLOBAL uint32_t *sup, uint32_t lane) {
-  const PAC_GLOBAL uint32_t *B = oadj + b0;
+  const PAC_GLOBAL uint32_t *B = okey + b0;
   const uint32_t total = nA + nb;
   const uint32_t L = (total + 63u) >> 6;
   uint32_t d0 = lane * L;
@@ -297,7 +299,7 @@ __device__ __forceinline__ void for_each_pair(const GDev &G, uint32_t p0, uint32
     if (kl < k_hi) {
       const u32x2 pi = G.pinfo[p0 + kl];
       bl = pi.x;
-      nl = pi.y & ~kPairFwd;
+      nl = pi.y;
       if (MODE != kModeTC) {
         yl = G.ppar[p0 + kl];
         el = G.pedge[p0 + kl];
@@ -332,7 +334,7 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
   uint32_t *A = sm.u.s.A;
   uint32_t *cnt = sm.u.s.cnt;
   for (uint32_t i = tid; i < nA; i += BS) {
-    A[i] = G.oadj[a0 + i];
+    A[i] = G.okey[a0 + i];
     if (MODE != kModeTC) cnt[i] = 0;
   }
   if (tid == 0) sm.rowtot = 0;
@@ -345,19 +347,17 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
     if (MODE == kModeSUP) clear_hit_words(G, nb, p, lane);
     if (nb != 0) {
       if (STRAT == kGallop)
-        h = probe_gallop<MODE>(G.oadj, A, nA, cnt, b0, nb, G.sup, lane);
+        h = probe_gallop<MODE>(G.okey, A, nA, cnt, b0, nb, G.sup, lane);
       else
-        h = probe_merge<MODE>(G.oadj, A, nA, cnt, b0, nb, G.sup, lane);
+        h = probe_merge<MODE>(G.okey, A, nA, cnt, b0, nb, G.sup, lane);
     }
     if (MODE != kModeTC) {
       const uint32_t ch = wave_sum(h);
       if (lane == 0 && ch) {
-        if (e - a0 < nA)
-          atomicAdd(&cnt[e - a0], ch);  // r->y: y = A[e - a0]
-        else if (MODE == kModeLCC)
-          gadd(&G.tvert[y], (unsigned long long)ch);  // y->r: credited to y
+        if (MODE == kModeLCC)
+          gadd(&G.tvert[y], (unsigned long long)ch);  // the pair's edge y->r: t(y)
         else
-          gadd(&G.sup[e], ch);  // y->r: credited to the edge
+          gadd(&G.sup[e], ch);  // the pair's edge y->r
       }
     }
     row_hits += h;
@@ -371,8 +371,8 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
   if (MODE != kModeTC) {
     for (uint32_t i = tid; i < nA; i += BS) {
       const uint32_t v = cnt[i];
-      if (v && MODE == kModeLCC) gadd(&G.tvert[A[i]], (unsigned long long)v);  // t(A[i])
-      else if (v) gadd(&G.sup[a0 + i], v);  // the edge r->A[i]
+      if (v && MODE == kModeLCC) gadd(&G.tvert[G.oadj[a0 + i]], (unsigned long long)v);  // t(w)
+      else if (v) gadd(&G.sup[a0 + i], v);  // the edge r->w
     }
     // an atomic even for a whole row: other rows' tail credits to r arrive meanwhile
     if (MODE == kModeLCC && tid == 0 && sm.rowtot) gadd(&G.tvert[r], (unsigned long long)sm.rowtot);
@@ -471,18 +471,16 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   uint32_t *const cnt16 = MODE == kModeTC ? H.cnt16 : H.cnt16 + share * (S::kSlots / WAVES / 2);
   if (c_hi > np) c_hi = np;
   // The wave's first batch of pairs is loaded before the table is built, so the latency
-  // overlaps it: per pair the bounds of the partner's list and the direction of its edge
-  // (stored with the pair: no gather through ooff), and when counts are kept the partner;
-  // SUP mode also needs the edge index and the first hit word.
+  // overlaps it: per pair the bounds of its stream (stored with the pair: no gather
+  // through ooff), and when counts are kept the partner; SUP mode also needs the edge
+  // index and the first hit word.
   uint32_t kl = c_lo + wave + PS * lane, yl = 0, el = 0, bl = 0, nl = 0;
-  bool fl = false;            // the pair's edge is r -> y (y is in the table)
   unsigned long long wl = 0;  // SUP mode: the pair's first hit word
   auto load_pair = [&]() {
     if (kl < c_hi) {
       const u32x2 pi = G.sinfo[p0 + kl];
       bl = pi.x;
-      nl = pi.y & ~kPairFwd;
-      fl = (pi.y & kPairFwd) != 0;
+      nl = pi.y;
       if (MODE != kModeTC) yl = G.ppar[p0 + kl];
       if (MODE == kModeSUP) {
         el = G.pedge[p0 + kl];
@@ -712,7 +710,6 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     if (t0 != 0) {
       kl = c_lo + wave + PS * (t0 + lane);
       yl = el = bl = nl = 0;
-      fl = false;
       wl = 0;
       load_pair();
     }
@@ -731,19 +728,13 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         stream_batch(std::integral_constant<uint32_t, 4u>(), nvalid, phl);
     }
     if (MODE != kModeTC && phl != 0) {
-      // The batch's pair credits, one lane per pair, all at once (one table lookup per
-      // lane rather than one wave-wide lookup per pair).  t(y) / the pair's edge: r->y adds
-      // to y's slot in the table; y->r goes to t(y) in LCC mode and to the edge's support in
-      // SUP mode, both atomically (each pair is visited once).
-      if (fl) {
-        uint32_t word, inc;
-        find(yl, word, inc);  // r -> y: y is a key of the table
-        atomicAdd(&cnt16[word], inc * phl);  // phl <= nA <= CAP < 2^16
-      } else if (MODE == kModeLCC) {
+      // The batch's pair credits, one lane per pair, all at once: the pair's edge y->r
+      // goes to t(y) in LCC mode and to the edge's support in SUP mode, atomically (other
+      // rows credit the same vertex meanwhile).
+      if (MODE == kModeLCC)
         gadd(&G.tvert[yl], (unsigned long long)phl);
-      } else {
+      else
         gadd(&G.sup[el], phl);
-      }
     }
     if (nvalid < 64) break;
   }
@@ -780,7 +771,7 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = SOLO ? 0u : tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
   const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
-  const PAC_GLOBAL uint32_t *A = G.oadj + a0;
+  const PAC_GLOBAL uint32_t *A = G.okey + a0;  // ranks: binary-searched
   if (c_hi > np) c_hi = np;
   uint32_t row_hits = 0;
   for_each_pair<SOLO ? 1 : WAVES, MODE>(G, p0, c_lo, c_hi, wave, lane,
@@ -788,11 +779,11 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
     uint32_t h = 0;
     if (MODE == kModeSUP) clear_hit_words(G, nb, p, lane);
     for (uint32_t j = lane; j < nb; j += 64) {
-      const uint32_t w = G.oadj[b0 + j];
+      const uint32_t w = G.okey[b0 + j];
       const uint32_t i = lower_bound_u32(A, nA, w);
       if (i < nA && A[i] == w) {
         ++h;
-        if (MODE == kModeLCC) gadd(&G.tvert[w], 1ull);  // t(w)
+        if (MODE == kModeLCC) gadd(&G.tvert[G.oadj[b0 + j]], 1ull);  // t(w)
         if (MODE == kModeSUP) gadd(&G.sup[a0 + i], 1u);  // edge r->w
         if (MODE == kModeSUP) gadd(&G.sup[b0 + j], 1u);  // edge y->w
       }
@@ -801,9 +792,9 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
       const uint32_t ch = wave_sum(h);
       if (lane == 0 && ch) {
         if (MODE == kModeLCC)
-          gadd(&G.tvert[y], (unsigned long long)ch);  // t(y), either direction
+          gadd(&G.tvert[y], (unsigned long long)ch);  // t(y)
         else
-          gadd(&G.sup[e], ch);  // the pair's edge, r->y or y->r
+          gadd(&G.sup[e], ch);  // the pair's edge y->r
       }
     }
     row_hits += h;
@@ -1546,18 +1537,20 @@ k_sup_big(const uint32_t *__restrict__ units, const uint32_t *__restrict__ ooff,
 
 
 // Device descriptor of a graph; its output pointers must already be allocated.
-GraphDev make_desc(Graph *g, uint32_t part_base, uint32_t es) {
+GraphDev make_desc(Graph *g, uint32_t part_base, uint32_t es, int mode) {
   GraphDev d;
   d.ooff = g->ooff.as<const uint32_t>();
   d.oadj = g->oadj.as<const uint32_t>();
+  d.okey = g->okey.as<const uint32_t>();
   d.poff = g->poff.as<const uint32_t>();
   d.ppar = g->ppar.as<const uint32_t>();
   d.pedge = g->pedge.as<const uint32_t>();
-  d.pinfo = g->pinfo.as<const uint2>();
+  // SUP mode streams from 64-element boundaries (its hit words are list columns)
+  d.pinfo = mode == kModeSUP ? g->pinfo64.as<const uint2>() : g->pinfo.as<const uint2>();
   // the 3-byte lists where the graph has them and the context streams them
   const bool packed = es == 3u;  // launch_elem_bytes checked that the lists exist
   d.sadj = packed ? g->opk.as<const uint8_t>() : g->oadj.as<const uint8_t>();
-  d.sinfo = packed ? g->pkinfo.as<const uint2>() : g->pinfo.as<const uint2>();
+  d.sinfo = packed ? g->pkinfo.as<const uint2>() : d.pinfo;
   d.selem = packed ? 3u : 4u;
   d.rows = g->rows.as<const uint32_t>();
   d.total = g->total.as<unsigned long long>();
@@ -1653,7 +1646,7 @@ int launch_plain(Ctx *ctx, Graph *g, int mode, const uint8_t *strategy_per_part,
                  uint64_t *launches) {
   Graph *gs[1] = {g};
   PAC_TRY(alloc_outputs(gs, 1, mode));
-  const GraphDev d = make_desc(g, 0, launch_elem_bytes(ctx, gs, 1, mode));
+  const GraphDev d = make_desc(g, 0, launch_elem_bytes(ctx, gs, 1, mode), mode);
   uint32_t k = 0;
   while (k < g->P) {
     uint32_t e = k + 1;
@@ -1680,7 +1673,7 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   bool any_long = false;
   const uint32_t cap = (uint32_t)kTiles[ctx->tile].cap;
   for (uint32_t i = 0; i < G; ++i) {
-    descs[i] = make_desc(gs[i], (uint32_t)nparts, es);
+    descs[i] = make_desc(gs[i], (uint32_t)nparts, es, mode);
     nparts += gs[i]->P;
     nitems += gs[i]->h_items.size();
     // rows over the tile's capacity, each queued at most once per item that holds it
@@ -1838,7 +1831,7 @@ int launch_persistent_noalloc(Ctx *ctx, Graph *g, int mode, const uint8_t *d_str
   PAC_CHECK_HIP(hipGetLastError());
   Graph *gs1[1] = {g};
   const uint32_t es = launch_elem_bytes(ctx, gs1, 1, mode);
-  GraphDev d = make_desc(g, 0, es);
+  GraphDev d = make_desc(g, 0, es, mode);
   d.total = d_total;
   d.tvert = mode == kModeLCC ? d_tvert : nullptr;
   d.sup = nullptr;

@@ -8,15 +8,20 @@
 //   is by (degree, id), which bounds every out-list by sqrt(2m); triangle counts do
 //   not depend on the orientation.
 //
-// Orientation = flags / scan / scatter into a uint32 CSR, edge-parallel throughout:
+// Orientation = flags / scan / keys / sort into a uint32 CSR, edge-parallel throughout:
 //   1. one keep bit per symmetric entry (u, v): (d(u),u) < (d(v),v), written as aligned
 //      64-bit words (word k holds entries 64k .. 64k+63); each lane finds its row by a
 //      short search between the rows of its chunk's first entry and the next chunk's;
 //   2. exclusive scan of the words' popcounts: the kept entry i goes to oadj position
-//      wpre[i/64] + (kept entries below it in its word), so
-//   3. the scatter needs no row at all, keeps every row's input (ID) order, and
-//      ooff[u] is that same ordinal of off[u].
+//      wpre[i/64] + (kept entries below it in its word), and ooff[u] is that same
+//      ordinal of off[u];
+//   3. each kept entry (u, v) writes the key (u, rank(v)) at its position, rank(v) being
+//      v's place among all vertices sorted by (degree, id), and one radix sort of the keys
+//      puts every row's out-neighbours in rank order -- the order of the orientation
+//      itself, so the neighbours of N+(t) that rank above one of them, h, are a suffix of
+//      the list: the only part of N+(t) that can meet N+(h).
 // The intersection pairs are built the same way from one ownership bit per entry.
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
@@ -211,38 +216,99 @@ __global__ void k_ordinal_offsets(const uint32_t *__restrict__ off,
   }
 }
 
-// 3. scatter: thread per entry, each wave on one aligned word (the grid stride is a
-// multiple of 64), the kept entries written in input order.
+// (degree, id) keys of the vertices; sorted, their low words list the vertices in rank
+// order.
+__global__ void k_vertex_keys(const uint32_t *__restrict__ deg, uint32_t V,
+                              unsigned long long *__restrict__ keys) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < V;
+       u += (uint64_t)gridDim.x * blockDim.x)
+    keys[u] = ((unsigned long long)deg[u] << 32) | u;
+}
+
+// rank[v] = the position of v in (degree, id) order, from the sorted keys.
+__global__ void k_rank_scatter(const unsigned long long *__restrict__ keys, uint32_t V,
+                               uint32_t *__restrict__ rank) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    rank[(uint32_t)keys[i]] = (uint32_t)i;
+}
+
+// 3. the key of each kept entry (t, v) at its oadj position: t << L | rank(v).  One wave
+// per chunk, as k_orient_flags (the row is needed for the key).
 __global__ void __launch_bounds__(256)
-k_orient_scatter(const uint32_t *__restrict__ adj, const unsigned long long *__restrict__ flags,
-                 const uint32_t *__restrict__ wpre, uint32_t nnz, uint32_t *__restrict__ oadj) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nnz;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t k = i >> 6;
-    const unsigned long long bits = flags[k];
-    if (bit_of(bits, (uint32_t)(i & 63u))) oadj[wpre[k] + lanes_below(bits)] = adj[i];
+k_orient_keys(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
+              const uint32_t *__restrict__ rank, const uint32_t *__restrict__ crow,
+              const unsigned long long *__restrict__ flags, const uint32_t *__restrict__ wpre,
+              uint32_t V, uint32_t nnz, uint32_t L, unsigned long long *__restrict__ keys) {
+  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
+  const uint64_t nchunks = num_chunks(nnz);
+  for (uint64_t c = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); c < nchunks;
+       c += (uint64_t)gridDim.x * wpb) {
+    uint32_t lo, hi;
+    chunk_rows(crow, c, nchunks, V, lo, hi);
+#pragma unroll
+    for (uint32_t j = 0; j < kChunkWords; ++j) {
+      const uint64_t k = c * kChunkWords + j;
+      const uint64_t i = 64u * k + lane;
+      if (64u * k >= nnz) break;  // uniform
+      const unsigned long long bits = flags[k];
+      if (i < nnz && bit_of(bits, lane)) {
+        const uint32_t t = row_of(off, lo, hi, (uint32_t)i);
+        keys[wpre[k] + lanes_below(bits)] = ((unsigned long long)t << L) | rank[adj[i]];
+      }
+    }
+  }
+}
+
+// After the sort: okey = the key's rank field, oadj = the vertex of that rank.
+__global__ void k_orient_unpack(const unsigned long long *__restrict__ keys, uint64_t m,
+                                uint32_t L, const unsigned long long *__restrict__ vkeys,
+                                uint32_t *__restrict__ okey, uint32_t *__restrict__ oadj) {
+  const unsigned long long mask = (1ull << L) - 1ull;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < m;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = (uint32_t)(keys[j] & mask);
+    okey[j] = r;
+    oadj[j] = (uint32_t)vkeys[r];
   }
 }
 
 // ---- intersection pairs --------------------------------------------------------------
-// The oriented edge t->h stands for the intersection N+(t) & N+(h).  Vertex x owns the
-// pair {x,u} (u in N(x)) when x's out-list is the longer of the two, ties going to the
-// tail:  x->u: d+(u) <= d+(x) (and d+(u) > 0, else the pair holds no triangle);
-//        u->x: d+(x) >  d+(u).
-// Exactly one endpoint owns each edge with d+(h) > 0 or d+(t) < d+(h).
-__device__ __forceinline__ bool owns(bool fwd, uint32_t ox, uint32_t ou) {
-  return fwd ? (ou <= ox && ou > 0u) : (ox > ou);
+// The oriented edge t->h stands for the intersection N+(t) & N+(h).  Every element of
+// N+(h) ranks above h, and N+(t) is in rank order with h in it, so only the suffix of
+// N+(t) after h can meet N+(h): the head h owns the pair, stages N+(h) and streams that
+// suffix.  Row x therefore owns the entries (x, u) with u->x (entries the orientation did
+// not keep), provided d+(x) > 0 and x is not the last element of N+(u) -- otherwise the
+// pair holds no triangle and is dropped.
+
+// The position of x (of rank rx) in N+(u) = okey[a .. a + n), which holds rx.
+__device__ __forceinline__ uint32_t list_pos(const uint32_t *__restrict__ okey, uint32_t a,
+                                             uint32_t n, uint32_t rx) {
+  uint32_t lo = a;
+  while (n > 0) {
+    const uint32_t half = n >> 1;
+    if (okey[lo + half] < rx) {
+      lo += half + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  return lo - a;
 }
 
-// Ownership bits of the entries (x, u) of row x, and each row's streamed length
-// rowwork[x] = sum of d+(u) over its pairs (rowwork zeroed before): the lanes of one row
-// are contiguous, so a segmented scan across the wave leaves each row's part of the word
-// in its last lane, which adds it.  fwd = the entry's keep bit from the orientation.
+// Ownership bits of the entries (x, u) of row x, the position of x in N+(u) of each owned
+// entry (epos[i]), and each row's streamed length rowwork[x] = the sum of its pairs'
+// suffix lengths (rowwork zeroed before): the lanes of one row are contiguous, so a
+// segmented scan across the wave leaves each row's part of the word in its last lane,
+// which adds it.
 __global__ void __launch_bounds__(256)
 k_pair_flags(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
-             const uint32_t *__restrict__ odeg, const unsigned long long *__restrict__ oflags,
+             const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ okey,
+             const uint32_t *__restrict__ rank, const unsigned long long *__restrict__ oflags,
              const uint32_t *__restrict__ crow, uint32_t V, uint32_t nnz,
-             unsigned long long *__restrict__ pflags, uint32_t *__restrict__ rowwork) {
+             unsigned long long *__restrict__ pflags, uint32_t *__restrict__ epos,
+             uint32_t *__restrict__ rowwork) {
   const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
   const uint64_t nchunks = num_chunks(nnz);
   for (uint64_t c = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); c < nchunks;
@@ -257,9 +323,16 @@ k_pair_flags(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
       uint32_t x = 0xFFFFFFFFu, w = 0;  // lanes past the end: no row
       if (i < nnz) {
         x = row_of(off, lo, hi, (uint32_t)i);
-        const uint32_t ou = odeg[adj[i]];
-        own = owns(bit_of(oflags[k], lane), odeg[x], ou);
-        w = own ? ou : 0u;
+        if (!bit_of(oflags[k], lane) && ooff[x + 1] > ooff[x]) {  // u -> x, N+(x) not empty
+          const uint32_t u = adj[i];
+          const uint32_t a = ooff[u], n = ooff[u + 1] - a;
+          const uint32_t pos = list_pos(okey, a, n, rank[x]);
+          if (pos + 1u < n) {
+            own = true;
+            w = n - pos - 1u;
+            epos[i] = pos;
+          }
+        }
       }
       const unsigned long long mask = __ballot(own);
       if (lane == 0 && 64u * k < nnz) pflags[k] = mask;
@@ -277,29 +350,26 @@ k_pair_flags(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
 
 // Pair lists: thread per entry (each wave on one aligned word); an owned entry (x, u)
 // becomes pair ordinal(i) of the ownership bits, in input order, so each row's pairs
-// come out sorted by partner.  The pair's edge index: for x->u the entry's position in
-// oadj (its orientation ordinal); for u->x kBackEdge until k_back_edges finds it.  pinfo:
-// u's out-list (first position, length | kPairFwd for x->u).
+// come out sorted by partner ID.  pedge = the position of u->x in oadj, pinfo = (the
+// position after it, the suffix length).
 __global__ void __launch_bounds__(256)
-k_pair_fill(const uint32_t *__restrict__ adj, const unsigned long long *__restrict__ oflags,
-            const uint32_t *__restrict__ owpre, const unsigned long long *__restrict__ pflags,
+k_pair_fill(const uint32_t *__restrict__ adj, const unsigned long long *__restrict__ pflags,
             const uint32_t *__restrict__ ppre, const uint32_t *__restrict__ ooff,
-            uint32_t nnz, uint32_t *__restrict__ ppar, uint32_t *__restrict__ pedge,
-            uint2 *__restrict__ pinfo) {
+            const uint32_t *__restrict__ epos, uint32_t nnz, uint32_t *__restrict__ ppar,
+            uint32_t *__restrict__ pedge, uint2 *__restrict__ pinfo) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nnz;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t k = i >> 6;
     const uint32_t b = (uint32_t)(i & 63u);
     const unsigned long long pb = pflags[k];
     if (bit_of(pb, b)) {
-      const unsigned long long ob = oflags[k];
       const uint32_t p = ppre[k] + lanes_below(pb);
       const uint32_t u = adj[i];
-      const bool fwd = bit_of(ob, b);
+      const uint32_t a = ooff[u], n = ooff[u + 1] - a;  // mostly one line per pair
+      const uint32_t e = a + epos[i];
       ppar[p] = u;
-      pedge[p] = fwd ? owpre[k] + lanes_below(ob) : kBackEdge;
-      const uint32_t a = ooff[u];  // both bounds from one array: mostly one line per pair
-      pinfo[p] = make_uint2(a, (ooff[u + 1] - a) | (fwd ? kPairFwd : 0u));
+      pedge[p] = e;
+      pinfo[p] = make_uint2(e + 1u, a + n - e - 1u);
     }
   }
 }
@@ -330,53 +400,18 @@ k_pack24(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj, u
   }
 }
 
-// Per pair: where the partner's 3-byte list starts (a 4-byte word of opk), with pinfo.y.
+// Per pair: where its stream starts in the 3-byte lists.  Element 4j of a list starts
+// word 3j of it, so the suffix start s (within the partner's list) is rounded down to a
+// multiple of 4 and the length grows by the s mod 4 elements before it, all ranked at most
+// the owner and so never found in its list.
 __global__ void __launch_bounds__(256)
-k_pack_info(const uint32_t *__restrict__ ppar, const uint2 *__restrict__ pinfo, uint32_t np,
-            uint2 *__restrict__ pkinfo) {
+k_pack_info(const uint32_t *__restrict__ ppar, const uint2 *__restrict__ pinfo,
+            const uint32_t *__restrict__ ooff, uint32_t np, uint2 *__restrict__ pkinfo) {
   for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < np;
        p += (uint64_t)gridDim.x * blockDim.x) {
     const uint2 pi = pinfo[p];
-    pkinfo[p] = make_uint2((uint32_t)pack_word(pi.x, ppar[p]), pi.y);
-  }
-}
-
-// Edge indices of the pairs owned by their head (u->x, entry (x, u) of row x): the
-// position of x in N+(u), found by a binary search of the sorted oriented row.
-__global__ void __launch_bounds__(256)
-k_back_edges(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
-             const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
-             const unsigned long long *__restrict__ oflags,
-             const unsigned long long *__restrict__ pflags, const uint32_t *__restrict__ ppre,
-             const uint32_t *__restrict__ crow, uint32_t V, uint32_t nnz,
-             uint32_t *__restrict__ pedge) {
-  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
-  const uint64_t nchunks = num_chunks(nnz);
-  for (uint64_t c = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); c < nchunks;
-       c += (uint64_t)gridDim.x * wpb) {
-    uint32_t lo, hi;
-    chunk_rows(crow, c, nchunks, V, lo, hi);
-    for (uint32_t j = 0; j < kChunkWords; ++j) {
-      const uint64_t k = c * kChunkWords + j;
-      const uint64_t i = 64u * k + lane;
-      if (64u * k >= nnz) break;  // uniform
-      const unsigned long long pb = pflags[k], ob = oflags[k];
-      if (i < nnz && bit_of(pb, lane) && !bit_of(ob, lane)) {
-        const uint32_t x = row_of(off, lo, hi, (uint32_t)i);
-        const uint32_t u = adj[i];
-        uint32_t a = ooff[u], n = ooff[u + 1] - a;  // x is in N+(u): lower_bound finds it
-        while (n > 0) {
-          const uint32_t half = n >> 1;
-          if (oadj[a + half] < x) {
-            a += half + 1;
-            n -= half + 1;
-          } else {
-            n = half;
-          }
-        }
-        pedge[ppre[k] + lanes_below(pb)] = a;
-      }
-    }
+    const uint32_t y = ppar[p], a = ooff[y], s = pi.x - a;
+    pkinfo[p] = make_uint2((uint32_t)(pack_word(a, y) + 3u * (s >> 2)), pi.y + (s & 3u));
   }
 }
 
@@ -501,16 +536,62 @@ int orient(Graph *g) {
   }
   // m entries plus one sentinel (kListEnd) that list streams past their end read
   PAC_TRY(g->oadj.reserve(sizeof(uint32_t) * (g->m + 1)));
-  if (nnz) {
-    hipLaunchKernelGGL(k_orient_scatter, dim3(grid_for_n(nnz, 256 * 4)), dim3(256), 0,
-                       ctx->stream, g->adj.as<const uint32_t>(),
-                       g->oflags.as<const unsigned long long>(), g->owpre.as<const uint32_t>(),
-                       (uint32_t)nnz, g->oadj.as<uint32_t>());
+  PAC_TRY(g->okey.reserve(sizeof(uint32_t) * (g->m + 1)));
+  PAC_TRY(g->rank.reserve(sizeof(uint32_t) * (V ? V : 1)));
+  if (V) {
+    // vertex ranks: the (degree, id) keys sorted; L bits hold a vertex or a rank
+    uint32_t L = 1;
+    while (L < 32 && (1ull << L) < V) ++L;
+    uint32_t dbits = 0;
+    while (dbits < 32 && (1ull << dbits) <= g->max_deg) ++dbits;
+    DevBuf vk, vks, tmp;
+    PAC_TRY(vk.reserve(sizeof(unsigned long long) * V));
+    PAC_TRY(vks.reserve(sizeof(unsigned long long) * V));
+    hipLaunchKernelGGL(k_vertex_keys, dim3(grid_for_n(V, 256 * 4)), dim3(256), 0, ctx->stream,
+                       g->deg.as<const uint32_t>(), V, vk.as<unsigned long long>());
     PAC_CHECK_HIP(hipGetLastError());
+    size_t tb = 0;
+    PAC_CHECK_HIP(rocprim::radix_sort_keys(nullptr, tb, vk.as<uint64_t>(), vks.as<uint64_t>(),
+                                           (size_t)V, 0u, 32u + dbits, ctx->stream));
+    PAC_TRY(tmp.reserve(tb));
+    PAC_CHECK_HIP(rocprim::radix_sort_keys(tmp.p, tb, vk.as<uint64_t>(), vks.as<uint64_t>(),
+                                           (size_t)V, 0u, 32u + dbits, ctx->stream));
+    hipLaunchKernelGGL(k_rank_scatter, dim3(grid_for_n(V, 256 * 4)), dim3(256), 0, ctx->stream,
+                       vks.as<const unsigned long long>(), V, g->rank.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+    if (g->m) {
+      // every kept entry's key (row, rank of the neighbour) at its position, one sort
+      DevBuf ek, eks;
+      PAC_TRY(ek.reserve(sizeof(unsigned long long) * g->m));
+      PAC_TRY(eks.reserve(sizeof(unsigned long long) * g->m));
+      hipLaunchKernelGGL(k_orient_keys, dim3(grid_for_n(nchunks, 4)), dim3(256), 0, ctx->stream,
+                         g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
+                         g->rank.as<const uint32_t>(), g->crow.as<const uint32_t>(),
+                         g->oflags.as<const unsigned long long>(), g->owpre.as<const uint32_t>(),
+                         V, (uint32_t)nnz, L, ek.as<unsigned long long>());
+      PAC_CHECK_HIP(hipGetLastError());
+      tb = 0;
+      PAC_CHECK_HIP(rocprim::radix_sort_keys(nullptr, tb, ek.as<uint64_t>(), eks.as<uint64_t>(),
+                                             (size_t)g->m, 0u, 2u * L, ctx->stream));
+      if (tb > tmp.bytes) {
+        PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // the first sort may still run
+        tmp.release();
+        PAC_TRY(tmp.reserve(tb));
+      }
+      PAC_CHECK_HIP(rocprim::radix_sort_keys(tmp.p, tb, ek.as<uint64_t>(), eks.as<uint64_t>(),
+                                             (size_t)g->m, 0u, 2u * L, ctx->stream));
+      hipLaunchKernelGGL(k_orient_unpack, dim3(grid_for_n(g->m, 256 * 4)), dim3(256), 0,
+                         ctx->stream, eks.as<const unsigned long long>(), g->m, L,
+                         vks.as<const unsigned long long>(), g->okey.as<uint32_t>(),
+                         g->oadj.as<uint32_t>());
+      PAC_CHECK_HIP(hipGetLastError());
+    }
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // the temporaries are freed on return
   }
   PAC_TRY(fill_u32(ctx->stream, g->oadj.as<uint32_t>() + g->m, 1, kListEnd));
+  PAC_TRY(fill_u32(ctx->stream, g->okey.as<uint32_t>() + g->m, 1, kListEnd));
   g->pairs_built = false;
-  g->back_edges_built = false;
+  g->hitw_built = false;
   return PAC_OK;
 }
 
@@ -526,12 +607,16 @@ int build_pairs(Graph *g) {
   PAC_TRY(g->rowwork.reserve(sizeof(uint32_t) * (V ? V : 1)));
   PAC_TRY(fill_u32(ctx->stream, g->pflags.as<uint32_t>() + 2 * nwords, 2, 0u));
   PAC_TRY(fill_u32(ctx->stream, g->rowwork.as<uint32_t>(), V, 0u));
+  DevBuf epos;  // per symmetric entry owned as a pair: the owner's position in the partner's list
+  PAC_TRY(epos.reserve(sizeof(uint32_t) * (nnz ? nnz : 1)));
   if (nnz) {
     hipLaunchKernelGGL(k_pair_flags, dim3(grid_for_n(nchunks, 4)), dim3(256), 0, ctx->stream,
                        g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
-                       g->odeg.as<const uint32_t>(), g->oflags.as<const unsigned long long>(),
+                       g->ooff.as<const uint32_t>(), g->okey.as<const uint32_t>(),
+                       g->rank.as<const uint32_t>(), g->oflags.as<const unsigned long long>(),
                        g->crow.as<const uint32_t>(), V, (uint32_t)nnz,
-                       g->pflags.as<unsigned long long>(), g->rowwork.as<uint32_t>());
+                       g->pflags.as<unsigned long long>(), epos.as<uint32_t>(),
+                       g->rowwork.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
   uint32_t np = 0;
@@ -559,11 +644,10 @@ int build_pairs(Graph *g) {
   PAC_TRY(g->pinfo.reserve(sizeof(uint2) * (np ? np : 1)));
   if (nnz) {
     hipLaunchKernelGGL(k_pair_fill, dim3(grid_for_n(nnz, 256 * 4)), dim3(256), 0, ctx->stream,
-                       g->adj.as<const uint32_t>(), g->oflags.as<const unsigned long long>(),
-                       g->owpre.as<const uint32_t>(), g->pflags.as<const unsigned long long>(),
+                       g->adj.as<const uint32_t>(), g->pflags.as<const unsigned long long>(),
                        g->ppre.as<const uint32_t>(), g->ooff.as<const uint32_t>(),
-                       (uint32_t)nnz, g->ppar.as<uint32_t>(), g->pedge.as<uint32_t>(),
-                       g->pinfo.as<uint2>());
+                       epos.as<const uint32_t>(), (uint32_t)nnz, g->ppar.as<uint32_t>(),
+                       g->pedge.as<uint32_t>(), g->pinfo.as<uint2>());
     PAC_CHECK_HIP(hipGetLastError());
   }
   if (V <= kPackMaxV) {
@@ -580,30 +664,12 @@ int build_pairs(Graph *g) {
     if (np)
       hipLaunchKernelGGL(k_pack_info, dim3(grid_for_n(np, 256 * 4)), dim3(256), 0,
                          ctx->stream, g->ppar.as<const uint32_t>(), g->pinfo.as<const uint2>(),
-                         np, g->pkinfo.as<uint2>());
+                         g->ooff.as<const uint32_t>(), np, g->pkinfo.as<uint2>());
     PAC_CHECK_HIP(hipGetLastError());
   }
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // epos is freed on return
   g->pairs_built = true;
-  g->back_edges_built = false;
-  return PAC_OK;
-}
-
-// The edge indices of the pairs owned by their head (needed by the per-edge supports).
-int build_back_edges(Graph *g) {
-  if (g->back_edges_built) return PAC_OK;
-  PAC_TRY(build_pairs(g));
-  const uint64_t nnz = g->nnz, nchunks = num_chunks(nnz);
-  if (nnz) {
-    hipLaunchKernelGGL(k_back_edges, dim3(grid_for_n(nchunks, 4)), dim3(256), 0, g->ctx->stream,
-                       g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
-                       g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
-                       g->oflags.as<const unsigned long long>(),
-                       g->pflags.as<const unsigned long long>(), g->ppre.as<const uint32_t>(),
-                       g->crow.as<const uint32_t>(), g->V, (uint32_t)nnz,
-                       g->pedge.as<uint32_t>());
-    PAC_CHECK_HIP(hipGetLastError());
-  }
-  g->back_edges_built = true;
+  g->hitw_built = false;
   return PAC_OK;
 }
 
@@ -755,18 +821,28 @@ __global__ void k_word_counts(const uint32_t *__restrict__ R, const uint32_t *__
     w[y] = y < V ? (unsigned long long)R[y] * ((odeg[y] + 63u) >> 6) : 0ull;
 }
 
-// First hit word of each pair: its partner's word base plus its rank among the pairs
-// streaming that partner times the words per stream.  The ranks come in any order (the
-// words of all of a vertex's streams are summed, so only their distinctness matters).
-__global__ void k_pair_words(const uint32_t *__restrict__ ppar, uint64_t np,
-                             const uint32_t *__restrict__ odeg,
+// SUP-mode stream of each pair and its first hit word.  The stream starts at the
+// 64-element boundary of N+(y) at or before the pair's suffix (the elements before the
+// suffix rank at most the owner and are never found), so each 64-element block of the
+// stream is a word column c of the list.  The pair's words: its partner's word base,
+// plus its rank among the pairs streaming that partner times the words per stream, plus
+// the column of its first block -- the columns before it are never written and stay zero.
+// The ranks come in any order (the words of all of a vertex's streams are summed, so only
+// their distinctness matters).
+__global__ void k_pair_words(const uint32_t *__restrict__ ppar, const uint2 *__restrict__ pinfo,
+                             uint64_t np, const uint32_t *__restrict__ ooff,
                              const unsigned long long *__restrict__ wbase,
-                             uint32_t *__restrict__ cursor, unsigned long long *__restrict__ pword) {
+                             uint32_t *__restrict__ cursor, unsigned long long *__restrict__ pword,
+                             uint2 *__restrict__ pinfo64) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
        p += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t y = ppar[p];
     const uint32_t r = atomicAdd(&cursor[y], 1u);
-    pword[p] = wbase[y] + (unsigned long long)r * ((odeg[y] + 63u) >> 6);
+    const uint32_t a = ooff[y], d = ooff[y + 1] - a;
+    const uint2 pi = pinfo[p];
+    const uint32_t c0 = (pi.x - a) >> 6;  // the column of the stream's first block
+    pword[p] = wbase[y] + (unsigned long long)r * ((d + 63u) >> 6) + c0;
+    pinfo64[p] = make_uint2(a + 64u * c0, d - 64u * c0);
   }
 }
 
@@ -779,13 +855,14 @@ uint32_t grid_elems(uint64_t n) {
 
 int build_hit_words(Graph *g) {
   if (g->hitw_built) return PAC_OK;
-  PAC_TRY(build_back_edges(g));
+  PAC_TRY(build_pairs(g));
   Ctx *ctx = g->ctx;
   const uint32_t V = g->V;
   uint32_t np = 0;
   PAC_CHECK_HIP(hipMemcpy(&np, g->poff.as<uint32_t>() + V, 4, hipMemcpyDeviceToHost));
   PAC_TRY(g->pstreams.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
   PAC_TRY(g->pword.reserve(sizeof(unsigned long long) * (np ? np : 1)));
+  PAC_TRY(g->pinfo64.reserve(sizeof(uint2) * (np ? np : 1)));
   PAC_TRY(g->wbase.reserve(sizeof(unsigned long long) * ((uint64_t)V + 1)));
   DevBuf words, cursor;
   PAC_TRY(words.reserve(sizeof(unsigned long long) * ((uint64_t)V + 1)));
@@ -811,9 +888,10 @@ int build_hit_words(Graph *g) {
                                         rocprim::plus<unsigned long long>(), ctx->stream));
   if (np) {
     hipLaunchKernelGGL(k_pair_words, dim3(grid_elems(np)), dim3(256), 0, ctx->stream,
-                       g->ppar.as<const uint32_t>(), (uint64_t)np, g->odeg.as<const uint32_t>(),
-                       g->wbase.as<const unsigned long long>(), cursor.as<uint32_t>(),
-                       g->pword.as<unsigned long long>());
+                       g->ppar.as<const uint32_t>(), g->pinfo.as<const uint2>(), (uint64_t)np,
+                       g->ooff.as<const uint32_t>(), g->wbase.as<const unsigned long long>(),
+                       cursor.as<uint32_t>(), g->pword.as<unsigned long long>(),
+                       g->pinfo64.as<uint2>());
     PAC_CHECK_HIP(hipGetLastError());
   }
   unsigned long long nwords = 0;
@@ -824,8 +902,8 @@ int build_hit_words(Graph *g) {
     PAC_CHECK_HIP(hipMemcpyAsync(R.data(), g->pstreams.p, 4ull * V, hipMemcpyDeviceToHost,
                                  ctx->stream));
   PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // words and cursor are freed on return
-  // Every pair writes all of its words in each SUP launch; the zero fill only makes the
-  // words of a graph that has never been solved well defined.
+  // Every pair writes the words of its stream's columns in each SUP launch; the zero fill
+  // gives the columns before its stream their value (no triangle) for good.
   // 8 words of padding: the column sums load 8 words per stream from any word position
   PAC_TRY(g->hitw.reserve(sizeof(unsigned long long) * (nwords + 8)));
   PAC_TRY(fill_u32(ctx->stream, g->hitw.as<uint32_t>(), 2ull * (nwords + 8), 0u));

@@ -313,38 +313,36 @@ def test_greedy_partition_2_minimax(P, orc):
 
 
 def brute_pairs(g):
-    """Pair ownership walked per vertex over its symmetric row, as the device builds it
-    (x owns {x, u} when x's out-list is the longer, ties to the tail; empty partner lists
-    hold no pair) -- a different formulation from the oracle's per-edge minimum."""
+    """Pair ownership walked per owner row over its symmetric row, as the device builds it
+    (x owns {x, u} when u->x, d+(x) > 0 and some out-neighbour of u ranks above x), the
+    suffix counted with Python sets -- a different formulation from the oracle's sorted
+    positions per tail."""
     d = g.deg()
     before = lambda a, b: (d[a], a) < (d[b], b)
-    dplus = np.zeros(g.V, np.int64)
-    for x in range(g.V):
-        for u in g.elist[g.vlist[x]:g.vlist[x + 1]].tolist():
-            dplus[x] += before(x, u)
+    nb = [set(g.elist[g.vlist[x]:g.vlist[x + 1]].tolist()) for x in range(g.V)]
+    outs = [{w for w in nb[x] if before(x, w)} for x in range(g.V)]
     pairs = rows = streamed = staged = 0
     for x in range(g.V):
         own = 0
-        for u in g.elist[g.vlist[x]:g.vlist[x + 1]].tolist():
-            if before(x, u):
-                mine = dplus[u] <= dplus[x] and dplus[u] > 0
-            else:
-                mine = dplus[x] > dplus[u]
-            if mine:
+        for u in nb[x]:
+            if not before(u, x) or not outs[x]:
+                continue
+            suffix = sum(1 for w in outs[u] if before(x, w))
+            if suffix:
                 own += 1
-                streamed += int(dplus[u])
+                streamed += suffix
         pairs += own
         if own:
             rows += 1
-            staged += int(dplus[x])
+            staged += len(outs[x])
     return {"pairs": pairs, "rows": rows, "streamed": streamed, "staged": staged}
 
 
 @pytest.mark.parametrize("n", [3, 4, 7, 12])
 def test_pair_stats_complete_graph(n, orc):
-    """K_n orients by ID (all degrees equal), d+(i) = n-1-i: every streamed element is a
-    hit, so the streamed total is C(n,3); pairs are the edges i<j<=n-2 (C(n-1,2)), owned
-    by rows 0..n-3 staging sum of (n-1-i)."""
+    """K_n orients by ID (all degrees equal), d+(i) = n-1-i: the pair t->h streams the
+    n-1-h IDs above h, every one a hit, so the streamed total is C(n,3); pairs are the
+    edges t<h<=n-2 (C(n-1,2)), owned by the heads 1..n-2 staging sum of (n-1-h)."""
     z = np.load(os.path.join(GOLDEN, "K12.npz")) if n == 12 else None
     g = orc.CSR(z["vlist"], z["elist"]) if z is not None else orc.gen_er(n, n - 1, seed=1)
     assert g.nnz == n * (n - 1)
@@ -352,7 +350,7 @@ def test_pair_stats_complete_graph(n, orc):
     assert s["streamed"] == math.comb(n, 3)
     assert s["pairs"] == math.comb(n - 1, 2)
     assert s["rows"] == max(0, n - 2)
-    assert s["staged"] == sum(n - 1 - i for i in range(n - 2))
+    assert s["staged"] == sum(n - 1 - h for h in range(1, n - 1))
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
