@@ -63,6 +63,8 @@ namespace {
 // counter: every wait for an LDS result would then wait for the outstanding HBM loads.
 #define PAC_GLOBAL __attribute__((address_space(1)))
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+// four IDs at any 4-byte boundary (a typedef may lower the alignment)
+typedef unsigned int u32x4a __attribute__((ext_vector_type(4), aligned(4)));
 struct GDev {
   const PAC_GLOBAL uint32_t *ooff, *oadj, *okey, *poff, *ppar, *pedge, *rows;
   const PAC_GLOBAL u32x2 *pinfo;
@@ -109,6 +111,8 @@ __device__ __forceinline__ void gadd(PAC_GLOBAL unsigned long long *p, unsigned 
 
 constexpr uint32_t kAllC = 0xFFFFFFFFu;
 constexpr uint32_t kPassPairs = 57344;  // pairs per hash pass (bounds the 16-bit counters)
+// TC/LCC hash path: pairs streaming at most this many elements go through the chunk path.
+constexpr uint32_t kChunkMax = 64;
 
 constexpr uint32_t pow2_at_least(uint32_t x) { return x <= 1 ? 1u : 2u * pow2_at_least((x + 1) / 2); }
 
@@ -166,6 +170,8 @@ struct alignas(8) Smem {
   uint32_t defer_slot;
   uint32_t homeless;
   uint32_t rowtot;  // the row's triangle count (LCC mode)
+  // LCC mode: per wave, the hit counts of its batch's small pairs (chunk path of row_hash)
+  uint32_t pcnt[MODE == kModeLCC ? WAVES : 1][MODE == kModeLCC ? 64 : 1];
 };
 
 __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
@@ -454,14 +460,16 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   constexpr uint32_t PS = SOLO ? 1u : (uint32_t)WAVES;  // pair stride between its waves
   const uint32_t tid = SOLO ? (threadIdx.x & 63u) : threadIdx.x;  // rank on the row
   const uint32_t lane = threadIdx.x & 63u, wave = SOLO ? 0u : threadIdx.x >> 6;
+  auto wave_sync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
   auto sync = [&]() {
-    if (SOLO) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
+    if (SOLO)
+      wave_sync();
+    else
       __syncthreads();
-    }
   };
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
   const uint32_t p0 = G.poff[r], np = G.poff[r + 1] - p0;
@@ -475,12 +483,20 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // through ooff), and when counts are kept the partner; SUP mode also needs the edge
   // index and the first hit word.
   uint32_t kl = c_lo + wave + PS * lane, yl = 0, el = 0, bl = 0, nl = 0;
+  uint32_t b4 = 0, n4 = 0;    // the pair's exact stream in oadj (its 4-byte list)
   unsigned long long wl = 0;  // SUP mode: the pair's first hit word
   auto load_pair = [&]() {
     if (kl < c_hi) {
       const u32x2 pi = G.sinfo[p0 + kl];
       bl = pi.x;
       nl = pi.y;
+      b4 = bl;
+      n4 = nl;
+      if (MODE != kModeSUP && (ESF == 3 || (ESF == 0 && G.selem == 3u))) {
+        const u32x2 q = G.pinfo[p0 + kl];  // 3-byte streams start on a 4-element boundary
+        b4 = q.x;
+        n4 = q.y;
+      }
       if (MODE != kModeTC) yl = G.ppar[p0 + kl];
       if (MODE == kModeSUP) {
         el = G.pedge[p0 + kl];
@@ -706,26 +722,110 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       if (jj < nvalid) step(sb, sa);
     } while ((uint32_t)__builtin_amdgcn_readfirstlane((int)jj) < nvalid);
   };
+  // Small pairs (at most kChunkMax elements) in chunks of four elements spread over the
+  // lanes: the batch's small pairs are cut into 4-element chunks of their 4-byte lists,
+  // numbered pair after pair (an exclusive scan of the chunk counts over the pair lanes),
+  // and lane l takes chunks l, l + 64, ...: its pair is the last pair lane whose first chunk
+  // is at most its chunk (a 6-step binary search across the lanes), its elements one
+  // 16-byte load, each looked up in the table as in the wave path.  Such pairs are half of
+  // the pairs but 7 % of the streamed elements at R-MAT scale 24 (the hubs' in-edges,
+  // whose suffixes above the hub are short); in the wave path each would take a whole step
+  // of four 64-element sub-blocks for a few lookups.  Returns the pair hit count of the
+  // lane's own pair (LCC mode; summed per pair in the wave's pcnt row in LDS).
+  auto chunk_path = [&](bool small) -> uint32_t {
+    const uint32_t c = small ? (n4 + 3u) >> 2 : 0u;
+    uint32_t incl = c;
+#pragma unroll
+    for (uint32_t sft = 1; sft < 64; sft <<= 1) {
+      const uint32_t o = (uint32_t)__shfl_up((int)incl, sft, 64);
+      if (lane >= sft) incl += o;
+    }
+    const uint32_t P = incl - c;  // the pair's first chunk
+    const uint32_t C = readlane(incl, 63);
+    uint32_t *pc = nullptr;
+    if constexpr (MODE == kModeLCC) {
+      pc = sm.pcnt[threadIdx.x >> 6];
+      pc[lane] = 0;
+      wave_sync();
+    }
+    auto bperm = [](uint32_t from, uint32_t v) -> uint32_t {
+      return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(4u * from), (int)v);
+    };
+    for (uint32_t q0 = 0; q0 < C; q0 += 64) {  // wave-uniform
+      const uint32_t q = q0 + lane;
+      uint32_t i = 0;
+#pragma unroll
+      for (uint32_t st = 32; st; st >>= 1)
+        if (bperm(i + st, P) <= q) i += st;
+      const uint32_t e0 = 4u * (q - bperm(i, P)), bi = bperm(i, b4), ni = bperm(i, n4);
+      const bool act = q < C;
+      u32x4a v = {kListEnd, kListEnd, kListEnd, kListEnd};
+      if (act) v = *(const PAC_GLOBAL u32x4a *)(G.oadj + bi + e0);  // oadj is padded
+      uint32_t h = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const Probe q4 = probe(act && e0 + u < ni ? v[u] : kListEnd);  // kListEnd: no key
+        const bool hit = q4.e10 | q4.e11 | q4.e20 | q4.e21;
+        row_hits += (uint32_t)__popcll(__ballot(hit));
+        if (MODE == kModeLCC) {
+          h += hit ? 1u : 0u;
+          if (hit) {
+            const uint32_t ca = ((q4.e10 | q4.e11) ? q4.a1 : q4.a2) >> 1;
+            const uint32_t inc = (q4.e11 | q4.e21) ? 0x10000u : 1u;
+            asm volatile("ds_add_u32 %0, %1 offset:%2" : : "v"(ca), "v"(inc), "i"(kCntBase)
+                         : "memory");
+          }
+        }
+      }
+      if (MODE == kModeLCC && h != 0u) atomicAdd(&pc[i], h);
+    }
+    if constexpr (MODE == kModeLCC) {
+      wave_sync();
+      return small ? pc[lane] : 0u;
+    }
+    return 0u;
+  };
   for (uint32_t t0 = 0;; t0 += 64) {
     if (t0 != 0) {
       kl = c_lo + wave + PS * (t0 + lane);
-      yl = el = bl = nl = 0;
+      yl = el = bl = nl = b4 = n4 = 0;
       wl = 0;
       load_pair();
     }
-    const uint32_t nvalid = (uint32_t)__popcll(__ballot(kl < c_hi));
+    const bool valid = kl < c_hi;
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
     if (nvalid == 0) break;
     uint32_t phl = 0;  // lane j: the hit count of the batch's pair j, once it is done
-    static_assert(!(MODE == kModeSUP && ESF == 3), "per-edge supports stream 4-byte lists");
-    if constexpr (ESF == 3) {
-      stream_batch(std::integral_constant<uint32_t, 3u>(), nvalid, phl);
-    } else if constexpr (ESF == 4 || MODE == kModeSUP) {
-      stream_batch(std::integral_constant<uint32_t, 4u>(), nvalid, phl);
-    } else {
-      if (G.selem == 3u)  // uniform
-        stream_batch(std::integral_constant<uint32_t, 3u>(), nvalid, phl);
-      else
-        stream_batch(std::integral_constant<uint32_t, 4u>(), nvalid, phl);
+    uint32_t nwave = nvalid;  // pairs left to the wave path (the batch's first lanes)
+    if constexpr (MODE != kModeSUP) {
+      const bool small = valid && n4 <= kChunkMax;
+      if (__ballot(small) != 0ull) {  // uniform
+        const uint32_t h = chunk_path(small);
+        if (MODE == kModeLCC && h != 0u) gadd(&G.tvert[yl], (unsigned long long)h);  // t(y)
+        // the other pairs move to the first lanes, in order (a permutation of the lanes)
+        const bool longp = valid && !small;
+        const unsigned long long lm = __ballot(longp);
+        nwave = (uint32_t)__popcll(lm);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u));
+        const uint32_t dst = longp ? below : nwave + lane - below;
+        bl = (uint32_t)__builtin_amdgcn_ds_permute((int)(4u * dst), (int)bl);
+        nl = (uint32_t)__builtin_amdgcn_ds_permute((int)(4u * dst), (int)nl);
+        if (MODE != kModeTC) yl = (uint32_t)__builtin_amdgcn_ds_permute((int)(4u * dst), (int)yl);
+      }
+    }
+    if (nwave != 0u) {  // uniform
+      static_assert(!(MODE == kModeSUP && ESF == 3), "per-edge supports stream 4-byte lists");
+      if constexpr (ESF == 3) {
+        stream_batch(std::integral_constant<uint32_t, 3u>(), nwave, phl);
+      } else if constexpr (ESF == 4 || MODE == kModeSUP) {
+        stream_batch(std::integral_constant<uint32_t, 4u>(), nwave, phl);
+      } else {
+        if (G.selem == 3u)  // uniform
+          stream_batch(std::integral_constant<uint32_t, 3u>(), nwave, phl);
+        else
+          stream_batch(std::integral_constant<uint32_t, 4u>(), nwave, phl);
+      }
     }
     if (MODE != kModeTC && phl != 0) {
       // The batch's pair credits, one lane per pair, all at once: the pair's edge y->r
@@ -864,12 +964,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_plain(GraphDev Gd, uint32_t row_
 // but queued (queue[1] counts them, at most defer_cap) for k_deferred, which runs with the
 // widest tile; should the queue be full the row is done here after all.  Launches whose
 // rows all fit use the variant without it.
-// Waves per SIMD that the workgroup's LDS allows (at most 6): the launch bound asks the
-// compiler to keep the registers within it too (the SUP-mode variants would otherwise
-// take 85 VGPRs, one wave fewer).
+// Waves per SIMD that the workgroup's LDS allows, at most 8 in TC mode and 6 otherwise:
+// the launch bound asks the compiler to keep the registers within it too.  TC fits 8 waves
+// in 64 VGPRs without spilling (R-MAT scale 24: 30.3 ms against 33.9 ms at the 7 waves of
+// its natural 66 VGPRs; 9 waves are not reached); LCC at 5 waves (no spills) took 46.0 ms
+// against 42.2 ms at 6, and the SUP-mode variants would otherwise take 85 VGPRs.
 template <int CAP, int MODE, int WAVES>
 constexpr int lds_waves() {
-  return std::max(1, std::min(6, (int)(163840 / sizeof(Smem<CAP, MODE, WAVES>)) * WAVES / 4));
+  return std::max(1, std::min(MODE == kModeTC ? 8 : 6,
+                              (int)(163840 / sizeof(Smem<CAP, MODE, WAVES>)) * WAVES / 4));
 }
 
 // With stamps (profiling only; null otherwise) the workgroup records, per item it, the

@@ -534,8 +534,9 @@ int orient(Graph *g) {
               (unsigned long long)g->m, (unsigned long long)g->nnz);
     return PAC_ERR_FORMAT;
   }
-  // m entries plus one sentinel (kListEnd) that list streams past their end read
-  PAC_TRY(g->oadj.reserve(sizeof(uint32_t) * (g->m + 1)));
+  // m entries plus sentinels (kListEnd) that list streams past their end read: one, and
+  // three more for the 16-byte loads of the lane path
+  PAC_TRY(g->oadj.reserve(sizeof(uint32_t) * (g->m + 4)));
   PAC_TRY(g->okey.reserve(sizeof(uint32_t) * (g->m + 1)));
   PAC_TRY(g->rank.reserve(sizeof(uint32_t) * (V ? V : 1)));
   if (V) {
@@ -588,7 +589,7 @@ int orient(Graph *g) {
     }
     PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // the temporaries are freed on return
   }
-  PAC_TRY(fill_u32(ctx->stream, g->oadj.as<uint32_t>() + g->m, 1, kListEnd));
+  PAC_TRY(fill_u32(ctx->stream, g->oadj.as<uint32_t>() + g->m, 4, kListEnd));
   PAC_TRY(fill_u32(ctx->stream, g->okey.as<uint32_t>() + g->m, 1, kListEnd));
   g->pairs_built = false;
   g->hitw_built = false;
