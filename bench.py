@@ -277,6 +277,10 @@ def main():
             ev[f"{name}_mispredict"] = round(float(np.mean(ch != pms.argmin(1))), 5)
             ev[f"{name}_regret"] = float(f"{(pms[idx, ch] - best).sum() / best.sum():.3e}")
             ev[f"{name}_counts"] = np.bincount(ch, minlength=3).tolist()
+        # the most any per-partition strategy choice can gain over the fixed heuristic on
+        # this graph: the fixed choice's time over the best strategy's, summed over partitions
+        ev["max_gain_any_dispatch"] = round(float((pms[idx, g.fixed_choice()[keep]]).sum()
+                                                  / best.sum()), 6)
         out["dispatch_eval"] = ev
     log("phase: stream copy")
     copy_bytes = 1 << 30

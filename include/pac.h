@@ -75,9 +75,10 @@ typedef struct {
   uint64_t num_items;      /* persistent work items after pac_graph_prepare          */
   double prepare_ms;       /* GPU time of the last pac_graph_prepare                 */
   /* The intersection pairs (after pac_graph_prepare; 0 before).  Each oriented edge
-   * t->h is one pair, owned by the endpoint with the longer out-list (ties: the tail),
-   * which stages that list while the partner's shorter list is streamed; pairs whose
-   * shorter list is empty are dropped.  These are the work model of the hash path:
+   * t->h is one pair, owned by the head h, which stages N+(h) while the suffix of N+(t)
+   * above h (the out-lists are in (degree, id) order; every element of N+(h) ranks above
+   * h) is streamed; pairs whose suffix or N+(h) is empty are dropped.  These are the
+   * work model of the hash path:
    * its algorithmic bytes per launch are S*streamed + 4*staged_item_elems +
    * 8*num_pairs (each pair's list bounds) + 20*num_rows + 32*num_items (TC), plus
    * 4*num_pairs (partner IDs) and 16*num_vertices (per-vertex counts read and written
@@ -86,7 +87,7 @@ typedef struct {
    * once) for per-edge supports; S = stream_elem_bytes, 4 for per-edge supports. */
   uint64_t num_pairs;
   uint64_t num_rows;          /* vertices owning at least one pair                     */
-  uint64_t streamed_elems;    /* sum over pairs of d+(partner) = sum of min(d+(t),d+(h)) */
+  uint64_t streamed_elems;    /* sum over pairs t->h of #{w in N+(t) ranked above h}   */
   uint64_t staged_elems;      /* sum of d+(x) over the owning vertices                 */
   uint64_t staged_item_elems; /* the same, once per work item holding the row (a row  *
                                * split over several items is staged by each)          */
@@ -225,7 +226,8 @@ int pac_tc(pac_graph *g, int dispatch, int launch, uint64_t *total, uint64_t *pe
  * (float)lcc64.  Either output may be NULL.  total (may be NULL) receives T. */
 int pac_lcc(pac_graph *g, int dispatch, int launch, float *lcc32, double *lcc64,
             uint64_t *total);
-/* Per-edge Jaccard over every undirected edge, in oriented-edge order (m entries):
+/* Per-edge Jaccard over every undirected edge, in oriented-edge order (m entries: rows in
+ * ID order, each row's edges in (degree, id) order of their heads):
  * src/dst (original IDs, src has the smaller (degree, id)), inter = |N(u)&N(v)|,
  * uni = d(u)+d(v)-inter, ratio = (float)((double)inter/(double)uni) -- the pair
  * formula of Graph::solve_jsim_merge (Graph.cpp:808-809) applied to each edge.  Any

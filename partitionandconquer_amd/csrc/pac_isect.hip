@@ -9,34 +9,35 @@
 //
 // Work decomposition.  The graph is oriented by degree: u->v iff (d(u),u) < (d(v),v),
 // so every triangle {t,h,w} is found exactly once, at the oriented edge t->h whose
-// tail t has the smallest (degree, id), as w in N+(t) & N+(h).  Each oriented edge is
-// an intersection pair owned by the endpoint whose out-list is longer (ties: the
-// tail); pairs whose other list is empty are dropped.  A workgroup owns a row x:
-// N+(x) (the list A) is staged once in LDS and each of x's pairs is given to one wave,
-// which streams the partner y's out-list N+(y) (the list B, never longer than A) from
-// HBM with coalesced loads and looks each element up in A.  Streaming the shorter list
-// of every pair moves sum(min(d+(t), d+(h))) elements from HBM rather than
-// sum(d+(h)).  The three strategies differ only in the lookup:
-//   kMerge  : wave64 merge-path merge of A (LDS) and B (HBM): each lane walks an
+// tail t has the smallest (degree, id), as w in N+(t) & N+(h).  Every out-list is kept in
+// that (degree, id) order (oadj holds the IDs, okey their ranks), so the elements of N+(t)
+// that can meet N+(h) -- those ranked above h -- are the suffix of N+(t) after h.  Each
+// oriented edge t->h is an intersection pair owned by its head h (pairs whose suffix or
+// N+(h) is empty are dropped): a workgroup owns a row x, stages N+(x) (the list A) once in
+// LDS, and streams for each pair (y, x) the suffix of N+(y) above x (the list B) from HBM,
+// looking each element up in A.  The suffixes total 0.6 of what streaming the shorter whole
+// list of every pair would.  The three strategies differ only in the lookup:
+//   kMerge  : wave64 merge-path merge of A (LDS) and B (HBM) by rank: each lane walks an
 //             equal share of the merged sequence after a diagonal binary search.
-//   kGallop : the shorter list's elements are binary-searched in the longer one
+//   kGallop : the shorter list's ranks are binary-searched in the longer one
 //             (B elements in A held in LDS, or A elements in B in HBM).
 //   kBitmap : A is held in a two-choice cuckoo hash table in LDS (two buckets of two
 //             slots per key, load at most 1/2); each B element is looked up with two
-//             8-byte LDS reads and four compares, no loop (exact).
+//             8-byte LDS reads and four compares, no loop (exact).  Pairs of at most
+//             kChunkMax elements are looked up in 4-element chunks spread over the lanes.
 // Counting is exact by construction in all three: a hit is only recorded for an
 // element equal to a list entry.
 //
-// Per-vertex and per-edge counts.  A hit w of pair (x, y) is the triangle {x, y, w}:
+// Per-vertex and per-edge counts.  A hit w of pair (y, x) is the triangle {y, x, w}:
 // t(x) += 1 (a register per row); t(w) goes to the LDS counter of w's place in A;
-// t(y) goes to the LDS counter of y's place in A when x->y, otherwise (y->x) to one
-// global atomic per pair.  After the row each nonzero LDS counter is added to the
-// global count of its vertex -- about one global atomic per pair instead of one per
-// triangle.  SUP mode (per-edge support, for Jaccard) keeps the same counters per
-// edge: x->w and x->y at their places in A, y->x per pair; the third edge y->w lies
-// in the streamed list, and the hash path records it as one hit bit per streamed
-// element (the pair's hit words), summed per edge after the launch (k_sup_small,
-// k_sup_big), where the sorted and HBM paths add one atomic per triangle instead.
+// t(y) goes to one global atomic per pair (the pair's edge y->x enters the owner).  After
+// the row each nonzero LDS counter is added to the global count of its vertex -- about
+// one global atomic per pair instead of one per triangle.  SUP mode (per-edge support,
+// for Jaccard) keeps the same counters per edge: x->w at its place in A, y->x per pair;
+// the third edge y->w lies in the streamed list, and the hash path records it as one hit
+// bit per streamed element (the pair's hit words), summed per edge after the launch
+// (k_sup_small, k_sup_big), where the sorted and HBM paths add one atomic per triangle
+// instead.
 #include <algorithm>
 #include <mutex>
 #include <type_traits>

@@ -55,7 +55,8 @@ class GraphInfo(C.Structure):
 
     def algorithmic_bytes(self, mode: int = 1) -> int:
         """Bytes the hash path's intersection launch must move in `mode` (0 TC, 1 LCC,
-        2 Jaccard): every pair's streamed list (stream_elem_bytes per element in TC and LCC:
+        2 Jaccard): every pair's stream, the suffix of its partner's out-list ranked above
+        its owner (stream_elem_bytes per element in TC and LCC:
         3 with packed lists, else 4; 4 in Jaccard) with the list's bounds stored
         beside the pair (8 B), every row's staged list once per work item holding it, the
         row's list and pair bounds and its row-list entry (20 B), the items (32 B).  LCC adds
