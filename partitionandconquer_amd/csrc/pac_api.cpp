@@ -353,6 +353,7 @@ int run_intersection(Graph *g, int mode, int dispatch, int launch, Timed &t) {
     PAC_TRY(rc);
   }
   if (mode == kModeSUP) PAC_TRY(sup_from_hit_words(ctx, g));
+  if (mode == kModeLCC) PAC_TRY(fold_low_credits(ctx, g));
   ctx->last.launches = launches;
   PAC_TRY(t.mark(2));
   return PAC_OK;
@@ -568,6 +569,8 @@ int solve_batch(Ctx *c, pac_graph *const *hs, uint32_t G, int mode, int dispatch
   PAC_TRY(rc);
   if (mode == kModeSUP)
     for (uint32_t i = 0; i < G; ++i) PAC_TRY(sup_from_hit_words(c, gs[i]));
+  if (mode == kModeLCC)
+    for (uint32_t i = 0; i < G; ++i) PAC_TRY(fold_low_credits(c, gs[i]));
   PAC_TRY(t.mark(2));
   PAC_TRY(t.mark(3));
   PAC_TRY(t.finish());

@@ -91,6 +91,11 @@ struct GraphDev {
   unsigned long long *total;  // T accumulator
   unsigned long long *tvert;  // LCC mode: per-vertex triangle counts, every credit added
                               // atomically by the intersection kernel
+  // LCC mode (may be null: then these credits go to tvert): the pairs' credits to their
+  // partners, i.e. per vertex the triangles in which it ranks lowest, as 32-bit atomics;
+  // folded into tvert after the launch.  A vertex t is lowest in at most C(d+(t), 2) < m
+  // triangles (d+(t)^2 <= the sum of its out-neighbours' degrees <= 2m), so they fit.
+  uint32_t *tlow;
   uint32_t *sup;              // per-oriented-edge support (SUP mode)
   // SUP mode: the hit words of the streamed lists.  The pair with rank r among the pairs
   // streaming N+(y) owns the 64-bit words hitw[wbase[y] + r*nw ..+ nw), nw = ceil(d+(y)/64);
@@ -194,7 +199,7 @@ struct Graph {
   DevBuf dstrat, dlearned;
   uint64_t dlearned_version = 0;
   // outputs
-  DevBuf total, tvert, sup, lcc32, lcc64, juni, jratio;
+  DevBuf total, tvert, tlow, sup, lcc32, lcc64, juni, jratio;
   double prepare_ms = 0;
 };
 
@@ -281,6 +286,8 @@ constexpr uint32_t kSupRun = 256;
 int build_hit_words(Graph *g);
 // sup[y->w] += number of pair streams of N+(y) that found w (after a SUP-mode launch).
 int sup_from_hit_words(Ctx *ctx, Graph *g);
+// tvert += tlow (after an LCC-mode launch).
+int fold_low_credits(Ctx *ctx, Graph *g);
 int graph_validate(const Graph *g, int level);
 
 // ---- gen.hip ---------------------------------------------------------------------

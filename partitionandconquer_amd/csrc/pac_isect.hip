@@ -72,6 +72,7 @@ struct GDev {
   const PAC_GLOBAL uint8_t *sadj;  // the streamed lists, selem bytes per element
   const PAC_GLOBAL u32x2 *sinfo;   // per pair: 4-byte word of sadj where its list starts
   PAC_GLOBAL unsigned long long *total, *tvert;
+  PAC_GLOBAL uint32_t *tlow;
   PAC_GLOBAL uint32_t *sup;
   const PAC_GLOBAL unsigned long long *pword;
   PAC_GLOBAL unsigned long long *hitw;
@@ -93,6 +94,7 @@ __device__ __forceinline__ GDev to_dev(const GraphDev &d) {
   g.rows = (const PAC_GLOBAL uint32_t *)d.rows;
   g.total = (PAC_GLOBAL unsigned long long *)d.total;
   g.tvert = (PAC_GLOBAL unsigned long long *)d.tvert;
+  g.tlow = (PAC_GLOBAL uint32_t *)d.tlow;
   g.sup = (PAC_GLOBAL uint32_t *)d.sup;
   g.pword = (const PAC_GLOBAL unsigned long long *)d.pword;
   g.hitw = (PAC_GLOBAL unsigned long long *)d.hitw;
@@ -108,6 +110,15 @@ __device__ __forceinline__ void gadd(PAC_GLOBAL uint32_t *p, uint32_t v) {
 }
 __device__ __forceinline__ void gadd(PAC_GLOBAL unsigned long long *p, unsigned long long v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// LCC: a pair's credit to its partner y (the triangles in which y ranks lowest): 32-bit
+// atomics into tlow when the launch has it, else into the 64-bit counts.
+__device__ __forceinline__ void credit_low(const GDev &G, uint32_t y, uint32_t v) {
+  if (G.tlow)
+    gadd(&G.tlow[y], v);
+  else
+    gadd(&G.tvert[y], (unsigned long long)v);
 }
 
 constexpr uint32_t kAllC = 0xFFFFFFFFu;
@@ -362,7 +373,7 @@ __device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k
       const uint32_t ch = wave_sum(h);
       if (lane == 0 && ch) {
         if (MODE == kModeLCC)
-          gadd(&G.tvert[y], (unsigned long long)ch);  // the pair's edge y->r: t(y)
+          credit_low(G, y, ch);  // the pair's edge y->r: t(y)
         else
           gadd(&G.sup[e], ch);  // the pair's edge y->r
       }
@@ -802,7 +813,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       const bool small = valid && n4 <= kChunkMax;
       if (__ballot(small) != 0ull) {  // uniform
         const uint32_t h = chunk_path(small);
-        if (MODE == kModeLCC && h != 0u) gadd(&G.tvert[yl], (unsigned long long)h);  // t(y)
+        if (MODE == kModeLCC && h != 0u) credit_low(G, yl, h);  // t(y)
         // the other pairs move to the first lanes, in order (a permutation of the lanes)
         const bool longp = valid && !small;
         const unsigned long long lm = __ballot(longp);
@@ -833,7 +844,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       // goes to t(y) in LCC mode and to the edge's support in SUP mode, atomically (other
       // rows credit the same vertex meanwhile).
       if (MODE == kModeLCC)
-        gadd(&G.tvert[yl], (unsigned long long)phl);
+        credit_low(G, yl, phl);
       else
         gadd(&G.sup[el], phl);
     }
@@ -893,7 +904,7 @@ __device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo
       const uint32_t ch = wave_sum(h);
       if (lane == 0 && ch) {
         if (MODE == kModeLCC)
-          gadd(&G.tvert[y], (unsigned long long)ch);  // t(y)
+          credit_low(G, y, ch);  // t(y)
         else
           gadd(&G.sup[e], ch);  // the pair's edge y->r
       }
@@ -1135,6 +1146,14 @@ __global__ void k_fill_u64(unsigned long long *p, uint64_t n) {
 }
 
 // ---- epilogues -----------------------------------------------------------------------
+// LCC: the lowest-vertex credits gathered in 32 bits during the launch, into the counts.
+__global__ void k_fold_low(const uint32_t *__restrict__ tlow, unsigned long long *tvert,
+                           uint32_t V) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < V;
+       u += (uint64_t)gridDim.x * blockDim.x)
+    tvert[u] += tlow[u];
+}
+
 __global__ void k_lcc(const unsigned long long *__restrict__ tvert,
                       const uint32_t *__restrict__ off, uint32_t V, float *lcc32,
                       double *lcc64) {
@@ -1659,6 +1678,7 @@ GraphDev make_desc(Graph *g, uint32_t part_base, uint32_t es, int mode) {
   d.rows = g->rows.as<const uint32_t>();
   d.total = g->total.as<unsigned long long>();
   d.tvert = g->tvert.as<unsigned long long>();
+  d.tlow = mode == kModeLCC ? g->tlow.as<uint32_t>() : nullptr;
   d.sup = g->sup.as<uint32_t>();
   d.pword = g->pword.as<const unsigned long long>();
   d.hitw = g->hitw.as<unsigned long long>();
@@ -1721,8 +1741,10 @@ int alloc_outputs(Graph *const *gs, uint32_t G, int mode) {
   for (uint32_t i = 0; i < G; ++i) {
     Graph *g = gs[i];
     PAC_TRY(g->total.reserve(sizeof(unsigned long long)));
-    if (mode == kModeLCC)
+    if (mode == kModeLCC) {
       PAC_TRY(g->tvert.reserve(sizeof(unsigned long long) * (size_t)(g->V ? g->V : 1)));
+      PAC_TRY(g->tlow.reserve(sizeof(uint32_t) * (size_t)(g->V ? g->V : 1)));
+    }
     if (mode == kModeSUP) PAC_TRY(g->sup.reserve(sizeof(uint32_t) * (size_t)(g->m ? g->m : 1)));
   }
   return PAC_OK;
@@ -1735,9 +1757,11 @@ int zero_outputs(Ctx *ctx, Graph *const *gs, uint32_t G, int mode) {
     Graph *g = gs[i];
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream,
                        g->total.as<unsigned long long>(), (uint64_t)1);
-    if (mode == kModeLCC && g->V)
+    if (mode == kModeLCC && g->V) {
       hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
                          g->tvert.as<unsigned long long>(), (uint64_t)g->V);
+      PAC_TRY(fill_u32(ctx->stream, g->tlow.as<uint32_t>(), g->V, 0u));
+    }
     if (mode == kModeSUP) PAC_TRY(fill_u32(ctx->stream, g->sup.as<uint32_t>(), g->m, 0u));
     PAC_CHECK_HIP(hipGetLastError());
   }
@@ -1839,6 +1863,7 @@ int launch_persistent(Ctx *ctx, Graph *const *gs, uint32_t G, int mode,
   key.push_back(reinterpret_cast<const void *>((uintptr_t)es));  // the kernels' list width
   for (uint32_t i = 0; i < G; ++i) {
     key.push_back(gs[i]->tvert.p);
+    key.push_back(gs[i]->tlow.p);
     key.push_back(gs[i]->sup.p);
     key.push_back(gs[i]->total.p);
   }
@@ -1938,6 +1963,7 @@ int launch_persistent_noalloc(Ctx *ctx, Graph *g, int mode, const uint8_t *d_str
   GraphDev d = make_desc(g, 0, es, mode);
   d.total = d_total;
   d.tvert = mode == kModeLCC ? d_tvert : nullptr;
+  d.tlow = nullptr;  // every credit straight into the caller's 64-bit counts
   d.sup = nullptr;
   d.pword = nullptr;
   d.hitw = nullptr;
@@ -1953,6 +1979,14 @@ int launch_persistent_noalloc(Ctx *ctx, Graph *g, int mode, const uint8_t *d_str
   b.defer_cap = (uint32_t)long_rows;
   return run_persistent_any(ctx, mode, es, g->items.as<const WorkItem>(), (uint32_t)nitems, b,
                             nullptr);
+}
+
+int fold_low_credits(Ctx *ctx, Graph *g) {
+  if (g->V == 0 || !g->tlow.p) return PAC_OK;
+  hipLaunchKernelGGL(k_fold_low, dim3(grid_for(g->V, 256 * 8)), dim3(256), 0, ctx->stream,
+                     g->tlow.as<const uint32_t>(), g->tvert.as<unsigned long long>(), g->V);
+  PAC_CHECK_HIP(hipGetLastError());
+  return PAC_OK;
 }
 
 int sup_from_hit_words(Ctx *ctx, Graph *g) {
