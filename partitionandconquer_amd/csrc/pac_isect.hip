@@ -763,25 +763,34 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     auto bperm = [](uint32_t from, uint32_t v) -> uint32_t {
       return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(4u * from), (int)v);
     };
+    // SUP mode: a small pair streams one 64-element block of its partner's list (its stream
+    // starts on a 64-element boundary), so its hits are one hit word, assembled by a
+    // segmented OR over the lanes holding its chunks (consecutive, at most 16) and stored by
+    // the segment's last lane, with the pair-edge credit = the word's popcount.  A pair cut
+    // by the end of a trip carries its part of the word to the next trip.
+    unsigned long long carry = 0;
+    uint32_t ckey = 0xFFFFFFFFu;  // the pair whose word continues into the next trip
     for (uint32_t q0 = 0; q0 < C; q0 += 64) {  // wave-uniform
       const uint32_t q = q0 + lane;
       uint32_t i = 0;
 #pragma unroll
       for (uint32_t st = 32; st; st >>= 1)
         if (bperm(i + st, P) <= q) i += st;
-      const uint32_t e0 = 4u * (q - bperm(i, P)), bi = bperm(i, b4), ni = bperm(i, n4);
+      const uint32_t Pi = bperm(i, P);
+      const uint32_t e0 = 4u * (q - Pi), bi = bperm(i, b4), ni = bperm(i, n4);
       const bool act = q < C;
       u32x4a v = {kListEnd, kListEnd, kListEnd, kListEnd};
       if (act) v = *(const PAC_GLOBAL u32x4a *)(G.oadj + bi + e0);  // oadj is padded
-      uint32_t h = 0;
+      uint32_t h = 0, hm = 0;
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
         const Probe q4 = probe(act && e0 + u < ni ? v[u] : kListEnd);  // kListEnd: no key
         const bool hit = q4.e10 | q4.e11 | q4.e20 | q4.e21;
         row_hits += (uint32_t)__popcll(__ballot(hit));
-        if (MODE == kModeLCC) {
+        if (MODE != kModeTC) {
           h += hit ? 1u : 0u;
-          if (hit) {
+          hm |= hit ? 1u << u : 0u;
+          if (hit) {  // t(w) / edge r->w
             const uint32_t ca = ((q4.e10 | q4.e11) ? q4.a1 : q4.a2) >> 1;
             const uint32_t inc = (q4.e11 | q4.e21) ? 0x10000u : 1u;
             asm volatile("ds_add_u32 %0, %1 offset:%2" : : "v"(ca), "v"(inc), "i"(kCntBase)
@@ -790,6 +799,32 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         }
       }
       if (MODE == kModeLCC && h != 0u) atomicAdd(&pc[i], h);
+      if constexpr (MODE == kModeSUP) {
+        const uint32_t key = act ? i : 0xFFFFFFFFu;
+        unsigned long long w = (unsigned long long)hm << e0;  // e0 < 64: one block
+#pragma unroll
+        for (uint32_t st = 1; st < 16; st <<= 1) {
+          const uint32_t kup = (uint32_t)__shfl_up((int)key, st, 64);
+          const unsigned long long wup = __shfl_up(w, st, 64);
+          if (lane >= st && kup == key) w |= wup;
+        }
+        if (key == ckey) w |= carry;
+        const uint32_t kdn = (uint32_t)__shfl_down((int)key, 1, 64);
+        // the pair's data, read across lanes while every lane is active (a lane permute
+        // reads nothing from a lane that is switched off)
+        const uint32_t ci = bperm(i, c), ei = bperm(i, el);
+        const uint32_t wlo = bperm(i, (uint32_t)wl), whi = bperm(i, (uint32_t)(wl >> 32));
+        // the pair's chunks end at Pi + ci; it goes on past this trip when that end lies
+        // beyond q0 + 64
+        const bool cont = act && lane == 63u && Pi + ci > q0 + 64u;
+        if (act && (lane == 63u || kdn != key) && !cont) {
+          G.hitw[((unsigned long long)whi << 32) | wlo] = w;  // edges y->w, summed later
+          const uint32_t cnt = (uint32_t)__popcll(w);
+          if (cnt) gadd(&G.sup[ei], cnt);  // the pair's edge y->r
+        }
+        carry = readlane64(w, 63);
+        ckey = __ballot(cont) ? readlane(key, 63) : 0xFFFFFFFFu;
+      }
     }
     if constexpr (MODE == kModeLCC) {
       wave_sync();
@@ -809,7 +844,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     if (nvalid == 0) break;
     uint32_t phl = 0;  // lane j: the hit count of the batch's pair j, once it is done
     uint32_t nwave = nvalid;  // pairs left to the wave path (the batch's first lanes)
-    if constexpr (MODE != kModeSUP) {
+    {
       const bool small = valid && n4 <= kChunkMax;
       if (__ballot(small) != 0ull) {  // uniform
         const uint32_t h = chunk_path(small);
@@ -820,10 +855,16 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         nwave = (uint32_t)__popcll(lm);
         const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u));
-        const uint32_t dst = longp ? below : nwave + lane - below;
-        bl = (uint32_t)__builtin_amdgcn_ds_permute((int)(4u * dst), (int)bl);
-        nl = (uint32_t)__builtin_amdgcn_ds_permute((int)(4u * dst), (int)nl);
-        if (MODE != kModeTC) yl = (uint32_t)__builtin_amdgcn_ds_permute((int)(4u * dst), (int)yl);
+        const int to = (int)(4u * (longp ? below : nwave + lane - below));
+        bl = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)bl);
+        nl = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)nl);
+        if (MODE != kModeTC) yl = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)yl);
+        if (MODE == kModeSUP) {
+          el = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)el);
+          const uint32_t wlo = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)(uint32_t)wl);
+          const uint32_t whi = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)(uint32_t)(wl >> 32));
+          wl = ((unsigned long long)whi << 32) | wlo;
+        }
       }
     }
     if (nwave != 0u) {  // uniform
