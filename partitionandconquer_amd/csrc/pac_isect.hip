@@ -184,6 +184,8 @@ struct alignas(8) Smem {
   uint32_t rowtot;  // the row's triangle count (LCC mode)
   // LCC mode: per wave, the hit counts of its batch's small pairs (chunk path of row_hash)
   uint32_t pcnt[MODE == kModeLCC ? WAVES : 1][MODE == kModeLCC ? 64 : 1];
+  // SUP mode: per wave, the hit words of its batch's small pairs (chunk path of row_hash)
+  unsigned long long phw[MODE == kModeSUP ? WAVES : 1][MODE == kModeSUP ? 64 : 1];
 };
 
 __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t lane) {
@@ -764,12 +766,15 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(4u * from), (int)v);
     };
     // SUP mode: a small pair streams one 64-element block of its partner's list (its stream
-    // starts on a 64-element boundary), so its hits are one hit word, assembled by a
-    // segmented OR over the lanes holding its chunks (consecutive, at most 16) and stored by
-    // the segment's last lane, with the pair-edge credit = the word's popcount.  A pair cut
-    // by the end of a trip carries its part of the word to the next trip.
-    unsigned long long carry = 0;
-    uint32_t ckey = 0xFFFFFFFFu;  // the pair whose word continues into the next trip
+    // starts on a 64-element boundary), so its hits are one hit word, gathered in the wave's
+    // phw row by LDS atomic ORs and stored, with the pair-edge credit = its popcount, by the
+    // pair's own lane after the last trip.
+    unsigned long long *pw = nullptr;
+    if constexpr (MODE == kModeSUP) {
+      pw = sm.phw[threadIdx.x >> 6];
+      pw[lane] = 0ull;
+      wave_sync();
+    }
     for (uint32_t q0 = 0; q0 < C; q0 += 64) {  // wave-uniform
       const uint32_t q = q0 + lane;
       uint32_t i = 0;
@@ -799,31 +804,15 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         }
       }
       if (MODE == kModeLCC && h != 0u) atomicAdd(&pc[i], h);
-      if constexpr (MODE == kModeSUP) {
-        const uint32_t key = act ? i : 0xFFFFFFFFu;
-        unsigned long long w = (unsigned long long)hm << e0;  // e0 < 64: one block
-#pragma unroll
-        for (uint32_t st = 1; st < 16; st <<= 1) {
-          const uint32_t kup = (uint32_t)__shfl_up((int)key, st, 64);
-          const unsigned long long wup = __shfl_up(w, st, 64);
-          if (lane >= st && kup == key) w |= wup;
-        }
-        if (key == ckey) w |= carry;
-        const uint32_t kdn = (uint32_t)__shfl_down((int)key, 1, 64);
-        // the pair's data, read across lanes while every lane is active (a lane permute
-        // reads nothing from a lane that is switched off)
-        const uint32_t ci = bperm(i, c), ei = bperm(i, el);
-        const uint32_t wlo = bperm(i, (uint32_t)wl), whi = bperm(i, (uint32_t)(wl >> 32));
-        // the pair's chunks end at Pi + ci; it goes on past this trip when that end lies
-        // beyond q0 + 64
-        const bool cont = act && lane == 63u && Pi + ci > q0 + 64u;
-        if (act && (lane == 63u || kdn != key) && !cont) {
-          G.hitw[((unsigned long long)whi << 32) | wlo] = w;  // edges y->w, summed later
-          const uint32_t cnt = (uint32_t)__popcll(w);
-          if (cnt) gadd(&G.sup[ei], cnt);  // the pair's edge y->r
-        }
-        carry = readlane64(w, 63);
-        ckey = __ballot(cont) ? readlane(key, 63) : 0xFFFFFFFFu;
+      if (MODE == kModeSUP && hm != 0u) atomicOr(&pw[i], (unsigned long long)hm << e0);  // e0 < 64
+    }
+    if constexpr (MODE == kModeSUP) {
+      wave_sync();
+      if (small) {
+        const unsigned long long w = pw[lane];
+        G.hitw[wl] = w;  // edges y->w, summed after the launch
+        const uint32_t cnt = (uint32_t)__popcll(w);
+        if (cnt) gadd(&G.sup[el], cnt);  // the pair's edge y->r
       }
     }
     if constexpr (MODE == kModeLCC) {
