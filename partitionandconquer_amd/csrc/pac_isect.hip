@@ -530,8 +530,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     sm.rowtot = 0;
   }
   bool lost = false;  // SOLO: this lane left a key homeless
-  // The thread's first element of A, loaded here and kept for the row's credits at the
-  // end (most rows have at most one element per thread).
+  // The thread's first element of A, loaded here and kept for the row's edge credits at
+  // the end in SUP mode (most rows have at most one element per thread).
   uint32_t key0 = 0;
   if (tid < nA) key0 = G.oadj[a0 + tid];
   sync();
@@ -885,16 +885,21 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   if (lane == 0) acc += row_hits;
   if (MODE == kModeLCC && !SOLO && lane == 0 && row_hits) atomicAdd(&sm.rowtot, row_hits);
   sync();
-  if (MODE != kModeTC) {
-    // per edge r->A[i], in A order: the credits of A[i]'s slot, to t(A[i]) (LCC mode) or
-    // to the edge (coalesced)
+  if (MODE == kModeLCC) {
+    // t(w) for each w of A: its slot's count, read in slot order (a slot with a nonzero
+    // count holds the key it was counted for), so no key is hashed and found again
+    for (uint32_t s = tid; s < nslot; s += BS) {
+      const uint32_t v = (cnt16[s >> 1] >> (16u * (s & 1u))) & 0xFFFFu;
+      if (v) gadd(&G.tvert[tab[s]], (unsigned long long)v);
+    }
+  } else if (MODE == kModeSUP) {
+    // per edge r->A[i], in A order: the credits of A[i]'s slot (coalesced)
     for (uint32_t i = tid; i < nA; i += BS) {
       const uint32_t x = i == tid ? key0 : G.oadj[a0 + i];
       uint32_t word, inc;
       find(x, word, inc);
       const uint32_t v = (cnt16[word] >> (inc == 1u ? 0u : 16u)) & 0xFFFFu;
-      if (v && MODE == kModeLCC) gadd(&G.tvert[x], (unsigned long long)v);
-      else if (v) gadd(&G.sup[a0 + i], v);  // the edge r->A[i]
+      if (v) gadd(&G.sup[a0 + i], v);  // the edge r->A[i]
     }
   }
   if (MODE == kModeLCC) {
