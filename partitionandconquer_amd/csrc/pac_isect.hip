@@ -413,32 +413,40 @@ __device__ __attribute__((noinline)) void row_sorted(const GDev &G, uint32_t r, 
 // other bucket is what keeps two keys from trading one slot back and forth: on R-MAT
 // out-lists it leaves about 2 rows in 100 000 with a key still homeless after
 // kMaxKicks evictions, where alternating between the two buckets by kick number left
-// 2 in 1 000, most of them rows of 129-512 keys).  Such a row is done by another path
-// instead (exact either way: the sorted lists for a workgroup's row, HBM lookups for a
-// one-wave row), so lookups never scan an overflow area.
+// 2 in 1 000, most of them rows of 129-512 keys).  A row with a homeless key is built
+// again with the next of kHashTries hash multipliers (the row's lookups then use that
+// one), and only a row homeless under all of them is done by another path (exact either
+// way: the sorted lists for a workgroup's row, HBM lookups for a one-wave row), so lookups
+// never scan an overflow area.
 // Each pair's list N+(y) is streamed by one wave, 4 x 64 elements per step with
 // coalesced loads, software-pipelined one step ahead across the wave's pairs; the
 // pair's hit count is a ballot popcount held in scalar registers.
 constexpr uint32_t kMaxKicks = 48;
 // The two buckets of x among 2^lg (sh = 32 - lg), from one full-rate 24-bit multiply-add
-// and no shift: h = (x mod 2^24) * 0x9E3778 + x, which for IDs below 2^24 is x times the odd
-// constant 0x9E3779 (the + x also carries ID bits above 23 into the top of h); b1 is the top
+// and no shift: h = (x mod 2^24) * M + x, which for IDs below 2^24 is x times the odd
+// constant M + 1 (the + x also carries ID bits above 23 into the top of h); b1 is the top
 // lg bits of h and b2 = b1 ^ d with d = bits [3, 3 + lg) of h made odd (lg <= 12, so the
 // two fields are disjoint) -- so the two choices always differ (one bucket of each parity,
-// the two tables of classic cuckoo hashing).  On the out-lists of R-MAT scales 20 and 22 a
-// sequential replay of the table build leaves 0.006 % of the rows with a homeless key (the
-// earlier (x mod 2^24) * 0x9E3779 + (x >> 20) with d from bits [4, 4 + lg): 0.0035 %), and
-// the streamed lookups need two vector instructions fewer per element: the bucket field of
-// d sits where a bucket's byte address has it (see probe in row_hash).  (__umul24 is
+// the two tables of classic cuckoo hashing).  The streamed lookups need two vector
+// instructions fewer per element than with d taken elsewhere: its bucket field sits where a
+// bucket's byte address has it (see probe in row_hash).  M is the row's multiplier, a
+// wave-uniform value, first 0x9E3778: on the out-lists of R-MAT scales 22, 24 and 25 a
+// sequential replay of the table build with it leaves 111, 297 and 843 rows with a homeless
+// key (0.003-0.005 %); at scale 25 those include hub rows owning up to 93 245 pairs, whose
+// fallback cost the per-edge Jaccard launch about a third of its time.  With the next
+// multipliers 7 of those rows still fail the second and none the third.  (__umul24 is
 // declared returning int: its result is taken as unsigned before any shift, so the bucket
 // indices stay below 2^lg.)
-constexpr uint32_t kHashMul = 0x9E3778u;
-__device__ __forceinline__ uint32_t hash_key(uint32_t x) {
-  return (uint32_t)__umul24(x, kHashMul) + x;
+constexpr uint32_t kHashTries = 4;
+__device__ __forceinline__ uint32_t hash_mul(uint32_t t) {
+  return t == 0 ? 0x9E3778u : t == 1 ? 0xC2B2AEu : t == 2 ? 0x85EBCAu : 0x27D4EAu;
 }
-__device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t lg, uint32_t &b1,
-                                        uint32_t &b2) {
-  const uint32_t h = hash_key(x);
+__device__ __forceinline__ uint32_t hash_key(uint32_t x, uint32_t mul) {
+  return (uint32_t)__umul24(x, mul) + x;
+}
+__device__ __forceinline__ void buckets(uint32_t x, uint32_t sh, uint32_t lg, uint32_t mul,
+                                        uint32_t &b1, uint32_t &b2) {
+  const uint32_t h = hash_key(x, mul);
   b1 = h >> sh;
   b2 = b1 ^ (__builtin_amdgcn_ubfe(h, 3u, lg) | 1u);
 }
@@ -525,45 +533,49 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   uint32_t lg = 4;
   while ((1u << lg) < nA) ++lg;  // 2^lg >= nA: load factor <= 1/2
   const uint32_t nslot = 2u << lg, sh = 32u - lg;
-  for (uint32_t i = tid; i < nslot; i += BS) tab[i] = kEmpty;
   if (MODE != kModeTC)
     for (uint32_t i = tid; i < nslot / 2; i += BS) cnt16[i] = 0;
-  if (!SOLO && tid == 0) {
-    sm.homeless = 0;
-    sm.rowtot = 0;
-  }
-  bool lost = false;  // SOLO: this lane left a key homeless
-  // The thread's first element of A, loaded here and kept for the row's edge credits at
-  // the end in SUP mode (most rows have at most one element per thread).
+  if (!SOLO && tid == 0) sm.rowtot = 0;
+  // The thread's first element of A, loaded here and kept for the builds and, in SUP mode,
+  // the row's edge credits at the end (most rows have at most one element per thread).
   uint32_t key0 = 0;
   if (tid < nA) key0 = G.oadj[a0 + tid];
-  sync();
-  for (uint32_t i = tid; i < nA; i += BS) {
-    uint32_t x = i == tid ? key0 : G.oadj[a0 + i];
-    uint32_t from = kEmpty;  // the bucket x was just evicted from (none yet)
-    for (uint32_t kick = 0;; ++kick) {
-      uint32_t b1, b2;
-      buckets(x, sh, lg, b1, b2);
-      if (atomicCAS(&tab[2u * b1], kEmpty, x) == kEmpty) break;
-      if (atomicCAS(&tab[2u * b1 + 1u], kEmpty, x) == kEmpty) break;
-      if (atomicCAS(&tab[2u * b2], kEmpty, x) == kEmpty) break;
-      if (atomicCAS(&tab[2u * b2 + 1u], kEmpty, x) == kEmpty) break;
-      if (kick == kMaxKicks) {
-        if (SOLO)
-          lost = true;
-        else
-          atomicAdd(&sm.homeless, 1u);
-        break;
+  uint32_t mul = 0;  // the row's hash multiplier (uniform)
+  for (uint32_t attempt = 0;; ++attempt) {
+    mul = hash_mul(attempt);
+    for (uint32_t i = tid; i < nslot; i += BS) tab[i] = kEmpty;
+    if (!SOLO && tid == 0) sm.homeless = 0;
+    bool lost = false;  // SOLO: this lane left a key homeless
+    sync();
+    for (uint32_t i = tid; i < nA; i += BS) {
+      uint32_t x = i == tid ? key0 : G.oadj[a0 + i];
+      uint32_t from = kEmpty;  // the bucket x was just evicted from (none yet)
+      for (uint32_t kick = 0;; ++kick) {
+        uint32_t b1, b2;
+        buckets(x, sh, lg, mul, b1, b2);
+        if (atomicCAS(&tab[2u * b1], kEmpty, x) == kEmpty) break;
+        if (atomicCAS(&tab[2u * b1 + 1u], kEmpty, x) == kEmpty) break;
+        if (atomicCAS(&tab[2u * b2], kEmpty, x) == kEmpty) break;
+        if (atomicCAS(&tab[2u * b2 + 1u], kEmpty, x) == kEmpty) break;
+        if (kick == kMaxKicks) {
+          if (SOLO)
+            lost = true;
+          else
+            atomicAdd(&sm.homeless, 1u);
+          break;
+        }
+        // evict from the bucket x did not just come from, alternating slots every two kicks
+        const uint32_t vb = from == b1 ? b2 : b1;
+        x = atomicExch(&tab[2u * vb + ((kick >> 1) & 1u)], x);
+        from = vb;
+        if (x == kEmpty) break;
       }
-      // evict from the bucket x did not just come from, alternating slots every two kicks
-      const uint32_t vb = from == b1 ? b2 : b1;
-      x = atomicExch(&tab[2u * vb + ((kick >> 1) & 1u)], x);
-      from = vb;
-      if (x == kEmpty) break;
     }
+    sync();
+    if (SOLO ? __ballot(lost) == 0ull : sm.homeless == 0) break;  // uniform
+    if (attempt + 1 == kHashTries) return false;
+    sync();  // every thread has read homeless before the next build resets it
   }
-  sync();
-  if (SOLO ? __ballot(lost) != 0ull : sm.homeless != 0) return false;  // uniform
   // The streamed lookups address the table by byte: bucket b is at tb + 8b, and the table
   // starts at a multiple of its size (the workgroup's LDS begins with it, and a wave's share
   // of it is a multiple of the largest table one wave builds alone), so bucket b1 ^ f lies at
@@ -588,7 +600,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // more in divergent scalar work than it saves.
   auto find = [&](uint32_t w, uint32_t &word, uint32_t &inc) -> bool {
     uint32_t b1, b2;
-    buckets(w, sh, lg, b1, b2);
+    buckets(w, sh, lg, mul, b1, b2);
     const uint2 q1 = reinterpret_cast<const uint2 *>(tab)[b1];
     const uint32_t k10 = q1.x, k11 = q1.y;
     const uint2 q2 = reinterpret_cast<const uint2 *>(tab)[b2];
@@ -611,7 +623,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   };
   auto probe = [&](uint32_t w) -> Probe {
     Probe q;
-    const uint32_t h = hash_key(w);
+    const uint32_t h = hash_key(w, mul);
     q.a1 = tb + ((h >> sh) << 3);
     q.a2 = q.a1 ^ ((h | 8u) & m3);
     const uint2 q1 = lds_load_u2(q.a1);
