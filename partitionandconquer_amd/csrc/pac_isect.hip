@@ -508,7 +508,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
   // through ooff), and when counts are kept the partner; SUP mode also needs the edge
   // index and the first hit word.
   uint32_t kl = c_lo + wave + PS * lane, yl = 0, el = 0, bl = 0, nl = 0;
-  uint32_t b4 = 0, n4 = 0;    // the pair's exact stream in oadj (its 4-byte list)
+  uint32_t b4 = 0, n4 = 0;    // the pair's exact stream (its suffix) in oadj
   unsigned long long wl = 0;  // SUP mode: the pair's first hit word
   auto load_pair = [&]() {
     if (kl < c_hi) {
@@ -517,8 +517,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       nl = pi.y;
       b4 = bl;
       n4 = nl;
-      if (MODE != kModeSUP && (ESF == 3 || (ESF == 0 && G.selem == 3u))) {
-        const u32x2 q = G.pinfo[p0 + kl];  // 3-byte streams start on a 4-element boundary
+      // 3-byte streams start on a 4-element boundary, SUP-mode streams on a 64-element one
+      if (MODE == kModeSUP || ESF == 3 || (ESF == 0 && G.selem == 3u)) {
+        const u32x2 q = G.pinfo[p0 + kl];
         b4 = q.x;
         n4 = q.y;
       }
@@ -780,10 +781,12 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     auto bperm = [](uint32_t from, uint32_t v) -> uint32_t {
       return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(4u * from), (int)v);
     };
-    // SUP mode: a small pair streams one 64-element block of its partner's list (its stream
-    // starts on a 64-element boundary), so its hits are one hit word, gathered in the wave's
-    // phw row by LDS atomic ORs and stored, with the pair-edge credit = its popcount, by the
-    // pair's own lane after the last trip.
+    // SUP mode: a small pair's suffix lies in one 64-element block of its partner's list
+    // (its SUP-mode stream, from the block's start, has at most 64 elements), so its hits are
+    // one hit word, at the suffix's offset in the block (b4 - bl); they are gathered in the
+    // wave's phw row by LDS atomic ORs and stored, with the pair-edge credit = its popcount,
+    // by the pair's own lane after the last trip.  Only the suffix is looked up: at R-MAT
+    // scale 20 the blocks hold twice its elements.
     unsigned long long *pw = nullptr;
     if constexpr (MODE == kModeSUP) {
       pw = sm.phw[threadIdx.x >> 6];
@@ -798,6 +801,9 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         if (bperm(i + st, P) <= q) i += st;
       const uint32_t Pi = bperm(i, P);
       const uint32_t e0 = 4u * (q - Pi), bi = bperm(i, b4), ni = bperm(i, n4);
+      // SUP mode: the chunk's first bit in its pair's hit word (read in every lane: a lane
+      // permute inside a branch would read lanes the branch left out)
+      const uint32_t bit0 = MODE == kModeSUP ? e0 + bperm(i, b4 - bl) : 0u;
       const bool act = q < C;
       u32x4a v = {kListEnd, kListEnd, kListEnd, kListEnd};
       if (act) v = *(const PAC_GLOBAL u32x4a *)(G.oadj + bi + e0);  // oadj is padded
@@ -819,7 +825,8 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         }
       }
       if (MODE == kModeLCC && h != 0u) atomicAdd(&pc[i], h);
-      if (MODE == kModeSUP && hm != 0u) atomicOr(&pw[i], (unsigned long long)hm << e0);  // e0 < 64
+      if (MODE == kModeSUP && hm != 0u)  // its bits land below 64: the suffix ends in the block
+        atomicOr(&pw[i], (unsigned long long)hm << bit0);
     }
     if constexpr (MODE == kModeSUP) {
       wave_sync();
@@ -849,7 +856,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
     uint32_t phl = 0;  // lane j: the hit count of the batch's pair j, once it is done
     uint32_t nwave = nvalid;  // pairs left to the wave path (the batch's first lanes)
     {
-      const bool small = valid && n4 <= kChunkMax;
+      const bool small = valid && (MODE == kModeSUP ? nl : n4) <= kChunkMax;
       if (__ballot(small) != 0ull) {  // uniform
         const uint32_t h = chunk_path(small);
         if (MODE == kModeLCC && h != 0u) credit_low(G, yl, h);  // t(y)
