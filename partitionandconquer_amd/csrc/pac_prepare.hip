@@ -281,31 +281,26 @@ __global__ void k_orient_unpack(const unsigned long long *__restrict__ keys, uin
 // not keep), provided d+(x) > 0 and x is not the last element of N+(u) -- otherwise the
 // pair holds no triangle and is dropped.
 
-// The position of x (of rank rx) in N+(u) = okey[a .. a + n), which holds rx.
-__device__ __forceinline__ uint32_t list_pos(const uint32_t *__restrict__ okey, uint32_t a,
-                                             uint32_t n, uint32_t rx) {
-  uint32_t lo = a;
-  while (n > 0) {
-    const uint32_t half = n >> 1;
-    if (okey[lo + half] < rx) {
-      lo += half + 1;
-      n -= half + 1;
-    } else {
-      n = half;
-    }
-  }
-  return lo - a;
+// out[e] = e: the values of the transposing sort.
+__global__ void k_iota(uint32_t *__restrict__ out, uint64_t n) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    out[e] = (uint32_t)e;
 }
 
 // Ownership bits of the entries (x, u) of row x, the position of x in N+(u) of each owned
 // entry (epos[i]), and each row's streamed length rowwork[x] = the sum of its pairs'
 // suffix lengths (rowwork zeroed before): the lanes of one row are contiguous, so a
 // segmented scan across the wave leaves each row's part of the word in its last lane,
-// which adds it.
+// which adds it.  The position comes from tin, the oriented edges' positions sorted
+// (stably) by head: an entry (x, u) whose edge is u -> x is the j-th such entry of the
+// symmetric lists, j = i less the kept entries before it, and the j-th position of tin
+// is that of its edge -- both orders run by head, then by tail (the rows of the symmetric
+// lists and of oadj ascend).
 __global__ void __launch_bounds__(256)
 k_pair_flags(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
-             const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ okey,
-             const uint32_t *__restrict__ rank, const unsigned long long *__restrict__ oflags,
+             const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ tin,
+             const unsigned long long *__restrict__ oflags, const uint32_t *__restrict__ owpre,
              const uint32_t *__restrict__ crow, uint32_t V, uint32_t nnz,
              unsigned long long *__restrict__ pflags, uint32_t *__restrict__ epos,
              uint32_t *__restrict__ rowwork) {
@@ -323,10 +318,11 @@ k_pair_flags(const uint32_t *__restrict__ off, const uint32_t *__restrict__ adj,
       uint32_t x = 0xFFFFFFFFu, w = 0;  // lanes past the end: no row
       if (i < nnz) {
         x = row_of(off, lo, hi, (uint32_t)i);
-        if (!bit_of(oflags[k], lane) && ooff[x + 1] > ooff[x]) {  // u -> x, N+(x) not empty
+        const unsigned long long kept = oflags[k];
+        if (!bit_of(kept, lane) && ooff[x + 1] > ooff[x]) {  // u -> x, N+(x) not empty
           const uint32_t u = adj[i];
           const uint32_t a = ooff[u], n = ooff[u + 1] - a;
-          const uint32_t pos = list_pos(okey, a, n, rank[x]);
+          const uint32_t pos = tin[(uint32_t)i - owpre[k] - lanes_below(kept)] - a;
           if (pos + 1u < n) {
             own = true;
             w = n - pos - 1u;
@@ -610,15 +606,36 @@ int build_pairs(Graph *g) {
   PAC_TRY(fill_u32(ctx->stream, g->rowwork.as<uint32_t>(), V, 0u));
   DevBuf epos;  // per symmetric entry owned as a pair: the owner's position in the partner's list
   PAC_TRY(epos.reserve(sizeof(uint32_t) * (nnz ? nnz : 1)));
-  if (nnz) {
+  if (nnz) {  // then m = nnz / 2 > 0
+    // tin: the positions in oadj sorted by head (a stable sort of e by oadj[e])
+    DevBuf tin, keys, vals, tmp;
+    PAC_TRY(tin.reserve(sizeof(uint32_t) * g->m));
+    PAC_TRY(keys.reserve(sizeof(uint32_t) * g->m));
+    PAC_TRY(vals.reserve(sizeof(uint32_t) * g->m));
+    hipLaunchKernelGGL(k_iota, dim3(grid_for_n(g->m, 256 * 4)), dim3(256), 0, ctx->stream,
+                       vals.as<uint32_t>(), (uint64_t)g->m);
+    PAC_CHECK_HIP(hipGetLastError());
+    uint32_t L = 1;
+    while (L < 32 && (1ull << L) < V) ++L;
+    size_t tb = 0;
+    PAC_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tb, g->oadj.as<const uint32_t>(),
+                                            keys.as<uint32_t>(), vals.as<const uint32_t>(),
+                                            tin.as<uint32_t>(), (size_t)g->m, 0u, L,
+                                            ctx->stream));
+    PAC_TRY(tmp.reserve(tb));
+    PAC_CHECK_HIP(rocprim::radix_sort_pairs(tmp.p, tb, g->oadj.as<const uint32_t>(),
+                                            keys.as<uint32_t>(), vals.as<const uint32_t>(),
+                                            tin.as<uint32_t>(), (size_t)g->m, 0u, L,
+                                            ctx->stream));
     hipLaunchKernelGGL(k_pair_flags, dim3(grid_for_n(nchunks, 4)), dim3(256), 0, ctx->stream,
                        g->off.as<const uint32_t>(), g->adj.as<const uint32_t>(),
-                       g->ooff.as<const uint32_t>(), g->okey.as<const uint32_t>(),
-                       g->rank.as<const uint32_t>(), g->oflags.as<const unsigned long long>(),
+                       g->ooff.as<const uint32_t>(), tin.as<const uint32_t>(),
+                       g->oflags.as<const unsigned long long>(), g->owpre.as<const uint32_t>(),
                        g->crow.as<const uint32_t>(), V, (uint32_t)nnz,
                        g->pflags.as<unsigned long long>(), epos.as<uint32_t>(),
                        g->rowwork.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // the sort's buffers are freed here
   }
   uint32_t np = 0;
   {
