@@ -343,11 +343,12 @@ __device__ __forceinline__ void clear_hit_words(const GDev &G, uint32_t nb, uint
 }
 
 // ---- one row, sorted-list strategies (merge, gallop): A staged in LDS ---------------
-// Not inlined, like row_hbm: in the persistent kernel both are the fallbacks of the hash
-// path, and as calls their code stays out of the kernel's body (R-MAT scale 24, LCC mode:
-// 33.7-34.0 ms against 34.3-34.7 ms inlined, same box).
+// (Inlined, like row_hbm: as calls, which kept their code out of the persistent kernel's
+// body, the LDS arrays reached them as generic pointers, and the launches whose dispatch
+// picks these strategies slowed down -- 256 mixed graphs 32.3 -> 36.5 ms -- while R-MAT
+// scale 24 LCC moved within its run-to-run spread.)
 template <int STRAT, int MODE, int WAVES, int CAP>
-__device__ __attribute__((noinline)) void row_sorted(const GDev &G, uint32_t r, uint32_t k_lo,
+__device__ __forceinline__ void row_sorted(const GDev &G, uint32_t r, uint32_t k_lo,
                                            uint32_t k_hi, Smem<CAP, MODE, WAVES> &sm,
                                            unsigned long long &acc) {
   constexpr uint32_t BS = 64u * WAVES;
@@ -935,7 +936,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
 // ---- one row whose N+(r) exceeds the LDS capacity: lookups in HBM ------------------
 // (SOLO: one wave does the whole row, as a fallback of the one-wave hash path.)
 template <int MODE, int WAVES, bool SOLO = false>
-__device__ __attribute__((noinline)) void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo,
+__device__ __forceinline__ void row_hbm(const GDev &G, uint32_t r, uint32_t c_lo,
                                         uint32_t c_hi, unsigned long long &acc) {
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = SOLO ? 0u : tid >> 6;
   const uint32_t a0 = G.ooff[r], nA = G.ooff[r + 1] - a0;
