@@ -100,6 +100,8 @@ struct GraphDev {
   // SUP mode: the hit words of the streamed lists.  The pair with rank r among the pairs
   // streaming N+(y) owns the 64-bit words hitw[wbase[y] + r*nw ..+ nw), nw = ceil(d+(y)/64);
   // bit l of word c is set when element 64c + l of N+(y) was found in the owner's list.
+  // The ranks follow the pairs' positions in N+(y), so the rows whose streams reach
+  // column c (start at or before it) are the first ones.
   const unsigned long long *pword;  // per pair: its first hit word, wbase[y] + r*nw
   unsigned long long *hitw;
   uint32_t V;
@@ -168,8 +170,10 @@ struct Graph {
   bool pairs_built = false;
   // Per-edge supports (SUP mode) from hit words instead of one atomic per triangle:
   // pstreams[y] = number of pairs streaming N+(y); wbase[y] = first word of y's streams;
-  // pword, hitw as in GraphDev.
-  DevBuf pstreams, pword, wbase, hitw, supbig;
+  // prank[e] = the number of pairs' edges before position e of oadj (the rank of the pair
+  // of edge e among its partner's pairs is prank[e] - prank[ooff[y]]); pword, hitw as in
+  // GraphDev.
+  DevBuf pstreams, pword, wbase, hitw, supbig, prank;
   std::vector<uint32_t> h_supbig;  // (y, first stream) units of kSupRun streams of the
                                    // vertices streamed by more than 64 pairs
   bool hitw_built = false;

@@ -1234,7 +1234,7 @@ __global__ void k_lcc(const unsigned long long *__restrict__ tvert,
 // vertices' first edges held one per lane.
 __global__ void __launch_bounds__(256)
 k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
-          uint32_t *__restrict__ sup, const uint32_t *__restrict__ deg,
+          const uint32_t *__restrict__ sup, const uint32_t *__restrict__ deg,
           const uint32_t *__restrict__ hdeg, uint32_t V, int32_t *src, int32_t *dst,
           uint32_t *inter, uint32_t *uni, float *ratio) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -1276,7 +1276,6 @@ k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
       for (int u = 0; u < 4; ++u) {
         const uint32_t dt = (uint32_t)__shfl((int)du, (int)t[u], 64);
         if (e[u] < E1) {
-          sup[e[u]] = c[u];
           const uint32_t un = dt + dv[u] - c[u];
           if (src) src[e[u]] = (int32_t)(v0 + t[u]);
           if (dst) dst[e[u]] = (int32_t)v[u];
@@ -1661,11 +1660,16 @@ k_sup_small(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ pstr
 // Vertices streamed by more than 64 pairs, one workgroup per unit (y, first stream r0) of
 // kSupRun streams; each wave takes bands of kBand word columns, its lanes reading kBand
 // consecutive words of one stream each (a row's words share cache lines, so a band costs
-// one line request per row instead of kBand); units add atomically.
+// one line request per row instead of kBand); units add atomically.  A band reads only
+// the rows whose streams reach it: the rows come in the order of the pairs' positions in
+// N+(y), and those with a position below 64 (c + 1) - 1 start at or before column c, so
+// they are the first prank[a0 + 64 (c + 1) - 1] - prank[a0] rows (R-MAT scale 25: about
+// half of the words).
 __global__ void __launch_bounds__(256)
 k_sup_big(const uint32_t *__restrict__ units, const uint32_t *__restrict__ ooff,
           const uint32_t *__restrict__ pstreams, const unsigned long long *__restrict__ wbase,
-          const unsigned long long *__restrict__ hitw, uint32_t *__restrict__ sup) {
+          const uint32_t *__restrict__ prank, const unsigned long long *__restrict__ hitw,
+          uint32_t *__restrict__ sup) {
   constexpr uint32_t kBand = 8;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
   const PAC_GLOBAL unsigned long long *W = (const PAC_GLOBAL unsigned long long *)hitw;
@@ -1673,20 +1677,23 @@ k_sup_big(const uint32_t *__restrict__ units, const uint32_t *__restrict__ ooff,
   const uint32_t R = pstreams[y];
   const uint32_t a0 = ooff[y], nd = ooff[y + 1] - a0, nw = (nd + 63u) >> 6;
   const unsigned long long base = wbase[y];
-  const uint32_t r1 = min(R, r0 + kSupRun);
+  const uint32_t r1 = min(R, r0 + kSupRun), pr0 = prank[a0];
   for (uint32_t c0 = wave * kBand; c0 < nw; c0 += wpb * kBand) {  // wave-uniform
     const uint32_t nc = min(kBand, nw - c0);
+    // the rows reaching the band's last column
+    const uint32_t rb = min(r1, prank[a0 + min(64u * (c0 + nc) - 1u, nd)] - pr0);
+    if (rb <= r0) continue;  // wave-uniform: no row reaches the band
     uint32_t acc[kBand];
 #pragma unroll
     for (uint32_t k = 0; k < kBand; ++k) acc[k] = 0;
-    for (uint32_t r = r0; r < r1; r += 64) {  // wave-uniform
+    for (uint32_t r = r0; r < rb; r += 64) {  // wave-uniform
       const uint32_t row = r + lane;
       const PAC_GLOBAL unsigned long long *src = W + base + (unsigned long long)row * nw + c0;
       unsigned long long w[kBand];
 #pragma unroll
       for (uint32_t k = 0; k < kBand; ++k) w[k] = 0ull;
       if (nc == kBand) {  // wave-uniform: whole band, read as 16-byte loads
-        if (row < r1) {
+        if (row < rb) {
 #pragma unroll
           for (uint32_t k = 0; k < kBand; k += 2) {
             HIP_vector_type<unsigned long long, 2> t;
@@ -1697,7 +1704,7 @@ k_sup_big(const uint32_t *__restrict__ units, const uint32_t *__restrict__ ooff,
         }
       } else {
 #pragma unroll
-        for (uint32_t k = 0; k < kBand; ++k) w[k] = row < r1 && k < nc ? src[k] : 0ull;
+        for (uint32_t k = 0; k < kBand; ++k) w[k] = row < rb && k < nc ? src[k] : 0ull;
       }
 #pragma unroll
       for (uint32_t k = 0; k < kBand; ++k) {
@@ -2058,7 +2065,7 @@ int sup_from_hit_words(Ctx *ctx, Graph *g) {
     hipLaunchKernelGGL(k_sup_big, dim3((uint32_t)(g->h_supbig.size() / 2)), dim3(256), 0,
                        ctx->stream, g->supbig.as<const uint32_t>(),
                        g->ooff.as<const uint32_t>(), g->pstreams.as<const uint32_t>(),
-                       g->wbase.as<const unsigned long long>(),
+                       g->wbase.as<const unsigned long long>(), g->prank.as<const uint32_t>(),
                        g->hitw.as<const unsigned long long>(), g->sup.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
@@ -2091,7 +2098,7 @@ int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *i
   }
   hipLaunchKernelGGL(k_jaccard, dim3(grid_for(g->V, 4 * 64)), dim3(256), 0, ctx->stream,
                      g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
-                     g->sup.as<uint32_t>(), g->deg.as<const uint32_t>(),
+                     g->sup.as<const uint32_t>(), g->deg.as<const uint32_t>(),
                      g->hdeg.as<const uint32_t>(), g->V, src, dst, inter, uni, ratio);
   PAC_CHECK_HIP(hipGetLastError());
   return PAC_OK;

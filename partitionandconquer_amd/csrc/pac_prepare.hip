@@ -822,12 +822,47 @@ int row_summary(Graph *g, uint64_t part_width, uint32_t P, RowSummary &out) {
 
 namespace {
 
-// Number of pairs streaming each vertex's out-list.
-__global__ void k_partner_count(const uint32_t *__restrict__ ppar, uint64_t np,
-                                uint32_t *__restrict__ R) {
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
-       p += (uint64_t)gridDim.x * blockDim.x)
-    atomicAdd(&R[ppar[p]], 1u);
+// Per oriented edge y->h: 1 when it is a pair's edge (the pair of owner h and partner y:
+// d+(h) > 0 and h is not the last element of N+(y)), else 0 -- here the first condition,
+// for every edge (a gather of odeg in edge order) ...
+__global__ void k_pair_edge_flags(const uint32_t *__restrict__ oadj,
+                                  const uint32_t *__restrict__ odeg, uint64_t m,
+                                  uint32_t *__restrict__ flag) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    flag[e] = odeg[oadj[e]] > 0u ? 1u : 0u;
+}
+// ... then the second: the last edge of every non-empty list is cleared.
+__global__ void k_clear_last_edges(const uint32_t *__restrict__ ooff, uint32_t V,
+                                   uint32_t *__restrict__ flag) {
+  for (uint64_t y = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; y < V;
+       y += (uint64_t)gridDim.x * blockDim.x)
+    if (ooff[y + 1] > ooff[y]) flag[ooff[y + 1] - 1u] = 0u;
+}
+
+// Per vertex y, what its pairs' hit words need: (ooff[y], d+(y)) and the word base shifted
+// by the rank of its first pair edge, wbase[y] - prank[ooff[y]] * nw (mod 2^64; nw =
+// ceil(d+(y) / 64)), so that a pair's row is prank[e] alone.
+__global__ void k_vertex_words(const uint32_t *__restrict__ ooff,
+                               const uint32_t *__restrict__ prank,
+                               const unsigned long long *__restrict__ wbase, uint32_t V,
+                               uint4 *__restrict__ vw) {
+  for (uint64_t y = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; y < V;
+       y += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t a = ooff[y], d = ooff[y + 1] - a;
+    const unsigned long long b =
+        wbase[y] - (unsigned long long)prank[a] * (unsigned long long)((d + 63u) >> 6);
+    vw[y] = make_uint4(a, d, (uint32_t)b, (uint32_t)(b >> 32));
+  }
+}
+
+// Pairs streaming each vertex's out-list: R[y] = prank[ooff[y+1]] - prank[ooff[y]].
+__global__ void k_partner_streams(const uint32_t *__restrict__ ooff,
+                                  const uint32_t *__restrict__ prank, uint32_t V,
+                                  uint32_t *__restrict__ R) {
+  for (uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; y < V;
+       y += (uint64_t)gridDim.x * blockDim.x)
+    R[y] = prank[ooff[y + 1]] - prank[ooff[y]];
 }
 
 // Hit words per vertex: pairs streaming it x words per stream; entry V is 0, so the
@@ -845,21 +880,21 @@ __global__ void k_word_counts(const uint32_t *__restrict__ R, const uint32_t *__
 // stream is a word column c of the list.  The pair's words: its partner's word base,
 // plus its rank among the pairs streaming that partner times the words per stream, plus
 // the column of its first block -- the columns before it are never written and stay zero.
-// The ranks come in any order (the words of all of a vertex's streams are summed, so only
-// their distinctness matters).
-__global__ void k_pair_words(const uint32_t *__restrict__ ppar, const uint2 *__restrict__ pinfo,
-                             uint64_t np, const uint32_t *__restrict__ ooff,
-                             const unsigned long long *__restrict__ wbase,
-                             uint32_t *__restrict__ cursor, unsigned long long *__restrict__ pword,
-                             uint2 *__restrict__ pinfo64) {
+// The rank is the pair's edge's position among its partner's pair edges (prank), so the
+// rows of a list's words come in the order of their first columns: the column sums read,
+// for column c, only the rows whose stream reaches back to it.
+__global__ void k_pair_words(const uint32_t *__restrict__ ppar, const uint32_t *__restrict__ pedge,
+                             const uint2 *__restrict__ pinfo, uint64_t np,
+                             const uint4 *__restrict__ vw, const uint32_t *__restrict__ prank,
+                             unsigned long long *__restrict__ pword, uint2 *__restrict__ pinfo64) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np;
        p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t y = ppar[p];
-    const uint32_t r = atomicAdd(&cursor[y], 1u);
-    const uint32_t a = ooff[y], d = ooff[y + 1] - a;
+    const uint4 w = vw[ppar[p]];  // one 16-byte read per pair for its partner
+    const uint32_t a = w.x, d = w.y;
     const uint2 pi = pinfo[p];
     const uint32_t c0 = (pi.x - a) >> 6;  // the column of the stream's first block
-    pword[p] = wbase[y] + (unsigned long long)r * ((d + 63u) >> 6) + c0;
+    pword[p] = (((unsigned long long)w.w << 32) | w.z) +
+               (unsigned long long)prank[pedge[p]] * ((d + 63u) >> 6) + c0;
     pinfo64[p] = make_uint2(a + 64u * c0, d - 64u * c0);
   }
 }
@@ -882,15 +917,44 @@ int build_hit_words(Graph *g) {
   PAC_TRY(g->pword.reserve(sizeof(unsigned long long) * (np ? np : 1)));
   PAC_TRY(g->pinfo64.reserve(sizeof(uint2) * (np ? np : 1)));
   PAC_TRY(g->wbase.reserve(sizeof(unsigned long long) * ((uint64_t)V + 1)));
-  DevBuf words, cursor;
+  PAC_TRY(g->prank.reserve(sizeof(uint32_t) * ((uint64_t)g->m + 1)));
+  DevBuf words;
   PAC_TRY(words.reserve(sizeof(unsigned long long) * ((uint64_t)V + 1)));
-  PAC_TRY(cursor.reserve(sizeof(uint32_t) * ((uint64_t)V + 1)));
   PAC_TRY(fill_u32(ctx->stream, g->pstreams.as<uint32_t>(), (uint64_t)V + 1, 0u));
-  PAC_TRY(fill_u32(ctx->stream, cursor.as<uint32_t>(), (uint64_t)V + 1, 0u));
-  if (np) {
-    hipLaunchKernelGGL(k_partner_count, dim3(grid_elems(np)), dim3(256), 0, ctx->stream,
-                       g->ppar.as<const uint32_t>(), (uint64_t)np, g->pstreams.as<uint32_t>());
+  {
+    // prank = the exclusive scan of the pair-edge flags over oadj
+    DevBuf flag;
+    PAC_TRY(flag.reserve(sizeof(uint32_t) * ((uint64_t)g->m + 1)));
+    PAC_TRY(fill_u32(ctx->stream, flag.as<uint32_t>() + g->m, 1, 0u));
+    if (g->m)
+      hipLaunchKernelGGL(k_pair_edge_flags, dim3(grid_elems(g->m)), dim3(256), 0, ctx->stream,
+                         g->oadj.as<const uint32_t>(), g->odeg.as<const uint32_t>(),
+                         (uint64_t)g->m, flag.as<uint32_t>());
+    if (V)
+      hipLaunchKernelGGL(k_clear_last_edges, dim3(grid_elems(V)), dim3(256), 0, ctx->stream,
+                         g->ooff.as<const uint32_t>(), V, flag.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
+    size_t tb = 0;
+    PAC_CHECK_HIP(rocprim::exclusive_scan(nullptr, tb, flag.as<const uint32_t>(),
+                                          g->prank.as<uint32_t>(), 0u, (size_t)g->m + 1,
+                                          rocprim::plus<uint32_t>(), ctx->stream));
+    PAC_TRY(ctx->scratch.reserve(tb));
+    PAC_CHECK_HIP(rocprim::exclusive_scan(ctx->scratch.p, tb, flag.as<const uint32_t>(),
+                                          g->prank.as<uint32_t>(), 0u, (size_t)g->m + 1,
+                                          rocprim::plus<uint32_t>(), ctx->stream));
+    if (V)
+      hipLaunchKernelGGL(k_partner_streams, dim3(grid_elems(V)), dim3(256), 0, ctx->stream,
+                         g->ooff.as<const uint32_t>(), g->prank.as<const uint32_t>(), V,
+                         g->pstreams.as<uint32_t>());
+    PAC_CHECK_HIP(hipGetLastError());
+    uint32_t nflag = 0;
+    PAC_CHECK_HIP(hipMemcpyAsync(&nflag, g->prank.as<uint32_t>() + g->m, 4,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // flag is freed here
+    if (nflag != np) {
+      set_error("hit-word layout: %u pair edges for %u pairs", nflag, np);
+      return PAC_ERR_HIP;
+    }
   }
   hipLaunchKernelGGL(k_word_counts, dim3(grid_elems((uint64_t)V + 1)), dim3(256), 0, ctx->stream,
                      g->pstreams.as<const uint32_t>(), g->odeg.as<const uint32_t>(), V,
@@ -905,12 +969,18 @@ int build_hit_words(Graph *g) {
                                         g->wbase.as<unsigned long long>(), 0ull, (size_t)V + 1,
                                         rocprim::plus<unsigned long long>(), ctx->stream));
   if (np) {
+    DevBuf vw;
+    PAC_TRY(vw.reserve(sizeof(uint4) * V));
+    hipLaunchKernelGGL(k_vertex_words, dim3(grid_elems(V)), dim3(256), 0, ctx->stream,
+                       g->ooff.as<const uint32_t>(), g->prank.as<const uint32_t>(),
+                       g->wbase.as<const unsigned long long>(), V, vw.as<uint4>());
     hipLaunchKernelGGL(k_pair_words, dim3(grid_elems(np)), dim3(256), 0, ctx->stream,
-                       g->ppar.as<const uint32_t>(), g->pinfo.as<const uint2>(), (uint64_t)np,
-                       g->ooff.as<const uint32_t>(), g->wbase.as<const unsigned long long>(),
-                       cursor.as<uint32_t>(), g->pword.as<unsigned long long>(),
+                       g->ppar.as<const uint32_t>(), g->pedge.as<const uint32_t>(),
+                       g->pinfo.as<const uint2>(), (uint64_t)np, vw.as<const uint4>(),
+                       g->prank.as<const uint32_t>(), g->pword.as<unsigned long long>(),
                        g->pinfo64.as<uint2>());
     PAC_CHECK_HIP(hipGetLastError());
+    PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // vw is freed here
   }
   unsigned long long nwords = 0;
   PAC_CHECK_HIP(hipMemcpyAsync(&nwords, g->wbase.as<unsigned long long>() + V, 8,
@@ -919,7 +989,7 @@ int build_hit_words(Graph *g) {
   if (V)
     PAC_CHECK_HIP(hipMemcpyAsync(R.data(), g->pstreams.p, 4ull * V, hipMemcpyDeviceToHost,
                                  ctx->stream));
-  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // words and cursor are freed on return
+  PAC_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // words is freed on return
   // Every pair writes the words of its stream's columns in each SUP launch; the zero fill
   // gives the columns before its stream their value (no triangle) for good.
   // 8 words of padding: the column sums load 8 words per stream from any word position
