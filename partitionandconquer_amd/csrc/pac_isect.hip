@@ -729,9 +729,12 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
         }
       }
       if (MODE == kModeSUP) {
-        // edges y->w: one hit word per 64 streamed elements, summed after the launch
+        // edges y->w: one hit word per 64 streamed elements, summed after the launch.  A
+        // zero word is not stored: the words start zeroed when the layout is built, and a
+        // pair's words are the same in every launch (or zeroed by the paths that credit the
+        // supports directly), so a word with no hit now never held one.
         const uint32_t nsub = min(4u, (rem + SSTEP - 1u) / SSTEP);
-        if (lane < nsub) G.hitw[hb + lane] = hw;
+        if (lane < nsub && hw != 0ull) G.hitw[hb + lane] = hw;
       }
       if (MODE != kModeTC && jn != jj) {
         // pair jj is done: its count goes to its lane, credited with the batch's others
@@ -833,7 +836,7 @@ __device__ __forceinline__ bool row_hash(const GDev &G, uint32_t r, uint32_t c_l
       wave_sync();
       if (small) {
         const unsigned long long w = pw[lane];
-        G.hitw[wl] = w;  // edges y->w, summed after the launch
+        if (w != 0ull) G.hitw[wl] = w;  // edges y->w, summed after the launch (zero: see stream_batch)
         const uint32_t cnt = (uint32_t)__popcll(w);
         if (cnt) gadd(&G.sup[el], cnt);  // the pair's edge y->r
       }
