@@ -23,8 +23,10 @@
 //             (B elements in A held in LDS, or A elements in B in HBM).
 //   kBitmap : A is held in a two-choice cuckoo hash table in LDS (two buckets of two
 //             slots per key, load at most 1/2); each B element is looked up with two
-//             8-byte LDS reads and four compares, no loop (exact).  Pairs of at most
-//             kChunkMax elements are looked up in 4-element chunks spread over the lanes.
+//             8-byte LDS reads and four compares, no loop (exact); a table that
+//             leaves a key homeless is built again with another hash multiplier.  Pairs of
+//             at most kChunkMax elements are looked up in 4-element chunks spread over the
+//             lanes.
 // Counting is exact by construction in all three: a hit is only recorded for an
 // element equal to a list entry.
 //
