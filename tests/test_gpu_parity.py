@@ -200,50 +200,145 @@ def test_degenerate_graphs(ctx, orc):
     assert inter.tolist() == [0] and uni.tolist() == [2] and ratio.tolist() == [0.0]
 
 
-def _cuckoo_buckets(x, lg):
-    """The two LDS hash-table buckets of key x among 2^lg (buckets() in pac_isect.hip),
-    used here only to build inputs whose tables cannot be filled."""
-    sh = 32 - lg
-    h = (((x & 0xFFFFFF) * 0x9E3779) + (x >> 20)) & 0xFFFFFFFF
-    b1 = h >> sh
-    return b1, b1 ^ (((h >> 4) & ((1 << lg) - 1)) | 1)
+# The hash multipliers of the LDS tables, in the order a row's builds try them, and the
+# two buckets of key x among 2^lg under one of them: a replica of hash_mul() and buckets()
+# in pac_isect.hip (h = (x mod 2^24) * M + x; b1 = the top lg bits of h, b2 = b1 xor (bits
+# [3, 3 + lg) of h, made odd)), used only to build inputs whose tables cannot be filled.
+HASH_MULS = (0x9E3778, 0xC2B2AE, 0x85EBCA, 0x27D4EA)
 
 
-def test_hash_table_build_failure_falls_back(ctx, orc):
-    """Rows whose out-lists hold six keys sharing both buckets (four slots between them):
-    two keys stay homeless, the hash build gives up and the row is counted by a fallback.
-    Vertex r joins the n keys of K, which form a clique, so r owns n pairs: rows of 16 and
-    40 pairs go one per wave (fallback: HBM lookups by that wave), the row of 80 pairs to
-    the whole workgroup (fallback: the sorted-list path).  Counts, LCC and Jaccard must
-    still equal the oracle's."""
-    rng = np.random.default_rng(7)
+def _cuckoo_buckets(x, lg, mul):
+    h = ((x & 0xFFFFFF) * mul + x) & 0xFFFFFFFF
+    b1 = h >> (32 - lg)
+    return b1, b1 ^ (((h >> 3) & ((1 << lg) - 1)) | 1)
+
+
+def _colliding(lg, mul, count, lo, taken):
+    """count IDs from lo up, none in taken, sharing both buckets under mul: more than four
+    such keys cannot all be placed (the two buckets hold four), whatever the order."""
+    by = {}
+    x = lo
+    while True:
+        if x not in taken:
+            b = _cuckoo_buckets(x, lg, mul)
+            by.setdefault(b, []).append(x)
+            if len(by[b]) == count:
+                return by[b]
+        x += 1
+
+
+def _owner_row_graph(rows, V):
+    """Per (h, K, q): vertex h stages N+(h) = K (a clique) and owns q pairs, one per
+    in-neighbour t (adjacent to h and to all of K, d(t) = |K| + 1 < d(h) = |K| + q), each
+    streaming all of K (the suffix of N+(t) after h), so row h's table must hold K."""
     src, dst = [], []
-    for r, n, lg, lo, hi in ((5, 16, 4, 1000, 1 << 19), (7, 40, 6, 1 << 19, 1 << 20),
-                             (9, 80, 7, 1 << 20, 3 << 19)):
-        by = {}
-        for x in range(lo, hi):
-            by.setdefault(_cuckoo_buckets(x, lg), []).append(x)
-            if len(by[_cuckoo_buckets(x, lg)]) == 6:
-                bad = by[_cuckoo_buckets(x, lg)]
-                break
-        pool = np.setdiff1d(np.arange(lo, hi), bad)
-        K = sorted(bad + rng.choice(pool, n - 6, replace=False).tolist())
+    t_next = V - 1
+    for h, K, q in rows:
         for i, a in enumerate(K):
-            src.append(r)
+            src.append(h)
             dst.append(a)
             for b in K[i + 1:]:
                 src.append(a)
                 dst.append(b)
-    ref = orc.edges_to_csr(3 << 19, src, dst)
+        for _ in range(q):
+            t = t_next
+            t_next -= 1
+            src.append(t)
+            dst.append(h)
+            for a in K:
+                src.append(t)
+                dst.append(a)
+    return src, dst
+
+
+def _check_hash_rows(ctx, orc, src, dst, V):
+    ref = orc.edges_to_csr(V, src, dst)
     g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
     T = check_all(ctx, g, ref, orc, P=4, strategies=[pac.FORCE_BITMAP, pac.DISPATCH_FIXED])
-    assert T == sum(math.comb(n, 3) + math.comb(n, 2) for n in (16, 40, 80))
     for l in LAUNCHES:
         l32, l64, t = g.lcc(pac.FORCE_BITMAP, l)
         assert t == T and bits_equal(l64, orc.lcc_merge(ref, NT))
         src2, dst2, inter, uni, ratio = g.jaccard_edges(pac.FORCE_BITMAP, l)
         ri, ru, rr = orc.jaccard_edges(ref, src2, dst2, NT)
         assert np.array_equal(inter, ri) and np.array_equal(uni, ru) and bits_equal(ratio, rr)
+    return T
+
+
+def _rows_expected_T(rows):
+    # per row: the clique, h with it, and each in-neighbour with h and the clique
+    return sum(math.comb(len(K), 3) + (q + 1) * math.comb(len(K), 2) + q * len(K)
+               for _, K, q in rows)
+
+
+def test_hash_table_rebuilt_with_the_next_multiplier(ctx, orc):
+    """Rows whose staged lists hold six keys sharing both buckets under the first hash
+    multiplier: the first build always leaves keys homeless, and the row is built again
+    with the second multiplier, under which (a sequential replay says) the list fits.  Rows
+    of 3 pairs go one per wave, rows of 80 and 100 pairs to the whole workgroup."""
+    V = 3 << 19
+    rng = np.random.default_rng(7)
+    rows, taken = [], set()
+    for h, n, q, lo in ((5, 16, 3, 1000), (7, 40, 80, 1 << 19), (9, 80, 100, 1 << 20)):
+        lg = max(4, (n - 1).bit_length())
+        bad = _colliding(lg, HASH_MULS[0], 6, lo, taken)
+        pool = np.setdiff1d(np.arange(lo, lo + (1 << 18)), np.array(sorted(taken | set(bad))))
+        K = sorted(bad + rng.choice(pool, n - 6, replace=False).tolist())
+        assert _replay_build(K, lg, HASH_MULS[0]) is False
+        assert _replay_build(K, lg, HASH_MULS[1]) is True
+        taken |= set(K)
+        rows.append((h, K, q))
+    src, dst = _owner_row_graph(rows, V)
+    assert _check_hash_rows(ctx, orc, src, dst, V) == _rows_expected_T(rows)
+
+
+def test_hash_table_build_failure_falls_back(ctx, orc):
+    """Rows whose staged lists hold, for each of the four hash multipliers, five keys sharing
+    both buckets under it: every build leaves a key homeless and the row is counted by a
+    fallback (HBM lookups by the wave for a row of 3 pairs, the sorted-list path for rows of
+    80 and 100 pairs).  Counts, LCC and Jaccard must still equal the oracle's."""
+    V = 3 << 19
+    rng = np.random.default_rng(11)
+    rows, taken = [], set()
+    for h, n, q, lo in ((5, 32, 3, 1000), (7, 40, 80, 1 << 19), (9, 80, 100, 1 << 20)):
+        lg = max(4, (n - 1).bit_length())
+        bad = []
+        for mul in HASH_MULS:
+            grp = _colliding(lg, mul, 5, lo, taken | set(bad))
+            bad += grp
+        pool = np.setdiff1d(np.arange(lo, lo + (1 << 18)), np.array(sorted(taken | set(bad))))
+        K = sorted(bad + rng.choice(pool, n - len(bad), replace=False).tolist())
+        assert len(set(K)) == n
+        for mul in HASH_MULS:
+            assert _replay_build(K, lg, mul) is False
+        taken |= set(K)
+        rows.append((h, K, q))
+    src, dst = _owner_row_graph(rows, V)
+    assert _check_hash_rows(ctx, orc, src, dst, V) == _rows_expected_T(rows)
+
+
+def _replay_build(K, lg, mul, max_kicks=48):
+    """A sequential replay of row_hash's table build: True when every key finds a slot."""
+    nb = 1 << lg
+    tab = [None] * (2 * nb)
+    for x in K:
+        frm = None
+        for kick in range(max_kicks + 1):
+            b1, b2 = _cuckoo_buckets(x, lg, mul)
+            placed = False
+            for s_ in (2 * b1, 2 * b1 + 1, 2 * b2, 2 * b2 + 1):
+                if tab[s_] is None:
+                    tab[s_] = x
+                    placed = True
+                    break
+            if placed:
+                break
+            if kick == max_kicks:
+                return False
+            vb = b2 if frm == b1 else b1
+            slot = 2 * vb + ((kick >> 1) & 1)
+            tab[slot], x = x, tab[slot]
+            frm = vb
+    return True
 
 
 def test_jaccard_edges_scale13(ctx, orc):
