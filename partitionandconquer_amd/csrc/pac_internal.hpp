@@ -177,7 +177,7 @@ struct Graph {
   std::vector<uint32_t> h_supbig;  // (y, first stream) units of kSupRun streams of the
                                    // vertices streamed by more than 64 pairs
   bool hitw_built = false;
-  DevBuf hdeg;  // per oriented edge, the degree of its head (Jaccard unions), built on first use
+  DevBuf dsum;  // per oriented edge t->h, d(t) + d(h) (Jaccard unions), built on first use
   // partitioning and work items
   bool prepared = false;
   uint32_t P = 0;

@@ -1188,14 +1188,6 @@ k_stream_copy(const v4u32 *__restrict__ src, v4u32 *__restrict__ dst, uint64_t n
   }
 }
 
-// out[i] = table[idx[i]]
-__global__ void k_gather_u32(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ table,
-                             uint64_t n, uint32_t *__restrict__ out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = table[idx[i]];
-}
-
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -1233,17 +1225,37 @@ __global__ void k_lcc(const unsigned long long *__restrict__ tvert,
 // Per-edge Jaccard (pair formula of Graph.cpp:808-809 on every oriented edge).
 // The support of edge e (in sup): its credits as an edge of the staged list and as a
 // pair's edge, added during the launch, and as an edge of the streamed lists, added from
-// the hit words after it.
-// Each wave takes 64 consecutive vertices and walks their out-edges (contiguous in the
-// oriented CSR) 64 at a time; an edge's tail is found by a binary search over the
-// vertices' first edges held one per lane.
+// the hit words after it.  The union is dsum[e] - sup[e], dsum[e] = d(t) + d(h) of the
+// edge t->h (built once per graph, k_edge_head_deg + k_edge_add_tail_deg).
+// k_jaccard_flat: thread per edge, when the edges' tails are not asked for.
+__global__ void k_jaccard_flat(const uint32_t *__restrict__ oadj, const uint32_t *__restrict__ sup,
+                               const uint32_t *__restrict__ dsum, uint64_t m, int32_t *dst,
+                               uint32_t *inter, uint32_t *uni, float *ratio) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = sup[e], un = dsum[e] - c;
+    if (dst) dst[e] = (int32_t)oadj[e];
+    if (inter) inter[e] = c;
+    if (uni) uni[e] = un;
+    if (ratio) ratio[e] = (float)((double)c / (double)un);
+  }
+}
+
+// dsum[e] = d(y) + d(oadj[e]) for the out-edges e of every vertex y: the heads' degrees
+// gathered in edge order, then each tail's degree added along its out-edges -- each wave
+// takes 64 consecutive vertices (lane i: vertex v0 + i, its first out-edge and degree) and
+// walks their out-edges 64 at a time, an edge's tail found by a 6-step search across the
+// lanes' first edges.
+__global__ void k_edge_head_deg(const uint32_t *__restrict__ oadj, const uint32_t *__restrict__ deg,
+                                uint64_t m, uint32_t *__restrict__ dsum) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    dsum[e] = deg[oadj[e]];
+}
 __global__ void __launch_bounds__(256)
-k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
-          const uint32_t *__restrict__ sup, const uint32_t *__restrict__ deg,
-          const uint32_t *__restrict__ hdeg, uint32_t V, int32_t *src, int32_t *dst,
-          uint32_t *inter, uint32_t *uni, float *ratio) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t wpb = blockDim.x >> 6;
+k_edge_add_tail_deg(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ deg,
+                    uint32_t V, uint32_t *__restrict__ dsum) {
+  const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
   for (uint64_t v0 = ((uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6)) * 64u; v0 < V;
        v0 += (uint64_t)gridDim.x * wpb * 64u) {
     const uint64_t vi = v0 + lane;
@@ -1251,10 +1263,38 @@ k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
     const uint32_t du = vi < V ? deg[vi] : 0u;
     const uint32_t E0 = readlane(a, 0);
     const uint32_t E1 = ooff[v0 + 64u < V ? v0 + 64u : V];
+    for (uint32_t e = E0 + lane; e - lane < E1; e += 64) {  // wave-uniform trip count
+      uint32_t t = 0;  // the last lane whose vertex starts at or before e
+#pragma unroll
+      for (uint32_t step = 32; step; step >>= 1) {
+        const uint32_t at = (uint32_t)__shfl((int)a, (int)(t + step), 64);
+        t = at <= e ? t + step : t;
+      }
+      const uint32_t dt = (uint32_t)__shfl((int)du, (int)t, 64);
+      if (e < E1) dsum[e] += dt;
+    }
+  }
+}
+
+// k_jaccard: with the tails (src) as well.  Each wave takes 64 consecutive vertices and
+// walks their out-edges (contiguous in the oriented CSR) 64 at a time; an edge's tail is
+// found by a binary search over the vertices' first edges held one per lane.
+__global__ void __launch_bounds__(256)
+k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
+          const uint32_t *__restrict__ sup, const uint32_t *__restrict__ dsum, uint32_t V,
+          int32_t *src, int32_t *dst, uint32_t *inter, uint32_t *uni, float *ratio) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wpb = blockDim.x >> 6;
+  for (uint64_t v0 = ((uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6)) * 64u; v0 < V;
+       v0 += (uint64_t)gridDim.x * wpb * 64u) {
+    const uint64_t vi = v0 + lane;
+    const uint32_t a = ooff[vi < V ? vi : V];  // first out-edge (the end past the last vertex)
+    const uint32_t E0 = readlane(a, 0);
+    const uint32_t E1 = ooff[v0 + 64u < V ? v0 + 64u : V];
     // Four chunks of 64 edges per trip, each step over all four before the next, so that
     // a wave has four loads of each kind in flight instead of one.
     for (uint32_t eb = E0; eb < E1; eb += 256) {  // wave-uniform
-      uint32_t e[4], t[4], v[4], c[4], dv[4];
+      uint32_t e[4], t[4], v[4], c[4], ds[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         e[u] = eb + 64u * u + lane;
@@ -1270,18 +1310,17 @@ k_jaccard(const uint32_t *__restrict__ ooff, const uint32_t *__restrict__ oadj,
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        v[u] = c[u] = dv[u] = 0;
+        v[u] = c[u] = ds[u] = 0;
         if (e[u] < E1) {
           if (dst) v[u] = oadj[e[u]];
           c[u] = sup[e[u]];
-          dv[u] = hdeg[e[u]];  // d(v), in edge order (no gather)
+          ds[u] = dsum[e[u]];
         }
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const uint32_t dt = (uint32_t)__shfl((int)du, (int)t[u], 64);
         if (e[u] < E1) {
-          const uint32_t un = dt + dv[u] - c[u];
+          const uint32_t un = ds[u] - c[u];
           if (src) src[e[u]] = (int32_t)(v0 + t[u]);
           if (dst) dst[e[u]] = (int32_t)v[u];
           if (inter) inter[e[u]] = c[u];
@@ -2092,19 +2131,27 @@ int epilogue_lcc(Ctx *ctx, Graph *g, bool want32, bool want64) {
 int epilogue_jaccard(Ctx *ctx, Graph *g, int32_t *src, int32_t *dst, uint32_t *inter,
                      uint32_t *uni, float *ratio) {
   if (g->V == 0) return PAC_OK;
-  if (!g->hdeg.p && g->m) {
-    // the head degrees in edge order, once per graph: a gather of deg over oadj that the
+  if (!g->dsum.p && g->m) {
+    // d(t) + d(h) per oriented edge, once per graph: a gather of deg over oadj that the
     // per-solve epilogue would otherwise repeat (a cache line per 4-byte read)
-    PAC_TRY(g->hdeg.reserve(sizeof(uint32_t) * g->m));
-    hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(g->m, 256)), dim3(256), 0, ctx->stream,
-                       g->oadj.as<const uint32_t>(), g->deg.as<const uint32_t>(), g->m,
-                       g->hdeg.as<uint32_t>());
+    PAC_TRY(g->dsum.reserve(sizeof(uint32_t) * g->m));
+    hipLaunchKernelGGL(k_edge_head_deg, dim3(grid_for(g->m, 256)), dim3(256), 0, ctx->stream,
+                       g->oadj.as<const uint32_t>(), g->deg.as<const uint32_t>(), (uint64_t)g->m,
+                       g->dsum.as<uint32_t>());
+    hipLaunchKernelGGL(k_edge_add_tail_deg, dim3(grid_for(g->V, 4 * 64)), dim3(256), 0,
+                       ctx->stream, g->ooff.as<const uint32_t>(), g->deg.as<const uint32_t>(),
+                       g->V, g->dsum.as<uint32_t>());
     PAC_CHECK_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_jaccard, dim3(grid_for(g->V, 4 * 64)), dim3(256), 0, ctx->stream,
-                     g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
-                     g->sup.as<const uint32_t>(), g->deg.as<const uint32_t>(),
-                     g->hdeg.as<const uint32_t>(), g->V, src, dst, inter, uni, ratio);
+  if (src)  // the tails need the per-vertex walk
+    hipLaunchKernelGGL(k_jaccard, dim3(grid_for(g->V, 4 * 64)), dim3(256), 0, ctx->stream,
+                       g->ooff.as<const uint32_t>(), g->oadj.as<const uint32_t>(),
+                       g->sup.as<const uint32_t>(), g->dsum.as<const uint32_t>(), g->V, src,
+                       dst, inter, uni, ratio);
+  else if (g->m)
+    hipLaunchKernelGGL(k_jaccard_flat, dim3(grid_for(g->m, 256 * 4)), dim3(256), 0, ctx->stream,
+                       g->oadj.as<const uint32_t>(), g->sup.as<const uint32_t>(),
+                       g->dsum.as<const uint32_t>(), (uint64_t)g->m, dst, inter, uni, ratio);
   PAC_CHECK_HIP(hipGetLastError());
   return PAC_OK;
 }
