@@ -358,6 +358,30 @@ def test_jaccard_edges_scale13(ctx, orc):
     assert len(np.unique(key)) == ref.nnz // 2
 
 
+def test_jaccard_edges_without_tails(ctx, orc):
+    """A solve asked for no tails (src) takes the elementwise epilogue over the per-edge
+    degree sums (as a solve that keeps its results on the device does): its heads,
+    intersections, unions and ratios equal those of the solve with tails, and the oracle's."""
+    ref = orc.gen_rmat(13, 16, 0.57, 0.19, 0.19, seed=131)
+    g = pac.Graph.from_csr(ctx, ref.vlist, ref.elist)
+    g.prepare(32)
+    src, dst, inter, uni, ratio = g.jaccard_edges(pac.DISPATCH_FIXED, pac.LAUNCH_GRAPH)
+    ri, ru, rr = orc.jaccard_edges(ref, src, dst, NT)
+    m = len(src)
+    dst2 = np.zeros(m, np.int32)
+    inter2 = np.zeros(m, np.uint32)
+    uni2 = np.zeros(m, np.uint32)
+    ratio2 = np.zeros(m, np.float32)
+    for l in LAUNCHES:
+        rc = pac.lib().pac_jaccard_edges(g._h, pac.DISPATCH_FIXED, l, None, dst2.ctypes.data,
+                                         inter2.ctypes.data, uni2.ctypes.data,
+                                         ratio2.ctypes.data)
+        assert rc == 0, l
+        assert np.array_equal(dst2, dst) and np.array_equal(inter2, ri)
+        assert np.array_equal(uni2, ru) and bits_equal(ratio2, rr)
+    g.jaccard_edges_device(pac.DISPATCH_FIXED, pac.LAUNCH_GRAPH)  # device-only: no copies
+
+
 def test_jaccard_strategy_switches(ctx, orc):
     """Per-edge supports stay exact when consecutive solves of one graph switch between the
     hash path (hit words of the streamed lists, summed after the launch) and the sorted
